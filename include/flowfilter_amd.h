@@ -1,0 +1,300 @@
+/*
+ * flowfilter_amd.h -- C ABI of the MI355X (gfx950) optical-flow filter
+ * library, libflowfilter_amd.so.
+ *
+ * The entry points mirror the C++ classes of namespace flowfilter::gpu
+ * one to one: each function replaces the reference method cited next to it
+ * (paths relative to the reference repository root). Plain pointers, sizes
+ * and opaque handles only; no HIP or C++ types cross this boundary.
+ *
+ * Conventions
+ *  - Every function returns ff_status. FF_OK is 0; on any other value
+ *    ff_last_error() returns a message for the calling thread. The library
+ *    never exits the process (the reference calls exit() on CUDA errors,
+ *    src/gpu/error.cu:15-24).
+ *  - Status codes map to the reference's exception types:
+ *    FF_ERR_INVALID_ARGUMENT <-> std::invalid_argument (shape mismatch,
+ *    wrong depth or item size), FF_ERR_LOGIC <-> std::logic_error
+ *    (unconfigured stage), FF_ERR_RUNTIME <-> bare std::exception thrown by
+ *    the setters (gamma <= 0, iterations <= 0, level out of range).
+ *  - Device images (ff_gpuimage) are reference-counted buffer handles with
+ *    the shared-pointer semantics of flowfilter::gpu::GPUImage
+ *    (include/flowfilter/gpu/image.h:47-123): every getter returns a new
+ *    handle sharing the buffer; release it with ff_gpuimage_destroy. Stages
+ *    hold their own references, so a buffer lives until its last holder is
+ *    gone.
+ *  - Work is enqueued on the library stream (ff_get_stream). compute()
+ *    returns without waiting; elapsed_time, download and upload calls wait
+ *    for the work they depend on. Objects are not thread-safe: use one
+ *    object per thread.
+ */
+#ifndef FLOWFILTER_AMD_H_
+#define FLOWFILTER_AMD_H_
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FF_API __attribute__((visibility("default")))
+
+typedef int ff_status;
+enum {
+    FF_OK = 0,
+    FF_ERR_INVALID_ARGUMENT = 1,
+    FF_ERR_LOGIC = 2,
+    FF_ERR_RUNTIME = 3,
+    FF_ERR_HIP = 4,
+    FF_ERR_BAD_ALLOC = 5,
+    FF_ERR_NULL_HANDLE = 6,
+    FF_ERR_WRONG_TYPE = 7
+};
+
+/* Host image descriptor, byte-identical to flowfilter::image_t
+ * (include/flowfilter/image.h:30-49; 40 bytes on LP64). */
+typedef struct ff_image_t {
+    int height;
+    int width;
+    int depth;
+    size_t pitch;    /* row pitch in bytes */
+    size_t itemSize; /* bytes per channel element */
+    void* data;
+} ff_image_t;
+
+typedef struct ff_gpuimage ff_gpuimage; /* flowfilter::gpu::GPUImage */
+typedef struct ff_stage ff_stage;       /* flowfilter::gpu::Stage and subclasses */
+
+/* ------------------------------------------------------------- library */
+FF_API const char* ff_last_error(void);
+FF_API ff_status ff_version(int* major, int* minor);
+/* Wait for all work enqueued by the library. */
+FF_API ff_status ff_synchronize(void);
+/* Enqueue on the given hipStream_t (NULL restores the library stream). */
+FF_API ff_status ff_set_stream(void* hip_stream);
+FF_API ff_status ff_get_stream(void** hip_stream);
+
+/* --------------------------------------------- host images (image.cpp) */
+/* createImage (include/flowfilter/image.h:59,69; src/image.cpp:14-40) */
+FF_API ff_status ff_create_image(int height, int width, int depth, size_t pixelSize, ff_image_t* out);
+/* destroyImage (include/flowfilter/image.h:78; src/image.cpp:43-52) */
+FF_API ff_status ff_destroy_image(ff_image_t* image);
+
+/* ------------------------------------ GPUImage (gpu/image.h:47-123) */
+/* GPUImage(height, width, depth, itemSize) (src/gpu/image.cu:32-42) */
+FF_API ff_status ff_gpuimage_create(int height, int width, int depth, int itemSize, ff_gpuimage** out);
+/* GPUImage() (src/gpu/image.cu:24-30): unallocated; upload allocates */
+FF_API ff_status ff_gpuimage_create_empty(ff_gpuimage** out);
+/* copy constructor: a second handle to the same buffer */
+FF_API ff_status ff_gpuimage_share(const ff_gpuimage* img, ff_gpuimage** out);
+FF_API ff_status ff_gpuimage_destroy(ff_gpuimage* img);
+/* height()/width()/depth()/pitch()/itemSize() (src/gpu/image.cu:52-70) */
+FF_API ff_status ff_gpuimage_height(const ff_gpuimage* img, int* out);
+FF_API ff_status ff_gpuimage_width(const ff_gpuimage* img, int* out);
+FF_API ff_status ff_gpuimage_depth(const ff_gpuimage* img, int* out);
+FF_API ff_status ff_gpuimage_pitch(const ff_gpuimage* img, int* out);
+FF_API ff_status ff_gpuimage_item_size(const ff_gpuimage* img, int* out);
+/* data() (src/gpu/image.cu:72-74): device pointer */
+FF_API ff_status ff_gpuimage_data(const ff_gpuimage* img, void** out);
+/* upload (src/gpu/image.cu:81-121) */
+FF_API ff_status ff_gpuimage_upload(ff_gpuimage* img, const ff_image_t* host);
+/* download (src/gpu/image.cu:123-150) */
+FF_API ff_status ff_gpuimage_download(const ff_gpuimage* img, ff_image_t* host);
+/* copyFrom (src/gpu/image.cu:152-170) */
+FF_API ff_status ff_gpuimage_copy_from(ff_gpuimage* img, const ff_gpuimage* src);
+/* clear (src/gpu/image.cu:172-176) */
+FF_API ff_status ff_gpuimage_clear(ff_gpuimage* img);
+/* Device-resident variants of upload/download (no reference counterpart):
+ * the source/destination is device memory with the given row pitch, the
+ * copy is enqueued on the library stream without waiting. */
+FF_API ff_status ff_gpuimage_upload_device(ff_gpuimage* img, const void* dev, size_t pitch);
+FF_API ff_status ff_gpuimage_download_device(const ff_gpuimage* img, void* dev, size_t pitch);
+
+/* ------------------------------------------ Stage (gpu/pipeline.h:29-78) */
+FF_API ff_status ff_stage_configure(ff_stage* s);              /* configure() */
+FF_API ff_status ff_stage_compute(ff_stage* s);                /* compute() */
+FF_API ff_status ff_stage_elapsed_time(ff_stage* s, float* ms); /* elapsedTime() */
+FF_API ff_status ff_stage_destroy(ff_stage* s);
+
+/* ------------------------------ ImageModel (gpu/imagemodel.h:22-63) */
+/* ImageModel() / ImageModel(GPUImage) (src/gpu/imagemodel.cu:24-43); input may be NULL */
+FF_API ff_status ff_imagemodel_create(const ff_gpuimage* input, ff_stage** out);
+FF_API ff_status ff_imagemodel_set_input_image(ff_stage* s, const ff_gpuimage* img);  /* :119-134 */
+FF_API ff_status ff_imagemodel_get_image_constant(ff_stage* s, ff_gpuimage** out);     /* :139-141 */
+FF_API ff_status ff_imagemodel_get_image_gradient(ff_stage* s, ff_gpuimage** out);     /* :144-146 */
+
+/* -------------------------- FlowPropagator (gpu/propagation.h:25-105) */
+/* FlowPropagator(GPUImage, iterations) (src/gpu/propagation.cu:20-44); inputFlow may be NULL */
+FF_API ff_status ff_flowpropagator_create(const ff_gpuimage* inputFlow, int iterations, ff_stage** out);
+FF_API ff_status ff_flowpropagator_set_iterations(ff_stage* s, int n);      /* :137-148 */
+FF_API ff_status ff_flowpropagator_get_iterations(ff_stage* s, int* out);   /* :151-153 */
+FF_API ff_status ff_flowpropagator_get_dt(ff_stage* s, float* out);        /* :156-158 */
+FF_API ff_status ff_flowpropagator_set_border(ff_stage* s, int border);     /* :160-170 */
+FF_API ff_status ff_flowpropagator_get_border(ff_stage* s, int* out);       /* :173-175 */
+FF_API ff_status ff_flowpropagator_set_invert_input_flow(ff_stage* s, int invert); /* :204-206 */
+FF_API ff_status ff_flowpropagator_get_invert_input_flow(ff_stage* s, int* out);   /* :209-211 */
+FF_API ff_status ff_flowpropagator_set_input_flow(ff_stage* s, const ff_gpuimage* flow); /* :178-195 */
+FF_API ff_status ff_flowpropagator_get_propagated_flow(ff_stage* s, ff_gpuimage** out);  /* :198-201 */
+
+/* ------------- FlowPropagatorPayload (gpu/propagation.h:108-200) */
+/* (src/gpu/propagation.cu:217-252); any input may be NULL */
+FF_API ff_status ff_flowpropagatorpayload_create(const ff_gpuimage* inputFlow, const ff_gpuimage* scalarPayload,
+                                                 const ff_gpuimage* vectorPayload, int iterations, ff_stage** out);
+FF_API ff_status ff_flowpropagatorpayload_set_iterations(ff_stage* s, int n);    /* :376-387 */
+FF_API ff_status ff_flowpropagatorpayload_get_iterations(ff_stage* s, int* out); /* :389-391 */
+FF_API ff_status ff_flowpropagatorpayload_get_dt(ff_stage* s, float* out);      /* :393-395 */
+FF_API ff_status ff_flowpropagatorpayload_set_border(ff_stage* s, int border);   /* :398-408 */
+FF_API ff_status ff_flowpropagatorpayload_get_border(ff_stage* s, int* out);     /* :411-413 */
+FF_API ff_status ff_flowpropagatorpayload_set_input_flow(ff_stage* s, const ff_gpuimage* flow);      /* :418-434 */
+FF_API ff_status ff_flowpropagatorpayload_set_scalar_payload(ff_stage* s, const ff_gpuimage* img);   /* :436-460 */
+FF_API ff_status ff_flowpropagatorpayload_set_vector_payload(ff_stage* s, const ff_gpuimage* img);   /* :462-486 */
+FF_API ff_status ff_flowpropagatorpayload_get_propagated_flow(ff_stage* s, ff_gpuimage** out);       /* :492-494 */
+FF_API ff_status ff_flowpropagatorpayload_get_propagated_scalar(ff_stage* s, ff_gpuimage** out);     /* :496-498 */
+FF_API ff_status ff_flowpropagatorpayload_get_propagated_vector(ff_stage* s, ff_gpuimage** out);     /* :500-502 */
+
+/* ------------------------------------ FlowUpdate (gpu/update.h:21-90) */
+/* FlowUpdate(inputFlow, inputImage, inputImageGradient, gamma, maxflow) (src/gpu/update.cu:20-50) */
+FF_API ff_status ff_flowupdate_create(const ff_gpuimage* inputFlow, const ff_gpuimage* inputImage,
+                                      const ff_gpuimage* inputImageGradient, float gamma, float maxflow,
+                                      ff_stage** out);
+FF_API ff_status ff_flowupdate_get_gamma(ff_stage* s, float* out);    /* :119-121 */
+FF_API ff_status ff_flowupdate_set_gamma(ff_stage* s, float gamma);   /* :124-132 */
+FF_API ff_status ff_flowupdate_get_maxflow(ff_stage* s, float* out);  /* :135-137 */
+FF_API ff_status ff_flowupdate_set_maxflow(ff_stage* s, float m);     /* :140-143 */
+FF_API ff_status ff_flowupdate_set_input_flow(ff_stage* s, const ff_gpuimage* img);           /* :146-162 */
+FF_API ff_status ff_flowupdate_set_input_image(ff_stage* s, const ff_gpuimage* img);          /* :165-181 */
+FF_API ff_status ff_flowupdate_set_input_image_gradient(ff_stage* s, const ff_gpuimage* img); /* :184-200 */
+FF_API ff_status ff_flowupdate_get_updated_flow(ff_stage* s, ff_gpuimage** out);              /* :203-206 */
+FF_API ff_status ff_flowupdate_get_updated_image(ff_stage* s, ff_gpuimage** out);             /* :209-212 */
+
+/* ------------------------------- DeltaFlowUpdate (gpu/update.h:93-174) */
+/* (src/gpu/update.cu:221-259) */
+FF_API ff_status ff_deltaflowupdate_create(const ff_gpuimage* inputFlow, const ff_gpuimage* inputDeltaFlow,
+                                           const ff_gpuimage* inputImageOld, const ff_gpuimage* inputImage,
+                                           const ff_gpuimage* inputImageGradient, float gamma, float maxflow,
+                                           ff_stage** out);
+FF_API ff_status ff_deltaflowupdate_get_gamma(ff_stage* s, float* out);    /* :351-353 */
+FF_API ff_status ff_deltaflowupdate_set_gamma(ff_stage* s, float gamma);   /* :355-363 */
+FF_API ff_status ff_deltaflowupdate_get_maxflow(ff_stage* s, float* out);  /* :366-368 */
+FF_API ff_status ff_deltaflowupdate_set_maxflow(ff_stage* s, float m);     /* :371-374 */
+FF_API ff_status ff_deltaflowupdate_set_input_flow(ff_stage* s, const ff_gpuimage* img);           /* :377-393 */
+FF_API ff_status ff_deltaflowupdate_set_input_delta_flow(ff_stage* s, const ff_gpuimage* img);     /* :396-412 */
+FF_API ff_status ff_deltaflowupdate_set_input_image_old(ff_stage* s, const ff_gpuimage* img);      /* :414-430 */
+FF_API ff_status ff_deltaflowupdate_set_input_image(ff_stage* s, const ff_gpuimage* img);          /* :433-449 */
+FF_API ff_status ff_deltaflowupdate_set_input_image_gradient(ff_stage* s, const ff_gpuimage* img); /* :452-468 */
+FF_API ff_status ff_deltaflowupdate_get_updated_flow(ff_stage* s, ff_gpuimage** out);              /* :471-474 */
+FF_API ff_status ff_deltaflowupdate_get_updated_delta_flow(ff_stage* s, ff_gpuimage** out);        /* :476-479 */
+FF_API ff_status ff_deltaflowupdate_get_updated_image(ff_stage* s, ff_gpuimage** out);             /* :482-485 */
+
+/* ------------------------- FlowSmoother (gpu/flowsmoothing.h:22-57) */
+/* FlowSmoother(inputFlow, iterations) (src/gpu/flowsmoothing.cu:20-38) */
+FF_API ff_status ff_flowsmoother_create(const ff_gpuimage* inputFlow, int iterations, ff_stage** out);
+FF_API ff_status ff_flowsmoother_get_iterations(ff_stage* s, int* out);   /* :111-114 */
+FF_API ff_status ff_flowsmoother_set_iterations(ff_stage* s, int n);      /* :117-127 */
+FF_API ff_status ff_flowsmoother_set_input_flow(ff_stage* s, const ff_gpuimage* img); /* :130-146 */
+FF_API ff_status ff_flowsmoother_get_smoothed_flow(ff_stage* s, ff_gpuimage** out);   /* :149-152 */
+
+/* -------------------------------- ImagePyramid (gpu/pyramid.h:22-89) */
+/* ImagePyramid(GPUImage, levels) (src/gpu/pyramid.cu:22-41) */
+FF_API ff_status ff_imagepyramid_create(const ff_gpuimage* image, int levels, ff_stage** out);
+FF_API ff_status ff_imagepyramid_set_input_image(ff_stage* s, const ff_gpuimage* img); /* :173-190 */
+FF_API ff_status ff_imagepyramid_set_levels(ff_stage* s, int levels);                  /* :192-201 */
+FF_API ff_status ff_imagepyramid_get_levels(ff_stage* s, int* out);                    /* :222-224 */
+FF_API ff_status ff_imagepyramid_get_image(ff_stage* s, int level, ff_gpuimage** out); /* :207-219 */
+
+/* ------------------------------ FlowFilter (gpu/flowfilter.h:31-145) */
+/* FlowFilter(h, w, smoothIterations, maxflow, gamma) (src/gpu/flowfilter.cu:45-76) */
+FF_API ff_status ff_flowfilter_create(int height, int width, int smoothIterations, float maxflow, float gamma,
+                                      ff_stage** out);
+/* FlowFilter(GPUImage inputImage) (src/gpu/flowfilter.cu:32-38) */
+FF_API ff_status ff_flowfilter_create_from_image(const ff_gpuimage* inputImage, ff_stage** out);
+FF_API ff_status ff_flowfilter_compute_image_model(ff_stage* s);  /* :155-162 */
+FF_API ff_status ff_flowfilter_compute_propagation(ff_stage* s);  /* :165-172 */
+FF_API ff_status ff_flowfilter_compute_update(ff_stage* s);       /* :175-197 */
+FF_API ff_status ff_flowfilter_set_input_image(ff_stage* s, const ff_gpuimage* img); /* :200-216 */
+FF_API ff_status ff_flowfilter_load_image(ff_stage* s, const ff_image_t* image);     /* :218-236 */
+FF_API ff_status ff_flowfilter_download_flow(ff_stage* s, ff_image_t* flow);         /* :238-240 */
+FF_API ff_status ff_flowfilter_download_image(ff_stage* s, ff_image_t* image);       /* :242-244 */
+FF_API ff_status ff_flowfilter_get_flow(ff_stage* s, ff_gpuimage** out);             /* :266-268 */
+FF_API ff_status ff_flowfilter_get_gamma(ff_stage* s, float* out);                   /* :271-273 */
+FF_API ff_status ff_flowfilter_set_gamma(ff_stage* s, float gamma);                  /* :276-285 */
+FF_API ff_status ff_flowfilter_get_maxflow(ff_stage* s, float* out);                 /* :288-290 */
+FF_API ff_status ff_flowfilter_set_maxflow(ff_stage* s, float maxflow);              /* :293-296 */
+FF_API ff_status ff_flowfilter_get_smooth_iterations(ff_stage* s, int* out);         /* :299-301 */
+FF_API ff_status ff_flowfilter_set_smooth_iterations(ff_stage* s, int n);            /* :304-306 */
+FF_API ff_status ff_flowfilter_set_propagation_border(ff_stage* s, int border);      /* :309-311 */
+FF_API ff_status ff_flowfilter_get_propagation_border(ff_stage* s, int* out);        /* :314-316 */
+FF_API ff_status ff_flowfilter_get_propagation_iterations(ff_stage* s, int* out);    /* :319-321 */
+FF_API ff_status ff_flowfilter_height(ff_stage* s, int* out);                        /* :323-326 */
+FF_API ff_status ff_flowfilter_width(ff_stage* s, int* out);                         /* :328-330 */
+/* loadImage from device memory (no reference counterpart; enqueued, no wait) */
+FF_API ff_status ff_flowfilter_load_image_device(ff_stage* s, const void* dev, size_t pitch);
+
+/* ------------------------- DeltaFlowFilter (gpu/flowfilter.h:148-235) */
+/* DeltaFlowFilter(inputImage, inputFlow) (src/gpu/flowfilter.cu:349-362) */
+FF_API ff_status ff_deltaflowfilter_create(const ff_gpuimage* inputImage, const ff_gpuimage* inputFlow,
+                                           ff_stage** out);
+FF_API ff_status ff_deltaflowfilter_compute_image_model(ff_stage* s);   /* :460-467 */
+FF_API ff_status ff_deltaflowfilter_compute_propagation(ff_stage* s);   /* :470-477 */
+FF_API ff_status ff_deltaflowfilter_compute_update(ff_stage* s);        /* :480-503 */
+FF_API ff_status ff_deltaflowfilter_set_input_image(ff_stage* s, const ff_gpuimage* img); /* :506-523 */
+FF_API ff_status ff_deltaflowfilter_set_input_flow(ff_stage* s, const ff_gpuimage* img);  /* :526-542 */
+FF_API ff_status ff_deltaflowfilter_get_flow(ff_stage* s, ff_gpuimage** out);             /* :545-547 */
+FF_API ff_status ff_deltaflowfilter_get_image(ff_stage* s, ff_gpuimage** out);            /* :549-551 */
+FF_API ff_status ff_deltaflowfilter_get_gamma(ff_stage* s, float* out);                   /* :554-556 */
+FF_API ff_status ff_deltaflowfilter_set_gamma(ff_stage* s, float gamma);                  /* :559-568 */
+FF_API ff_status ff_deltaflowfilter_get_maxflow(ff_stage* s, float* out);                 /* :571-573 */
+FF_API ff_status ff_deltaflowfilter_set_maxflow(ff_stage* s, float maxflow);              /* :576-579 */
+FF_API ff_status ff_deltaflowfilter_get_smooth_iterations(ff_stage* s, int* out);         /* :582-584 */
+FF_API ff_status ff_deltaflowfilter_set_smooth_iterations(ff_stage* s, int n);            /* :587-589 */
+FF_API ff_status ff_deltaflowfilter_set_propagation_border(ff_stage* s, int border);      /* :592-594 */
+FF_API ff_status ff_deltaflowfilter_get_propagation_border(ff_stage* s, int* out);        /* :597-599 */
+FF_API ff_status ff_deltaflowfilter_get_propagation_iterations(ff_stage* s, int* out);    /* :602-604 */
+FF_API ff_status ff_deltaflowfilter_height(ff_stage* s, int* out);                        /* :607-610 */
+FF_API ff_status ff_deltaflowfilter_width(ff_stage* s, int* out);                         /* :613-615 */
+
+/* --------------------- PyramidalFlowFilter (gpu/flowfilter.h:238-332) */
+/* PyramidalFlowFilter(h, w, levels) (src/gpu/flowfilter.cu:631-641) */
+FF_API ff_status ff_pyramidal_create(int height, int width, int levels, ff_stage** out);
+FF_API ff_status ff_pyramidal_get_flow(ff_stage* s, ff_gpuimage** out);              /* :718-725 */
+FF_API ff_status ff_pyramidal_load_image(ff_stage* s, const ff_image_t* image);      /* :728-731 */
+FF_API ff_status ff_pyramidal_download_flow(ff_stage* s, ff_image_t* flow);          /* :734-741 */
+FF_API ff_status ff_pyramidal_download_image(ff_stage* s, ff_image_t* image);        /* :744-751 */
+FF_API ff_status ff_pyramidal_get_gamma(ff_stage* s, int level, float* out);         /* :754-766 */
+FF_API ff_status ff_pyramidal_set_gamma(ff_stage* s, int level, float gamma);        /* :769-781 */
+FF_API ff_status ff_pyramidal_set_gamma_vector(ff_stage* s, const float* gamma, int n);        /* :784-794 */
+FF_API ff_status ff_pyramidal_get_smooth_iterations(ff_stage* s, int level, int* out);         /* :797-809 */
+FF_API ff_status ff_pyramidal_set_smooth_iterations(ff_stage* s, int level, int n);            /* :812-824 */
+FF_API ff_status ff_pyramidal_set_smooth_iterations_vector(ff_stage* s, const int* iters, int n); /* :826-836 */
+FF_API ff_status ff_pyramidal_get_maxflow(ff_stage* s, float* out);                  /* :839-846 */
+FF_API ff_status ff_pyramidal_set_maxflow(ff_stage* s, float maxflow);               /* :849-866 */
+FF_API ff_status ff_pyramidal_set_propagation_border(ff_stage* s, int border);       /* :869-877 */
+FF_API ff_status ff_pyramidal_get_propagation_border(ff_stage* s, int* out);         /* :880-882 */
+FF_API ff_status ff_pyramidal_height(ff_stage* s, int* out);                         /* :885-887 */
+FF_API ff_status ff_pyramidal_width(ff_stage* s, int* out);                          /* :890-892 */
+FF_API ff_status ff_pyramidal_levels(ff_stage* s, int* out);                         /* :895-897 */
+/* loadImage from device memory (no reference counterpart; enqueued, no wait) */
+FF_API ff_status ff_pyramidal_load_image_device(ff_stage* s, const void* dev, size_t pitch);
+
+/* ----------------------------------------- execution controls (new) */
+/* Select how compute() runs a filter: 1 = fused per-level kernels (the
+ * default), 0 = one launch per reference stage kernel. Both produce
+ * identical results. Applies to FlowFilter and PyramidalFlowFilter. */
+FF_API ff_status ff_filter_set_fused(ff_stage* s, int fused);
+FF_API ff_status ff_filter_get_fused(ff_stage* s, int* out);
+
+/* Kernel profiling (no reference counterpart; the reference times whole
+ * stages only, src/gpu/pipeline.cu:63-89). Between begin and end, every
+ * kernel launch whose name contains `filter` is bracketed by HIP events on
+ * its stream; end waits for them and returns the summed milliseconds and
+ * the number of launches. Kernel names: image_model, propagate_x/_y,
+ * propagate_payload_x/_y, update, delta_update, smooth_x/_y, down_x/_y,
+ * scale_f2, copy2d, fused_top, fused_delta. */
+FF_API ff_status ff_profile_begin(const char* filter);
+FF_API ff_status ff_profile_end(double* total_ms, long long* count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FLOWFILTER_AMD_H_ */

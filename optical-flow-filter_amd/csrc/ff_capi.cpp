@@ -1,0 +1,530 @@
+// ff_capi.cpp -- extern "C" entry points declared in
+// include/flowfilter_amd.h. Each one validates its handles, calls the C++
+// object and converts exceptions into a status code plus a thread-local
+// message; nothing propagates across the C boundary.
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/flowfilter_amd.h"
+#include "ff_filters.hpp"
+
+using namespace ffamd;
+
+struct ff_gpuimage {
+    Image img;
+};
+struct ff_stage {
+    std::unique_ptr<Stage> obj;
+};
+
+namespace {
+
+thread_local std::string t_error;
+
+ff_status fail(int code, const std::string& msg) {
+    t_error = msg;
+    return code;
+}
+
+template <class F>
+ff_status guard(F&& f) {
+    try {
+        f();
+        return FF_OK;
+    } catch (const Error& e) {
+        return fail(e.code(), e.what());
+    } catch (const std::bad_alloc& e) {
+        return fail(FF_ERR_BAD_ALLOC, e.what());
+    } catch (const std::exception& e) {
+        return fail(FF_ERR_RUNTIME, e.what());
+    } catch (...) {
+        return fail(FF_ERR_RUNTIME, "unknown C++ exception");
+    }
+}
+
+Error null_handle(const char* what) { return Error(FF_ERR_NULL_HANDLE, std::string("null ") + what); }
+
+const Image& img_of(const ff_gpuimage* h) {
+    if (!h) throw null_handle("ff_gpuimage");
+    return h->img;
+}
+Image& img_of(ff_gpuimage* h) {
+    if (!h) throw null_handle("ff_gpuimage");
+    return h->img;
+}
+
+template <class T>
+T& as(ff_stage* s, const char* type) {
+    if (!s || !s->obj) throw null_handle("ff_stage");
+    T* p = dynamic_cast<T*>(s->obj.get());
+    if (!p)
+        throw Error(FF_ERR_WRONG_TYPE,
+                    std::string("stage is a ") + s->obj->name() + ", expected a " + type);
+    return *p;
+}
+
+template <class T>
+void out_ptr(T* p, const char* what) {
+    if (!p) throw Error(FF_ERR_INVALID_ARGUMENT, std::string("null output pointer: ") + what);
+}
+
+void new_image(const Image& img, ff_gpuimage** out) {
+    out_ptr(out, "ff_gpuimage**");
+    *out = new ff_gpuimage{img};
+}
+
+template <class T, class... A>
+void new_stage(ff_stage** out, A&&... args) {
+    out_ptr(out, "ff_stage**");
+    std::unique_ptr<Stage> obj(new T(std::forward<A>(args)...));
+    *out = new ff_stage{std::move(obj)};
+}
+
+const ff_image_t& host_img(const ff_image_t* p) {
+    if (!p) throw Error(FF_ERR_INVALID_ARGUMENT, "null ff_image_t");
+    return *p;
+}
+ff_image_t& host_img(ff_image_t* p) {
+    if (!p) throw Error(FF_ERR_INVALID_ARGUMENT, "null ff_image_t");
+    return *p;
+}
+
+}  // namespace
+
+extern "C" {
+
+// ----------------------------------------------------------------- library
+const char* ff_last_error(void) { return t_error.c_str(); }
+
+ff_status ff_version(int* major, int* minor) {
+    return guard([&] {
+        out_ptr(major, "major");
+        out_ptr(minor, "minor");
+        *major = 1;
+        *minor = 0;
+    });
+}
+
+ff_status ff_synchronize(void) { return guard([] { synchronize(); }); }
+ff_status ff_set_stream(void* s) { return guard([&] { set_stream((hipStream_t)s); }); }
+ff_status ff_get_stream(void** s) {
+    return guard([&] {
+        out_ptr(s, "stream");
+        *s = (void*)stream();
+    });
+}
+
+// ------------------------------------------------------------- host images
+// createImage (src/image.cpp:14-40): pitch = width * depth * pixelSize.
+ff_status ff_create_image(int height, int width, int depth, size_t pixelSize, ff_image_t* out) {
+    return guard([&] {
+        out_ptr(out, "ff_image_t*");
+        if (height <= 0 || width <= 0 || depth <= 0 || pixelSize == 0)
+            throw Error(FF_ERR_RUNTIME, "createImage(): dimensions must be positive");
+        const size_t pitch = (size_t)width * depth * pixelSize;
+        void* buf = std::calloc((size_t)height, pitch);
+        if (!buf) throw std::bad_alloc();
+        out->height = height;
+        out->width = width;
+        out->depth = depth;
+        out->pitch = pitch;
+        out->itemSize = pixelSize;
+        out->data = buf;
+    });
+}
+
+// destroyImage (src/image.cpp:43-52); frees what ff_create_image allocated.
+ff_status ff_destroy_image(ff_image_t* image) {
+    return guard([&] {
+        out_ptr(image, "ff_image_t*");
+        std::free(image->data);
+        image->data = nullptr;
+    });
+}
+
+// ---------------------------------------------------------------- GPUImage
+ff_status ff_gpuimage_create(int h, int w, int depth, int itemSize, ff_gpuimage** out) {
+    return guard([&] { new_image(Image(h, w, depth, itemSize), out); });
+}
+ff_status ff_gpuimage_create_empty(ff_gpuimage** out) {
+    return guard([&] { new_image(Image(), out); });
+}
+ff_status ff_gpuimage_share(const ff_gpuimage* img, ff_gpuimage** out) {
+    return guard([&] { new_image(img_of(img), out); });
+}
+ff_status ff_gpuimage_destroy(ff_gpuimage* img) {
+    return guard([&] { delete img; });
+}
+#define GETI(NAME, EXPR)                                         \
+    ff_status ff_gpuimage_##NAME(const ff_gpuimage* img, int* out) { \
+        return guard([&] {                                       \
+            out_ptr(out, #NAME);                                 \
+            *out = (int)(img_of(img).EXPR);                      \
+        });                                                      \
+    }
+GETI(height, height())
+GETI(width, width())
+GETI(depth, depth())
+GETI(pitch, pitch())
+GETI(item_size, itemSize())
+#undef GETI
+ff_status ff_gpuimage_data(const ff_gpuimage* img, void** out) {
+    return guard([&] {
+        out_ptr(out, "data");
+        *out = img_of(img).data();
+    });
+}
+ff_status ff_gpuimage_upload(ff_gpuimage* img, const ff_image_t* host) {
+    return guard([&] { img_of(img).upload(host_img(host)); });
+}
+ff_status ff_gpuimage_download(const ff_gpuimage* img, ff_image_t* host) {
+    return guard([&] { img_of(img).download(host_img(host)); });
+}
+ff_status ff_gpuimage_copy_from(ff_gpuimage* img, const ff_gpuimage* src) {
+    return guard([&] { img_of(img).copyFrom(img_of(src)); });
+}
+ff_status ff_gpuimage_clear(ff_gpuimage* img) {
+    return guard([&] { img_of(img).clear(); });
+}
+ff_status ff_gpuimage_upload_device(ff_gpuimage* img, const void* dev, size_t pitch) {
+    return guard([&] { img_of(img).upload_device(dev, pitch); });
+}
+ff_status ff_gpuimage_download_device(const ff_gpuimage* img, void* dev, size_t pitch) {
+    return guard([&] { img_of(img).download_device(dev, pitch); });
+}
+
+// ------------------------------------------------------------------- Stage
+ff_status ff_stage_configure(ff_stage* s) {
+    return guard([&] { as<Stage>(s, "Stage").configure(); });
+}
+ff_status ff_stage_compute(ff_stage* s) {
+    return guard([&] { as<Stage>(s, "Stage").compute(); });
+}
+ff_status ff_stage_elapsed_time(ff_stage* s, float* ms) {
+    return guard([&] {
+        out_ptr(ms, "ms");
+        *ms = as<Stage>(s, "Stage").elapsedTime();
+    });
+}
+ff_status ff_stage_destroy(ff_stage* s) {
+    return guard([&] { delete s; });
+}
+
+// -------------------------------------------------------- helper macros
+#define SETTER(FN, CLS, TYPE, CALL)                                  \
+    ff_status FN(ff_stage* s, TYPE v) {                              \
+        return guard([&] { as<CLS>(s, #CLS).CALL(v); });             \
+    }
+#define GETTER(FN, CLS, TYPE, CALL)                                  \
+    ff_status FN(ff_stage* s, TYPE* out) {                           \
+        return guard([&] {                                           \
+            out_ptr(out, #FN);                                       \
+            *out = (TYPE)as<CLS>(s, #CLS).CALL();                    \
+        });                                                          \
+    }
+#define IMG_SETTER(FN, CLS, CALL)                                    \
+    ff_status FN(ff_stage* s, const ff_gpuimage* img) {              \
+        return guard([&] { as<CLS>(s, #CLS).CALL(img_of(img)); });   \
+    }
+#define IMG_GETTER(FN, CLS, CALL)                                    \
+    ff_status FN(ff_stage* s, ff_gpuimage** out) {                   \
+        return guard([&] { new_image(as<CLS>(s, #CLS).CALL(), out); }); \
+    }
+#define ACTION(FN, CLS, CALL)                                        \
+    ff_status FN(ff_stage* s) {                                      \
+        return guard([&] { as<CLS>(s, #CLS).CALL(); });              \
+    }
+
+// -------------------------------------------------------------- ImageModel
+ff_status ff_imagemodel_create(const ff_gpuimage* input, ff_stage** out) {
+    return guard([&] {
+        if (input) new_stage<ImageModel>(out, input->img);
+        else new_stage<ImageModel>(out);
+    });
+}
+IMG_SETTER(ff_imagemodel_set_input_image, ImageModel, setInputImage)
+IMG_GETTER(ff_imagemodel_get_image_constant, ImageModel, getImageConstant)
+IMG_GETTER(ff_imagemodel_get_image_gradient, ImageModel, getImageGradient)
+
+// ---------------------------------------------------------- FlowPropagator
+ff_status ff_flowpropagator_create(const ff_gpuimage* inputFlow, int iterations, ff_stage** out) {
+    return guard([&] {
+        if (inputFlow) new_stage<FlowPropagator>(out, inputFlow->img, iterations);
+        else new_stage<FlowPropagator>(out);
+    });
+}
+SETTER(ff_flowpropagator_set_iterations, FlowPropagator, int, setIterations)
+GETTER(ff_flowpropagator_get_iterations, FlowPropagator, int, getIterations)
+GETTER(ff_flowpropagator_get_dt, FlowPropagator, float, getDt)
+SETTER(ff_flowpropagator_set_border, FlowPropagator, int, setBorder)
+GETTER(ff_flowpropagator_get_border, FlowPropagator, int, getBorder)
+ff_status ff_flowpropagator_set_invert_input_flow(ff_stage* s, int invert) {
+    return guard([&] { as<FlowPropagator>(s, "FlowPropagator").setInvertInputFlow(invert != 0); });
+}
+GETTER(ff_flowpropagator_get_invert_input_flow, FlowPropagator, int, getInvertInputFlow)
+IMG_SETTER(ff_flowpropagator_set_input_flow, FlowPropagator, setInputFlow)
+IMG_GETTER(ff_flowpropagator_get_propagated_flow, FlowPropagator, getPropagatedFlow)
+
+// --------------------------------------------------- FlowPropagatorPayload
+ff_status ff_flowpropagatorpayload_create(const ff_gpuimage* flow, const ff_gpuimage* scalar,
+                                          const ff_gpuimage* vector, int iterations, ff_stage** out) {
+    return guard([&] {
+        if (flow && scalar && vector)
+            new_stage<FlowPropagatorPayload>(out, flow->img, scalar->img, vector->img, iterations);
+        else new_stage<FlowPropagatorPayload>(out);
+    });
+}
+SETTER(ff_flowpropagatorpayload_set_iterations, FlowPropagatorPayload, int, setIterations)
+GETTER(ff_flowpropagatorpayload_get_iterations, FlowPropagatorPayload, int, getIterations)
+GETTER(ff_flowpropagatorpayload_get_dt, FlowPropagatorPayload, float, getDt)
+SETTER(ff_flowpropagatorpayload_set_border, FlowPropagatorPayload, int, setBorder)
+GETTER(ff_flowpropagatorpayload_get_border, FlowPropagatorPayload, int, getBorder)
+IMG_SETTER(ff_flowpropagatorpayload_set_input_flow, FlowPropagatorPayload, setInputFlow)
+IMG_SETTER(ff_flowpropagatorpayload_set_scalar_payload, FlowPropagatorPayload, setScalarPayload)
+IMG_SETTER(ff_flowpropagatorpayload_set_vector_payload, FlowPropagatorPayload, setVectorPayload)
+IMG_GETTER(ff_flowpropagatorpayload_get_propagated_flow, FlowPropagatorPayload, getPropagatedFlow)
+IMG_GETTER(ff_flowpropagatorpayload_get_propagated_scalar, FlowPropagatorPayload, getPropagatedScalar)
+IMG_GETTER(ff_flowpropagatorpayload_get_propagated_vector, FlowPropagatorPayload, getPropagatedVector)
+
+// -------------------------------------------------------------- FlowUpdate
+ff_status ff_flowupdate_create(const ff_gpuimage* flow, const ff_gpuimage* image, const ff_gpuimage* grad,
+                               float gamma, float maxflow, ff_stage** out) {
+    return guard([&] {
+        if (flow && image && grad) {
+            new_stage<FlowUpdate>(out, flow->img, image->img, grad->img, gamma, maxflow);
+        } else {
+            // FlowUpdate() then the parameter setters (python/flowfilter/gpu/update.pyx:14-30)
+            auto* u = new FlowUpdate();
+            std::unique_ptr<Stage> keep(u);
+            u->setGamma(gamma);
+            u->setMaxFlow(maxflow);
+            out_ptr(out, "ff_stage**");
+            *out = new ff_stage{std::move(keep)};
+        }
+    });
+}
+GETTER(ff_flowupdate_get_gamma, FlowUpdate, float, getGamma)
+SETTER(ff_flowupdate_set_gamma, FlowUpdate, float, setGamma)
+GETTER(ff_flowupdate_get_maxflow, FlowUpdate, float, getMaxFlow)
+SETTER(ff_flowupdate_set_maxflow, FlowUpdate, float, setMaxFlow)
+IMG_SETTER(ff_flowupdate_set_input_flow, FlowUpdate, setInputFlow)
+IMG_SETTER(ff_flowupdate_set_input_image, FlowUpdate, setInputImage)
+IMG_SETTER(ff_flowupdate_set_input_image_gradient, FlowUpdate, setInputImageGradient)
+IMG_GETTER(ff_flowupdate_get_updated_flow, FlowUpdate, getUpdatedFlow)
+IMG_GETTER(ff_flowupdate_get_updated_image, FlowUpdate, getUpdatedImage)
+
+// --------------------------------------------------------- DeltaFlowUpdate
+ff_status ff_deltaflowupdate_create(const ff_gpuimage* flow, const ff_gpuimage* dflow, const ff_gpuimage* old,
+                                    const ff_gpuimage* image, const ff_gpuimage* grad, float gamma, float maxflow,
+                                    ff_stage** out) {
+    return guard([&] {
+        if (flow && dflow && old && image && grad) {
+            new_stage<DeltaFlowUpdate>(out, flow->img, dflow->img, old->img, image->img, grad->img, gamma, maxflow);
+        } else {
+            auto* u = new DeltaFlowUpdate();
+            std::unique_ptr<Stage> keep(u);
+            u->setGamma(gamma);
+            u->setMaxFlow(maxflow);
+            out_ptr(out, "ff_stage**");
+            *out = new ff_stage{std::move(keep)};
+        }
+    });
+}
+GETTER(ff_deltaflowupdate_get_gamma, DeltaFlowUpdate, float, getGamma)
+SETTER(ff_deltaflowupdate_set_gamma, DeltaFlowUpdate, float, setGamma)
+GETTER(ff_deltaflowupdate_get_maxflow, DeltaFlowUpdate, float, getMaxFlow)
+SETTER(ff_deltaflowupdate_set_maxflow, DeltaFlowUpdate, float, setMaxFlow)
+IMG_SETTER(ff_deltaflowupdate_set_input_flow, DeltaFlowUpdate, setInputFlow)
+IMG_SETTER(ff_deltaflowupdate_set_input_delta_flow, DeltaFlowUpdate, setInputDeltaFlow)
+IMG_SETTER(ff_deltaflowupdate_set_input_image_old, DeltaFlowUpdate, setInputImageOld)
+IMG_SETTER(ff_deltaflowupdate_set_input_image, DeltaFlowUpdate, setInputImage)
+IMG_SETTER(ff_deltaflowupdate_set_input_image_gradient, DeltaFlowUpdate, setInputImageGradient)
+IMG_GETTER(ff_deltaflowupdate_get_updated_flow, DeltaFlowUpdate, getUpdatedFlow)
+IMG_GETTER(ff_deltaflowupdate_get_updated_delta_flow, DeltaFlowUpdate, getUpdatedDeltaFlow)
+IMG_GETTER(ff_deltaflowupdate_get_updated_image, DeltaFlowUpdate, getUpdatedImage)
+
+// ------------------------------------------------------------ FlowSmoother
+ff_status ff_flowsmoother_create(const ff_gpuimage* flow, int iterations, ff_stage** out) {
+    return guard([&] {
+        if (flow) new_stage<FlowSmoother>(out, flow->img, iterations);
+        else new_stage<FlowSmoother>(out);
+    });
+}
+GETTER(ff_flowsmoother_get_iterations, FlowSmoother, int, getIterations)
+SETTER(ff_flowsmoother_set_iterations, FlowSmoother, int, setIterations)
+IMG_SETTER(ff_flowsmoother_set_input_flow, FlowSmoother, setInputFlow)
+IMG_GETTER(ff_flowsmoother_get_smoothed_flow, FlowSmoother, getSmoothedFlow)
+
+// ------------------------------------------------------------ ImagePyramid
+ff_status ff_imagepyramid_create(const ff_gpuimage* image, int levels, ff_stage** out) {
+    return guard([&] {
+        if (image) new_stage<ImagePyramid>(out, image->img, levels);
+        else {
+            auto* p = new ImagePyramid();
+            std::unique_ptr<Stage> keep(p);
+            p->setLevels(levels);
+            out_ptr(out, "ff_stage**");
+            *out = new ff_stage{std::move(keep)};
+        }
+    });
+}
+IMG_SETTER(ff_imagepyramid_set_input_image, ImagePyramid, setInputImage)
+SETTER(ff_imagepyramid_set_levels, ImagePyramid, int, setLevels)
+GETTER(ff_imagepyramid_get_levels, ImagePyramid, int, getLevels)
+ff_status ff_imagepyramid_get_image(ff_stage* s, int level, ff_gpuimage** out) {
+    return guard([&] { new_image(as<ImagePyramid>(s, "ImagePyramid").getImage(level), out); });
+}
+
+// -------------------------------------------------------------- FlowFilter
+ff_status ff_flowfilter_create(int h, int w, int smooth, float maxflow, float gamma, ff_stage** out) {
+    return guard([&] { new_stage<FlowFilter>(out, h, w, smooth, maxflow, gamma); });
+}
+ff_status ff_flowfilter_create_from_image(const ff_gpuimage* input, ff_stage** out) {
+    return guard([&] { new_stage<FlowFilter>(out, img_of(input)); });
+}
+ACTION(ff_flowfilter_compute_image_model, FlowFilter, computeImageModel)
+ACTION(ff_flowfilter_compute_propagation, FlowFilter, computePropagation)
+ACTION(ff_flowfilter_compute_update, FlowFilter, computeUpdate)
+IMG_SETTER(ff_flowfilter_set_input_image, FlowFilter, setInputImage)
+ff_status ff_flowfilter_load_image(ff_stage* s, const ff_image_t* image) {
+    return guard([&] { as<FlowFilter>(s, "FlowFilter").loadImage(host_img(image)); });
+}
+ff_status ff_flowfilter_download_flow(ff_stage* s, ff_image_t* flow) {
+    return guard([&] { as<FlowFilter>(s, "FlowFilter").downloadFlow(host_img(flow)); });
+}
+ff_status ff_flowfilter_download_image(ff_stage* s, ff_image_t* image) {
+    return guard([&] { as<FlowFilter>(s, "FlowFilter").downloadImage(host_img(image)); });
+}
+IMG_GETTER(ff_flowfilter_get_flow, FlowFilter, getFlow)
+GETTER(ff_flowfilter_get_gamma, FlowFilter, float, getGamma)
+SETTER(ff_flowfilter_set_gamma, FlowFilter, float, setGamma)
+GETTER(ff_flowfilter_get_maxflow, FlowFilter, float, getMaxFlow)
+SETTER(ff_flowfilter_set_maxflow, FlowFilter, float, setMaxFlow)
+GETTER(ff_flowfilter_get_smooth_iterations, FlowFilter, int, getSmoothIterations)
+SETTER(ff_flowfilter_set_smooth_iterations, FlowFilter, int, setSmoothIterations)
+SETTER(ff_flowfilter_set_propagation_border, FlowFilter, int, setPropagationBorder)
+GETTER(ff_flowfilter_get_propagation_border, FlowFilter, int, getPropagationBorder)
+GETTER(ff_flowfilter_get_propagation_iterations, FlowFilter, int, getPropagationIterations)
+GETTER(ff_flowfilter_height, FlowFilter, int, height)
+GETTER(ff_flowfilter_width, FlowFilter, int, width)
+ff_status ff_flowfilter_load_image_device(ff_stage* s, const void* dev, size_t pitch) {
+    return guard([&] { as<FlowFilter>(s, "FlowFilter").loadImageDevice(dev, pitch); });
+}
+
+// --------------------------------------------------------- DeltaFlowFilter
+ff_status ff_deltaflowfilter_create(const ff_gpuimage* image, const ff_gpuimage* flow, ff_stage** out) {
+    return guard([&] {
+        if (image && flow) new_stage<DeltaFlowFilter>(out, image->img, flow->img);
+        else new_stage<DeltaFlowFilter>(out);
+    });
+}
+ACTION(ff_deltaflowfilter_compute_image_model, DeltaFlowFilter, computeImageModel)
+ACTION(ff_deltaflowfilter_compute_propagation, DeltaFlowFilter, computePropagation)
+ACTION(ff_deltaflowfilter_compute_update, DeltaFlowFilter, computeUpdate)
+IMG_SETTER(ff_deltaflowfilter_set_input_image, DeltaFlowFilter, setInputImage)
+IMG_SETTER(ff_deltaflowfilter_set_input_flow, DeltaFlowFilter, setInputFlow)
+IMG_GETTER(ff_deltaflowfilter_get_flow, DeltaFlowFilter, getFlow)
+IMG_GETTER(ff_deltaflowfilter_get_image, DeltaFlowFilter, getImage)
+GETTER(ff_deltaflowfilter_get_gamma, DeltaFlowFilter, float, getGamma)
+SETTER(ff_deltaflowfilter_set_gamma, DeltaFlowFilter, float, setGamma)
+GETTER(ff_deltaflowfilter_get_maxflow, DeltaFlowFilter, float, getMaxFlow)
+SETTER(ff_deltaflowfilter_set_maxflow, DeltaFlowFilter, float, setMaxFlow)
+GETTER(ff_deltaflowfilter_get_smooth_iterations, DeltaFlowFilter, int, getSmoothIterations)
+SETTER(ff_deltaflowfilter_set_smooth_iterations, DeltaFlowFilter, int, setSmoothIterations)
+SETTER(ff_deltaflowfilter_set_propagation_border, DeltaFlowFilter, int, setPropagationBorder)
+GETTER(ff_deltaflowfilter_get_propagation_border, DeltaFlowFilter, int, getPropagationBorder)
+GETTER(ff_deltaflowfilter_get_propagation_iterations, DeltaFlowFilter, int, getPropagationIterations)
+GETTER(ff_deltaflowfilter_height, DeltaFlowFilter, int, height)
+GETTER(ff_deltaflowfilter_width, DeltaFlowFilter, int, width)
+
+// ----------------------------------------------------- PyramidalFlowFilter
+ff_status ff_pyramidal_create(int h, int w, int levels, ff_stage** out) {
+    return guard([&] { new_stage<PyramidalFlowFilter>(out, h, w, levels); });
+}
+IMG_GETTER(ff_pyramidal_get_flow, PyramidalFlowFilter, getFlow)
+ff_status ff_pyramidal_load_image(ff_stage* s, const ff_image_t* image) {
+    return guard([&] { as<PyramidalFlowFilter>(s, "PyramidalFlowFilter").loadImage(host_img(image)); });
+}
+ff_status ff_pyramidal_download_flow(ff_stage* s, ff_image_t* flow) {
+    return guard([&] { as<PyramidalFlowFilter>(s, "PyramidalFlowFilter").downloadFlow(host_img(flow)); });
+}
+ff_status ff_pyramidal_download_image(ff_stage* s, ff_image_t* image) {
+    return guard([&] { as<PyramidalFlowFilter>(s, "PyramidalFlowFilter").downloadImage(host_img(image)); });
+}
+ff_status ff_pyramidal_get_gamma(ff_stage* s, int level, float* out) {
+    return guard([&] {
+        out_ptr(out, "gamma");
+        *out = as<PyramidalFlowFilter>(s, "PyramidalFlowFilter").getGamma(level);
+    });
+}
+ff_status ff_pyramidal_set_gamma(ff_stage* s, int level, float gamma) {
+    return guard([&] { as<PyramidalFlowFilter>(s, "PyramidalFlowFilter").setGamma(level, gamma); });
+}
+ff_status ff_pyramidal_set_gamma_vector(ff_stage* s, const float* gamma, int n) {
+    return guard([&] {
+        if (n > 0 && !gamma) throw Error(FF_ERR_INVALID_ARGUMENT, "null gamma vector");
+        std::vector<float> v(gamma, gamma + (n > 0 ? n : 0));
+        as<PyramidalFlowFilter>(s, "PyramidalFlowFilter").setGamma(v);
+    });
+}
+ff_status ff_pyramidal_get_smooth_iterations(ff_stage* s, int level, int* out) {
+    return guard([&] {
+        out_ptr(out, "iterations");
+        *out = as<PyramidalFlowFilter>(s, "PyramidalFlowFilter").getSmoothIterations(level);
+    });
+}
+ff_status ff_pyramidal_set_smooth_iterations(ff_stage* s, int level, int n) {
+    return guard([&] { as<PyramidalFlowFilter>(s, "PyramidalFlowFilter").setSmoothIterations(level, n); });
+}
+ff_status ff_pyramidal_set_smooth_iterations_vector(ff_stage* s, const int* iters, int n) {
+    return guard([&] {
+        if (n > 0 && !iters) throw Error(FF_ERR_INVALID_ARGUMENT, "null iterations vector");
+        std::vector<int> v(iters, iters + (n > 0 ? n : 0));
+        as<PyramidalFlowFilter>(s, "PyramidalFlowFilter").setSmoothIterations(v);
+    });
+}
+GETTER(ff_pyramidal_get_maxflow, PyramidalFlowFilter, float, getMaxFlow)
+SETTER(ff_pyramidal_set_maxflow, PyramidalFlowFilter, float, setMaxFlow)
+SETTER(ff_pyramidal_set_propagation_border, PyramidalFlowFilter, int, setPropagationBorder)
+GETTER(ff_pyramidal_get_propagation_border, PyramidalFlowFilter, int, getPropagationBorder)
+GETTER(ff_pyramidal_height, PyramidalFlowFilter, int, height)
+GETTER(ff_pyramidal_width, PyramidalFlowFilter, int, width)
+GETTER(ff_pyramidal_levels, PyramidalFlowFilter, int, levels)
+ff_status ff_pyramidal_load_image_device(ff_stage* s, const void* dev, size_t pitch) {
+    return guard([&] { as<PyramidalFlowFilter>(s, "PyramidalFlowFilter").loadImageDevice(dev, pitch); });
+}
+
+// ------------------------------------------------------- execution controls
+ff_status ff_filter_set_fused(ff_stage* s, int fused) {
+    return guard([&] {
+        if (!s || !s->obj) throw null_handle("ff_stage");
+        if (auto* f = dynamic_cast<FlowFilter*>(s->obj.get())) f->setFused(fused != 0);
+        else if (auto* p = dynamic_cast<PyramidalFlowFilter*>(s->obj.get())) p->setFused(fused != 0);
+        else if (auto* d = dynamic_cast<DeltaFlowFilter*>(s->obj.get())) d->setFused(fused != 0);
+        else throw Error(FF_ERR_WRONG_TYPE, std::string("stage is a ") + s->obj->name() + ", expected a filter");
+    });
+}
+ff_status ff_filter_get_fused(ff_stage* s, int* out) {
+    return guard([&] {
+        out_ptr(out, "fused");
+        if (!s || !s->obj) throw null_handle("ff_stage");
+        if (auto* f = dynamic_cast<FlowFilter*>(s->obj.get())) *out = f->fused();
+        else if (auto* p = dynamic_cast<PyramidalFlowFilter*>(s->obj.get())) *out = p->fused();
+        else throw Error(FF_ERR_WRONG_TYPE, std::string("stage is a ") + s->obj->name() + ", expected a filter");
+    });
+}
+
+// --------------------------------------------------------------- profiling
+ff_status ff_profile_begin(const char* filter) {
+    return guard([&] { profile_begin(filter ? filter : ""); });
+}
+ff_status ff_profile_end(double* total_ms, long long* count) {
+    return guard([&] { profile_end(total_ms, count); });
+}
+
+}  // extern "C"

@@ -1,0 +1,538 @@
+// ff_filters.cpp -- see ff_filters.hpp. Reference: src/gpu/flowfilter.cu
+// (line numbers per method).
+#include "ff_filters.hpp"
+
+#include <cmath>
+
+namespace ffamd {
+
+namespace {
+std::string num(long v) { return std::to_string(v); }
+
+void ensureLike(Image& spare, const Image& like) {
+    if (!spare.allocated() || !spare.sameShape(like) || spare.pitch() != like.pitch())
+        spare = Image(like.height(), like.width(), like.depth(), like.itemSize());
+}
+}  // namespace
+
+// ================================================================ FlowFilter
+
+// FlowFilter(GPUImage) (flowfilter.cu:32-38).
+FlowFilter::FlowFilter(const Image& inputImage) {
+    setInputImage(inputImage);
+    configure();
+}
+
+// FlowFilter(h, w, smoothIterations, maxflow, gamma) (flowfilter.cu:45-76).
+FlowFilter::FlowFilter(int height, int width, int smoothIterations, float maxflow, float gamma) {
+    if (height <= 0)
+        throw invalid_argument("FlowFilter::FlowFilter(): height should be greater than zero, got: " + num(height));
+    if (width <= 0)
+        throw invalid_argument("FlowFilter::FlowFilter(): width should be greater than zero, got: " + num(width));
+    setInputImage(Image(height, width, 1, 1));
+    configure();
+    setGamma(gamma);
+    setMaxFlow(maxflow);
+    setSmoothIterations(smoothIterations);
+}
+
+// FlowFilter::configure (flowfilter.cu:84-122): stages wired by shared
+// buffers; the update's input flow is the propagator's output.
+void FlowFilter::configure() {
+    if (!inputSet_) throw logic_error("FlowFilter::configure(): input image has not been set");
+    model_ = ImageModel(input_);
+    Image dummyFlow(h_, w_, 2, 4);
+    update_ = FlowUpdate(dummyFlow, model_.getImageConstant(), model_.getImageGradient(), 1.0f, 1.0f);
+    smoother_ = FlowSmoother(update_.getUpdatedFlow(), 1);
+    propagator_ = FlowPropagator(smoother_.getSmoothedFlow(), 1);
+    update_.setInputFlow(propagator_.getPropagatedFlow());
+    propagator_.getPropagatedFlow().clear();
+    update_.getUpdatedFlow().clear();
+    update_.getUpdatedImage().clear();
+    smoother_.getSmoothedFlow().clear();
+    spareFlow_ = Image();
+    spareImage_ = Image();
+    configured_ = true;
+    firstLoad_ = true;
+}
+
+void FlowFilter::requireConfigured(const char* who) const {
+    if (!configured_) throw logic_error(std::string("FlowFilter::") + who + ": filter not configured");
+}
+
+void FlowFilter::runImageModel() {
+    requireConfigured("computeImageModel()");
+    model_.run();
+}
+
+void FlowFilter::runPropagation() {
+    requireConfigured("computePropagation()");
+    propagator_.run();
+}
+
+// FlowFilter::computeUpdate (flowfilter.cu:175-197).
+void FlowFilter::runUpdate() {
+    requireConfigured("computeUpdate()");
+    if (firstLoad_) {
+        update_.getUpdatedImage().copyFrom(model_.getImageConstant());
+        firstLoad_ = false;
+    }
+    update_.run();
+    smoother_.run();
+}
+
+bool FlowFilter::fusedSupported() {
+    if (!configured_ || firstLoad_) return false;
+    return fused_level_supported(fusedArgs());
+}
+
+LevelArgs FlowFilter::fusedArgs() {
+    Image S = smoother_.getSmoothedFlow();
+    Image A = update_.getUpdatedImage();
+    ensureLike(spareFlow_, S);
+    ensureLike(spareImage_, A);
+    LevelArgs a{};
+    a.img = model_.boundImage().view();
+    a.img_u8 = model_.boundImage().itemSize() == 1;
+    a.flow_s_in = S.view();
+    a.a0_in = A.view();
+    a.flow_s_out = spareFlow_.view();
+    a.a0_out = spareImage_.view();
+    a.flow_upd_out = update_.getUpdatedFlow().view();
+    a.gamma = update_.getGamma();
+    a.maxflow = update_.getMaxFlow();
+    a.prop_iters = propagator_.getIterations();
+    a.dt = propagator_.getDt();
+    a.border = propagator_.getBorder();
+    a.smooth_iters = smoother_.getIterations();
+    a.delta = 0;
+    return a;
+}
+
+void FlowFilter::fusedCommit() {
+    Image S = smoother_.getSmoothedFlow();
+    Image A = update_.getUpdatedImage();
+    S.swapBuffers(spareFlow_);
+    A.swapBuffers(spareImage_);
+}
+
+// FlowFilter::compute (flowfilter.cu:125-153).
+void FlowFilter::compute() {
+    requireConfigured("compute()");
+    if (model_.boundImage().height() != h_ || model_.boundImage().width() != w_)
+        throw logic_error("FlowFilter::compute(): input image shape changed since configure()");
+    startTiming();
+    if (fused_ && fusedSupported()) {
+        launch_fused_level(fusedArgs(), stream());
+        check_launch("FlowFilter::compute (fused)");
+        fusedCommit();
+    } else {
+        model_.run();
+        if (firstLoad_) {
+            update_.getUpdatedImage().copyFrom(model_.getImageConstant());
+            firstLoad_ = false;
+        }
+        propagator_.run();
+        update_.run();
+        smoother_.run();
+    }
+    stopTiming();
+}
+
+void FlowFilter::computeImageModel() {
+    startTiming();
+    runImageModel();
+    stopTiming();
+}
+
+void FlowFilter::computePropagation() {
+    startTiming();
+    runPropagation();
+    stopTiming();
+}
+
+void FlowFilter::computeUpdate() {
+    startTiming();
+    runUpdate();
+    stopTiming();
+}
+
+// FlowFilter::setInputImage (flowfilter.cu:200-216).
+void FlowFilter::setInputImage(const Image& img) {
+    if (img.depth() != 1)
+        throw invalid_argument("FlowFilter::setInputImage(): input image should have depth 1, got: " +
+                               num(img.depth()));
+    if (img.itemSize() != 1 && img.itemSize() != 4)
+        throw invalid_argument("FlowFilter::setInputImage(): item size should be 1 or 4: " + num(img.itemSize()));
+    input_ = img;
+    h_ = img.height();
+    w_ = img.width();
+    inputSet_ = true;
+}
+
+void FlowFilter::loadImage(const ff_image_t& img) { input_.upload(img); }
+void FlowFilter::loadImageDevice(const void* dev, size_t pitch) { input_.upload_device(dev, pitch); }
+void FlowFilter::downloadFlow(ff_image_t& flow) { smoother_.getSmoothedFlow().download(flow); }
+void FlowFilter::downloadImage(ff_image_t& img) { update_.getUpdatedImage().download(img); }
+
+// FlowFilter::setGamma (flowfilter.cu:276-285).
+void FlowFilter::setGamma(float g) {
+    if (input_.itemSize() == 1) update_.setGamma(g / (255.0f * 255.0f));
+    else update_.setGamma(g);
+}
+
+// FlowFilter::setMaxFlow (flowfilter.cu:293-296).
+void FlowFilter::setMaxFlow(float m) {
+    update_.setMaxFlow(m);
+    propagator_.setIterations(int(ceilf(m)));
+}
+
+// =========================================================== DeltaFlowFilter
+
+// DeltaFlowFilter(inputImage, inputFlow) (flowfilter.cu:349-362).
+DeltaFlowFilter::DeltaFlowFilter(const Image& inputImage, const Image& inputFlow) {
+    setInputImage(inputImage);
+    setInputFlow(inputFlow);
+    configure();
+}
+
+// DeltaFlowFilter::configure (flowfilter.cu:371-426).
+void DeltaFlowFilter::configure() {
+    if (!flowSet_) throw runtime_error("DeltaFlowFilter::configure(): input flow not set");
+    if (!imageSet_) throw runtime_error("DeltaFlowFilter::configure(): input image not set");
+    const int h = input_.height(), w = input_.width();
+    model_ = ImageModel(input_);
+    Image dummyDelta(h, w, 2, 4), dummyOld(h, w, 1, 4);
+    update_ = DeltaFlowUpdate(inputFlow_, dummyDelta, dummyOld, model_.getImageConstant(),
+                              model_.getImageGradient());
+    smoother_ = FlowSmoother(update_.getUpdatedFlow(), 1);
+    propagator_ = FlowPropagatorPayload(smoother_.getSmoothedFlow(), update_.getUpdatedImage(),
+                                        update_.getUpdatedDeltaFlow());
+    update_.setInputDeltaFlow(propagator_.getPropagatedVector());
+    update_.setInputImageOld(propagator_.getPropagatedScalar());
+    model_.getImageConstant().clear();
+    model_.getImageGradient().clear();
+    propagator_.getPropagatedFlow().clear();
+    propagator_.getPropagatedScalar().clear();
+    propagator_.getPropagatedVector().clear();
+    update_.getUpdatedFlow().clear();
+    update_.getUpdatedDeltaFlow().clear();
+    update_.getUpdatedImage().clear();
+    smoother_.getSmoothedFlow().clear();
+    spareFlow_ = Image();
+    spareImage_ = Image();
+    spareDelta_ = Image();
+    configured_ = true;
+    firstLoad_ = true;
+}
+
+void DeltaFlowFilter::requireConfigured(const char* who) const {
+    if (!configured_) throw logic_error(std::string("DeltaFlowFilter::") + who + ": filter not configured");
+}
+
+void DeltaFlowFilter::runImageModel() {
+    requireConfigured("computeImageModel()");
+    model_.run();
+}
+
+void DeltaFlowFilter::runPropagation() {
+    requireConfigured("computePropagation()");
+    propagator_.run();
+}
+
+// DeltaFlowFilter::computeUpdate (flowfilter.cu:480-503).
+void DeltaFlowFilter::runUpdate() {
+    requireConfigured("computeUpdate()");
+    if (firstLoad_) {
+        update_.getUpdatedImage().copyFrom(model_.getImageConstant());
+        propagator_.getPropagatedScalar().copyFrom(model_.getImageConstant());
+        firstLoad_ = false;
+    }
+    update_.run();
+    smoother_.run();
+}
+
+bool DeltaFlowFilter::fusedSupported() {
+    if (!configured_ || firstLoad_) return false;
+    return fused_level_supported(fusedArgs());
+}
+
+LevelArgs DeltaFlowFilter::fusedArgs() {
+    Image S = smoother_.getSmoothedFlow();
+    Image A = update_.getUpdatedImage();
+    Image D = update_.getUpdatedDeltaFlow();
+    ensureLike(spareFlow_, S);
+    ensureLike(spareImage_, A);
+    ensureLike(spareDelta_, D);
+    LevelArgs a{};
+    a.img = model_.boundImage().view();
+    a.img_u8 = model_.boundImage().itemSize() == 1;
+    a.flow_s_in = S.view();
+    a.a0_in = A.view();
+    a.dflow_in = D.view();
+    a.flow_s_out = spareFlow_.view();
+    a.a0_out = spareImage_.view();
+    a.dflow_out = spareDelta_.view();
+    a.flow_upd_out = update_.getUpdatedFlow().view();
+    a.upper = update_.boundUpperFlow().view();
+    a.xmap = update_.xmap();
+    a.ymap = update_.ymap();
+    a.gamma = update_.getGamma();
+    a.maxflow = update_.getMaxFlow();
+    a.prop_iters = propagator_.getIterations();
+    a.dt = propagator_.getDt();
+    a.border = propagator_.getBorder();
+    a.smooth_iters = smoother_.getIterations();
+    a.delta = 1;
+    return a;
+}
+
+void DeltaFlowFilter::fusedCommit() {
+    Image S = smoother_.getSmoothedFlow();
+    Image A = update_.getUpdatedImage();
+    Image D = update_.getUpdatedDeltaFlow();
+    S.swapBuffers(spareFlow_);
+    A.swapBuffers(spareImage_);
+    D.swapBuffers(spareDelta_);
+}
+
+// DeltaFlowFilter::compute (flowfilter.cu:429-457). The first-load copy of
+// this method precedes propagation (unlike computeUpdate's).
+void DeltaFlowFilter::compute() {
+    requireConfigured("compute()");
+    startTiming();
+    if (fused_ && fusedSupported()) {
+        launch_fused_level(fusedArgs(), stream());
+        check_launch("DeltaFlowFilter::compute (fused)");
+        fusedCommit();
+    } else {
+        model_.run();
+        if (firstLoad_) {
+            update_.getUpdatedImage().copyFrom(model_.getImageConstant());
+            firstLoad_ = false;
+        }
+        propagator_.run();
+        update_.run();
+        smoother_.run();
+    }
+    stopTiming();
+}
+
+void DeltaFlowFilter::computeImageModel() {
+    startTiming();
+    runImageModel();
+    stopTiming();
+}
+
+void DeltaFlowFilter::computePropagation() {
+    startTiming();
+    runPropagation();
+    stopTiming();
+}
+
+void DeltaFlowFilter::computeUpdate() {
+    startTiming();
+    runUpdate();
+    stopTiming();
+}
+
+// DeltaFlowFilter::setInputImage / setInputFlow (flowfilter.cu:506-542).
+void DeltaFlowFilter::setInputImage(const Image& img) {
+    if (img.depth() != 1)
+        throw runtime_error("DeltaFlowFilter::setInputImage(): input image should have depth 1: " + num(img.depth()));
+    if (img.itemSize() != 1 && img.itemSize() != 4)
+        throw runtime_error("DeltaFlowFilter::setInputImage(): input image should have item size 1 or 4: " +
+                            num(img.itemSize()));
+    input_ = img;
+    imageSet_ = true;
+}
+
+void DeltaFlowFilter::setInputFlow(const Image& flow) {
+    if (flow.depth() != 2)
+        throw runtime_error("DeltaFlowFilter::setInputFlow(): input flow should have depth 2: " + num(flow.depth()));
+    if (flow.itemSize() != 4)
+        throw runtime_error("DeltaFlowFilter::setInputFlow(): input flow should have item size 4: " +
+                            num(flow.itemSize()));
+    inputFlow_ = flow;
+    flowSet_ = true;
+}
+
+// DeltaFlowFilter::setGamma / setMaxFlow (flowfilter.cu:559-579).
+void DeltaFlowFilter::setGamma(float g) {
+    if (input_.itemSize() == 1) update_.setGamma(g / (255.0f * 255.0f));
+    else update_.setGamma(g);
+}
+
+void DeltaFlowFilter::setMaxFlow(float m) {
+    update_.setMaxFlow(m);
+    propagator_.setIterations(int(ceilf(m)));
+}
+
+// ======================================================= PyramidalFlowFilter
+
+// PyramidalFlowFilter(h, w, levels) (flowfilter.cu:631-641).
+PyramidalFlowFilter::PyramidalFlowFilter(int height, int width, int levels)
+    : h_(height), w_(width), levels_(levels) {
+    configure();
+}
+
+// PyramidalFlowFilter::configure (flowfilter.cu:650-682): the top level is a
+// FlowFilter on the coarsest image; each lower level a DeltaFlowFilter fed
+// by the top's unsmoothed flow (first) or the next coarser level's
+// smoothed flow.
+void PyramidalFlowFilter::configure() {
+    input_ = Image(h_, w_, 1, 1);
+    pyramid_ = ImagePyramid(input_, levels_);
+    top_ = FlowFilter(pyramid_.getImage(levels_ - 1));
+    low_.clear();
+    if (levels_ > 1) {
+        low_.resize(levels_ - 1);
+        Image levelInputFlow = top_.getFlow();
+        levelInputFlow.clear();
+        for (int l = levels_ - 2; l >= 0; --l) {
+            low_[l] = DeltaFlowFilter(pyramid_.getImage(l), levelInputFlow);
+            levelInputFlow = low_[l].getFlow();
+        }
+    }
+    input_.clear();
+    for (int l = 0; l < levels_; ++l) pyramid_.getImage(l).clear();
+    setFused(fused_);
+    configured_ = true;
+}
+
+void PyramidalFlowFilter::setFused(bool f) {
+    fused_ = f;
+    top_.setFused(f);
+    for (auto& d : low_) d.setFused(f);
+}
+
+// PyramidalFlowFilter::compute (flowfilter.cu:685-716).
+void PyramidalFlowFilter::computeStaged() {
+    pyramid_.run();
+    if (levels_ == 1) {
+        // FlowFilter::compute (flowfilter.cu:125-153); its first-load copy
+        // reads only the image model output, so it may follow propagation.
+        top_.runImageModel();
+        top_.runPropagation();
+        top_.runUpdate();
+        return;
+    }
+    top_.runImageModel();
+    top_.runPropagation();
+    for (int l = 0; l < levels_ - 1; ++l) {
+        low_[l].runImageModel();
+        low_[l].runPropagation();
+    }
+    top_.runUpdate();
+    for (int l = 0; l < levels_ - 1; ++l) low_[l].runUpdate();
+}
+
+// Same frame with one fused kernel per level: the top level first, then
+// levels 0 .. L-2 in ascending order. Level l reads level l+1's smoothed
+// flow before level l+1 has advanced (the reference's update order) and the
+// top level's unsmoothed flow of this frame.
+void PyramidalFlowFilter::computeFused() {
+    pyramid_.run();
+    hipStream_t s = stream();
+    launch_fused_level(top_.fusedArgs(), s);
+    check_launch("PyramidalFlowFilter::compute (fused top)");
+    top_.fusedCommit();
+    for (int l = 0; l < levels_ - 1; ++l) {
+        launch_fused_level(low_[l].fusedArgs(), s);
+        check_launch("PyramidalFlowFilter::compute (fused level)");
+        low_[l].fusedCommit();
+    }
+}
+
+void PyramidalFlowFilter::compute() {
+    if (!configured_) throw logic_error("PyramidalFlowFilter::compute(): filter not configured");
+    startTiming();
+    bool fused = fused_ && top_.fusedSupported();
+    for (auto& d : low_) fused = fused && d.fusedSupported();
+    if (fused) computeFused();
+    else computeStaged();
+    stopTiming();
+}
+
+Image PyramidalFlowFilter::getFlow() const { return levels_ == 1 ? top_.getFlow() : low_[0].getFlow(); }
+
+void PyramidalFlowFilter::loadImage(const ff_image_t& img) { input_.upload(img); }
+void PyramidalFlowFilter::loadImageDevice(const void* dev, size_t pitch) { input_.upload_device(dev, pitch); }
+
+void PyramidalFlowFilter::downloadFlow(ff_image_t& flow) {
+    if (levels_ == 1) top_.downloadFlow(flow);
+    else low_[0].getFlow().download(flow);
+}
+
+void PyramidalFlowFilter::downloadImage(ff_image_t& img) {
+    if (levels_ == 1) top_.downloadImage(img);
+    else low_[0].getImage().download(img);
+}
+
+void PyramidalFlowFilter::checkLevel(int level, const char* who) const {
+    if (level < 0 || level >= levels_)
+        throw runtime_error(std::string("PyramidalFlowFilter::") + who + ": level index out of bounds: " + num(level));
+}
+
+float PyramidalFlowFilter::getGamma(int level) const {
+    checkLevel(level, "getGamma()");
+    return level == levels_ - 1 ? top_.getGamma() : low_[level].getGamma();
+}
+
+void PyramidalFlowFilter::setGamma(int level, float g) {
+    checkLevel(level, "setGamma()");
+    if (level == levels_ - 1) top_.setGamma(g);
+    else low_[level].setGamma(g);
+}
+
+void PyramidalFlowFilter::setGamma(const std::vector<float>& g) {
+    if ((int)g.size() != levels_)
+        throw runtime_error("PyramidalFlowFilter::setGamma(): gamma vector should be size " + num(levels_) +
+                            ", got: " + num((long)g.size()));
+    for (int l = 0; l < levels_; ++l) setGamma(l, g[l]);
+}
+
+int PyramidalFlowFilter::getSmoothIterations(int level) const {
+    checkLevel(level, "getSmoothIterations()");
+    return level == levels_ - 1 ? top_.getSmoothIterations() : low_[level].getSmoothIterations();
+}
+
+void PyramidalFlowFilter::setSmoothIterations(int level, int n) {
+    checkLevel(level, "setSmoothIterations()");
+    if (level == levels_ - 1) top_.setSmoothIterations(n);
+    else low_[level].setSmoothIterations(n);
+}
+
+void PyramidalFlowFilter::setSmoothIterations(const std::vector<int>& n) {
+    if ((int)n.size() != levels_)
+        throw runtime_error("PyramidalFlowFilter::setSmoothIterations(): iterations vector should be size " +
+                            num(levels_) + ", got: " + num((long)n.size()));
+    for (int l = 0; l < levels_; ++l) setSmoothIterations(l, n[l]);
+}
+
+float PyramidalFlowFilter::getMaxFlow() const { return levels_ == 1 ? top_.getMaxFlow() : low_[0].getMaxFlow(); }
+
+// PyramidalFlowFilter::setMaxFlow (flowfilter.cu:849-866): level 0 gets m,
+// each coarser level half of the previous, the top the last halving.
+void PyramidalFlowFilter::setMaxFlow(float m) {
+    if (levels_ == 1) {
+        top_.setMaxFlow(m);
+        return;
+    }
+    float ml = m;
+    for (int l = 0; l < levels_ - 1; ++l) {
+        low_[l].setMaxFlow(ml);
+        ml /= 2.0f;
+    }
+    top_.setMaxFlow(ml);
+}
+
+// PyramidalFlowFilter::setPropagationBorder (flowfilter.cu:869-877). The
+// reference loops h < levels over levels-1 filters (reading one past the
+// end); here every existing level is set.
+void PyramidalFlowFilter::setPropagationBorder(int b) {
+    top_.setPropagationBorder(b);
+    for (auto& d : low_) d.setPropagationBorder(b);
+}
+
+}  // namespace ffamd

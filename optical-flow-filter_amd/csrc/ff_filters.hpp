@@ -1,0 +1,160 @@
+// ff_filters.hpp -- the filter classes of flowfilter::gpu
+// (include/flowfilter/gpu/flowfilter.h:31-332): FlowFilter,
+// DeltaFlowFilter and PyramidalFlowFilter.
+//
+// Two execution paths produce identical results:
+//  - staged: one launch per reference stage kernel, the reference's exact
+//    sequence (used for the first frame, for the split compute methods and
+//    when the fused kernels do not cover a parameter set);
+//  - fused: one kernel per pyramid level and frame (image model,
+//    propagation, update and smoothing in one pass over HBM), with the
+//    level state ping-ponged between two buffers.
+#pragma once
+
+#include <vector>
+
+#include "ff_stages.hpp"
+
+namespace ffamd {
+
+class FlowFilter : public Stage {
+public:
+    FlowFilter() = default;
+    explicit FlowFilter(const Image& inputImage);
+    FlowFilter(int height, int width, int smoothIterations = 1, float maxflow = 1.0f, float gamma = 1.0f);
+    void configure() override;
+    void compute() override;
+    const char* name() const override { return "FlowFilter"; }
+    void computeImageModel();
+    void computePropagation();
+    void computeUpdate();
+    void setInputImage(const Image& img);
+    Image getFlow() const { return update_.getUpdatedFlow(); }
+    void loadImage(const ff_image_t& img);
+    void loadImageDevice(const void* dev, size_t pitch);
+    void downloadFlow(ff_image_t& flow);
+    void downloadImage(ff_image_t& img);
+    float getGamma() const { return update_.getGamma(); }
+    void setGamma(float g);
+    float getMaxFlow() const { return update_.getMaxFlow(); }
+    void setMaxFlow(float m);
+    int getSmoothIterations() const { return smoother_.getIterations(); }
+    void setSmoothIterations(int n) { smoother_.setIterations(n); }
+    void setPropagationBorder(int b) { propagator_.setBorder(b); }
+    int getPropagationBorder() const { return propagator_.getBorder(); }
+    int getPropagationIterations() const { return propagator_.getIterations(); }
+    int height() const { return h_; }
+    int width() const { return w_; }
+    void setFused(bool f) { fused_ = f; }
+    bool fused() const { return fused_; }
+
+    // Used by PyramidalFlowFilter: the same work without the timer.
+    void runImageModel();
+    void runPropagation();
+    void runUpdate();
+    bool firstLoad() const { return firstLoad_; }
+    // Arguments of the fused kernel for this level (allocates the spare
+    // state buffers on first use) and the state exchange after it ran.
+    LevelArgs fusedArgs();
+    void fusedCommit();
+    bool fusedSupported();
+
+private:
+    void requireConfigured(const char* who) const;
+    int h_ = 0, w_ = 0;
+    bool configured_ = false, firstLoad_ = false, inputSet_ = false, fused_ = true;
+    Image input_;
+    ImageModel model_;
+    FlowUpdate update_;
+    FlowSmoother smoother_;
+    FlowPropagator propagator_;
+    Image spareFlow_, spareImage_;
+};
+
+class DeltaFlowFilter : public Stage {
+public:
+    DeltaFlowFilter() = default;
+    DeltaFlowFilter(const Image& inputImage, const Image& inputFlow);
+    void configure() override;
+    void compute() override;
+    const char* name() const override { return "DeltaFlowFilter"; }
+    void computeImageModel();
+    void computePropagation();
+    void computeUpdate();
+    void setInputImage(const Image& img);
+    void setInputFlow(const Image& flow);
+    Image getFlow() const { return smoother_.getSmoothedFlow(); }
+    Image getImage() const { return update_.getUpdatedImage(); }
+    float getGamma() const { return update_.getGamma(); }
+    void setGamma(float g);
+    float getMaxFlow() const { return update_.getMaxFlow(); }
+    void setMaxFlow(float m);
+    int getSmoothIterations() const { return smoother_.getIterations(); }
+    void setSmoothIterations(int n) { smoother_.setIterations(n); }
+    void setPropagationBorder(int b) { propagator_.setBorder(b); }
+    int getPropagationBorder() const { return propagator_.getBorder(); }
+    int getPropagationIterations() const { return propagator_.getIterations(); }
+    int height() const { return input_.height(); }
+    int width() const { return input_.width(); }
+    void setFused(bool f) { fused_ = f; }
+
+    void runImageModel();
+    void runPropagation();
+    void runUpdate();
+    bool firstLoad() const { return firstLoad_; }
+    LevelArgs fusedArgs();
+    void fusedCommit();
+    bool fusedSupported();
+
+private:
+    void requireConfigured(const char* who) const;
+    bool configured_ = false, firstLoad_ = true, imageSet_ = false, flowSet_ = false, fused_ = true;
+    Image input_, inputFlow_;
+    ImageModel model_;
+    DeltaFlowUpdate update_;
+    FlowSmoother smoother_;
+    FlowPropagatorPayload propagator_;
+    Image spareFlow_, spareImage_, spareDelta_;
+};
+
+class PyramidalFlowFilter : public Stage {
+public:
+    PyramidalFlowFilter() = default;
+    PyramidalFlowFilter(int height, int width, int levels);
+    void configure() override;
+    void compute() override;
+    const char* name() const override { return "PyramidalFlowFilter"; }
+    Image getFlow() const;
+    void loadImage(const ff_image_t& img);
+    void loadImageDevice(const void* dev, size_t pitch);
+    void downloadFlow(ff_image_t& flow);
+    void downloadImage(ff_image_t& img);
+    float getGamma(int level) const;
+    void setGamma(int level, float g);
+    void setGamma(const std::vector<float>& g);
+    int getSmoothIterations(int level) const;
+    void setSmoothIterations(int level, int n);
+    void setSmoothIterations(const std::vector<int>& n);
+    float getMaxFlow() const;
+    void setMaxFlow(float m);
+    void setPropagationBorder(int b);
+    int getPropagationBorder() const { return top_.getPropagationBorder(); }
+    int height() const { return h_; }
+    int width() const { return w_; }
+    int levels() const { return levels_; }
+    void setFused(bool f);
+    bool fused() const { return fused_; }
+
+private:
+    void checkLevel(int level, const char* who) const;
+    void computeStaged();
+    void computeFused();
+    bool configured_ = false, fused_ = true;
+    int h_ = 0, w_ = 0, levels_ = 0;
+    Image input_;
+    ImagePyramid pyramid_;
+    FlowFilter top_;
+    std::vector<DeltaFlowFilter> low_;
+};
+
+}  // namespace ffamd

@@ -1,0 +1,96 @@
+// ff_kernels.hpp -- host-side launchers of the HIP kernels (gfx950).
+//
+// Every launcher takes plain device pointers, sizes and pitches plus the
+// stream to enqueue on; none of them allocates, synchronizes or throws on
+// the device side. Shapes are validated by the stage classes before a
+// launch.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace ffamd {
+
+// Device view of one pitched 2-D image (element type implied by the call).
+struct DevImg {
+    void* data = nullptr;
+    int h = 0;
+    int w = 0;
+    size_t pitch = 0;  // bytes between rows
+};
+
+// ------------------------------------------------------------ stage kernels
+// One launch per reference kernel family; used by the stage classes
+// (ImageModel, FlowPropagator, ...) and by the split compute methods.
+
+// imagePrefilter_k + imageModel_k: a0 (float) and gradient (float2) of a u8
+// (normalized read) or float image.
+void launch_image_model(const DevImg& img, bool u8, const DevImg& a0, const DevImg& grad,
+                        hipStream_t s);
+
+// scalarProductF2_k: out = scalar * in on float2 images.
+void launch_scale_f2(const DevImg& in, float scalar, const DevImg& out, hipStream_t s);
+
+// One upwind sub-pass (axis 0 = x, 1 = y) of the flow and optional payloads
+// (pass nullptr scalar/vector views for the plain propagator).
+void launch_propagate_pass(int axis, const DevImg& flow_in, const DevImg& flow_out,
+                           const DevImg* scalar_in, const DevImg* scalar_out, const DevImg* vec_in,
+                           const DevImg* vec_out, float dt, int border, hipStream_t s);
+
+// flowUpdate_k. image_out may alias a0old (same pixel read then written).
+void launch_update(const DevImg& a0, const DevImg& grad, const DevImg& a0old, const DevImg& flow_old,
+                   const DevImg& image_out, const DevImg& flow_out, float gamma, float maxflow,
+                   hipStream_t s);
+
+// deltaFlowUpdate_k. xmap/ymap hold the upper-level texel index of every
+// column/row (the normalized-coordinate point-sampling rule evaluated on the
+// host).
+void launch_delta_update(const DevImg& a0, const DevImg& grad, const DevImg& a0old,
+                         const DevImg& dflow_old, const DevImg& upper, const int* xmap,
+                         const int* ymap, const DevImg& image_out, const DevImg& dflow_out,
+                         const DevImg& flow_out, float gamma, float maxflow, hipStream_t s);
+
+// One box-smoothing pass (axis 0 = x, 1 = y) on float2 images.
+void launch_smooth_pass(int axis, const DevImg& in, const DevImg& out, hipStream_t s);
+
+// One pyramid half-resolution pass (axis 0 = x, 1 = y), u8 or float.
+void launch_down_pass(int axis, bool u8, const DevImg& in, const DevImg& out, hipStream_t s);
+
+// Byte copy of a pitched 2-D region as a kernel (usable under graph capture,
+// unlike a memset/memcpy node). row_bytes bytes per row.
+void launch_copy2d(const DevImg& src, const DevImg& dst, size_t row_bytes, hipStream_t s);
+
+// --------------------------------------------------------- fused level kernels
+
+// Parameters of one pyramid level for the fused per-frame kernel: image
+// model, propagation (with payloads on delta levels), update and smoothing
+// of one level and frame.
+struct LevelArgs {
+    DevImg img;              // this level's input image
+    int img_u8 = 1;          // 1: u8 (normalized read), 0: float
+    // previous state (read)
+    DevImg flow_s_in;        // smoothed flow, float2
+    DevImg a0_in;            // updated image (old brightness), float
+    DevImg dflow_in;         // delta flow, float2 (delta levels only)
+    // new state (written)
+    DevImg flow_s_out;
+    DevImg a0_out;
+    DevImg dflow_out;        // delta levels only
+    DevImg flow_upd_out;     // unsmoothed updated flow
+    // upper level flow (delta levels only) and its texel index maps
+    DevImg upper;
+    const int* xmap = nullptr;
+    const int* ymap = nullptr;
+    float gamma = 1.0f, maxflow = 1.0f, dt = 1.0f;
+    int prop_iters = 1, smooth_iters = 1, border = 3;
+    int delta = 0;           // 1: DeltaFlowFilter level, 0: top level (FlowFilter)
+};
+
+// Returns false when the level's parameters are outside what the fused
+// kernels are compiled for (the caller then runs the stage kernels).
+bool fused_level_supported(const LevelArgs& a);
+void launch_fused_level(const LevelArgs& a, hipStream_t s);
+
+}  // namespace ffamd

@@ -1,0 +1,283 @@
+// ff_runtime.cpp -- see ff_runtime.hpp.
+#include "ff_runtime.hpp"
+
+#include <cstring>
+#include <mutex>
+#include <sstream>
+#include <vector>
+
+namespace ffamd {
+
+void throw_hip(hipError_t e, const char* what, const char* file, int line) {
+    std::ostringstream os;
+    os << "HIP error '" << hipGetErrorString(e) << "' (" << (int)e << ") from " << what << " at " << file << ":"
+       << line;
+    if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) throw Error(FF_ERR_BAD_ALLOC, os.str());
+    throw Error(FF_ERR_HIP, os.str());
+}
+
+void check_launch(const char* what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) throw_hip(e, what, __FILE__, __LINE__);
+}
+
+// ------------------------------------------------------------------ stream
+
+namespace {
+std::mutex g_mu;
+hipStream_t g_own = nullptr;
+hipStream_t g_user = nullptr;
+bool g_has_user = false;
+}  // namespace
+
+hipStream_t stream() {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (g_has_user) return g_user;
+    if (!g_own) FF_HIP(hipStreamCreateWithFlags(&g_own, hipStreamNonBlocking));
+    return g_own;
+}
+
+void set_stream(hipStream_t s) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_user = s;
+    g_has_user = s != nullptr;
+}
+
+void synchronize() { FF_HIP(hipStreamSynchronize(stream())); }
+
+// ----------------------------------------------------------------- storage
+
+Storage::~Storage() {
+    if (ptr) (void)hipFree(ptr);
+}
+
+size_t pitch_for(int width, int depth, int itemSize) {
+    const size_t row = (size_t)width * depth * itemSize;
+    return (row + 255) / 256 * 256;
+}
+
+Image::Image(int height, int width, int depth, int itemSize) : h_(height), w_(width), d_(depth), item_(itemSize) {
+    allocate();
+}
+
+void Image::allocate() {
+    if (h_ < 0 || w_ < 0 || d_ < 0 || item_ < 0) throw invalid_argument("GPUImage: negative dimension " + shapeString());
+    pitch_ = pitch_for(w_, d_, item_);
+    st_ = std::make_shared<Storage>();
+    const size_t bytes = pitch_ * (size_t)h_;
+    if (bytes > 0) {
+        FF_HIP(hipMalloc(&st_->ptr, bytes));
+        st_->bytes = bytes;
+    }
+}
+
+std::string Image::shapeString() const {
+    std::ostringstream os;
+    os << "[" << h_ << ", " << w_ << ", " << d_ << "][" << item_ << "]";
+    return os.str();
+}
+
+std::string Image::shapeString(const ff_image_t& img) {
+    std::ostringstream os;
+    os << "[" << img.height << ", " << img.width << ", " << img.depth << "][" << img.itemSize << "]";
+    return os.str();
+}
+
+// GPUImage::upload (src/gpu/image.cu:81-121): an unallocated image takes the
+// host image's shape; otherwise shapes must match.
+void Image::upload(const ff_image_t& host) {
+    if (!st_) {
+        h_ = host.height;
+        w_ = host.width;
+        d_ = host.depth;
+        item_ = (int)host.itemSize;
+        allocate();
+    }
+    if (!matches(host))
+        throw invalid_argument("GPUImage::upload(): shapes do not match. Required: " + shapeString() +
+                               ", passed: " + shapeString(host));
+    if (rowBytes() == 0 || h_ == 0) return;
+    if (!host.data) throw invalid_argument("GPUImage::upload(): null host data");
+    if (host.pitch < rowBytes()) throw invalid_argument("GPUImage::upload(): host pitch smaller than a row");
+    hipStream_t s = stream();
+    FF_HIP(hipMemcpy2DAsync(data(), pitch_, host.data, host.pitch, rowBytes(), h_, hipMemcpyHostToDevice, s));
+    FF_HIP(hipStreamSynchronize(s));
+}
+
+// GPUImage::download (src/gpu/image.cu:123-150). The reference prints and
+// returns on an unallocated image; here it is an error.
+void Image::download(ff_image_t& host) const {
+    if (!st_) throw logic_error("GPUImage::download(): unallocated image");
+    if (!matches(host))
+        throw invalid_argument("GPUImage::download(): shapes do not match. Required: " + shapeString() +
+                               ", passed: " + shapeString(host));
+    if (rowBytes() == 0 || h_ == 0) return;
+    if (!host.data) throw invalid_argument("GPUImage::download(): null host data");
+    if (host.pitch < rowBytes()) throw invalid_argument("GPUImage::download(): host pitch smaller than a row");
+    hipStream_t s = stream();
+    FF_HIP(hipMemcpy2DAsync(host.data, host.pitch, data(), pitch_, rowBytes(), h_, hipMemcpyDeviceToHost, s));
+    FF_HIP(hipStreamSynchronize(s));
+}
+
+void Image::upload_device(const void* dev, size_t pitch) {
+    if (!st_) throw logic_error("GPUImage::upload_device(): unallocated image");
+    if (rowBytes() == 0 || h_ == 0) return;
+    if (!dev) throw invalid_argument("GPUImage::upload_device(): null device pointer");
+    if (pitch < rowBytes()) throw invalid_argument("GPUImage::upload_device(): pitch smaller than a row");
+    FF_HIP(hipMemcpy2DAsync(data(), pitch_, dev, pitch, rowBytes(), h_, hipMemcpyDeviceToDevice, stream()));
+}
+
+void Image::download_device(void* dev, size_t pitch) const {
+    if (!st_) throw logic_error("GPUImage::download_device(): unallocated image");
+    if (rowBytes() == 0 || h_ == 0) return;
+    if (!dev) throw invalid_argument("GPUImage::download_device(): null device pointer");
+    if (pitch < rowBytes()) throw invalid_argument("GPUImage::download_device(): pitch smaller than a row");
+    FF_HIP(hipMemcpy2DAsync(dev, pitch, data(), pitch_, rowBytes(), h_, hipMemcpyDeviceToDevice, stream()));
+}
+
+// GPUImage::copyFrom (src/gpu/image.cu:152-170): same shape and pitch
+// required (compareShapeGPU, :207-214). Enqueued as a copy kernel so that it
+// orders with the filter kernels on the library stream.
+void Image::copyFrom(const Image& src) {
+    if (!sameShape(src) || pitch_ != src.pitch_)
+        throw invalid_argument("GPUImage::copyFrom(): shapes do not match. Required: " + shapeString() +
+                               ", passed: " + src.shapeString());
+    if (!st_ || !src.st_) throw logic_error("GPUImage::copyFrom(): unallocated image");
+    if (rowBytes() == 0 || h_ == 0) return;
+    launch_copy2d(src.view(), view(), rowBytes(), stream());
+    check_launch("copyFrom");
+}
+
+// GPUImage::clear (src/gpu/image.cu:172-176).
+void Image::clear() {
+    if (!st_ || !st_->ptr) return;
+    FF_HIP(hipMemset2DAsync(data(), pitch_, 0, rowBytes(), h_, stream()));
+}
+
+void Image::swapBuffers(Image& o) {
+    if (!sameShape(o) || pitch_ != o.pitch_) throw logic_error("swapBuffers: shapes differ");
+    if (!st_ || !o.st_) throw logic_error("swapBuffers: unallocated image");
+    std::swap(st_->ptr, o.st_->ptr);
+    std::swap(st_->bytes, o.st_->bytes);
+}
+
+void DeviceArray::assign(const void* host, size_t bytes) {
+    st_ = std::make_shared<Storage>();
+    if (bytes == 0) return;
+    FF_HIP(hipMalloc(&st_->ptr, bytes));
+    st_->bytes = bytes;
+    hipStream_t s = stream();
+    FF_HIP(hipMemcpyAsync(st_->ptr, host, bytes, hipMemcpyHostToDevice, s));
+    FF_HIP(hipStreamSynchronize(s));
+}
+
+// ---------------------------------------------------------------- profiler
+
+namespace {
+struct Profiler {
+    std::mutex mu;
+    bool on = false;
+    std::string filter;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pool;
+    size_t used = 0;
+};
+Profiler& prof() {
+    static Profiler p;
+    return p;
+}
+}  // namespace
+
+void profile_begin(const std::string& filter) {
+    Profiler& p = prof();
+    std::lock_guard<std::mutex> lk(p.mu);
+    p.on = true;
+    p.filter = filter;
+    p.used = 0;
+}
+
+void profile_end(double* total_ms, long long* count) {
+    Profiler& p = prof();
+    std::lock_guard<std::mutex> lk(p.mu);
+    double tot = 0.0;
+    for (size_t i = 0; i < p.used; ++i) {
+        FF_HIP(hipEventSynchronize(p.pool[i].second));
+        float ms = 0.0f;
+        FF_HIP(hipEventElapsedTime(&ms, p.pool[i].first, p.pool[i].second));
+        tot += ms;
+    }
+    if (total_ms) *total_ms = tot;
+    if (count) *count = (long long)p.used;
+    p.on = false;
+    p.used = 0;
+}
+
+ProfileScope::ProfileScope(const char* kernel, hipStream_t s) : s_(s) {
+    Profiler& p = prof();
+    std::lock_guard<std::mutex> lk(p.mu);
+    if (!p.on || std::string(kernel).find(p.filter) == std::string::npos) return;
+    if (p.used == p.pool.size()) {
+        hipEvent_t a, b;
+        FF_HIP(hipEventCreate(&a));
+        FF_HIP(hipEventCreate(&b));
+        p.pool.emplace_back(a, b);
+    }
+    slot_ = (int)p.used++;
+    FF_HIP(hipEventRecord(p.pool[slot_].first, s_));
+}
+
+ProfileScope::~ProfileScope() {
+    if (slot_ < 0) return;
+    Profiler& p = prof();
+    (void)hipEventRecord(p.pool[slot_].second, s_);
+}
+
+// ------------------------------------------------------------------- timer
+
+Timer::~Timer() {
+    if (start_) (void)hipEventDestroy(start_);
+    if (stop_) (void)hipEventDestroy(stop_);
+}
+
+Timer& Timer::operator=(Timer&& o) noexcept {
+    if (this != &o) {
+        if (start_) (void)hipEventDestroy(start_);
+        if (stop_) (void)hipEventDestroy(stop_);
+        start_ = o.start_;
+        stop_ = o.stop_;
+        pending_ = o.pending_;
+        last_ = o.last_;
+        o.start_ = o.stop_ = nullptr;
+        o.pending_ = false;
+    }
+    return *this;
+}
+
+void Timer::ensure() {
+    if (!start_) FF_HIP(hipEventCreate(&start_));
+    if (!stop_) FF_HIP(hipEventCreate(&stop_));
+}
+
+void Timer::start() {
+    ensure();
+    FF_HIP(hipEventRecord(start_, stream()));
+}
+
+void Timer::stop() {
+    ensure();
+    FF_HIP(hipEventRecord(stop_, stream()));
+    pending_ = true;
+}
+
+// Stage::elapsedTime (src/gpu/pipeline.cu:87-89) after the event wait the
+// reference performs in stopTiming (:72-82); here the wait happens only
+// when the time is asked for.
+float Timer::elapsed() {
+    if (pending_) {
+        FF_HIP(hipEventSynchronize(stop_));
+        FF_HIP(hipEventElapsedTime(&last_, start_, stop_));
+        pending_ = false;
+    }
+    return last_;
+}
+
+}  // namespace ffamd

@@ -1,0 +1,53 @@
+"""Common base of the stage wrappers: owns one ff_stage handle."""
+import ctypes
+
+from .. import _native as N
+from .image import GPUImage
+
+
+class Stage(object):
+    """flowfilter::gpu::Stage (include/flowfilter/gpu/pipeline.h:29-78)."""
+
+    _h = None
+
+    def _set_handle(self, h):
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            try:
+                N.lib.ff_stage_destroy(ctypes.c_void_p(h))
+            except Exception:
+                pass
+            self._h = None
+
+    @property
+    def handle(self):
+        if not self._h:
+            raise RuntimeError("stage has no underlying object")
+        return ctypes.c_void_p(self._h)
+
+    def configure(self):
+        N.call("ff_stage_configure", self.handle)
+
+    def compute(self):
+        N.call("ff_stage_compute", self.handle)
+
+    def elapsedTime(self):
+        """Milliseconds of the last compute() (waits for it to finish)."""
+        return N.get_float("ff_stage_elapsed_time", self.handle)
+
+    # helpers for subclasses
+    def _img(self, fn):
+        return GPUImage._wrap(N.get_handle(fn, self.handle))
+
+    def _geti(self, fn):
+        return N.get_int(fn, self.handle)
+
+    def _getf(self, fn):
+        return N.get_float(fn, self.handle)
+
+
+def _hnd(img):
+    return img.handle if img is not None else None

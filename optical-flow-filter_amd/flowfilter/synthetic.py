@@ -1,0 +1,95 @@
+"""Seeded synthetic image sequences with known motion.
+
+The reference ships no test data (its notebook reads private Blender
+renders), so parity tests and benchmarks run on these generated sequences:
+
+* ``translating_noise``: band-limited noise (white noise low-passed with a
+  Gaussian, rescaled to [0, 255]) translated by a constant velocity per frame
+  (bilinear resampling of the float frame 0). Ground truth is the velocity.
+  Default: 640x480, (0.7, -0.4) px/frame, sigma 3, seed 1.
+* ``blob_affine``: a sum of random Gaussian blobs on mid-grey, advanced per
+  frame by an affine motion about the image centre (rotation plus
+  expansion, displacement capped), with fresh Gaussian sensor noise per
+  frame. Default: 2048x1088, 400 blobs (sigma 2-20 px, amplitude 20-80),
+  0.5 deg rotation + 1% expansion per frame, cap 4 px, noise sigma 2, seed 2.
+
+All randomness comes from numpy.random.default_rng(seed) (PCG64), so a seed
+gives the same frames on every machine with this numpy.
+"""
+import numpy as np
+from scipy import ndimage
+
+
+def _quantize(f):
+    return np.clip(np.rint(f), 0, 255).astype(np.uint8)
+
+
+def band_limited_noise(h, w, sigma=3.0, seed=1, pad=0):
+    """Float image of low-passed white noise rescaled to [0, 255]."""
+    rng = np.random.default_rng(seed)
+    H, W = h + 2 * pad, w + 2 * pad
+    n = rng.standard_normal((H, W))
+    n = ndimage.gaussian_filter(n, sigma, mode="reflect", truncate=4.0)
+    n -= n.min()
+    n *= 255.0 / max(n.max(), 1e-12)
+    return n
+
+
+def translating_noise(h=480, w=640, frames=200, velocity=(0.7, -0.4), sigma=3.0, seed=1):
+    """Yield uint8 frames of noise translated by ``velocity`` (vx, vy) per frame."""
+    vx, vy = velocity
+    pad = int(np.ceil(max(abs(vx), abs(vy)) * frames)) + 16
+    base = band_limited_noise(h, w, sigma, seed, pad)
+    yy, xx = np.mgrid[0:h, 0:w].astype(np.float64)
+    for t in range(frames):
+        # frame t(x) = base(x - v t): content moves by +v per frame
+        coords = np.stack([yy + pad - vy * t, xx + pad - vx * t])
+        yield _quantize(ndimage.map_coordinates(base, coords, order=1, mode="nearest"))
+
+
+def blob_texture(h, w, blobs=400, seed=2, sigma_range=(2.0, 20.0), amp_range=(20.0, 80.0), background=128.0):
+    rng = np.random.default_rng(seed)
+    img = np.full((h, w), background, np.float64)
+    cy = rng.uniform(0, h, blobs)
+    cx = rng.uniform(0, w, blobs)
+    sg = rng.uniform(sigma_range[0], sigma_range[1], blobs)
+    am = rng.uniform(amp_range[0], amp_range[1], blobs)
+    for y0, x0, s, a in zip(cy, cx, sg, am):
+        r = int(np.ceil(4 * s))
+        ya, yb = max(0, int(y0) - r), min(h, int(y0) + r + 1)
+        xa, xb = max(0, int(x0) - r), min(w, int(x0) + r + 1)
+        if ya >= yb or xa >= xb:
+            continue
+        yy, xx = np.mgrid[ya:yb, xa:xb]
+        img[ya:yb, xa:xb] += a * np.exp(-((yy - y0) ** 2 + (xx - x0) ** 2) / (2 * s * s))
+    return np.clip(img, 0, 255)
+
+
+def affine_displacement(h, w, rot_deg=0.5, scale=1.01, cap=4.0):
+    """Per-pixel displacement (dx, dy) of the affine motion about the centre, capped in norm."""
+    th = np.deg2rad(rot_deg)
+    A = scale * np.array([[np.cos(th), -np.sin(th)], [np.sin(th), np.cos(th)]]) - np.eye(2)
+    yy, xx = np.mgrid[0:h, 0:w].astype(np.float64)
+    px, py = xx - (w - 1) / 2.0, yy - (h - 1) / 2.0
+    dx = A[0, 0] * px + A[0, 1] * py
+    dy = A[1, 0] * px + A[1, 1] * py
+    n = np.sqrt(dx * dx + dy * dy)
+    f = np.where(n > cap, cap / np.maximum(n, 1e-12), 1.0)
+    return dx * f, dy * f
+
+
+def blob_affine(h=1088, w=2048, frames=300, blobs=400, seed=2, rot_deg=0.5, scale=1.01, cap=4.0,
+                noise=2.0):
+    """Yield uint8 frames of a blob texture advected by a capped affine motion."""
+    f = blob_texture(h, w, blobs, seed)
+    dx, dy = affine_displacement(h, w, rot_deg, scale, cap)
+    yy, xx = np.mgrid[0:h, 0:w].astype(np.float64)
+    coords = np.stack([yy - dy, xx - dx])
+    for t in range(frames):
+        rng = np.random.default_rng(seed + t)
+        yield _quantize(f + rng.normal(0.0, noise, (h, w)))
+        f = ndimage.map_coordinates(f, coords, order=1, mode="nearest")
+
+
+def random_u8(h, w, seed=0):
+    return np.random.default_rng(seed).integers(0, 256, (h, w), dtype=np.uint8)

@@ -1,0 +1,256 @@
+"""GPU parity of the filters (FlowFilter, DeltaFlowFilter, PyramidalFlowFilter)
+against the CPU oracle over whole synthetic sequences, through the C ABI.
+
+Bar: max |flow_gpu - flow_oracle_fp32| <= 1e-4 px on
+every frame. The kernels follow the oracle's rounding points, so the
+observed difference is expected to be exactly 0; both are asserted. The
+GPU result is also compared with the FP64 oracle and the error printed.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-4  # px, max abs flow error per frame
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def seq_noise(h, w, n, v=(0.7, -0.4)):
+    from flowfilter import synthetic
+    return list(synthetic.translating_noise(h, w, n, v))
+
+
+def seq_blobs(h, w, n):
+    from flowfilter import synthetic
+    return list(synthetic.blob_affine(h, w, n, blobs=max(20, h * w // 5600)))
+
+
+def check_frame(t, got, want, label):
+    d = float(np.abs(got - want).max()) if got.size else 0.0
+    assert d <= TOL, "%s frame %d: max |diff| %.3g > %.g" % (label, t, d, TOL)
+    assert np.array_equal(got, want), "%s frame %d: not bitwise equal (max |diff| %.3g)" % (label, t, d)
+    return d
+
+
+# ---------------------------------------------------------------- FlowFilter
+
+@pytest.mark.parametrize("shape,frames", [((48, 64), 12), ((480, 640), 30)])
+@pytest.mark.parametrize("fused", [True, False])
+def test_flowfilter_sequence(ffo, shape, frames, fused):
+    import flowfilter.gpu.flowfilters as gf
+    h, w = shape
+    say("flowfilter", shape, frames, "fused" if fused else "staged")
+    g = gf.FlowFilter(h, w, 2, 4.0, 10.0)
+    g.fused = fused
+    assert (g.height, g.width, g.smoothIterations, g.propagationIterations) == (h, w, 2, 4)
+    assert g.maxflow == 4.0 and abs(g.gamma - 10 / 65025.0) < 1e-12
+    o = ffo.FlowFilter(h, w, 2, 4.0, 10.0)
+    o64 = ffo.FlowFilter(h, w, 2, 4.0, 10.0, prec="f64")
+    worst64 = 0.0
+    for t, img in enumerate(seq_noise(h, w, frames)):
+        g.loadImage(img)
+        g.compute()
+        o.compute(img)
+        o64.compute(img)
+        got = g.getFlow()
+        check_frame(t, got, o.downloadFlow(), "FlowFilter flow")
+        check_frame(t, g.getImage(), o.downloadImage(), "FlowFilter image")
+        worst64 = max(worst64, float(np.abs(got - o64.downloadFlow()).max()))
+    # getFlowDevice is the unsmoothed updated flow (FlowFilter::getFlow)
+    check_frame(frames, g.getFlowDevice().download(np.float32), o.getFlow(), "FlowFilter updated flow")
+    assert g.elapsedTime() > 0.0
+    say("  max |gpu - fp64 oracle| = %.3g px" % worst64)
+    assert worst64 < 1e-3
+
+
+def test_flowfilter_split_methods_match_compute(ffo):
+    """computeImageModel + computePropagation + computeUpdate == compute()."""
+    import flowfilter.gpu.flowfilters as gf
+    h, w = 60, 90
+    a, b = gf.FlowFilter(h, w, 2, 3.0, 5.0), gf.FlowFilter(h, w, 2, 3.0, 5.0)
+    for img in seq_noise(h, w, 8, (0.5, 0.3)):
+        a.loadImage(img)
+        b.loadImage(img)
+        a.compute()
+        b.computeImageModel()
+        b.computePropagation()
+        b.computeUpdate()
+        assert np.array_equal(a.getFlow(), b.getFlow())
+
+
+def test_flowfilter_float_input(ffo):
+    import flowfilter.gpu.flowfilters as gf
+    from flowfilter.gpu.image import GPUImage
+    h, w = 40, 56
+    inp = GPUImage((h, w), 4)
+    g = gf.FlowFilter.fromGPUImage(inp)
+    # FlowFilter(GPUImage): raw gamma 1.0, maxflow 1.0, one smoothing iteration
+    assert g.gamma == 1.0 and g.maxflow == 1.0 and g.smoothIterations == 1
+    g.gamma = 0.01
+    g.maxflow = 2.0
+    o = ffo.FlowFilter(h, w, 1, 2.0, 0.01, u8=False)
+    for t, img in enumerate(seq_noise(h, w, 8)):
+        f = (img / 255.0).astype(np.float32)
+        inp.upload(f)
+        g.compute()
+        o.compute(f)
+        check_frame(t, g.getFlow(), o.downloadFlow(), "float input")
+
+
+def test_flowfilter_device_input(ffo):
+    import torch
+    import flowfilter.gpu.flowfilters as gf
+    h, w = 64, 100
+    g = gf.FlowFilter(h, w, 2, 4.0, 10.0)
+    o = ffo.FlowFilter(h, w, 2, 4.0, 10.0)
+    frames = seq_noise(h, w, 6)
+    dev = torch.from_numpy(np.stack(frames)).cuda()
+    torch.cuda.synchronize()
+    for t, img in enumerate(frames):
+        g.loadImageDevice(dev[t].data_ptr(), dev.stride(1))
+        g.compute()
+        o.compute(img)
+        check_frame(t, g.getFlow(), o.downloadFlow(), "device input")
+
+
+def test_flowfilter_errors():
+    import flowfilter.gpu.flowfilters as gf
+    from flowfilter import errors
+    g = gf.FlowFilter(16, 16)
+    with pytest.raises(errors.InvalidArgument):
+        g.loadImage(np.zeros((16, 17), np.uint8))
+    with pytest.raises(errors.FilterError):
+        g.gamma = 0.0
+    with pytest.raises(errors.FilterError):
+        g.maxflow = 0.0
+    with pytest.raises(errors.FilterError):
+        g.smoothIterations = 0
+    with pytest.raises(RuntimeError):
+        g.height = 3
+    with pytest.raises(RuntimeError):
+        g.propagationIterations = 3
+    with pytest.raises(errors.InvalidArgument):
+        g.getFlow(np.zeros((16, 16, 2), np.float64))
+
+
+# ------------------------------------------------------- PyramidalFlowFilter
+
+def _pyr_pair(ffo, h, w, levels, gamma, maxflow, smooth, fused=True):
+    import flowfilter.gpu.flowfilters as gf
+    g = gf.PyramidalFlowFilter(h, w, levels)
+    g.fused = fused
+    g.gamma = gamma if levels > 1 else gamma[0]
+    g.maxflow = maxflow
+    g.smoothIterations = smooth if levels > 1 else smooth[0]
+    o = ffo.PyramidalFlowFilter(h, w, levels)
+    o.configure_like(gamma, maxflow, smooth)
+    return g, o
+
+
+@pytest.mark.parametrize("shape,levels,frames", [((68, 128), 2, 16), ((272, 512), 2, 12), ((96, 160), 3, 12),
+                                                  ((50, 70), 1, 8), ((75, 131), 2, 8)])
+@pytest.mark.parametrize("fused", [True, False])
+def test_pyramidal_sequence(ffo, shape, levels, frames, fused):
+    h, w = shape
+    say("pyramidal", shape, levels, frames, "fused" if fused else "staged")
+    gamma = [10.0, 50.0, 80.0][:levels]
+    smooth = [2, 2, 1][:levels]
+    g, o = _pyr_pair(ffo, h, w, levels, gamma, 4.0, smooth, fused)
+    for t, img in enumerate(seq_blobs(h, w, frames)):
+        g.loadImage(img)
+        g.compute()
+        o.compute(img)
+        check_frame(t, g.getFlow(), o.downloadFlow(), "pyramidal flow")
+        check_frame(t, g.getImage(), o.downloadImage(), "pyramidal image")
+        for lvl in range(levels):
+            o_s = o.level_state(lvl, 0)
+            assert o_s.shape[:2] == ffo.level_sizes(h, w, levels)[lvl]
+    dev = g.getFlowDevice().download(np.float32)
+    check_frame(frames, dev, o.getFlow(), "pyramidal getFlowDevice")
+
+
+def test_pyramidal_full_size_cfg2(ffo):
+    """2048x1088, 2 levels, gamma [10, 50], maxflow 4, smooth [2, 2]: the benchmark workload."""
+    h, w, frames = 1088, 2048, 10
+    ffo.set_threads(16)
+    say("pyramidal full size", h, w, frames)
+    g, o = _pyr_pair(ffo, h, w, 2, [10.0, 50.0], 4.0, [2, 2])
+    o64 = ffo.PyramidalFlowFilter(h, w, 2, prec="f64")
+    o64.configure_like([10.0, 50.0], 4.0, [2, 2])
+    worst64 = 0.0
+    for t, img in enumerate(seq_blobs(h, w, frames)):
+        g.loadImage(img)
+        g.compute()
+        o.compute(img)
+        got = g.getFlow()
+        check_frame(t, got, o.downloadFlow(), "cfg2 flow")
+        o64.compute(img)
+        worst64 = max(worst64, float(np.abs(got - o64.downloadFlow()).max()))
+        say("  frame", t, "ok")
+    say("  max |gpu - fp64 oracle| = %.3g px" % worst64)
+
+
+def test_pyramidal_parameters_and_errors(ffo):
+    import flowfilter.gpu.flowfilters as gf
+    from flowfilter import errors
+    g = gf.PyramidalFlowFilter(64, 96, 3)
+    assert (g.height, g.width, g.levels) == (64, 96, 3)
+    # defaults: raw gamma 1.0, maxflow 1.0, one smoothing iteration per level
+    assert g.gamma == [1.0, 1.0, 1.0] and g.smoothIterations == [1, 1, 1]
+    g.gamma = [10, 50, 100]
+    assert np.allclose(g.gamma, np.array([10, 50, 100]) / 65025.0)
+    g.maxflow = 8.0
+    assert g.maxflow == 8.0
+    g.setGammaVector([1.0, 2.0, 3.0])
+    assert np.allclose(g.gamma, np.array([1, 2, 3]) / 65025.0)
+    with pytest.raises(errors.FilterError):
+        g.setGammaVector([1.0, 2.0])
+    from flowfilter import _native as N
+    with pytest.raises(errors.FilterError):
+        N.get_float("ff_pyramidal_get_gamma", g.handle, 3)
+    g.propagationBorder = 2
+    assert g.propagationBorder == 2
+    with pytest.raises(errors.FilterError):
+        gf.PyramidalFlowFilter(64, 96, 0)
+
+
+def test_pyramidal_device_input_and_timing(ffo):
+    import torch
+    h, w = 136, 256
+    g, o = _pyr_pair(ffo, h, w, 2, [10.0, 50.0], 4.0, [2, 2])
+    frames = seq_blobs(h, w, 6)
+    dev = torch.from_numpy(np.stack(frames)).cuda()
+    torch.cuda.synchronize()
+    for t, img in enumerate(frames):
+        g.loadImageDevice(dev[t].data_ptr(), dev.stride(1))
+        g.compute()
+        o.compute(img)
+        check_frame(t, g.getFlow(), o.downloadFlow(), "pyramidal device input")
+    assert g.elapsedTime() > 0.0
+
+
+def test_delta_flow_filter_standalone(ffo):
+    """DeltaFlowFilter.compute() on its own (image model, propagation, update, smoothing)."""
+    import flowfilter.gpu.flowfilters as gf
+    from flowfilter.gpu.image import GPUImage
+    h, w = 48, 64
+    img_g = GPUImage((h, w), 1)
+    up = GPUImage((h // 2, w // 2, 2), 4)
+    upv = np.random.default_rng(3).uniform(-1, 1, (h // 2, w // 2, 2)).astype(np.float32)
+    up.upload(upv)
+    d = gf.DeltaFlowFilter(img_g, up)
+    d.gamma = 50.0
+    d.maxflow = 2.0
+    assert d.propagationIterations == 2 and (d.height, d.width) == (h, w)
+    frames = seq_noise(h, w, 5)
+    for img in frames:
+        img_g.upload(img)
+        d.compute()
+    out = d.getFlow().download(np.float32)
+    assert np.all(np.isfinite(out)) and np.abs(out).max() <= 2.0
