@@ -101,3 +101,23 @@ def test_version():
     a, b = ctypes.c_int(), ctypes.c_int()
     N.call("ff_version", ctypes.byref(a), ctypes.byref(b))
     assert a.value >= 1
+
+
+def _build_cpp_demo(out_dir):
+    import shutil
+    import subprocess
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("g++ not available")
+    pkg = os.path.join(ROOT, "optical-flow-filter_amd")
+    exe = os.path.join(str(out_dir), "api_demo")
+    subprocess.check_call([gxx, "-std=c++17", "-O2", "-Wall", "-Werror", "-I" + os.path.join(ROOT, "include"),
+                           os.path.join(ROOT, "tests", "cpp", "api_demo.cpp"), "-L" + pkg, "-lflowfilter_amd",
+                           "-Wl,-rpath," + pkg, "-o", exe])
+    return exe
+
+
+def test_cpp_header_compiles_and_links(tmp_path):
+    """include/flowfilter_amd.hpp (the flowfilter::gpu classes) builds against the library."""
+    exe = _build_cpp_demo(tmp_path)
+    assert os.path.exists(exe)

@@ -254,3 +254,47 @@ def test_delta_flow_filter_standalone(ffo):
         d.compute()
     out = d.getFlow().download(np.float32)
     assert np.all(np.isfinite(out)) and np.abs(out).max() <= 2.0
+
+
+def test_cpp_api_demo(tmp_path):
+    """The C++ classes of include/flowfilter_amd.hpp drive the filter end to end."""
+    import subprocess
+    from test_abi import _build_cpp_demo
+    exe = _build_cpp_demo(tmp_path)
+    out = subprocess.run([exe, "240", "320", "2", "8"], capture_output=True, text=True, timeout=120)
+    say(out.stdout.strip(), out.stderr.strip()[-500:])
+    assert out.returncode == 0 and out.stdout.startswith("ok")
+
+
+# ------------------------------------------------ fused kernel coverage
+# The fused per-level kernel needs W >= 64 and H >= 128 at every level; these
+# cases exercise it (tile clamping at the right/bottom edges, odd sizes,
+# several propagation/smoothing counts and borders) against the oracle.
+
+@pytest.mark.parametrize("shape,levels,maxflow,smooth,border", [
+    ((272, 512), 2, 4.0, [2, 2], 3),
+    ((300, 200), 2, 4.0, [2, 2], 3),
+    ((257, 333), 2, 2.5, [1, 3], 3),
+    ((512, 700), 3, 4.0, [2, 2, 1], 3),
+    ((480, 640), 1, 4.0, [2], 3),
+    ((256, 130), 2, 1.0, [2, 2], 1),
+    ((260, 400), 2, 6.0, [3, 2], 5),
+])
+def test_fused_matches_oracle(ffo, shape, levels, maxflow, smooth, border):
+    h, w = shape
+    say("fused", shape, levels, maxflow, smooth, border)
+    gamma = [10.0, 50.0, 80.0][:levels]
+    g, o = _pyr_pair(ffo, h, w, levels, gamma, maxflow, smooth, fused=True)
+    s, _ = _pyr_pair(ffo, h, w, levels, gamma, maxflow, smooth, fused=False)
+    g.propagationBorder = border
+    s.propagationBorder = border
+    o.setPropagationBorder(border)
+    for t, img in enumerate(seq_blobs(h, w, 10)):
+        for f in (g, s):
+            f.loadImage(img)
+            f.compute()
+        o.compute(img)
+        want = o.downloadFlow()
+        check_frame(t, g.getFlow(), want, "fused")
+        check_frame(t, s.getFlow(), want, "staged")
+        check_frame(t, g.getImage(), o.downloadImage(), "fused image")
