@@ -10,7 +10,8 @@ noise), uploaded to HBM before timing and cycled.
 
 One step = loadImage from device memory (the frame's HBM copy into the
 filter's input image) + compute() of one frame. W warm-up steps, then K
-timed steps bracketed by torch.cuda.synchronize() on the filter's stream.
+timed steps between two HIP events on the library stream, with the stream
+synchronized before and after.
 
 Prints one JSON line: Mpix/s of level-0 input pixels, the roofline of the
 dominant kernel (algorithmic bytes per launch / its average duration from
@@ -74,6 +75,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=30)
     ap.add_argument("--frames", type=int, default=24, help="distinct synthetic frames kept in HBM")
     ap.add_argument("--staged", action="store_true", help="one launch per stage kernel instead of fused")
+    ap.add_argument("--no-graphs", action="store_true", help="launch the fused kernels one by one")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     args = ap.parse_args()
@@ -83,46 +85,44 @@ def main():
         sys.exit(2)
 
     import numpy as np
-    import torch
-    torch.cuda.set_device(0)
     import flowfilter.gpu.flowfilters as gf
     from flowfilter import _native as N, synthetic
+    from flowfilter.gpu.image import GPUImage
 
     t0 = time.perf_counter()
     frames = np.stack(list(synthetic.blob_affine(H, W, args.frames)))
     print("generated %d frames in %.1f s" % (len(frames), time.perf_counter() - t0), file=sys.stderr, flush=True)
-    dev = torch.from_numpy(frames).cuda()
-    stream = torch.cuda.current_stream()
-    N.set_stream(stream.cuda_stream)
-    torch.cuda.synchronize()
+    F = len(frames)
+    # all frames resident in HBM as one tall u8 image, frame t at rows t*H..
+    dev = GPUImage((F * H, W), 1)
+    dev.upload(frames.reshape(F * H, W))
+    base, pitch = dev.data, dev.pitch
 
     f = gf.PyramidalFlowFilter(H, W, LEVELS)
     f.gamma = GAMMA
     f.maxflow = MAXFLOW
     f.smoothIterations = SMOOTH
     f.fused = not args.staged
-    pitch = dev.stride(1)
-    F = len(frames)
+    f.graphs = not args.no_graphs
 
     def step(i):
-        f.loadImageDevice(dev[i % F].data_ptr(), pitch)
+        f.loadImageDevice(base + (i % F) * H * pitch, pitch)
         f.compute()
 
     for i in range(args.warmup):
         step(i)
-    torch.cuda.synchronize()
+    N.synchronize()
     print("warm-up done", file=sys.stderr, flush=True)
 
     def timed(first):
-        torch.cuda.synchronize()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        N.synchronize()
         w0 = time.perf_counter()
-        e0.record(stream)
+        m0 = N.Marker()
         for i in range(args.steps):
             step(first + i)
-        e1.record(stream)
-        torch.cuda.synchronize()
-        return e0.elapsed_time(e1), time.perf_counter() - w0
+        m1 = N.Marker()
+        ms = m0.elapsed_ms(m1)
+        return ms, time.perf_counter() - w0
 
     ms, wall = timed(args.warmup)
     ms_per_step = ms / args.steps

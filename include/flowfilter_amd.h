@@ -282,6 +282,10 @@ FF_API ff_status ff_pyramidal_load_image_device(ff_stage* s, const void* dev, si
  * identical results. Applies to FlowFilter and PyramidalFlowFilter. */
 FF_API ff_status ff_filter_set_fused(ff_stage* s, int fused);
 FF_API ff_status ff_filter_get_fused(ff_stage* s, int* out);
+/* PyramidalFlowFilter only: replay each fused frame as a captured HIP graph
+ * (1, the default) or launch its kernels one by one (0). */
+FF_API ff_status ff_filter_set_graphs(ff_stage* s, int graphs);
+FF_API ff_status ff_filter_get_graphs(ff_stage* s, int* out);
 
 /* Kernel profiling (no reference counterpart; the reference times whole
  * stages only, src/gpu/pipeline.cu:63-89). Between begin and end, every
@@ -292,6 +296,14 @@ FF_API ff_status ff_filter_get_fused(ff_stage* s, int* out);
  * scale_f2, copy2d, fused_top, fused_delta. */
 FF_API ff_status ff_profile_begin(const char* filter);
 FF_API ff_status ff_profile_end(double* total_ms, long long* count);
+
+/* Stream markers (no reference counterpart): a timing event recorded on the
+ * library stream; elapsed waits for `stop` and returns the milliseconds
+ * between the two markers. For timing loops of compute() calls. */
+typedef struct ff_marker ff_marker;
+FF_API ff_status ff_marker_record(ff_marker** out);
+FF_API ff_status ff_marker_elapsed(ff_marker* start, ff_marker* stop, float* ms);
+FF_API ff_status ff_marker_destroy(ff_marker* m);
 
 #ifdef __cplusplus
 }

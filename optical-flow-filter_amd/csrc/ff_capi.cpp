@@ -519,12 +519,50 @@ ff_status ff_filter_get_fused(ff_stage* s, int* out) {
     });
 }
 
+ff_status ff_filter_set_graphs(ff_stage* s, int graphs) {
+    return guard([&] { as<PyramidalFlowFilter>(s, "PyramidalFlowFilter").setGraphs(graphs != 0); });
+}
+ff_status ff_filter_get_graphs(ff_stage* s, int* out) {
+    return guard([&] {
+        out_ptr(out, "graphs");
+        *out = as<PyramidalFlowFilter>(s, "PyramidalFlowFilter").graphs();
+    });
+}
+
 // --------------------------------------------------------------- profiling
 ff_status ff_profile_begin(const char* filter) {
     return guard([&] { profile_begin(filter ? filter : ""); });
 }
 ff_status ff_profile_end(double* total_ms, long long* count) {
     return guard([&] { profile_end(total_ms, count); });
+}
+
+// ------------------------------------------------------------ stream markers
+ff_status ff_marker_record(ff_marker** out) {
+    return guard([&] {
+        out_ptr(out, "ff_marker**");
+        hipEvent_t e = nullptr;
+        FF_HIP(hipEventCreate(&e));
+        const hipError_t r = hipEventRecord(e, stream());
+        if (r != hipSuccess) {
+            (void)hipEventDestroy(e);
+            throw_hip(r, "hipEventRecord", __FILE__, __LINE__);
+        }
+        *out = reinterpret_cast<ff_marker*>(e);
+    });
+}
+ff_status ff_marker_elapsed(ff_marker* start, ff_marker* stop, float* ms) {
+    return guard([&] {
+        if (!start || !stop) throw null_handle("ff_marker");
+        out_ptr(ms, "ms");
+        FF_HIP(hipEventSynchronize(reinterpret_cast<hipEvent_t>(stop)));
+        FF_HIP(hipEventElapsedTime(ms, reinterpret_cast<hipEvent_t>(start), reinterpret_cast<hipEvent_t>(stop)));
+    });
+}
+ff_status ff_marker_destroy(ff_marker* m) {
+    return guard([&] {
+        if (m) FF_HIP(hipEventDestroy(reinterpret_cast<hipEvent_t>(m)));
+    });
 }
 
 }  // extern "C"

@@ -397,11 +397,13 @@ void PyramidalFlowFilter::configure() {
     input_.clear();
     for (int l = 0; l < levels_; ++l) pyramid_.getImage(l).clear();
     setFused(fused_);
+    graphs_.clear();
     configured_ = true;
 }
 
 void PyramidalFlowFilter::setFused(bool f) {
     fused_ = f;
+    graphs_.clear();
     top_.setFused(f);
     for (auto& d : low_) d.setFused(f);
 }
@@ -427,21 +429,104 @@ void PyramidalFlowFilter::computeStaged() {
     for (int l = 0; l < levels_ - 1; ++l) low_[l].runUpdate();
 }
 
+// ---------------------------------------------------------------- GraphCache
+
+namespace {
+bool same_img(const DevImg& a, const DevImg& b) {
+    return a.data == b.data && a.h == b.h && a.w == b.w && a.pitch == b.pitch;
+}
+bool same_args(const LevelArgs& a, const LevelArgs& b) {
+    return same_img(a.img, b.img) && a.img_u8 == b.img_u8 && same_img(a.flow_s_in, b.flow_s_in) &&
+           same_img(a.a0_in, b.a0_in) && same_img(a.dflow_in, b.dflow_in) && same_img(a.flow_s_out, b.flow_s_out) &&
+           same_img(a.a0_out, b.a0_out) && same_img(a.dflow_out, b.dflow_out) &&
+           same_img(a.flow_upd_out, b.flow_upd_out) && same_img(a.upper, b.upper) && a.xmap == b.xmap &&
+           a.ymap == b.ymap && a.gamma == b.gamma && a.maxflow == b.maxflow && a.dt == b.dt &&
+           a.prop_iters == b.prop_iters && a.smooth_iters == b.smooth_iters && a.border == b.border &&
+           a.delta == b.delta;
+}
+}  // namespace
+
+GraphCache& GraphCache::operator=(GraphCache&& o) noexcept {
+    if (this != &o) {
+        clear();
+        for (int i = 0; i < 2; ++i) {
+            slot_[i] = std::move(o.slot_[i]);
+            o.slot_[i].exec = nullptr;
+            o.slot_[i].key.clear();
+        }
+        next_ = o.next_;
+    }
+    return *this;
+}
+
+hipGraphExec_t GraphCache::find(const std::vector<LevelArgs>& key) const {
+    for (const Entry& e : slot_) {
+        if (!e.exec || e.key.size() != key.size()) continue;
+        bool eq = true;
+        for (size_t i = 0; i < key.size() && eq; ++i) eq = same_args(e.key[i], key[i]);
+        if (eq) return e.exec;
+    }
+    return nullptr;
+}
+
+void GraphCache::insert(const std::vector<LevelArgs>& key, hipGraphExec_t exec) {
+    Entry& e = slot_[next_];
+    if (e.exec) (void)hipGraphExecDestroy(e.exec);
+    e.exec = exec;
+    e.key = key;
+    next_ ^= 1;
+}
+
+void GraphCache::clear() {
+    for (Entry& e : slot_) {
+        if (e.exec) (void)hipGraphExecDestroy(e.exec);
+        e.exec = nullptr;
+        e.key.clear();
+    }
+}
+
 // Same frame with one fused kernel per level: the top level first, then
 // levels 0 .. L-2 in ascending order. Level l reads level l+1's smoothed
 // flow before level l+1 has advanced (the reference's update order) and the
-// top level's unsmoothed flow of this frame.
-void PyramidalFlowFilter::computeFused() {
+// top level's unsmoothed flow of this frame; every level writes its new
+// state to its spare buffers, exchanged with the state after the launches.
+void PyramidalFlowFilter::enqueueFused(const std::vector<LevelArgs>& args, hipStream_t s) {
     pyramid_.run();
-    hipStream_t s = stream();
-    launch_fused_level(top_.fusedArgs(), s);
-    check_launch("PyramidalFlowFilter::compute (fused top)");
-    top_.fusedCommit();
-    for (int l = 0; l < levels_ - 1; ++l) {
-        launch_fused_level(low_[l].fusedArgs(), s);
+    for (const LevelArgs& a : args) {
+        launch_fused_level(a, s);
         check_launch("PyramidalFlowFilter::compute (fused level)");
-        low_[l].fusedCommit();
     }
+}
+
+void PyramidalFlowFilter::computeFused() {
+    std::vector<LevelArgs> args;
+    args.push_back(top_.fusedArgs());
+    for (int l = 0; l < levels_ - 1; ++l) args.push_back(low_[l].fusedArgs());
+    hipStream_t s = stream();
+    if (graphs_on_ && !profiling_active()) {
+        hipGraphExec_t exec = graphs_.find(args);
+        if (!exec) {
+            FF_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            hipGraph_t graph = nullptr;
+            try {
+                enqueueFused(args, s);
+            } catch (...) {
+                (void)hipStreamEndCapture(s, &graph);
+                if (graph) (void)hipGraphDestroy(graph);
+                throw;
+            }
+            FF_HIP(hipStreamEndCapture(s, &graph));
+            const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (e != hipSuccess) throw_hip(e, "hipGraphInstantiate", __FILE__, __LINE__);
+            graphs_.insert(args, exec);
+        }
+        FF_HIP(hipGraphLaunch(exec, s));
+    } else {
+        enqueueFused(args, s);
+    }
+    top_.fusedCommit();
+    for (int l = 0; l < levels_ - 1; ++l) low_[l].fusedCommit();
 }
 
 void PyramidalFlowFilter::compute() {

@@ -117,6 +117,31 @@ private:
     Image spareFlow_, spareImage_, spareDelta_;
 };
 
+// Instantiated HIP graphs of one frame's fused launch sequence, keyed by the
+// exact kernel arguments. The level state alternates between two buffers,
+// so a steady run needs two graphs (even and odd frames); any parameter or
+// buffer change produces a new key and a new capture.
+class GraphCache {
+public:
+    GraphCache() = default;
+    GraphCache(const GraphCache&) = delete;
+    GraphCache& operator=(const GraphCache&) = delete;
+    GraphCache(GraphCache&& o) noexcept { *this = std::move(o); }
+    GraphCache& operator=(GraphCache&& o) noexcept;
+    ~GraphCache() { clear(); }
+    hipGraphExec_t find(const std::vector<LevelArgs>& key) const;
+    void insert(const std::vector<LevelArgs>& key, hipGraphExec_t exec);
+    void clear();
+
+private:
+    struct Entry {
+        std::vector<LevelArgs> key;
+        hipGraphExec_t exec = nullptr;
+    };
+    Entry slot_[2];
+    int next_ = 0;
+};
+
 class PyramidalFlowFilter : public Stage {
 public:
     PyramidalFlowFilter() = default;
@@ -144,12 +169,20 @@ public:
     int levels() const { return levels_; }
     void setFused(bool f);
     bool fused() const { return fused_; }
+    // Replay the fused frame as a captured HIP graph (default on).
+    void setGraphs(bool g) {
+        graphs_on_ = g;
+        graphs_.clear();
+    }
+    bool graphs() const { return graphs_on_; }
 
 private:
     void checkLevel(int level, const char* who) const;
     void computeStaged();
     void computeFused();
-    bool configured_ = false, fused_ = true;
+    void enqueueFused(const std::vector<LevelArgs>& args, hipStream_t s);
+    bool configured_ = false, fused_ = true, graphs_on_ = true;
+    GraphCache graphs_;
     int h_ = 0, w_ = 0, levels_ = 0;
     Image input_;
     ImagePyramid pyramid_;

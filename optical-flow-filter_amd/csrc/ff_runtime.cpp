@@ -211,6 +211,12 @@ void profile_end(double* total_ms, long long* count) {
     p.used = 0;
 }
 
+bool profiling_active() {
+    Profiler& p = prof();
+    std::lock_guard<std::mutex> lk(p.mu);
+    return p.on;
+}
+
 ProfileScope::ProfileScope(const char* kernel, hipStream_t s) : s_(s) {
     Profiler& p = prof();
     std::lock_guard<std::mutex> lk(p.mu);

@@ -132,6 +132,7 @@ private:
 // duration and the launch count.
 void profile_begin(const std::string& filter);
 void profile_end(double* total_ms, long long* count);
+bool profiling_active();
 
 class ProfileScope {
 public:

@@ -190,9 +190,15 @@ SIGNATURES = {
     # execution controls
     "ff_filter_set_fused": [_S, _i],
     "ff_filter_get_fused": [_S, _pi],
+    "ff_filter_set_graphs": [_S, _i],
+    "ff_filter_get_graphs": [_S, _pi],
     # profiling
     "ff_profile_begin": [ctypes.c_char_p],
     "ff_profile_end": [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_longlong)],
+    # stream markers
+    "ff_marker_record": [_pv],
+    "ff_marker_elapsed": [_v, _v, _pf],
+    "ff_marker_destroy": [_v],
 }
 
 
@@ -249,6 +255,28 @@ def set_stream(stream_ptr):
 
 def get_stream():
     return get_handle("ff_get_stream")
+
+
+class Marker(object):
+    """A timing event recorded on the library stream at construction."""
+
+    def __init__(self):
+        self._h = get_handle("ff_marker_record")
+
+    def elapsed_ms(self, stop):
+        """Milliseconds from this marker to ``stop`` (waits for ``stop``)."""
+        v = ctypes.c_float()
+        call("ff_marker_elapsed", ctypes.c_void_p(self._h), ctypes.c_void_p(stop._h), ctypes.byref(v))
+        return v.value
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            try:
+                lib.ff_marker_destroy(ctypes.c_void_p(h))
+            except Exception:
+                pass
+            self._h = None
 
 
 def profile_begin(kernel_filter):

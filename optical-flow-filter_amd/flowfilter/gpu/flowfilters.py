@@ -6,7 +6,7 @@ getFlowDevice and the gamma/maxflow/smoothIterations/propagationBorder
 properties. Also exposes DeltaFlowFilter (C++ only in the reference).
 
 Additions without a reference counterpart: ``loadImageDevice`` (input from
-device memory, e.g. a torch tensor's data_ptr) and the ``fused`` property
+device memory, e.g. another GPUImage's data pointer) and the ``fused`` property
 selecting fused per-level kernels (default) or one launch per stage kernel.
 """
 import ctypes
@@ -292,3 +292,12 @@ class PyramidalFlowFilter(Stage):
     @fused.setter
     def fused(self, v):
         N.call("ff_filter_set_fused", self.handle, int(bool(v)))
+
+    @property
+    def graphs(self):
+        """Replay each fused frame as a captured HIP graph (default True)."""
+        return bool(self._geti("ff_filter_get_graphs"))
+
+    @graphs.setter
+    def graphs(self, v):
+        N.call("ff_filter_set_graphs", self.handle, int(bool(v)))

@@ -20,14 +20,6 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an AMD GPU (MI355X) and the built HIP library")
 
 
-def gpu_available():
-    try:
-        import torch
-        return torch.cuda.is_available()
-    except Exception:
-        return False
-
-
 @pytest.fixture(scope="session")
 def ffo():
     import ffo as mod

@@ -49,7 +49,7 @@ def test_flowfilter_sequence(ffo, shape, frames, fused):
     g = gf.FlowFilter(h, w, 2, 4.0, 10.0)
     g.fused = fused
     assert (g.height, g.width, g.smoothIterations, g.propagationIterations) == (h, w, 2, 4)
-    assert g.maxflow == 4.0 and abs(g.gamma - 10 / 65025.0) < 1e-12
+    assert g.maxflow == 4.0 and g.gamma == np.float32(np.float32(10) / (np.float32(255) * np.float32(255)))
     o = ffo.FlowFilter(h, w, 2, 4.0, 10.0)
     o64 = ffo.FlowFilter(h, w, 2, 4.0, 10.0, prec="f64")
     worst64 = 0.0
@@ -103,17 +103,24 @@ def test_flowfilter_float_input(ffo):
         check_frame(t, g.getFlow(), o.downloadFlow(), "float input")
 
 
+def device_frames(frames):
+    """All frames in one tall device image; returns (image, base pointer, pitch)."""
+    from flowfilter.gpu.image import GPUImage
+    f = np.stack(frames)
+    dev = GPUImage((f.shape[0] * f.shape[1], f.shape[2]), 1)
+    dev.upload(f.reshape(-1, f.shape[2]))
+    return dev, dev.data, dev.pitch
+
+
 def test_flowfilter_device_input(ffo):
-    import torch
     import flowfilter.gpu.flowfilters as gf
-    h, w = 64, 100
+    h, w = 200, 100
     g = gf.FlowFilter(h, w, 2, 4.0, 10.0)
     o = ffo.FlowFilter(h, w, 2, 4.0, 10.0)
     frames = seq_noise(h, w, 6)
-    dev = torch.from_numpy(np.stack(frames)).cuda()
-    torch.cuda.synchronize()
+    dev, base, pitch = device_frames(frames)
     for t, img in enumerate(frames):
-        g.loadImageDevice(dev[t].data_ptr(), dev.stride(1))
+        g.loadImageDevice(base + t * h * pitch, pitch)
         g.compute()
         o.compute(img)
         check_frame(t, g.getFlow(), o.downloadFlow(), "device input")
@@ -220,15 +227,17 @@ def test_pyramidal_parameters_and_errors(ffo):
         gf.PyramidalFlowFilter(64, 96, 0)
 
 
-def test_pyramidal_device_input_and_timing(ffo):
-    import torch
-    h, w = 136, 256
+@pytest.mark.parametrize("graphs", [True, False])
+def test_pyramidal_device_input_and_timing(ffo, graphs):
+    """Device-resident frames (the benchmark's input path), graph replay on and off."""
+    h, w = 272, 320
     g, o = _pyr_pair(ffo, h, w, 2, [10.0, 50.0], 4.0, [2, 2])
-    frames = seq_blobs(h, w, 6)
-    dev = torch.from_numpy(np.stack(frames)).cuda()
-    torch.cuda.synchronize()
+    g.graphs = graphs
+    assert g.graphs == graphs
+    frames = seq_blobs(h, w, 9)
+    dev, base, pitch = device_frames(frames)
     for t, img in enumerate(frames):
-        g.loadImageDevice(dev[t].data_ptr(), dev.stride(1))
+        g.loadImageDevice(base + t * h * pitch, pitch)
         g.compute()
         o.compute(img)
         check_frame(t, g.getFlow(), o.downloadFlow(), "pyramidal device input")

@@ -85,18 +85,21 @@ def test_gpuimage_copy_clear_share():
 
 
 def test_gpuimage_device_copies():
-    import torch
+    """Device-to-device upload/download with the source/destination pitch honoured."""
     from flowfilter import _native as N
     from flowfilter.gpu.image import GPUImage
     a = rng(4).integers(0, 255, (10, 70), dtype=np.uint8)
-    t = torch.from_numpy(a).cuda()
-    torch.cuda.synchronize()
+    src = gimg(a)
     g = GPUImage((10, 70), 1)
-    g.uploadDevice(t.data_ptr(), t.stride(0))
-    out = torch.zeros_like(t)
-    g.downloadDevice(out.data_ptr(), out.stride(0))
+    g.uploadDevice(src.data, src.pitch)
+    out = GPUImage((10, 70), 1)
+    g.downloadDevice(out.data, out.pitch)
     N.synchronize()
-    assert np.array_equal(out.cpu().numpy(), a)
+    assert np.array_equal(out.download(np.uint8), a)
+    # a window of a wider device image (pitch larger than the row)
+    wide = gimg(rng(5).integers(0, 255, (10, 300), dtype=np.uint8))
+    g.uploadDevice(wide.data + 17, wide.pitch)
+    assert np.array_equal(g.download(np.uint8), wide.download(np.uint8)[:, 17:87])
 
 
 # ---------------------------------------------------------------- ImageModel
