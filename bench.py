@@ -61,11 +61,31 @@ def cpu_baseline(frames, budget_s=12.0, threads=None):
         o.compute(frames[n % len(frames)])
         n += 1
         el = time.perf_counter() - t0
-        if (el >= budget_s and n >= 2) or n >= 64:
+        if (el >= budget_s and n >= 2) or n >= 2000:
             break
     return {"value": round(H * W * n / el / 1e6, 3), "unit": "Mpix/s", "cores": threads, "kind": "port",
             "sample": "%d frames of the same 2048x1088 2-level workload, FP32 CPU oracle "
                       "(oracle/ff_oracle.hpp, OpenMP over rows), %.1f s" % (n, el)}
+
+
+def copy_bench(args, N, GPUImage):
+    """GPUImage.copyFrom of one float2 flow image, K times: a kernel whose HBM
+    traffic is known exactly (H*W*8 bytes read, the same written)."""
+    import numpy as np
+    a, b = GPUImage((H, W, 2), 4), GPUImage((H, W, 2), 4)
+    a.upload(np.random.default_rng(0).random((H, W, 2), dtype=np.float32))
+    for _ in range(args.warmup):
+        b.copyFrom(a)
+    N.synchronize()
+    m0 = N.Marker()
+    for _ in range(args.steps):
+        b.copyFrom(a)
+    m1 = N.Marker()
+    ms = m0.elapsed_ms(m1) / args.steps
+    nbytes = 2 * H * W * 8
+    print(json.dumps({"kernel": "k_copy2d_u32", "bytes_read": H * W * 8, "bytes_written": H * W * 8,
+                      "ms_per_launch": round(ms, 5), "GB/s": round(nbytes / (ms / 1e3) / 1e9, 2),
+                      "launches": args.steps}), flush=True)
 
 
 def main():
@@ -78,6 +98,9 @@ def main():
     ap.add_argument("--no-graphs", action="store_true", help="launch the fused kernels one by one")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--copy", action="store_true",
+                    help="instead of the filter, time a plain device copy of one 2048x1088 flow image "
+                         "(17.8 MB read + 17.8 MB written per launch; the reference kernel for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this path has no multi-GPU form "
@@ -88,6 +111,10 @@ def main():
     import flowfilter.gpu.flowfilters as gf
     from flowfilter import _native as N, synthetic
     from flowfilter.gpu.image import GPUImage
+
+    if args.copy:
+        copy_bench(args, N, GPUImage)
+        return
 
     t0 = time.perf_counter()
     frames = np.stack(list(synthetic.blob_affine(H, W, args.frames)))

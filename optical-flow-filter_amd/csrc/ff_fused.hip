@@ -14,17 +14,24 @@
 // is the buffer minus a halo h0 = max(N,2) + 2K on every side (the distance
 // an error at the buffer edge travels inward: one pixel per propagation
 // pass, two per smoothing pass, two for the image model's x taps).
-//  - X neighbours come from DPP wave shifts (a lane at the buffer edge reads
-//    itself; the resulting error stays inside the halo).
-//  - Y neighbours are registers of the same thread; the rows above and below
-//    a wave's strip are exchanged through LDS once per Y pass.
+//  - X neighbours come from DPP wave shifts; Y neighbours are registers of
+//    the same thread, and the rows above and below a wave's strip are
+//    exchanged through LDS once per Y pass.
 //  - Every buffer lies inside the image: buffers at the image edges are
-//    clamped against the edge and their output range extends to it. A lane
-//    without a DPP source keeps its own value and the first/last wave of a
-//    workgroup uses its own first/last row for the missing neighbour rows,
-//    which is exactly the clamp-to-edge read of the reference at an image
-//    edge (and a bounded error inside the halo elsewhere). The fused path
-//    therefore needs W >= 64 and H >= NW*R.
+//    clamped against the edge and their output range extends to it. At an
+//    image edge the missing neighbour must be the clamp-to-edge read of the
+//    reference: the first/last wave use their own first/last row, and the
+//    x taps of the image model and smoothing use DPP shifts in which the
+//    edge lane keeps its own value. The propagation passes use zero-filling
+//    shifts (a single instruction the compiler folds into the arithmetic):
+//    with border >= 1 the pixels next to an image edge keep their value, so
+//    the filled lane only affects pixels inside the halo.
+//  - All global accesses are buffer instructions with 32-bit offsets; stores
+//    of pixels outside the tile's output range use an offset beyond the
+//    buffer's size, which the hardware discards (no branches).
+//  - Image rows, the upper-level flow and the old brightness are loaded
+//    before they are needed so their latency overlaps computation.
+//  The fused path therefore needs W >= 64, H >= NW*R and border >= 1.
 //
 // Arithmetic per pixel is the reference's (same operations, order and
 // rounding points as the stage kernels and the CPU oracle); results are
@@ -40,6 +47,7 @@ namespace ffamd {
 namespace {
 
 constexpr int LANES = 64;
+constexpr int OOB = (int)0x80000000;  // buffer offset beyond any image: access dropped
 
 struct FusedParams {
     const void* img;
@@ -56,43 +64,73 @@ struct FusedParams {
     // row pitches in bytes
     int img_pitch, flow_in_pitch, a0_in_pitch, dflow_in_pitch, flow_out_pitch, a0_out_pitch, dflow_out_pitch,
         flow_upd_pitch, upper_pitch;
-    int W, H, N, K, border, h0, TW, TH, tiles_x, tiles_y;
+    int W, H, Hu, N, K, border, h0, TW, TH, tiles_x, tiles_y;
     float dt, gamma, maxflow;
 };
 
-template <class T>
-__device__ __forceinline__ T ld(const T* base, int pitch, int y, int x) {
-    return reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + (size_t)y * pitch)[x];
+// ------------------------------------------------------------ buffer access
+
+typedef __amdgpu_buffer_rsrc_t Rsrc;
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ Rsrc rsrc(const void* p, int bytes) {
+    // word 3 of the buffer descriptor for gfx9-family raw (untyped) buffers
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, bytes, 0x00020000);
 }
-template <class T>
-__device__ __forceinline__ void st(T* base, int pitch, int y, int x, T v) {
-    reinterpret_cast<T*>(reinterpret_cast<char*>(base) + (size_t)y * pitch)[x] = v;
+__device__ __forceinline__ float ld1(Rsrc r, int off) {
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+__device__ __forceinline__ float2 ld2(Rsrc r, int off) {
+    const u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
+    return make_float2(__uint_as_float(v.x), __uint_as_float(v.y));
+}
+__device__ __forceinline__ unsigned ldb(Rsrc r, int off) { return __builtin_amdgcn_raw_buffer_load_b8(r, off, 0, 0); }
+__device__ __forceinline__ void st1(Rsrc r, int off, float v) {
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, off, 0, 0);
+}
+__device__ __forceinline__ void st2(Rsrc r, int off, float a, float b) {
+    const u32x2 v = {__float_as_uint(a), __float_as_uint(b)};
+    __builtin_amdgcn_raw_buffer_store_b64(v, r, off, 0, 0);
 }
 
-// DPP wave shifts: lane i receives lane i-1 (shr) or lane i+1 (shl); the lane
-// without a source keeps its own value.
-__device__ __forceinline__ float wshr(float v) {
+// ---------------------------------------------------------- lane shifts
+// Lane i receives lane i-1 (shr) or lane i+1 (shl). The zero-filling forms
+// give 0 to the lane without a source; the clamping forms let it keep its
+// own value (the clamp-to-edge read at an image edge).
+__device__ __forceinline__ float zshr(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xf, 0xf, true));
+}
+__device__ __forceinline__ float zshl(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x130, 0xf, 0xf, true));
+}
+__device__ __forceinline__ float cshr(float v) {
     return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), 0x138, 0xf, 0xf, false));
 }
-__device__ __forceinline__ float wshl(float v) {
+__device__ __forceinline__ float cshl(float v) {
     return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), 0x130, 0xf, 0xf, false));
 }
-
-__device__ __forceinline__ float rdlane(float v, int lane) {
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+template <bool CLAMP>
+__device__ __forceinline__ float shr(float v) {
+    if constexpr (CLAMP) return cshr(v);
+    else return zshr(v);
+}
+template <bool CLAMP>
+__device__ __forceinline__ float shl(float v) {
+    if constexpr (CLAMP) return cshl(v);
+    else return zshl(v);
 }
 
 // 5-tap correlation along x in the order of the reference loops (c = -2..2):
 // acc = 0; acc = fma(w[c], v(x+c), acc).
-template <class Wf>
+template <bool CLAMP, class Wf>
 __device__ __forceinline__ float taps_x(float v, Wf w) {
-    const float m1 = wshr(v), m2 = wshr(m1), p1 = wshl(v), p2 = wshl(p1);
+    const float m1 = shr<CLAMP>(v), p1 = shl<CLAMP>(v);
     float a = 0.0f;
-    a = fmaf(w(0), m2, a);
+    a = fmaf(w(0), shr<CLAMP>(m1), a);
     a = fmaf(w(1), m1, a);
     a = fmaf(w(2), v, a);
     a = fmaf(w(3), p1, a);
-    a = fmaf(w(4), p2, a);
+    a = fmaf(w(4), shl<CLAMP>(p1), a);
     return a;
 }
 
@@ -116,7 +154,7 @@ __device__ __forceinline__ int xcd_tile(int b, int nt) {
 
 // Per-thread view of the tile.
 struct Geo {
-    int lane, wave, x, xc, yw, h0;
+    int lane, wave, x, yw;
     bool xin;      // column strictly inside the propagation border
     bool wave_in;  // every pixel of the wave strictly inside the border
 };
@@ -136,17 +174,17 @@ __device__ __forceinline__ void prop_x(State<DELTA, R>& s, const Geo& g, const F
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const float zu = s.fu[r], zv = s.fv[r];
-        const float mu = wshr(zu), pu = wshl(zu);
+        const float mu = zshr(zu), pu = zshl(zu);
         const float Ud = (fabsf(pu) - fabsf(mu)) > 0.0f ? pu : mu;
         const float nd = -(p.dt * Ud);
         const bool back = Ud >= 0.0f;
         float ou = upwind(mu, zu, pu, back, nd);
-        float ov = upwind(wshr(zv), zv, wshl(zv), back, nd);
+        float ov = upwind(zshr(zv), zv, zshl(zv), back, nd);
         float os = 0.f, odu = 0.f, odv = 0.f;
         if constexpr (DELTA) {
-            os = upwind(wshr(s.sa[r]), s.sa[r], wshl(s.sa[r]), back, nd);
-            odu = upwind(wshr(s.du[r]), s.du[r], wshl(s.du[r]), back, nd);
-            odv = upwind(wshr(s.dv[r]), s.dv[r], wshl(s.dv[r]), back, nd);
+            os = upwind(zshr(s.sa[r]), s.sa[r], zshl(s.sa[r]), back, nd);
+            odu = upwind(zshr(s.du[r]), s.du[r], zshl(s.du[r]), back, nd);
+            odv = upwind(zshr(s.dv[r]), s.dv[r], zshl(s.dv[r]), back, nd);
         }
         if constexpr (BORDER) {
             const int y = g.yw + r;
@@ -225,6 +263,71 @@ __device__ __forceinline__ void prop_y(State<DELTA, R>& s, const Geo& g, const F
     }
 }
 
+// Image model (imagePrefilter_k / imageModel_k, imagemodel_k.cu:31-133) and
+// update (flowUpdate_k, update_k.cu:15-79, or deltaFlowUpdate_k, :82-173)
+// of every row of the thread. im[j] is image row yw-2+j (clamped reads),
+// aux[r] the old brightness (top level) or upper-level flow (delta level)
+// of row r. Writes the new brightness (and delta flow) of the tile's output
+// pixels; leaves the updated flow in s.fu/s.fv.
+template <bool DELTA, bool CLAMP, int R>
+__device__ __forceinline__ void model_update(State<DELTA, R>& s, const float (&im)[R + 4], const float (&aold)[R],
+                                             const float2 (&up)[R], const Geo& g, const FusedParams& p,
+                                             const Rsrc& ra0, const Rsrc& rdf, const Rsrc& rfu, const int (&offa)[R],
+                                             const int (&offf)[R]) {
+    float sx[R + 4];
+#pragma unroll
+    for (int j = 0; j < R + 4; ++j) sx[j] = taps_x<CLAMP>(im[j], WSmooth{});
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        float sy = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 5; ++k) sy = fmaf(smooth_w(k), im[r + k], sy);
+        const float a0 = taps_x<CLAMP>(sy, WSmooth{});
+        const float gx = taps_x<CLAMP>(sy, WDiff{});
+        float gy = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 5; ++k) gy = fmaf(diff_w(k), sx[r + k], gy);
+        float ox, oy;
+        if constexpr (DELTA) {
+            const float Yt = s.sa[r] - a0;
+            solve2x2(gx, gy, Yt, s.du[r], s.dv[r], p.gamma, ox, oy);
+            ox = not_finite(ox) ? 0.0f : ox;
+            oy = not_finite(oy) ? 0.0f : oy;
+            const float half = 0.5f * p.maxflow;
+            ox = fmaxf(-half, fminf(ox, half));
+            oy = fmaxf(-half, fminf(oy, half));
+            float fx = ox + 2.0f * up[r].x, fy = oy + 2.0f * up[r].y;
+            fx = fmaxf(-p.maxflow, fminf(fx, p.maxflow));
+            fy = fmaxf(-p.maxflow, fminf(fy, p.maxflow));
+            st1(ra0, offa[r], a0);
+            st2(rdf, offf[r], ox, oy);
+            ox = fx;
+            oy = fy;
+        } else {
+            const float Yt = aold[r] - a0;
+            solve2x2(gx, gy, Yt, s.fu[r], s.fv[r], p.gamma, ox, oy);
+            ox = fmaxf(-p.maxflow, fminf(ox, p.maxflow));
+            oy = fmaxf(-p.maxflow, fminf(oy, p.maxflow));
+            ox = not_finite(ox) ? 0.0f : ox;
+            oy = not_finite(oy) ? 0.0f : oy;
+            st1(ra0, offa[r], a0);
+            if (p.flow_upd) st2(rfu, offf[r], ox, oy);
+        }
+        s.fu[r] = ox;
+        s.fv[r] = oy;
+    }
+}
+
+// X pass of the box smoother (flowSmoothX_k, flowsmoothing_k.cu:21-53).
+template <bool CLAMP, int R>
+__device__ __forceinline__ void smooth_x(float (&fu)[R], float (&fv)[R]) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        fu[r] = taps_x<CLAMP>(fu[r], WBox{});
+        fv[r] = taps_x<CLAMP>(fv[r], WBox{});
+    }
+}
+
 template <bool DELTA, bool U8, int R, int NW>
 __global__ void __launch_bounds__(NW* LANES) k_fused_level(FusedParams p) {
     // row exchange between the waves of the workgroup, double buffered
@@ -240,32 +343,44 @@ __global__ void __launch_bounds__(NW* LANES) k_fused_level(FusedParams p) {
     const int oy0 = tyi * p.TH, oy1 = min(oy0 + p.TH, p.H);
     const int x0 = clampi(ox0 - p.h0, 0, p.W - LANES);
     const int y0 = clampi(oy0 - p.h0, 0, p.H - NW * R);
-    g.h0 = p.h0;
     g.x = x0 + g.lane;
-    g.xc = g.x;
     g.yw = y0 + g.wave * R;  // image row of this thread's first row (wave-uniform)
     const int b = p.border;
     g.xin = g.x >= b && g.x < p.W - b;
     g.wave_in = (x0 >= b) && (x0 + LANES - 1 < p.W - b) && (g.yw >= b) && (g.yw + R - 1 < p.H - b);
+    const bool xedge = (x0 == 0) || (x0 + LANES == p.W);
     const int lane = g.lane, wave = g.wave;
     const bool out_x = g.x >= ox0 && g.x < ox1;
-    auto out_row = [&](int r) { return g.yw + r >= oy0 && g.yw + r < oy1; };
+
+    const Rsrc rimg = rsrc(p.img, p.img_pitch * p.H);
+    const Rsrc rfi = rsrc(p.flow_in, p.flow_in_pitch * p.H);
+    const Rsrc rai = rsrc(p.a0_in, p.a0_in_pitch * p.H);
+    const Rsrc rfo = rsrc(p.flow_out, p.flow_out_pitch * p.H);
+    const Rsrc rao = rsrc(p.a0_out, p.a0_out_pitch * p.H);
 
     State<DELTA, R> s;
 
-    // ------------------------------------------------------ load the state
+    // ------------------------------------- load the state and the image rows
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-        const int yc = g.yw + r;
-        const float2 f = ld(p.flow_in, p.flow_in_pitch, yc, g.xc);
+        const int y = g.yw + r;
+        const float2 f = ld2(rfi, y * p.flow_in_pitch + g.x * 8);
         s.fu[r] = f.x;
         s.fv[r] = f.y;
         if constexpr (DELTA) {
-            s.sa[r] = ld(p.a0_in, p.a0_in_pitch, yc, g.xc);
-            const float2 d = ld(p.dflow_in, p.dflow_in_pitch, yc, g.xc);
+            s.sa[r] = ld1(rai, y * p.a0_in_pitch + g.x * 4);
+            const float2 d = ld2(rsrc(p.dflow_in, p.dflow_in_pitch * p.H), y * p.dflow_in_pitch + g.x * 8);
             s.du[r] = d.x;
             s.dv[r] = d.y;
         }
+    }
+    float im[R + 4];  // image rows yw-2 .. yw+R+1, clamped
+    unsigned imb[U8 ? R + 4 : 1];
+#pragma unroll
+    for (int j = 0; j < R + 4; ++j) {
+        const int yc = clampi(g.yw - 2 + j, 0, p.H - 1);
+        if constexpr (U8) imb[j] = ldb(rimg, yc * p.img_pitch + g.x);
+        else im[j] = ld1(rimg, yc * p.img_pitch + g.x * 4);
     }
 
     int slot = 0;
@@ -327,85 +442,40 @@ __global__ void __launch_bounds__(NW* LANES) k_fused_level(FusedParams p) {
     }
 
     // ------------------------------------------------ image model + update
-    // imagePrefilter_k / imageModel_k (imagemodel_k.cu:31-133), then
-    // flowUpdate_k (update_k.cu:15-79) or deltaFlowUpdate_k (:82-173), one
-    // row at a time with a 5-row sliding window of image rows (w*) and of
-    // their X-smoothed values (x*), read with clamped coordinates.
-    auto img_row = [&](int j) -> float {  // image row yw - 2 + j
-        const int yc = clampi(g.yw - 2 + j, 0, p.H - 1);
-        if constexpr (U8) return unorm8(ld(reinterpret_cast<const uint8_t*>(p.img), p.img_pitch, yc, g.xc));
-        else return ld(reinterpret_cast<const float*>(p.img), p.img_pitch, yc, g.xc);
-    };
-    float w0 = img_row(0), w1 = img_row(1), w2 = img_row(2), w3 = img_row(3);
-    float x0s = taps_x(w0, WSmooth{}), x1s = taps_x(w1, WSmooth{}), x2s = taps_x(w2, WSmooth{}),
-          x3s = taps_x(w3, WSmooth{});
-    const int xmx = DELTA ? p.xmap[g.xc] : 0;
+    float aold[R];  // unused (and removed) on delta levels
+    float2 up[R];
+    int offa[R], offf[R];
+    const Rsrc rdf = rsrc(p.dflow_out, p.dflow_out_pitch * p.H);
+    const Rsrc rfu = rsrc(p.flow_upd, p.flow_upd_pitch * p.H);
+    const int xmx = DELTA ? p.xmap[g.x] : 0;
+    const Rsrc rup = rsrc(p.upper, p.upper_pitch * p.Hu);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-        const float w4 = img_row(r + 4);
-        const float x4s = taps_x(w4, WSmooth{});
         const int y = g.yw + r;
-        const int yc = y;
-        float sy = 0.0f;
-        sy = fmaf(smooth_w(0), w0, sy);
-        sy = fmaf(smooth_w(1), w1, sy);
-        sy = fmaf(smooth_w(2), w2, sy);
-        sy = fmaf(smooth_w(3), w3, sy);
-        sy = fmaf(smooth_w(4), w4, sy);
-        const float a0 = taps_x(sy, WSmooth{});
-        const float gx = taps_x(sy, WDiff{});
-        float gy = 0.0f;
-        gy = fmaf(diff_w(0), x0s, gy);
-        gy = fmaf(diff_w(1), x1s, gy);
-        gy = fmaf(diff_w(2), x2s, gy);
-        gy = fmaf(diff_w(3), x3s, gy);
-        gy = fmaf(diff_w(4), x4s, gy);
-        w0 = w1; w1 = w2; w2 = w3; w3 = w4;
-        x0s = x1s; x1s = x2s; x2s = x3s; x3s = x4s;
-        const bool out = out_x && out_row(r);
-        float ox, oy;
+        const bool out = out_x && y >= oy0 && y < oy1;
+        offa[r] = out ? y * p.a0_out_pitch + g.x * 4 : OOB;
         if constexpr (DELTA) {
-            const float Yt = s.sa[r] - a0;
-            solve2x2(gx, gy, Yt, s.du[r], s.dv[r], p.gamma, ox, oy);
-            ox = not_finite(ox) ? 0.0f : ox;
-            oy = not_finite(oy) ? 0.0f : oy;
-            const float half = 0.5f * p.maxflow;
-            ox = fmaxf(-half, fminf(ox, half));
-            oy = fmaxf(-half, fminf(oy, half));
-            const float2 up = ld(p.upper, p.upper_pitch, p.ymap[yc], xmx);
-            float fx = ox + 2.0f * up.x, fy = oy + 2.0f * up.y;
-            fx = fmaxf(-p.maxflow, fminf(fx, p.maxflow));
-            fy = fmaxf(-p.maxflow, fminf(fy, p.maxflow));
-            if (out) {
-                st(p.a0_out, p.a0_out_pitch, y, g.x, a0);
-                st(p.dflow_out, p.dflow_out_pitch, y, g.x, make_float2(ox, oy));
-            }
-            ox = fx;
-            oy = fy;
+            offf[r] = out ? y * p.dflow_out_pitch + g.x * 8 : OOB;
+            up[r] = ld2(rup, p.ymap[y] * p.upper_pitch + xmx * 8);
+            aold[r] = 0.f;
         } else {
-            const float Yt = ld(p.a0_in, p.a0_in_pitch, yc, g.xc) - a0;
-            solve2x2(gx, gy, Yt, s.fu[r], s.fv[r], p.gamma, ox, oy);
-            ox = fmaxf(-p.maxflow, fminf(ox, p.maxflow));
-            oy = fmaxf(-p.maxflow, fminf(oy, p.maxflow));
-            ox = not_finite(ox) ? 0.0f : ox;
-            oy = not_finite(oy) ? 0.0f : oy;
-            if (out) {
-                st(p.a0_out, p.a0_out_pitch, y, g.x, a0);
-                if (p.flow_upd) st(p.flow_upd, p.flow_upd_pitch, y, g.x, make_float2(ox, oy));
-            }
+            offf[r] = out ? y * p.flow_upd_pitch + g.x * 8 : OOB;
+            aold[r] = ld1(rai, y * p.a0_in_pitch + g.x * 4);
+            up[r] = make_float2(0.f, 0.f);
         }
-        s.fu[r] = ox;
-        s.fv[r] = oy;
     }
+    if constexpr (U8) {
+#pragma unroll
+        for (int j = 0; j < R + 4; ++j) im[j] = unorm8(imb[j]);
+    }
+    if (xedge) model_update<DELTA, true, R>(s, im, aold, up, g, p, rao, rdf, rfu, offa, offf);
+    else model_update<DELTA, false, R>(s, im, aold, up, g, p, rao, rdf, rfu, offa, offf);
 
     // ------------------------------------------------------------ smoothing
     // flowSmoothX_k / flowSmoothY_k (flowsmoothing_k.cu:21-87), K times.
     for (int it = 0; it < p.K; ++it) {
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            s.fu[r] = taps_x(s.fu[r], WBox{});
-            s.fv[r] = taps_x(s.fv[r], WBox{});
-        }
+        if (xedge) smooth_x<true, R>(s.fu, s.fv);
+        else smooth_x<false, R>(s.fu, s.fv);
         {
             float* my = &xch[slot][wave][0][0];
             my[0 * LANES + lane] = s.fu[0];
@@ -466,7 +536,9 @@ __global__ void __launch_bounds__(NW* LANES) k_fused_level(FusedParams p) {
     // ------------------------------------------------- write the new state
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-        if (out_x && out_row(r)) st(p.flow_out, p.flow_out_pitch, g.yw + r, g.x, make_float2(s.fu[r], s.fv[r]));
+        const int y = g.yw + r;
+        const bool out = out_x && y >= oy0 && y < oy1;
+        st2(rfo, out ? y * p.flow_out_pitch + g.x * 8 : OOB, s.fu[r], s.fv[r]);
     }
 }
 
@@ -499,10 +571,11 @@ bool fused_level_supported(const LevelArgs& a) {
         for (const DevImg* d : {&a.dflow_in, &a.dflow_out})
             if (d->h != H || d->w != W) return false;
     if (a.flow_upd_out.data && (a.flow_upd_out.h != H || a.flow_upd_out.w != W)) return false;
+    // every buffer must be addressable with 32-bit offsets
     const DevImg* all[] = {&a.img, &a.flow_s_in, &a.a0_in, &a.dflow_in, &a.flow_s_out, &a.a0_out, &a.dflow_out,
                            &a.flow_upd_out, &a.upper};
     for (const DevImg* d : all)
-        if (d->pitch > 0x7fffffff) return false;
+        if ((unsigned long long)d->pitch * (unsigned long long)(d->h > 0 ? d->h : 0) >= 0x7fffffffULL) return false;
     return true;
 }
 
@@ -531,6 +604,7 @@ void launch_fused_level(const LevelArgs& a, hipStream_t s) {
     p.ymap = a.ymap;
     p.W = a.img.w;
     p.H = a.img.h;
+    p.Hu = a.upper.h;
     p.N = a.prop_iters;
     p.K = a.smooth_iters;
     p.border = a.border;
