@@ -37,14 +37,16 @@ def algorithmic_bytes():
 
     Per pixel of its level: delta level 43 B (u8 image 1, flow 8, delta flow 8,
     old brightness 4, upper-level flow 8 B at 1/4 density = 2, new flow 8,
-    delta flow 8, brightness 4); top level 33 B (image 1, flow 8, brightness 4,
-    new smoothed flow 8, brightness 4, unsmoothed flow 8); one payload
-    propagation pass 40 B (read and write flow 8, scalar 4, vector 8)."""
+    delta flow 8, brightness 4); top level 32 B when it reduces the level-0
+    image while loading it (flow 8, brightness 4, new smoothed flow 8,
+    brightness 4, unsmoothed flow 8; its image bytes are the level-0 image
+    already counted at level 0). The fused frame kernel runs both levels:
+    43 B per level-0 pixel + 32 B per level-1 pixel. One payload propagation
+    pass of the staged path: 40 B (read and write flow 8, scalar 4, vector 8)."""
     n0 = H * W
     n1 = (H // 2) * (W // 2)
     return {
-        "fused_delta": 43 * n0,
-        "fused_top": 33 * n1,
+        "fused_frame": 43 * n0 + 32 * n1,
         "propagate_payload": 40 * n0,
     }
 
@@ -158,7 +160,7 @@ def main():
           flush=True)
 
     # roofline of the dominant kernel: second pass, HIP events per launch
-    dominant = "propagate_payload" if args.staged else "fused_delta"
+    dominant = "propagate_payload" if args.staged else "fused_frame"
     N.profile_begin(dominant)
     timed(args.warmup + args.steps)
     tot_ms, launches = N.profile_end()

@@ -286,6 +286,11 @@ FF_API ff_status ff_filter_get_fused(ff_stage* s, int* out);
  * (1, the default) or launch its kernels one by one (0). */
 FF_API ff_status ff_filter_set_graphs(ff_stage* s, int graphs);
 FF_API ff_status ff_filter_get_graphs(ff_stage* s, int* out);
+/* PyramidalFlowFilter only (diagnostic, no reference counterpart): number of
+ * times a level of a fused frame stopped waiting for the level above it
+ * after the spin limit instead of seeing it complete; any nonzero count
+ * means the flow of those frames is not valid. Waits for the device. */
+FF_API ff_status ff_filter_get_wait_timeouts(ff_stage* s, unsigned long long* out);
 
 /* Kernel profiling (no reference counterpart; the reference times whole
  * stages only, src/gpu/pipeline.cu:63-89). Between begin and end, every

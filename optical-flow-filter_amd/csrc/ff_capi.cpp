@@ -529,6 +529,13 @@ ff_status ff_filter_get_graphs(ff_stage* s, int* out) {
     });
 }
 
+ff_status ff_filter_get_wait_timeouts(ff_stage* s, unsigned long long* out) {
+    return guard([&] {
+        out_ptr(out, "timeouts");
+        *out = as<PyramidalFlowFilter>(s, "PyramidalFlowFilter").waitTimeouts();
+    });
+}
+
 // --------------------------------------------------------------- profiling
 ff_status ff_profile_begin(const char* filter) {
     return guard([&] { profile_begin(filter ? filter : ""); });

@@ -93,7 +93,7 @@ LevelArgs FlowFilter::fusedArgs() {
     ensureLike(spareImage_, A);
     LevelArgs a{};
     a.img = model_.boundImage().view();
-    a.img_u8 = model_.boundImage().itemSize() == 1;
+    a.img_mode = model_.boundImage().itemSize() == 1 ? IMG_U8 : IMG_F32;
     a.flow_s_in = S.view();
     a.a0_in = A.view();
     a.flow_s_out = spareFlow_.view();
@@ -123,7 +123,8 @@ void FlowFilter::compute() {
         throw logic_error("FlowFilter::compute(): input image shape changed since configure()");
     startTiming();
     if (fused_ && fusedSupported()) {
-        launch_fused_level(fusedArgs(), stream());
+        const int t = frame_.prepare({fusedArgs()});
+        frame_.launch(t, stream());
         check_launch("FlowFilter::compute (fused)");
         fusedCommit();
     } else {
@@ -266,7 +267,7 @@ LevelArgs DeltaFlowFilter::fusedArgs() {
     ensureLike(spareDelta_, D);
     LevelArgs a{};
     a.img = model_.boundImage().view();
-    a.img_u8 = model_.boundImage().itemSize() == 1;
+    a.img_mode = model_.boundImage().itemSize() == 1 ? IMG_U8 : IMG_F32;
     a.flow_s_in = S.view();
     a.a0_in = A.view();
     a.dflow_in = D.view();
@@ -302,7 +303,8 @@ void DeltaFlowFilter::compute() {
     requireConfigured("compute()");
     startTiming();
     if (fused_ && fusedSupported()) {
-        launch_fused_level(fusedArgs(), stream());
+        const int t = frame_.prepare({fusedArgs()});
+        frame_.launch(t, stream());
         check_launch("DeltaFlowFilter::compute (fused)");
         fusedCommit();
     } else {
@@ -436,7 +438,7 @@ bool same_img(const DevImg& a, const DevImg& b) {
     return a.data == b.data && a.h == b.h && a.w == b.w && a.pitch == b.pitch;
 }
 bool same_args(const LevelArgs& a, const LevelArgs& b) {
-    return same_img(a.img, b.img) && a.img_u8 == b.img_u8 && same_img(a.flow_s_in, b.flow_s_in) &&
+    return same_img(a.img, b.img) && a.img_mode == b.img_mode && a.wait_on == b.wait_on && same_img(a.flow_s_in, b.flow_s_in) &&
            same_img(a.a0_in, b.a0_in) && same_img(a.dflow_in, b.dflow_in) && same_img(a.flow_s_out, b.flow_s_out) &&
            same_img(a.a0_out, b.a0_out) && same_img(a.dflow_out, b.dflow_out) &&
            same_img(a.flow_upd_out, b.flow_upd_out) && same_img(a.upper, b.upper) && a.xmap == b.xmap &&
@@ -453,15 +455,16 @@ GraphCache& GraphCache::operator=(GraphCache&& o) noexcept {
             slot_[i] = std::move(o.slot_[i]);
             o.slot_[i].exec = nullptr;
             o.slot_[i].key.clear();
+            o.slot_[i].token = 0;
         }
         next_ = o.next_;
     }
     return *this;
 }
 
-hipGraphExec_t GraphCache::find(const std::vector<LevelArgs>& key) const {
+hipGraphExec_t GraphCache::find(const std::vector<LevelArgs>& key, unsigned long long token) const {
     for (const Entry& e : slot_) {
-        if (!e.exec || e.key.size() != key.size()) continue;
+        if (!e.exec || e.token != token || e.key.size() != key.size()) continue;
         bool eq = true;
         for (size_t i = 0; i < key.size() && eq; ++i) eq = same_args(e.key[i], key[i]);
         if (eq) return e.exec;
@@ -469,11 +472,12 @@ hipGraphExec_t GraphCache::find(const std::vector<LevelArgs>& key) const {
     return nullptr;
 }
 
-void GraphCache::insert(const std::vector<LevelArgs>& key, hipGraphExec_t exec) {
+void GraphCache::insert(const std::vector<LevelArgs>& key, unsigned long long token, hipGraphExec_t exec) {
     Entry& e = slot_[next_];
     if (e.exec) (void)hipGraphExecDestroy(e.exec);
     e.exec = exec;
     e.key = key;
+    e.token = token;
     next_ ^= 1;
 }
 
@@ -482,34 +486,55 @@ void GraphCache::clear() {
         if (e.exec) (void)hipGraphExecDestroy(e.exec);
         e.exec = nullptr;
         e.key.clear();
+        e.token = 0;
     }
 }
 
-// Same frame with one fused kernel per level: the top level first, then
-// levels 0 .. L-2 in ascending order. Level l reads level l+1's smoothed
-// flow before level l+1 has advanced (the reference's update order) and the
-// top level's unsmoothed flow of this frame; every level writes its new
-// state to its spare buffers, exchanged with the state after the launches.
-void PyramidalFlowFilter::enqueueFused(const std::vector<LevelArgs>& args, hipStream_t s) {
-    pyramid_.run();
-    for (const LevelArgs& a : args) {
-        launch_fused_level(a, s);
-        check_launch("PyramidalFlowFilter::compute (fused level)");
+// The same frame as one fused launch. Jobs: the top level first, then
+// levels 0 .. L-2. Level l reads level l+1's smoothed flow before level l+1
+// has advanced (the reference's update order); level L-2 reads the top
+// level's unsmoothed flow of this frame and waits for it inside the launch.
+// Every level writes its new state to its spare buffers, exchanged with the
+// state after the launch. With two levels the top level reduces the input
+// image while loading it, so the pyramid kernels do not run.
+bool PyramidalFlowFilter::fusedPlan(std::vector<LevelArgs>& args) {
+    args.clear();
+    if (!configured_ || top_.firstLoad()) return false;
+    for (const DeltaFlowFilter& d : low_)
+        if (d.firstLoad()) return false;
+    LevelArgs top = top_.fusedArgs();
+    if (levels_ == 2 && top.img_mode == IMG_U8) {
+        top.img = input_.view();
+        top.img_mode = IMG_U8_DOWN;
     }
+    args.push_back(top);
+    for (int l = 0; l < levels_ - 1; ++l) {
+        LevelArgs a = low_[l].fusedArgs();
+        if (l == levels_ - 2) a.wait_on = 0;
+        args.push_back(a);
+    }
+    for (const LevelArgs& a : args)
+        if (!fused_level_supported(a)) return false;
+    return true;
 }
 
-void PyramidalFlowFilter::computeFused() {
-    std::vector<LevelArgs> args;
-    args.push_back(top_.fusedArgs());
-    for (int l = 0; l < levels_ - 1; ++l) args.push_back(low_[l].fusedArgs());
+void PyramidalFlowFilter::enqueueFused(int table, hipStream_t s) {
+    if (levels_ > 2) pyramid_.run();
+    frame_.launch(table, s);
+    check_launch("PyramidalFlowFilter::compute (fused frame)");
+}
+
+void PyramidalFlowFilter::computeFused(const std::vector<LevelArgs>& args) {
+    const int table = frame_.prepare(args);
+    const unsigned long long token = frame_.token(table);
     hipStream_t s = stream();
     if (graphs_on_ && !profiling_active()) {
-        hipGraphExec_t exec = graphs_.find(args);
+        hipGraphExec_t exec = graphs_.find(args, token);
         if (!exec) {
             FF_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
             hipGraph_t graph = nullptr;
             try {
-                enqueueFused(args, s);
+                enqueueFused(table, s);
             } catch (...) {
                 (void)hipStreamEndCapture(s, &graph);
                 if (graph) (void)hipGraphDestroy(graph);
@@ -519,11 +544,11 @@ void PyramidalFlowFilter::computeFused() {
             const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
             (void)hipGraphDestroy(graph);
             if (e != hipSuccess) throw_hip(e, "hipGraphInstantiate", __FILE__, __LINE__);
-            graphs_.insert(args, exec);
+            graphs_.insert(args, token, exec);
         }
         FF_HIP(hipGraphLaunch(exec, s));
     } else {
-        enqueueFused(args, s);
+        enqueueFused(table, s);
     }
     top_.fusedCommit();
     for (int l = 0; l < levels_ - 1; ++l) low_[l].fusedCommit();
@@ -532,9 +557,8 @@ void PyramidalFlowFilter::computeFused() {
 void PyramidalFlowFilter::compute() {
     if (!configured_) throw logic_error("PyramidalFlowFilter::compute(): filter not configured");
     startTiming();
-    bool fused = fused_ && top_.fusedSupported();
-    for (auto& d : low_) fused = fused && d.fusedSupported();
-    if (fused) computeFused();
+    std::vector<LevelArgs> args;
+    if (fused_ && fusedPlan(args)) computeFused(args);
     else computeStaged();
     stopTiming();
 }

@@ -6,9 +6,9 @@
 //  - staged: one launch per reference stage kernel, the reference's exact
 //    sequence (used for the first frame, for the split compute methods and
 //    when the fused kernels do not cover a parameter set);
-//  - fused: one kernel per pyramid level and frame (image model,
-//    propagation, update and smoothing in one pass over HBM), with the
-//    level state ping-ponged between two buffers.
+//  - fused: one kernel launch per frame for all levels (image model,
+//    propagation, update and smoothing of each level in one pass over HBM),
+//    with the level state ping-ponged between two buffers.
 #pragma once
 
 #include <vector>
@@ -69,6 +69,7 @@ private:
     FlowSmoother smoother_;
     FlowPropagator propagator_;
     Image spareFlow_, spareImage_;
+    FusedFrame frame_;
 };
 
 class DeltaFlowFilter : public Stage {
@@ -115,12 +116,14 @@ private:
     FlowSmoother smoother_;
     FlowPropagatorPayload propagator_;
     Image spareFlow_, spareImage_, spareDelta_;
+    FusedFrame frame_;
 };
 
 // Instantiated HIP graphs of one frame's fused launch sequence, keyed by the
-// exact kernel arguments. The level state alternates between two buffers,
-// so a steady run needs two graphs (even and odd frames); any parameter or
-// buffer change produces a new key and a new capture.
+// exact level arguments and the identity of the job table's device memory.
+// The level state alternates between two buffers, so a steady run needs two
+// graphs (even and odd frames); any parameter or buffer change produces a
+// new key and a new capture.
 class GraphCache {
 public:
     GraphCache() = default;
@@ -129,13 +132,14 @@ public:
     GraphCache(GraphCache&& o) noexcept { *this = std::move(o); }
     GraphCache& operator=(GraphCache&& o) noexcept;
     ~GraphCache() { clear(); }
-    hipGraphExec_t find(const std::vector<LevelArgs>& key) const;
-    void insert(const std::vector<LevelArgs>& key, hipGraphExec_t exec);
+    hipGraphExec_t find(const std::vector<LevelArgs>& key, unsigned long long token) const;
+    void insert(const std::vector<LevelArgs>& key, unsigned long long token, hipGraphExec_t exec);
     void clear();
 
 private:
     struct Entry {
         std::vector<LevelArgs> key;
+        unsigned long long token = 0;
         hipGraphExec_t exec = nullptr;
     };
     Entry slot_[2];
@@ -175,14 +179,19 @@ public:
         graphs_.clear();
     }
     bool graphs() const { return graphs_on_; }
+    unsigned long long waitTimeouts() const { return frame_.timeouts(); }
 
 private:
     void checkLevel(int level, const char* who) const;
     void computeStaged();
-    void computeFused();
-    void enqueueFused(const std::vector<LevelArgs>& args, hipStream_t s);
+    // Level arguments of a fused frame; false when some level is not ready
+    // (first frame) or not covered by the fused kernel.
+    bool fusedPlan(std::vector<LevelArgs>& args);
+    void computeFused(const std::vector<LevelArgs>& args);
+    void enqueueFused(int table, hipStream_t s);
     bool configured_ = false, fused_ = true, graphs_on_ = true;
     GraphCache graphs_;
+    FusedFrame frame_;
     int h_ = 0, w_ = 0, levels_ = 0;
     Image input_;
     ImagePyramid pyramid_;

@@ -1,19 +1,32 @@
-// ff_fused.hip -- one kernel per pyramid level and frame for gfx950.
+// ff_fused.hip -- the fused frame kernel for gfx950.
 //
-// The kernel runs, for one output tile, the whole per-level recurrence of
-// the reference (FlowFilter::compute, src/gpu/flowfilter.cu:125-153, and the
-// DeltaFlowFilter update path, :429-503): N upwind propagation iterations of
-// the previous smoothed flow (with the old brightness and delta flow as
-// payloads on delta levels), the image model, the 2x2 update and K box
-// smoothing iterations. State is read once and written once per frame;
-// everything in between stays in registers.
+// One launch advances every level of a frame. Each level is a job; its
+// output tiles are workgroups of the launch. A tile runs the whole per-level
+// recurrence of the reference (FlowFilter::compute, src/gpu/flowfilter.cu:
+// 125-153, and the DeltaFlowFilter update path, :429-503): N upwind
+// propagation iterations of the previous smoothed flow (with the old
+// brightness and delta flow as payloads on delta levels), the image model,
+// the 2x2 update and K box smoothing iterations. State is read once and
+// written once per frame; everything in between stays in registers.
 //
-// Geometry: a workgroup is NW waves stacked vertically; lane l of every wave
-// owns buffer column l (64 columns), and each thread owns R consecutive rows
-// of its wave's strip, so the buffer is 64 x (NW*R) pixels. The output tile
-// is the buffer minus a halo h0 = max(N,2) + 2K on every side (the distance
-// an error at the buffer edge travels inward: one pixel per propagation
-// pass, two per smoothing pass, two for the image model's x taps).
+// Jobs and their order. The top level of a pyramid comes first; the level
+// below it reads the top level's unsmoothed flow of the same frame
+// (flowfilter.cu:663-672), so its tiles wait, after their propagation
+// phase and before the update, until every tile of the top level has
+// signalled completion (a counter in device memory). Blocks are dispatched
+// in order on each XCD and producers never wait, so every producer a
+// consumer waits for is running or finished; the wait is also bounded by a
+// spin limit. Lower levels read the previous frame's flow of the level
+// above (the reference's ascending update order) and do not wait.
+//
+// Tile geometry: a workgroup is NW waves stacked vertically; lane l of every
+// wave owns buffer column l (64 columns), and each thread owns R consecutive
+// rows of its wave's strip, so the buffer is 64 x (NW*R) pixels. The output
+// tile is the buffer minus a halo h0 = max(N, 2 + e) + 2K on every side (the
+// distance an error at the buffer edge travels inward: one pixel per
+// propagation pass, two per smoothing pass, two for the image model's x
+// taps, plus e = 1 when the image is reduced while loading, because then
+// the buffer's first column also lacks its left source pixel).
 //  - X neighbours come from DPP wave shifts; Y neighbours are registers of
 //    the same thread, and the rows above and below a wave's strip are
 //    exchanged through LDS once per Y pass.
@@ -38,6 +51,11 @@
 // bitwise identical to the stage-by-stage path.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+
 #include "ff_device.hpp"
 #include "ff_kernels.hpp"
 #include "ff_runtime.hpp"
@@ -48,8 +66,11 @@ namespace {
 
 constexpr int LANES = 64;
 constexpr int OOB = (int)0x80000000;  // buffer offset beyond any image: access dropped
+constexpr int kSpinLimit = 1 << 23;   // about 2 s of polling at s_sleep 8
 
-struct FusedParams {
+// One level of one frame, as the kernel reads it. Blocks first_block ..
+// first_block + tiles_x * tiles_y - 1 of the launch are its tiles.
+struct LevelJob {
     const void* img;
     const float2* flow_in;
     const float* a0_in;
@@ -57,15 +78,21 @@ struct FusedParams {
     float2* flow_out;
     float* a0_out;
     float2* dflow_out;
-    float2* flow_upd;  // may be null
+    float2* flow_upd;  // may be null on a top level
     const float2* upper;
     const int* xmap;
     const int* ymap;
     // row pitches in bytes
     int img_pitch, flow_in_pitch, a0_in_pitch, dflow_in_pitch, flow_out_pitch, a0_out_pitch, dflow_out_pitch,
         flow_upd_pitch, upper_pitch;
-    int W, H, Hu, N, K, border, h0, TW, TH, tiles_x, tiles_y;
+    int W, H, Hu, Hsrc;  // level size, upper level height, image height
+    int N, K, border, h0, TW, TH, tiles_x, tiles_y;
+    int delta, img_mode, first_block;
+    // completion counters: wait for counter wait_idx to reach wait_target
+    // (wait_consumers tiles wait on it in total); signal counter signal_idx
+    int wait_idx, wait_target, wait_consumers, signal_idx;
     float dt, gamma, maxflow;
+    int pad_;
 };
 
 // ------------------------------------------------------------ buffer access
@@ -85,6 +112,7 @@ __device__ __forceinline__ float2 ld2(Rsrc r, int off) {
     return make_float2(__uint_as_float(v.x), __uint_as_float(v.y));
 }
 __device__ __forceinline__ unsigned ldb(Rsrc r, int off) { return __builtin_amdgcn_raw_buffer_load_b8(r, off, 0, 0); }
+__device__ __forceinline__ unsigned ldh(Rsrc r, int off) { return __builtin_amdgcn_raw_buffer_load_b16(r, off, 0, 0); }
 __device__ __forceinline__ void st1(Rsrc r, int off, float v) {
     __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, off, 0, 0);
 }
@@ -108,6 +136,13 @@ __device__ __forceinline__ float cshr(float v) {
 }
 __device__ __forceinline__ float cshl(float v) {
     return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), 0x130, 0xf, 0xf, false));
+}
+// Lane i receives lane i-1's v; lane 0 receives `own`. (A select between a
+// shifted value and a lane's own value must not be written as a condition:
+// the compiler may then run the shift with some lanes disabled, and a lane
+// reading a disabled lane receives 0.)
+__device__ __forceinline__ unsigned shr_or(unsigned v, unsigned own) {
+    return (unsigned)__builtin_amdgcn_update_dpp((int)own, (int)v, 0x138, 0xf, 0xf, false);
 }
 template <bool CLAMP>
 __device__ __forceinline__ float shr(float v) {
@@ -144,13 +179,56 @@ struct WBox {
     __device__ __forceinline__ float operator()(int) const { return kBox; }
 };
 
-// Map block ids so that the tiles of one XCD (blocks are dealt round-robin
-// over the 8 XCDs) form a contiguous band of the image.
+// Map a job's tile index so that the tiles of one XCD (blocks are dealt
+// round-robin over the 8 XCDs) form a contiguous band of the image.
 __device__ __forceinline__ int xcd_tile(int b, int nt) {
     const int q = nt / 8;
     if (b >= 8 * q) return b;
     return (b % 8) * q + b / 8;
 }
+
+// ------------------------------------------------- completion counters
+
+// Wait until the producer job's tiles have all signalled, then make their
+// stores visible to this workgroup. Polling uses relaxed loads (an acquire
+// per poll would invalidate the XCD's L2 each time); one acquire follows the
+// wait, by the polling thread, and the workgroup barrier orders every
+// wave's later loads after it. The last consumer resets the counters for
+// the next launch.
+__device__ __forceinline__ void wait_upper(const LevelJob& p, unsigned* counters, int timeout_idx) {
+    if (threadIdx.x == 0) {
+        unsigned* done = counters + 2 * p.wait_idx;
+        unsigned* used = done + 1;
+        int spins = 0;
+        while (__hip_atomic_load(done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < (unsigned)p.wait_target) {
+            __builtin_amdgcn_s_sleep(8);
+            if (++spins > kSpinLimit) {
+                __hip_atomic_fetch_add(counters + timeout_idx, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                break;
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        const unsigned u = __hip_atomic_fetch_add(used, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (u + 1 == (unsigned)p.wait_consumers) {
+            __hip_atomic_store(done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(used, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+    __syncthreads();
+}
+
+// Publish this tile's stores, then count the tile as done: the barrier
+// waits for every wave's stores to complete, and one agent-scope release
+// (an L2 write-back) precedes the count.
+__device__ __forceinline__ void signal_done(const LevelJob& p, unsigned* counters) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __hip_atomic_fetch_add(counters + 2 * p.signal_idx, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// ------------------------------------------------------------ tile body
 
 // Per-thread view of the tile.
 struct Geo {
@@ -170,7 +248,7 @@ struct State {
 // One upwind X pass (flowPropagateX_k / flowPropagatePayloadX_k,
 // propagation_k.cu:16-72, 134-232).
 template <bool DELTA, bool BORDER, int R>
-__device__ __forceinline__ void prop_x(State<DELTA, R>& s, const Geo& g, const FusedParams& p) {
+__device__ __forceinline__ void prop_x(State<DELTA, R>& s, const Geo& g, const LevelJob& p) {
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const float zu = s.fu[r], zv = s.fv[r];
@@ -216,7 +294,7 @@ struct Edge5 {
 // propagation_k.cu:75-131, 235-334); rows top-down, m* keep the old values
 // of row r-1.
 template <bool DELTA, bool BORDER, int R>
-__device__ __forceinline__ void prop_y(State<DELTA, R>& s, const Geo& g, const FusedParams& p, const Edge5& a,
+__device__ __forceinline__ void prop_y(State<DELTA, R>& s, const Geo& g, const LevelJob& p, const Edge5& a,
                                        const Edge5& b) {
     float mu = a.u, mv = a.v, ms = a.s, mdu = a.du, mdv = a.dv;
 #pragma unroll
@@ -266,13 +344,13 @@ __device__ __forceinline__ void prop_y(State<DELTA, R>& s, const Geo& g, const F
 // Image model (imagePrefilter_k / imageModel_k, imagemodel_k.cu:31-133) and
 // update (flowUpdate_k, update_k.cu:15-79, or deltaFlowUpdate_k, :82-173)
 // of every row of the thread. im[j] is image row yw-2+j (clamped reads),
-// aux[r] the old brightness (top level) or upper-level flow (delta level)
-// of row r. Writes the new brightness (and delta flow) of the tile's output
-// pixels; leaves the updated flow in s.fu/s.fv.
+// aold[r] the old brightness (top level) and up[r] the upper-level flow
+// (delta level) of row r. Writes the new brightness (and delta flow) of the
+// tile's output pixels; leaves the updated flow in s.fu/s.fv.
 template <bool DELTA, bool CLAMP, int R>
 __device__ __forceinline__ void model_update(State<DELTA, R>& s, const float (&im)[R + 4], const float (&aold)[R],
-                                             const float2 (&up)[R], const Geo& g, const FusedParams& p,
-                                             const Rsrc& ra0, const Rsrc& rdf, const Rsrc& rfu, const int (&offa)[R],
+                                             const float2 (&up)[R], const LevelJob& p, const Rsrc& ra0,
+                                             const Rsrc& rdf, const Rsrc& rfu, const int (&offa)[R],
                                              const int (&offf)[R]) {
     float sx[R + 4];
 #pragma unroll
@@ -328,15 +406,105 @@ __device__ __forceinline__ void smooth_x(float (&fu)[R], float (&fv)[R]) {
     }
 }
 
-template <bool DELTA, bool U8, int R, int NW>
-__global__ void __launch_bounds__(NW* LANES) k_fused_level(FusedParams p) {
-    // row exchange between the waves of the workgroup, double buffered
-    __shared__ float xch[2][NW][10][LANES];
+// Image rows yw-2 .. yw+R+1 (clamped to the level) at the thread's column.
+// load_image issues the loads; finish_image turns them into normalized
+// values once they are needed, so the latency of the F32 and U8 loads
+// overlaps the propagation phase.
+//  IMG_F32:     the float texel.
+//  IMG_U8:      the u8 texel.
+//  IMG_U8_DOWN: the texel of the level one pyramid step below (twice the
+//               size): imageDownX_uint8_k then imageDownY_uint8_k
+//               (pyramid_k.cu:14-78) evaluated for this texel. Row yy needs
+//               source rows 2yy-1 (clamped), 2yy, 2yy+1; each of those the
+//               source columns 2x-1 (clamped), 2x, 2x+1, read as one 16-bit
+//               load of columns 2x, 2x+1 and the high byte of the left
+//               lane's load. With W = Wsrc/2 and H = Hsrc/2 (integer
+//               halving) the columns 2x+1 and rows 2yy+1 are always inside
+//               the source image. The reduction runs right after the loads
+//               (three source rows per texel would not fit in registers
+//               through the propagation phase).
 
+// One pyramid step along x of a source row: columns 2x-1 (clamped), 2x, 2x+1.
+// Column 2x-1 is the high byte of the left lane's pair; lane 0 takes its own
+// low byte instead, which is the clamped read when x = 0 (for a buffer that
+// does not start at the image edge lane 0 lies in the halo).
+__device__ __forceinline__ unsigned down_x(unsigned pair) {
+    const unsigned lo = pair & 0xffu, hi = pair >> 8;
+    const unsigned left = (shr_or(pair, pair << 8) >> 8) & 0xffu;
+    return down_u8(left, lo, hi);
+}
+
+// Registers holding the image rows between the loads and their use: the
+// float texels, or the u8 texels packed four to a register (the byte
+// conversion instructions read them in place).
+template <int MODE, int R>
+struct ImageRegs {
+    static constexpr int kN = MODE == IMG_F32 ? R + 4 : (R + 4 + 3) / 4;
+    unsigned v[kN];
+};
+
+template <int MODE, int R>
+__device__ __forceinline__ void load_image(ImageRegs<MODE, R>& ir, const LevelJob& p, const Geo& g) {
+    const Rsrc rimg = rsrc(p.img, p.img_pitch * p.Hsrc);
+    if constexpr (MODE == IMG_F32) {
+#pragma unroll
+        for (int j = 0; j < R + 4; ++j) {
+            const int yc = clampi(g.yw - 2 + j, 0, p.H - 1);
+            ir.v[j] = __builtin_amdgcn_raw_buffer_load_b32(rimg, yc * p.img_pitch + g.x * 4, 0, 0);
+        }
+        return;
+    } else {
+        unsigned t[R + 4];
+        if constexpr (MODE == IMG_U8_DOWN) {
+            unsigned src[R + 4][3];
+#pragma unroll
+            for (int j = 0; j < R + 4; ++j) {
+                const int r1 = 2 * clampi(g.yw - 2 + j, 0, p.H - 1);
+                const int r0 = r1 > 0 ? r1 - 1 : 0;
+                src[j][0] = ldh(rimg, r0 * p.img_pitch + 2 * g.x);
+                src[j][1] = ldh(rimg, r1 * p.img_pitch + 2 * g.x);
+                src[j][2] = ldh(rimg, (r1 + 1) * p.img_pitch + 2 * g.x);
+            }
+#pragma unroll
+            for (int j = 0; j < R + 4; ++j) t[j] = down_u8(down_x(src[j][0]), down_x(src[j][1]), down_x(src[j][2]));
+        } else {
+#pragma unroll
+            for (int j = 0; j < R + 4; ++j) {
+                const int yc = clampi(g.yw - 2 + j, 0, p.H - 1);
+                t[j] = ldb(rimg, yc * p.img_pitch + g.x);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < ImageRegs<MODE, R>::kN; ++k) {
+            unsigned w = 0;
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                if (4 * k + i < R + 4) w |= t[4 * k + i] << (8 * i);
+            // pack here: left to the compiler, the bytes (or, reduced on
+            // load, three source values per row) would stay in separate
+            // registers through the propagation phase
+            asm volatile("" : "+v"(w));
+            ir.v[k] = w;
+        }
+    }
+}
+
+template <int MODE, int R>
+__device__ __forceinline__ void finish_image(const ImageRegs<MODE, R>& ir, float (&im)[R + 4]) {
+#pragma unroll
+    for (int j = 0; j < R + 4; ++j) {
+        if constexpr (MODE == IMG_F32) im[j] = __uint_as_float(ir.v[j]);
+        else im[j] = unorm8((ir.v[j / 4] >> (8 * (j % 4))) & 0xffu);
+    }
+}
+
+template <bool DELTA, int MODE, int R, int NW>
+__device__ __forceinline__ void level_tile(const LevelJob& p, int tile, float (&xch)[2][NW][10][LANES],
+                                           unsigned* counters, int timeout_idx) {
     Geo g;
     g.lane = threadIdx.x & (LANES - 1);
     g.wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int t = xcd_tile(blockIdx.x, p.tiles_x * p.tiles_y);
+    const int t = xcd_tile(tile, p.tiles_x * p.tiles_y);
     const int tyi = t / p.tiles_x, txi = t - tyi * p.tiles_x;
     // output range of the tile and the buffer around it, clamped into the image
     const int ox0 = txi * p.TW, ox1 = min(ox0 + p.TW, p.W);
@@ -352,7 +520,6 @@ __global__ void __launch_bounds__(NW* LANES) k_fused_level(FusedParams p) {
     const int lane = g.lane, wave = g.wave;
     const bool out_x = g.x >= ox0 && g.x < ox1;
 
-    const Rsrc rimg = rsrc(p.img, p.img_pitch * p.H);
     const Rsrc rfi = rsrc(p.flow_in, p.flow_in_pitch * p.H);
     const Rsrc rai = rsrc(p.a0_in, p.a0_in_pitch * p.H);
     const Rsrc rfo = rsrc(p.flow_out, p.flow_out_pitch * p.H);
@@ -374,14 +541,8 @@ __global__ void __launch_bounds__(NW* LANES) k_fused_level(FusedParams p) {
             s.dv[r] = d.y;
         }
     }
-    float im[R + 4];  // image rows yw-2 .. yw+R+1, clamped
-    unsigned imb[U8 ? R + 4 : 1];
-#pragma unroll
-    for (int j = 0; j < R + 4; ++j) {
-        const int yc = clampi(g.yw - 2 + j, 0, p.H - 1);
-        if constexpr (U8) imb[j] = ldb(rimg, yc * p.img_pitch + g.x);
-        else im[j] = ld1(rimg, yc * p.img_pitch + g.x * 4);
-    }
+    ImageRegs<MODE, R> ir;
+    load_image<MODE, R>(ir, p, g);
 
     int slot = 0;
 
@@ -442,6 +603,7 @@ __global__ void __launch_bounds__(NW* LANES) k_fused_level(FusedParams p) {
     }
 
     // ------------------------------------------------ image model + update
+    if (DELTA && p.wait_idx >= 0) wait_upper(p, counters, timeout_idx);
     float aold[R];  // unused (and removed) on delta levels
     float2 up[R];
     int offa[R], offf[R];
@@ -464,12 +626,10 @@ __global__ void __launch_bounds__(NW* LANES) k_fused_level(FusedParams p) {
             up[r] = make_float2(0.f, 0.f);
         }
     }
-    if constexpr (U8) {
-#pragma unroll
-        for (int j = 0; j < R + 4; ++j) im[j] = unorm8(imb[j]);
-    }
-    if (xedge) model_update<DELTA, true, R>(s, im, aold, up, g, p, rao, rdf, rfu, offa, offf);
-    else model_update<DELTA, false, R>(s, im, aold, up, g, p, rao, rdf, rfu, offa, offf);
+    float im[R + 4];
+    finish_image<MODE, R>(ir, im);
+    if (xedge) model_update<DELTA, true, R>(s, im, aold, up, p, rao, rdf, rfu, offa, offf);
+    else model_update<DELTA, false, R>(s, im, aold, up, p, rao, rdf, rfu, offa, offf);
 
     // ------------------------------------------------------------ smoothing
     // flowSmoothX_k / flowSmoothY_k (flowsmoothing_k.cu:21-87), K times.
@@ -540,6 +700,28 @@ __global__ void __launch_bounds__(NW* LANES) k_fused_level(FusedParams p) {
         const bool out = out_x && y >= oy0 && y < oy1;
         st2(rfo, out ? y * p.flow_out_pitch + g.x * 8 : OOB, s.fu[r], s.fv[r]);
     }
+    if (!DELTA && p.signal_idx >= 0) signal_done(p, counters);
+}
+
+// Blocks are grouped by tile body: top levels with float, u8 and reduced u8
+// images, then delta levels with float and u8 images; ends[i] is the first
+// block after group i. (Selecting the body from the block index rather than
+// from a job field also keeps the compiler's register allocation of each
+// body as tight as in a kernel of its own.)
+template <int R, int NW>
+__global__ void __launch_bounds__(NW* LANES)
+    k_fused_frame(const LevelJob* __restrict__ jobs, const int* __restrict__ block_job, unsigned* counters,
+                  int timeout_idx, int end0, int end1, int end2, int end3) {
+    // row exchange between the waves of the workgroup, double buffered
+    __shared__ float xch[2][NW][10][LANES];
+    const LevelJob& p = jobs[block_job[blockIdx.x]];
+    const int tile = (int)blockIdx.x - p.first_block;
+    const unsigned b = blockIdx.x;
+    if (b < (unsigned)end0) level_tile<false, IMG_F32, R, NW>(p, tile, xch, counters, timeout_idx);
+    else if (b < (unsigned)end1) level_tile<false, IMG_U8, R, NW>(p, tile, xch, counters, timeout_idx);
+    else if (b < (unsigned)end2) level_tile<false, IMG_U8_DOWN, R, NW>(p, tile, xch, counters, timeout_idx);
+    else if (b < (unsigned)end3) level_tile<true, IMG_F32, R, NW>(p, tile, xch, counters, timeout_idx);
+    else level_tile<true, IMG_U8, R, NW>(p, tile, xch, counters, timeout_idx);
 }
 
 #ifndef FF_FUSED_R
@@ -551,7 +733,20 @@ __global__ void __launch_bounds__(NW* LANES) k_fused_level(FusedParams p) {
 constexpr int kR = FF_FUSED_R;
 constexpr int kNW = FF_FUSED_NW;
 
-int halo_of(const LevelArgs& a) { return (a.prop_iters > 2 ? a.prop_iters : 2) + 2 * a.smooth_iters; }
+// Level size of a job (the image is twice as large when reduced on load).
+void level_size(const LevelArgs& a, int& H, int& W) {
+    H = a.flow_s_in.h;
+    W = a.flow_s_in.w;
+}
+
+int halo_of(const LevelArgs& a) {
+    const int lo = a.img_mode == IMG_U8_DOWN ? 3 : 2;
+    return (a.prop_iters > lo ? a.prop_iters : lo) + 2 * a.smooth_iters;
+}
+
+bool fits_i32(const DevImg& d) {
+    return (unsigned long long)d.pitch * (unsigned long long)(d.h > 0 ? d.h : 0) < 0x7fffffffULL;
+}
 
 }  // namespace
 
@@ -559,71 +754,212 @@ bool fused_level_supported(const LevelArgs& a) {
     const int h0 = halo_of(a);
     if (a.border < 1 || a.prop_iters < 1 || a.smooth_iters < 1) return false;
     if (2 * h0 >= LANES || 2 * h0 >= kNW * kR) return false;
-    const int H = a.img.h, W = a.img.w;
+    int H, W;
+    level_size(a, H, W);
     if (W < LANES || H < kNW * kR) return false;
     if (!a.img.data || !a.flow_s_in.data || !a.a0_in.data || !a.flow_s_out.data || !a.a0_out.data) return false;
+    switch (a.img_mode) {
+    case IMG_F32:
+    case IMG_U8:
+        if (a.img.h != H || a.img.w != W) return false;
+        break;
+    case IMG_U8_DOWN:
+        if (a.img.h / 2 != H || a.img.w / 2 != W || a.delta) return false;
+        break;
+    default:
+        return false;
+    }
     if (a.delta && (!a.dflow_in.data || !a.dflow_out.data || !a.upper.data || !a.xmap || !a.ymap ||
                     a.upper.h <= 0 || a.upper.w <= 0))
         return false;
-    for (const DevImg* d : {&a.flow_s_in, &a.a0_in, &a.flow_s_out, &a.a0_out})
+    if (!a.delta && a.wait_on >= 0) return false;
+    for (const DevImg* d : {&a.a0_in, &a.flow_s_out, &a.a0_out})
         if (d->h != H || d->w != W) return false;
     if (a.delta)
         for (const DevImg* d : {&a.dflow_in, &a.dflow_out})
             if (d->h != H || d->w != W) return false;
     if (a.flow_upd_out.data && (a.flow_upd_out.h != H || a.flow_upd_out.w != W)) return false;
     // every buffer must be addressable with 32-bit offsets
-    const DevImg* all[] = {&a.img, &a.flow_s_in, &a.a0_in, &a.dflow_in, &a.flow_s_out, &a.a0_out, &a.dflow_out,
-                           &a.flow_upd_out, &a.upper};
-    for (const DevImg* d : all)
-        if ((unsigned long long)d->pitch * (unsigned long long)(d->h > 0 ? d->h : 0) >= 0x7fffffffULL) return false;
+    for (const DevImg* d : {&a.img, &a.flow_s_in, &a.a0_in, &a.dflow_in, &a.flow_s_out, &a.a0_out, &a.dflow_out,
+                            &a.flow_upd_out, &a.upper})
+        if (!fits_i32(*d)) return false;
     return true;
 }
 
-void launch_fused_level(const LevelArgs& a, hipStream_t s) {
-    ProfileScope ps(a.delta ? "fused_delta" : "fused_top", s);
-    FusedParams p{};
-    p.img = a.img.data;
-    p.img_pitch = (int)a.img.pitch;
-    p.flow_in = (const float2*)a.flow_s_in.data;
-    p.flow_in_pitch = (int)a.flow_s_in.pitch;
-    p.a0_in = (const float*)a.a0_in.data;
-    p.a0_in_pitch = (int)a.a0_in.pitch;
-    p.dflow_in = (const float2*)a.dflow_in.data;
-    p.dflow_in_pitch = (int)a.dflow_in.pitch;
-    p.flow_out = (float2*)a.flow_s_out.data;
-    p.flow_out_pitch = (int)a.flow_s_out.pitch;
-    p.a0_out = (float*)a.a0_out.data;
-    p.a0_out_pitch = (int)a.a0_out.pitch;
-    p.dflow_out = (float2*)a.dflow_out.data;
-    p.dflow_out_pitch = (int)a.dflow_out.pitch;
-    p.flow_upd = (float2*)a.flow_upd_out.data;
-    p.flow_upd_pitch = (int)a.flow_upd_out.pitch;
-    p.upper = (const float2*)a.upper.data;
-    p.upper_pitch = (int)a.upper.pitch;
-    p.xmap = a.xmap;
-    p.ymap = a.ymap;
-    p.W = a.img.w;
-    p.H = a.img.h;
-    p.Hu = a.upper.h;
-    p.N = a.prop_iters;
-    p.K = a.smooth_iters;
-    p.border = a.border;
-    p.h0 = halo_of(a);
-    p.TW = LANES - 2 * p.h0;
-    p.TH = kNW * kR - 2 * p.h0;
-    p.tiles_x = (p.W + p.TW - 1) / p.TW;
-    p.tiles_y = (p.H + p.TH - 1) / p.TH;
-    p.dt = a.dt;
-    p.gamma = a.gamma;
-    p.maxflow = a.maxflow;
-    const dim3 grid(p.tiles_x * p.tiles_y), block(kNW * LANES);
-    if (a.delta) {
-        if (a.img_u8) hipLaunchKernelGGL((k_fused_level<true, true, kR, kNW>), grid, block, 0, s, p);
-        else hipLaunchKernelGGL((k_fused_level<true, false, kR, kNW>), grid, block, 0, s, p);
-    } else {
-        if (a.img_u8) hipLaunchKernelGGL((k_fused_level<false, true, kR, kNW>), grid, block, 0, s, p);
-        else hipLaunchKernelGGL((k_fused_level<false, false, kR, kNW>), grid, block, 0, s, p);
+// ------------------------------------------------------------- FusedFrame
+
+struct FusedFrame::Impl {
+    struct Table {
+        std::vector<char> key;  // job records + block map, as uploaded
+        DeviceArray dev;
+        int njobs = 0, blocks = 0;
+        int ends[4] = {0, 0, 0, 0};
+        size_t map_offset = 0;
+        unsigned long long token = 0;
+    };
+    Table slot[2];
+    int next = 0;
+    unsigned long long uploads = 0;
+    DeviceArray counters;
+    int ncounters = 0;  // pairs
+};
+
+FusedFrame::FusedFrame() : impl_(new Impl) {}
+FusedFrame::~FusedFrame() { delete impl_; }
+FusedFrame::FusedFrame(FusedFrame&& o) noexcept : impl_(o.impl_) { o.impl_ = new Impl; }
+FusedFrame& FusedFrame::operator=(FusedFrame&& o) noexcept {
+    std::swap(impl_, o.impl_);
+    return *this;
+}
+
+namespace {
+// Tile body group of a job (the order of the kernel's dispatch).
+int body_group(const LevelArgs& a) {
+    if (!a.delta) return a.img_mode == IMG_F32 ? 0 : (a.img_mode == IMG_U8 ? 1 : 2);
+    return a.img_mode == IMG_F32 ? 3 : 4;
+}
+}  // namespace
+
+int FusedFrame::prepare(const std::vector<LevelArgs>& jobs) {
+    if (jobs.empty()) throw std::invalid_argument("fused frame: no levels");
+    // launch order: grouped by tile body, stable within a group; every top
+    // level precedes every delta level, so producers precede consumers
+    std::vector<int> order(jobs.size()), pos(jobs.size());
+    for (size_t i = 0; i < jobs.size(); ++i) order[i] = (int)i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int x, int y) { return body_group(jobs[x]) < body_group(jobs[y]); });
+    for (size_t i = 0; i < order.size(); ++i) pos[order[i]] = (int)i;
+    std::vector<LevelJob> rec(jobs.size());
+    int blocks = 0;
+    int ends[4] = {0, 0, 0, 0};
+    for (size_t i = 0; i < jobs.size(); ++i) {
+        const LevelArgs& a = jobs[order[i]];
+        if (!fused_level_supported(a)) throw std::logic_error("fused frame: unsupported level parameters");
+        LevelJob& p = rec[i];
+        std::memset(&p, 0, sizeof(p));
+        p.img = a.img.data;
+        p.img_pitch = (int)a.img.pitch;
+        p.flow_in = (const float2*)a.flow_s_in.data;
+        p.flow_in_pitch = (int)a.flow_s_in.pitch;
+        p.a0_in = (const float*)a.a0_in.data;
+        p.a0_in_pitch = (int)a.a0_in.pitch;
+        p.dflow_in = (const float2*)a.dflow_in.data;
+        p.dflow_in_pitch = (int)a.dflow_in.pitch;
+        p.flow_out = (float2*)a.flow_s_out.data;
+        p.flow_out_pitch = (int)a.flow_s_out.pitch;
+        p.a0_out = (float*)a.a0_out.data;
+        p.a0_out_pitch = (int)a.a0_out.pitch;
+        p.dflow_out = (float2*)a.dflow_out.data;
+        p.dflow_out_pitch = (int)a.dflow_out.pitch;
+        p.flow_upd = (float2*)a.flow_upd_out.data;
+        p.flow_upd_pitch = (int)a.flow_upd_out.pitch;
+        p.upper = (const float2*)a.upper.data;
+        p.upper_pitch = (int)a.upper.pitch;
+        p.xmap = a.xmap;
+        p.ymap = a.ymap;
+        level_size(a, p.H, p.W);
+        p.Hu = a.upper.h;
+        p.Hsrc = a.img.h;
+        p.N = a.prop_iters;
+        p.K = a.smooth_iters;
+        p.border = a.border;
+        p.h0 = halo_of(a);
+        p.TW = LANES - 2 * p.h0;
+        p.TH = kNW * kR - 2 * p.h0;
+        p.tiles_x = (p.W + p.TW - 1) / p.TW;
+        p.tiles_y = (p.H + p.TH - 1) / p.TH;
+        p.delta = a.delta;
+        p.img_mode = a.img_mode;
+        p.first_block = blocks;
+        p.wait_idx = -1;
+        p.signal_idx = -1;
+        p.dt = a.dt;
+        p.gamma = a.gamma;
+        p.maxflow = a.maxflow;
+        blocks += p.tiles_x * p.tiles_y;
+        for (int g = body_group(a); g < 4; ++g) ends[g] = blocks;
     }
+    for (size_t i = 0; i < jobs.size(); ++i) {
+        const int k = jobs[i].wait_on;
+        if (k < 0) continue;
+        if (k >= (int)i) throw std::logic_error("fused frame: a level waits on a later level");
+        if (!jobs[i].delta || jobs[k].delta) throw std::logic_error("fused frame: only delta levels wait on a top level");
+        LevelJob& c = rec[pos[i]];
+        LevelJob& q = rec[pos[k]];
+        c.wait_idx = pos[k];
+        c.wait_target = q.tiles_x * q.tiles_y;
+        q.signal_idx = pos[k];
+    }
+    for (size_t i = 0; i < rec.size(); ++i) {
+        if (rec[i].wait_idx < 0) continue;
+        int consumers = 0;
+        for (size_t j = 0; j < rec.size(); ++j)
+            if (rec[j].wait_idx == rec[i].wait_idx) consumers += rec[j].tiles_x * rec[j].tiles_y;
+        rec[i].wait_consumers = consumers;
+    }
+    std::vector<int> map(blocks);
+    for (size_t i = 0; i < rec.size(); ++i)
+        for (int b = 0; b < rec[i].tiles_x * rec[i].tiles_y; ++b) map[rec[i].first_block + b] = (int)i;
+
+    const size_t jobs_bytes = rec.size() * sizeof(LevelJob);
+    const size_t map_offset = (jobs_bytes + 255) & ~size_t(255);
+    std::vector<char> key(map_offset + map.size() * sizeof(int), 0);
+    std::memcpy(key.data(), rec.data(), jobs_bytes);
+    std::memcpy(key.data() + map_offset, map.data(), map.size() * sizeof(int));
+    // the group ends are kernel arguments, part of the table's identity
+    for (int g = 0; g < 4; ++g) {
+        const char* e = reinterpret_cast<const char*>(&ends[g]);
+        key.insert(key.end(), e, e + sizeof(int));
+    }
+
+    for (int i = 0; i < 2; ++i)
+        if (impl_->slot[i].key == key) return i;
+
+    // a table (or the counters) may be in use by launches still in flight
+    FF_HIP(hipDeviceSynchronize());
+    const int need = (int)jobs.size() + 1;
+    if (impl_->ncounters < need) {
+        const int n = need < 64 ? 64 : need;
+        std::vector<unsigned> zero(2 * (size_t)n, 0u);
+        impl_->counters.assign(zero.data(), zero.size() * sizeof(unsigned));
+        impl_->ncounters = n;
+        // launches recorded with the old counters must not be replayed
+        for (Impl::Table& o : impl_->slot) o.token = ++impl_->uploads;
+    }
+    const int s = impl_->next;
+    Impl::Table& t = impl_->slot[s];
+    t.dev.assign(key.data(), key.size());
+    t.key = std::move(key);
+    t.njobs = (int)rec.size();
+    t.blocks = blocks;
+    t.map_offset = map_offset;
+    for (int g = 0; g < 4; ++g) t.ends[g] = ends[g];
+    t.token = ++impl_->uploads;
+    impl_->next ^= 1;
+    return s;
+}
+
+int FusedFrame::blocks(int table) const { return impl_->slot[table & 1].blocks; }
+unsigned long long FusedFrame::token(int table) const { return impl_->slot[table & 1].token; }
+
+void FusedFrame::launch(int table, hipStream_t s) const {
+    const Impl::Table& t = impl_->slot[table & 1];
+    if (!t.blocks) throw std::logic_error("fused frame: launch of an unprepared table");
+    ProfileScope ps("fused_frame", s);
+    const char* base = (const char*)t.dev.data();
+    const int timeout_idx = 2 * (impl_->ncounters - 1);
+    hipLaunchKernelGGL((k_fused_frame<kR, kNW>), dim3(t.blocks), dim3(kNW * LANES), 0, s,
+                       (const LevelJob*)base, (const int*)(base + t.map_offset), (unsigned*)impl_->counters.data(),
+                       timeout_idx, t.ends[0], t.ends[1], t.ends[2], t.ends[3]);
+}
+
+unsigned long long FusedFrame::timeouts() const {
+    if (!impl_->ncounters) return 0;
+    FF_HIP(hipDeviceSynchronize());
+    unsigned v = 0;
+    const unsigned* c = (const unsigned*)impl_->counters.data();
+    FF_HIP(hipMemcpy(&v, c + 2 * (impl_->ncounters - 1), sizeof(v), hipMemcpyDeviceToHost));
+    return v;
 }
 
 }  // namespace ffamd

@@ -10,6 +10,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <vector>
 
 namespace ffamd {
 
@@ -62,14 +63,25 @@ void launch_down_pass(int axis, bool u8, const DevImg& in, const DevImg& out, hi
 // unlike a memset/memcpy node). row_bytes bytes per row.
 void launch_copy2d(const DevImg& src, const DevImg& dst, size_t row_bytes, hipStream_t s);
 
-// --------------------------------------------------------- fused level kernels
+// ------------------------------------------------------------ fused frame
+// One kernel launch runs every level of a frame (and, for batched filters,
+// of many frames): each level is a job whose output tiles are workgroups of
+// the launch. Within a tile the whole per-level recurrence runs in
+// registers: image model, propagation (with payloads on delta levels),
+// update and smoothing.
 
-// Parameters of one pyramid level for the fused per-frame kernel: image
-// model, propagation (with payloads on delta levels), update and smoothing
-// of one level and frame.
+// How a job reads its image.
+enum ImgMode : int {
+    IMG_F32 = 0,      // float image of the level's size
+    IMG_U8 = 1,       // u8 image of the level's size (normalized read)
+    IMG_U8_DOWN = 2,  // u8 image of twice the level's size, reduced by one
+                      // pyramid step (imageDown{X,Y}_uint8_k) while loading
+};
+
+// Parameters of one level of one frame.
 struct LevelArgs {
-    DevImg img;              // this level's input image
-    int img_u8 = 1;          // 1: u8 (normalized read), 0: float
+    DevImg img;              // input image (see img_mode)
+    int img_mode = IMG_U8;
     // previous state (read)
     DevImg flow_s_in;        // smoothed flow, float2
     DevImg a0_in;            // updated image (old brightness), float
@@ -78,7 +90,7 @@ struct LevelArgs {
     DevImg flow_s_out;
     DevImg a0_out;
     DevImg dflow_out;        // delta levels only
-    DevImg flow_upd_out;     // unsmoothed updated flow
+    DevImg flow_upd_out;     // unsmoothed updated flow (may be empty on a top level)
     // upper level flow (delta levels only) and its texel index maps
     DevImg upper;
     const int* xmap = nullptr;
@@ -86,11 +98,44 @@ struct LevelArgs {
     float gamma = 1.0f, maxflow = 1.0f, dt = 1.0f;
     int prop_iters = 1, smooth_iters = 1, border = 3;
     int delta = 0;           // 1: DeltaFlowFilter level, 0: top level (FlowFilter)
+    // Index (in the same launch) of the job whose unsmoothed flow this job
+    // reads as `upper`, or -1 when `upper` is not written in this launch.
+    // The job must come earlier in the list.
+    int wait_on = -1;
 };
 
 // Returns false when the level's parameters are outside what the fused
-// kernels are compiled for (the caller then runs the stage kernels).
+// kernel handles (the caller then runs the stage kernels).
 bool fused_level_supported(const LevelArgs& a);
-void launch_fused_level(const LevelArgs& a, hipStream_t s);
+
+// Device tables and completion counters of fused frame launches. prepare()
+// builds the job table of a list of levels (uploading it, after a device
+// synchronization, when it matches none of the two most recent tables) and
+// must not run under stream capture; launch() only enqueues the kernel.
+class FusedFrame {
+public:
+    FusedFrame();
+    ~FusedFrame();
+    FusedFrame(const FusedFrame&) = delete;
+    FusedFrame& operator=(const FusedFrame&) = delete;
+    FusedFrame(FusedFrame&&) noexcept;
+    FusedFrame& operator=(FusedFrame&&) noexcept;
+
+    int prepare(const std::vector<LevelArgs>& jobs);
+    void launch(int table, hipStream_t s) const;
+    // Waits that gave up after the spin limit (a scheduling fault; results
+    // of such a frame are not valid). Synchronizes the device.
+    unsigned long long timeouts() const;
+    // Workgroups of a prepared table.
+    int blocks(int table) const;
+    // Identity of the device memory a launch of this table uses; it changes
+    // whenever that memory is replaced, so recorded launches (graphs) keyed
+    // by it never refer to freed memory.
+    unsigned long long token(int table) const;
+
+private:
+    struct Impl;
+    Impl* impl_;
+};
 
 }  // namespace ffamd

@@ -192,6 +192,7 @@ SIGNATURES = {
     "ff_filter_get_fused": [_S, _pi],
     "ff_filter_set_graphs": [_S, _i],
     "ff_filter_get_graphs": [_S, _pi],
+    "ff_filter_get_wait_timeouts": [_S, ctypes.POINTER(ctypes.c_ulonglong)],
     # profiling
     "ff_profile_begin": [ctypes.c_char_p],
     "ff_profile_end": [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_longlong)],

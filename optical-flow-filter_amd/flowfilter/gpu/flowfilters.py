@@ -301,3 +301,10 @@ class PyramidalFlowFilter(Stage):
     @graphs.setter
     def graphs(self, v):
         N.call("ff_filter_set_graphs", self.handle, int(bool(v)))
+
+    def waitTimeouts(self):
+        """Waits of a fused frame that ended at the spin limit (0 when every
+        level saw the level above it complete). Synchronizes the device."""
+        v = ctypes.c_ulonglong(0)
+        N.call("ff_filter_get_wait_timeouts", self.handle, ctypes.byref(v))
+        return v.value

@@ -290,6 +290,7 @@ def test_cpp_api_demo(tmp_path):
     ((260, 400), 2, 6.0, [3, 2], 5),
 ])
 def test_fused_matches_oracle(ffo, shape, levels, maxflow, smooth, border):
+    from flowfilter import _native as N
     h, w = shape
     say("fused", shape, levels, maxflow, smooth, border)
     gamma = [10.0, 50.0, 80.0][:levels]
@@ -298,12 +299,21 @@ def test_fused_matches_oracle(ffo, shape, levels, maxflow, smooth, border):
     g.propagationBorder = border
     s.propagationBorder = border
     o.setPropagationBorder(border)
-    for t, img in enumerate(seq_blobs(h, w, 10)):
+    frames = 10
+    for t, img in enumerate(seq_blobs(h, w, frames)):
         for f in (g, s):
+            if f is g:
+                N.profile_begin("fused_frame")
             f.loadImage(img)
             f.compute()
+            if f is g:
+                _, n = N.profile_end()
+                # the first frame runs the stage kernels, every later frame
+                # one fused frame launch
+                assert n == (0 if t == 0 else 1), (t, n)
         o.compute(img)
         want = o.downloadFlow()
         check_frame(t, g.getFlow(), want, "fused")
         check_frame(t, s.getFlow(), want, "staged")
         check_frame(t, g.getImage(), o.downloadImage(), "fused image")
+    assert g.waitTimeouts() == 0
