@@ -292,6 +292,23 @@ FF_API ff_status ff_filter_get_graphs(ff_stage* s, int* out);
  * means the flow of those frames is not valid. Waits for the device. */
 FF_API ff_status ff_filter_get_wait_timeouts(ff_stage* s, unsigned long long* out);
 
+/* Batched filters (no reference counterpart; replaces a caller's loop of
+ * PyramidalFlowFilter::compute over camera streams, e.g. the per-camera
+ * filters of demos/highSpeedDemo/src/main.cpp:140-142). A batch holds
+ * shared references to PyramidalFlowFilters of any size and depth; each
+ * ff_stage_compute(batch) advances every filter by one frame with a single
+ * fused frame launch, replayed as a HIP graph (ff_filter_set_graphs and
+ * ff_filter_get_wait_timeouts accept a batch). Results are identical to
+ * calling ff_stage_compute on each filter. Frames are loaded, and results
+ * read, through the filters' own handles; ff_stage_elapsed_time(batch) is
+ * the time of the whole step. Destroy with ff_stage_destroy; a filter stays
+ * valid while its handle or a batch holding it exists. */
+FF_API ff_status ff_batch_create(ff_stage** out);
+FF_API ff_status ff_batch_add(ff_stage* batch, ff_stage* pyramidal_filter);
+FF_API ff_status ff_batch_size(ff_stage* batch, int* out);
+/* Workgroups of the last fused launch (diagnostic). */
+FF_API ff_status ff_batch_get_blocks(ff_stage* batch, int* out);
+
 /* Kernel profiling (no reference counterpart; the reference times whole
  * stages only, src/gpu/pipeline.cu:63-89). Between begin and end, every
  * kernel launch whose name contains `filter` is bracketed by HIP events on

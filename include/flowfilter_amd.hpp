@@ -413,6 +413,22 @@ public:
     }
 };
 
+// Many PyramidalFlowFilters advanced by one compute() in a single fused
+// launch (no reference counterpart; see ff_batch_create). The batch shares
+// ownership of the filters added; frames and results go through the
+// filters themselves.
+class BatchedFlowFilter : public Stage {
+public:
+    BatchedFlowFilter() {
+        ff_stage* s = nullptr;
+        check(ff_batch_create(&s));
+        reset(s);
+    }
+    void add(const PyramidalFlowFilter& f) { check(ff_batch_add(s_.get(), f.handle())); }
+    int size() const { return geti(ff_batch_size); }
+    void setGraphs(bool g) { check(ff_filter_set_graphs(s_.get(), g ? 1 : 0)); }
+};
+
 }  // namespace gpu
 }  // namespace flowfilter
 

@@ -19,7 +19,7 @@ struct ff_gpuimage {
     Image img;
 };
 struct ff_stage {
-    std::unique_ptr<Stage> obj;
+    std::shared_ptr<Stage> obj;  // shared with batches that hold the filter
 };
 
 namespace {
@@ -81,7 +81,7 @@ void new_image(const Image& img, ff_gpuimage** out) {
 template <class T, class... A>
 void new_stage(ff_stage** out, A&&... args) {
     out_ptr(out, "ff_stage**");
-    std::unique_ptr<Stage> obj(new T(std::forward<A>(args)...));
+    std::shared_ptr<Stage> obj = std::make_shared<T>(std::forward<A>(args)...);
     *out = new ff_stage{std::move(obj)};
 }
 
@@ -520,19 +520,51 @@ ff_status ff_filter_get_fused(ff_stage* s, int* out) {
 }
 
 ff_status ff_filter_set_graphs(ff_stage* s, int graphs) {
-    return guard([&] { as<PyramidalFlowFilter>(s, "PyramidalFlowFilter").setGraphs(graphs != 0); });
+    return guard([&] {
+        if (!s || !s->obj) throw null_handle("ff_stage");
+        if (auto* b = dynamic_cast<BatchedFlowFilter*>(s->obj.get())) b->setGraphs(graphs != 0);
+        else as<PyramidalFlowFilter>(s, "PyramidalFlowFilter").setGraphs(graphs != 0);
+    });
 }
 ff_status ff_filter_get_graphs(ff_stage* s, int* out) {
     return guard([&] {
         out_ptr(out, "graphs");
-        *out = as<PyramidalFlowFilter>(s, "PyramidalFlowFilter").graphs();
+        if (!s || !s->obj) throw null_handle("ff_stage");
+        if (auto* b = dynamic_cast<BatchedFlowFilter*>(s->obj.get())) *out = b->graphs();
+        else *out = as<PyramidalFlowFilter>(s, "PyramidalFlowFilter").graphs();
     });
 }
 
 ff_status ff_filter_get_wait_timeouts(ff_stage* s, unsigned long long* out) {
     return guard([&] {
         out_ptr(out, "timeouts");
-        *out = as<PyramidalFlowFilter>(s, "PyramidalFlowFilter").waitTimeouts();
+        if (!s || !s->obj) throw null_handle("ff_stage");
+        if (auto* b = dynamic_cast<BatchedFlowFilter*>(s->obj.get())) *out = b->waitTimeouts();
+        else *out = as<PyramidalFlowFilter>(s, "PyramidalFlowFilter").waitTimeouts();
+    });
+}
+
+// ------------------------------------------------------------------ batches
+ff_status ff_batch_create(ff_stage** out) {
+    return guard([&] { new_stage<BatchedFlowFilter>(out); });
+}
+ff_status ff_batch_add(ff_stage* batch, ff_stage* filter) {
+    return guard([&] {
+        BatchedFlowFilter& b = as<BatchedFlowFilter>(batch, "BatchedFlowFilter");
+        as<PyramidalFlowFilter>(filter, "PyramidalFlowFilter");
+        b.add(std::static_pointer_cast<PyramidalFlowFilter>(filter->obj));
+    });
+}
+ff_status ff_batch_size(ff_stage* batch, int* out) {
+    return guard([&] {
+        out_ptr(out, "size");
+        *out = as<BatchedFlowFilter>(batch, "BatchedFlowFilter").size();
+    });
+}
+ff_status ff_batch_get_blocks(ff_stage* batch, int* out) {
+    return guard([&] {
+        out_ptr(out, "blocks");
+        *out = as<BatchedFlowFilter>(batch, "BatchedFlowFilter").lastBlocks();
     });
 }
 

@@ -495,21 +495,28 @@ void GraphCache::clear() {
 // has advanced (the reference's update order); level L-2 reads the top
 // level's unsmoothed flow of this frame and waits for it inside the launch.
 // Every level writes its new state to its spare buffers, exchanged with the
-// state after the launch. With two levels the top level reduces the input
-// image while loading it, so the pyramid kernels do not run.
+// state after the launch.
 bool PyramidalFlowFilter::fusedPlan(std::vector<LevelArgs>& args) {
     args.clear();
     if (!configured_ || top_.firstLoad()) return false;
     for (const DeltaFlowFilter& d : low_)
         if (d.firstLoad()) return false;
+    // Levels 1 and 2 reduce the input image while loading it (one or two
+    // pyramid steps), so with up to three levels the pyramid kernels do not
+    // run; deeper pyramids read the pyramid's images.
+    const bool reduce = levels_ <= 3;
     LevelArgs top = top_.fusedArgs();
-    if (levels_ == 2 && top.img_mode == IMG_U8) {
+    if (reduce && levels_ > 1 && top.img_mode == IMG_U8) {
         top.img = input_.view();
-        top.img_mode = IMG_U8_DOWN;
+        top.img_mode = levels_ == 2 ? IMG_U8_DOWN : IMG_U8_DOWN2;
     }
     args.push_back(top);
     for (int l = 0; l < levels_ - 1; ++l) {
         LevelArgs a = low_[l].fusedArgs();
+        if (reduce && l == 1 && a.img_mode == IMG_U8) {
+            a.img = input_.view();
+            a.img_mode = IMG_U8_DOWN;
+        }
         if (l == levels_ - 2) a.wait_on = 0;
         args.push_back(a);
     }
@@ -518,40 +525,58 @@ bool PyramidalFlowFilter::fusedPlan(std::vector<LevelArgs>& args) {
     return true;
 }
 
+void PyramidalFlowFilter::enqueuePreFused() {
+    if (levels_ > 3) pyramid_.run();
+}
+
+void PyramidalFlowFilter::commitFused() {
+    top_.fusedCommit();
+    for (int l = 0; l < levels_ - 1; ++l) low_[l].fusedCommit();
+}
+
 void PyramidalFlowFilter::enqueueFused(int table, hipStream_t s) {
-    if (levels_ > 2) pyramid_.run();
+    enqueuePreFused();
     frame_.launch(table, s);
     check_launch("PyramidalFlowFilter::compute (fused frame)");
 }
 
+namespace {
+// Enqueue `work` on s, through a graph of `cache` keyed by (args, token)
+// when graphs are on and no kernel profile is open (profiling brackets each
+// launch with events, which a replay would not do).
+template <class F>
+void run_maybe_graphed(bool graphs_on, GraphCache& cache, const std::vector<LevelArgs>& args,
+                       unsigned long long token, hipStream_t s, F work) {
+    if (!graphs_on || profiling_active()) {
+        work();
+        return;
+    }
+    hipGraphExec_t exec = cache.find(args, token);
+    if (!exec) {
+        FF_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        hipGraph_t graph = nullptr;
+        try {
+            work();
+        } catch (...) {
+            (void)hipStreamEndCapture(s, &graph);
+            if (graph) (void)hipGraphDestroy(graph);
+            throw;
+        }
+        FF_HIP(hipStreamEndCapture(s, &graph));
+        const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) throw_hip(e, "hipGraphInstantiate", __FILE__, __LINE__);
+        cache.insert(args, token, exec);
+    }
+    FF_HIP(hipGraphLaunch(exec, s));
+}
+}  // namespace
+
 void PyramidalFlowFilter::computeFused(const std::vector<LevelArgs>& args) {
     const int table = frame_.prepare(args);
-    const unsigned long long token = frame_.token(table);
     hipStream_t s = stream();
-    if (graphs_on_ && !profiling_active()) {
-        hipGraphExec_t exec = graphs_.find(args, token);
-        if (!exec) {
-            FF_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-            hipGraph_t graph = nullptr;
-            try {
-                enqueueFused(table, s);
-            } catch (...) {
-                (void)hipStreamEndCapture(s, &graph);
-                if (graph) (void)hipGraphDestroy(graph);
-                throw;
-            }
-            FF_HIP(hipStreamEndCapture(s, &graph));
-            const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(graph);
-            if (e != hipSuccess) throw_hip(e, "hipGraphInstantiate", __FILE__, __LINE__);
-            graphs_.insert(args, token, exec);
-        }
-        FF_HIP(hipGraphLaunch(exec, s));
-    } else {
-        enqueueFused(table, s);
-    }
-    top_.fusedCommit();
-    for (int l = 0; l < levels_ - 1; ++l) low_[l].fusedCommit();
+    run_maybe_graphed(graphs_on_, graphs_, args, frame_.token(table), s, [&] { enqueueFused(table, s); });
+    commitFused();
 }
 
 void PyramidalFlowFilter::compute() {
@@ -642,6 +667,57 @@ void PyramidalFlowFilter::setMaxFlow(float m) {
 void PyramidalFlowFilter::setPropagationBorder(int b) {
     top_.setPropagationBorder(b);
     for (auto& d : low_) d.setPropagationBorder(b);
+}
+
+// ------------------------------------------------------- BatchedFlowFilter
+
+int BatchedFlowFilter::add(const std::shared_ptr<PyramidalFlowFilter>& f) {
+    if (!f) throw invalid_argument("BatchedFlowFilter::add(): null filter");
+    for (const auto& g : streams_)
+        if (g == f) throw invalid_argument("BatchedFlowFilter::add(): filter already in the batch");
+    streams_.push_back(f);
+    graphs_.clear();
+    return (int)streams_.size() - 1;
+}
+
+const std::shared_ptr<PyramidalFlowFilter>& BatchedFlowFilter::stream(int i) const {
+    if (i < 0 || i >= (int)streams_.size())
+        throw invalid_argument("BatchedFlowFilter::stream(): index out of range: " + num(i));
+    return streams_[i];
+}
+
+// One frame step of every stream: the jobs of all streams whose next frame
+// can run fused go into one launch (their level jobs keep their order and
+// their wait on their own top level); the others run their stage kernels.
+void BatchedFlowFilter::compute() {
+    if (streams_.empty()) throw logic_error("BatchedFlowFilter::compute(): no streams");
+    startTiming();
+    std::vector<LevelArgs> all, one;
+    std::vector<PyramidalFlowFilter*> fused;
+    for (const auto& f : streams_) {
+        if (f->fused() && f->fusedPlan(one)) {
+            const int off = (int)all.size();
+            for (LevelArgs a : one) {
+                if (a.wait_on >= 0) a.wait_on += off;
+                all.push_back(a);
+            }
+            fused.push_back(f.get());
+        } else {
+            f->computeStaged();
+        }
+    }
+    if (!all.empty()) {
+        const int table = frame_.prepare(all);
+        last_blocks_ = frame_.blocks(table);
+        hipStream_t s = ffamd::stream();
+        run_maybe_graphed(graphs_on_, graphs_, all, frame_.token(table), s, [&] {
+            for (PyramidalFlowFilter* f : fused) f->enqueuePreFused();
+            frame_.launch(table, s);
+            check_launch("BatchedFlowFilter::compute (fused frame)");
+        });
+        for (PyramidalFlowFilter* f : fused) f->commitFused();
+    }
+    stopTiming();
 }
 
 }  // namespace ffamd

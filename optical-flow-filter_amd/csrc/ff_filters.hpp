@@ -11,6 +11,7 @@
 //    with the level state ping-ponged between two buffers.
 #pragma once
 
+#include <memory>
 #include <vector>
 
 #include "ff_stages.hpp"
@@ -146,6 +147,8 @@ private:
     int next_ = 0;
 };
 
+class BatchedFlowFilter;
+
 class PyramidalFlowFilter : public Stage {
 public:
     PyramidalFlowFilter() = default;
@@ -182,8 +185,13 @@ public:
     unsigned long long waitTimeouts() const { return frame_.timeouts(); }
 
 private:
+    friend class BatchedFlowFilter;
     void checkLevel(int level, const char* who) const;
     void computeStaged();
+    // Pyramid kernels a fused frame needs before its launch (deeper pyramids)
+    // and the state exchange after it.
+    void enqueuePreFused();
+    void commitFused();
     // Level arguments of a fused frame; false when some level is not ready
     // (first frame) or not covered by the fused kernel.
     bool fusedPlan(std::vector<LevelArgs>& args);
@@ -197,6 +205,37 @@ private:
     ImagePyramid pyramid_;
     FlowFilter top_;
     std::vector<DeltaFlowFilter> low_;
+};
+
+// Many PyramidalFlowFilters (camera streams of any size and depth) advanced
+// together: compute() moves every stream one frame forward with a single
+// fused frame launch (replayed as a HIP graph); each stream's results are
+// identical to its own compute(). Streams in their first frame run their
+// stage kernels first. No reference counterpart.
+class BatchedFlowFilter : public Stage {
+public:
+    BatchedFlowFilter() = default;
+    void configure() override {}
+    void compute() override;
+    const char* name() const override { return "BatchedFlowFilter"; }
+    int add(const std::shared_ptr<PyramidalFlowFilter>& f);
+    int size() const { return (int)streams_.size(); }
+    const std::shared_ptr<PyramidalFlowFilter>& stream(int i) const;
+    void setGraphs(bool g) {
+        graphs_on_ = g;
+        graphs_.clear();
+    }
+    bool graphs() const { return graphs_on_; }
+    unsigned long long waitTimeouts() const { return frame_.timeouts(); }
+    // Workgroups of the last fused launch (0 before the first).
+    int lastBlocks() const { return last_blocks_; }
+
+private:
+    std::vector<std::shared_ptr<PyramidalFlowFilter>> streams_;
+    bool graphs_on_ = true;
+    int last_blocks_ = 0;
+    GraphCache graphs_;
+    FusedFrame frame_;
 };
 
 }  // namespace ffamd

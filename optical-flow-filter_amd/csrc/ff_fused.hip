@@ -30,8 +30,9 @@
 //  - X neighbours come from DPP wave shifts; Y neighbours are registers of
 //    the same thread, and the rows above and below a wave's strip are
 //    exchanged through LDS once per Y pass.
-//  - Every buffer lies inside the image: buffers at the image edges are
-//    clamped against the edge and their output range extends to it. At an
+//  - On levels at least as large as the buffer every buffer lies inside the
+//    image: buffers at the image edges are clamped against the edge and
+//    their output range extends to it. At an
 //    image edge the missing neighbour must be the clamp-to-edge read of the
 //    reference: the first/last wave use their own first/last row, and the
 //    x taps of the image model and smoothing use DPP shifts in which the
@@ -44,7 +45,10 @@
 //    buffer's size, which the hardware discards (no branches).
 //  - Image rows, the upper-level flow and the old brightness are loaded
 //    before they are needed so their latency overlaps computation.
-//  The fused path therefore needs W >= 64, H >= NW*R and border >= 1.
+//  - A level narrower or shorter than the buffer is a single tile across or
+//    down; the buffer then extends past the image's last column or row,
+//    whose values the loads replicate (see level_tile).
+//  The fused path needs border >= 1.
 //
 // Arithmetic per pixel is the reference's (same operations, order and
 // rounding points as the stage kernels and the CPU oracle); results are
@@ -233,6 +237,8 @@ __device__ __forceinline__ void signal_done(const LevelJob& p, unsigned* counter
 // Per-thread view of the tile.
 struct Geo {
     int lane, wave, x, yw;
+    int xl;        // column read: x, or the last column for x beyond the image
+    int xlast;     // lane of the image's last column (>= LANES - 1: not in the buffer)
     bool xin;      // column strictly inside the propagation border
     bool wave_in;  // every pixel of the wave strictly inside the border
 };
@@ -396,6 +402,18 @@ __device__ __forceinline__ void model_update(State<DELTA, R>& s, const float (&i
     }
 }
 
+// Lanes beyond the image's last column (lane xlast) take its values.
+template <int R>
+__device__ __forceinline__ void copy_last_column(float (&fu)[R], float (&fv)[R], int lane, int xlast) {
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const float eu = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(fu[r]), xlast));
+        const float ev = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(fv[r]), xlast));
+        fu[r] = lane > xlast ? eu : fu[r];
+        fv[r] = lane > xlast ? ev : fv[r];
+    }
+}
+
 // X pass of the box smoother (flowSmoothX_k, flowsmoothing_k.cu:21-53).
 template <bool CLAMP, int R>
 __device__ __forceinline__ void smooth_x(float (&fu)[R], float (&fv)[R]) {
@@ -403,6 +421,46 @@ __device__ __forceinline__ void smooth_x(float (&fu)[R], float (&fv)[R]) {
     for (int r = 0; r < R; ++r) {
         fu[r] = taps_x<CLAMP>(fu[r], WBox{});
         fv[r] = taps_x<CLAMP>(fv[r], WBox{});
+    }
+}
+
+// Y pass of the box smoother (flowSmoothY_k, flowsmoothing_k.cu:55-87) over
+// the thread's rows; a2/a1 are the rows above, b1/b2 the rows below. With
+// YCLAMP, rows after rlast (outside the image) are replaced by row rlast, the
+// clamped read; rows after rlast themselves are not needed.
+template <bool YCLAMP, int R>
+__device__ __forceinline__ void smooth_y(float (&fu)[R], float (&fv)[R], float a2u, float a2v, float a1u, float a1v,
+                                         float b1u, float b1v, float b2u, float b2v, int rlast) {
+    float m2u = a2u, m2v = a2v, m1u = a1u, m1v = a1v;  // old rows r-2, r-1
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const float zu = fu[r], zv = fv[r];
+        float p1u = r + 1 < R ? fu[r + 1] : b1u, p1v = r + 1 < R ? fv[r + 1] : b1v;
+        float p2u = r + 2 < R ? fu[r + 2] : (r + 2 == R ? b1u : b2u);
+        float p2v = r + 2 < R ? fv[r + 2] : (r + 2 == R ? b1v : b2v);
+        if constexpr (YCLAMP) {
+            p2u = r + 2 <= rlast ? p2u : (r + 1 <= rlast ? p1u : zu);
+            p2v = r + 2 <= rlast ? p2v : (r + 1 <= rlast ? p1v : zv);
+            p1u = r + 1 <= rlast ? p1u : zu;
+            p1v = r + 1 <= rlast ? p1v : zv;
+        }
+        float su = 0.0f, sv = 0.0f;
+        su = fmaf(kBox, m2u, su);
+        sv = fmaf(kBox, m2v, sv);
+        su = fmaf(kBox, m1u, su);
+        sv = fmaf(kBox, m1v, sv);
+        su = fmaf(kBox, zu, su);
+        sv = fmaf(kBox, zv, sv);
+        su = fmaf(kBox, p1u, su);
+        sv = fmaf(kBox, p1v, sv);
+        su = fmaf(kBox, p2u, su);
+        sv = fmaf(kBox, p2v, sv);
+        m2u = m1u;
+        m2v = m1v;
+        m1u = zu;
+        m1v = zv;
+        fu[r] = su;
+        fv[r] = sv;
     }
 }
 
@@ -450,20 +508,54 @@ __device__ __forceinline__ void load_image(ImageRegs<MODE, R>& ir, const LevelJo
 #pragma unroll
         for (int j = 0; j < R + 4; ++j) {
             const int yc = clampi(g.yw - 2 + j, 0, p.H - 1);
-            ir.v[j] = __builtin_amdgcn_raw_buffer_load_b32(rimg, yc * p.img_pitch + g.x * 4, 0, 0);
+            ir.v[j] = __builtin_amdgcn_raw_buffer_load_b32(rimg, yc * p.img_pitch + g.xl * 4, 0, 0);
         }
         return;
     } else {
         unsigned t[R + 4];
-        if constexpr (MODE == IMG_U8_DOWN) {
+        if constexpr (MODE == IMG_U8_DOWN2) {
+            // source rows 4yy-3 .. 4yy+3 (level-1 rows 2yy-1 .. 2yy+1, each
+            // from its three source rows), one 32-bit load of columns
+            // 4x .. 4x+3 per row
+            const int H1 = p.Hsrc / 2;
+#pragma unroll
+            for (int j = 0; j < R + 4; ++j) {
+                const int yy = clampi(g.yw - 2 + j, 0, p.H - 1);
+                unsigned l1a[3], l1b[3];  // level-1 texels at columns 2x, 2x+1 of rows 2yy-1+k
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    const int r1 = clampi(2 * yy - 1 + k, 0, H1 - 1);
+                    unsigned xa[3], xb[3];
+#pragma unroll
+                    for (int m = 0; m < 3; ++m) {
+                        const int r0 = max(2 * r1 - 1 + m, 0);
+                        const unsigned wd = __builtin_amdgcn_raw_buffer_load_b32(rimg, r0 * p.img_pitch + 4 * g.xl, 0, 0);
+                        const unsigned b0 = wd & 0xffu, b1 = (wd >> 8) & 0xffu, b2 = (wd >> 16) & 0xffu, b3 = wd >> 24;
+                        const unsigned left = (shr_or(wd, wd << 24) >> 24) & 0xffu;  // column 4x-1 (clamped)
+                        xa[m] = down_u8(left, b0, b1);
+                        xb[m] = down_u8(b1, b2, b3);
+                    }
+                    l1a[k] = down_u8(xa[0], xa[1], xa[2]);
+                    l1b[k] = down_u8(xb[0], xb[1], xb[2]);
+                }
+                unsigned yv[3];
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    // level-1 column 2x-1 is the left lane's 2x+1 (clamped at x = 0)
+                    const unsigned lm = shr_or(l1b[k], l1a[k]);
+                    yv[k] = down_u8(lm, l1a[k], l1b[k]);
+                }
+                t[j] = down_u8(yv[0], yv[1], yv[2]);
+            }
+        } else if constexpr (MODE == IMG_U8_DOWN) {
             unsigned src[R + 4][3];
 #pragma unroll
             for (int j = 0; j < R + 4; ++j) {
                 const int r1 = 2 * clampi(g.yw - 2 + j, 0, p.H - 1);
                 const int r0 = r1 > 0 ? r1 - 1 : 0;
-                src[j][0] = ldh(rimg, r0 * p.img_pitch + 2 * g.x);
-                src[j][1] = ldh(rimg, r1 * p.img_pitch + 2 * g.x);
-                src[j][2] = ldh(rimg, (r1 + 1) * p.img_pitch + 2 * g.x);
+                src[j][0] = ldh(rimg, r0 * p.img_pitch + 2 * g.xl);
+                src[j][1] = ldh(rimg, r1 * p.img_pitch + 2 * g.xl);
+                src[j][2] = ldh(rimg, (r1 + 1) * p.img_pitch + 2 * g.xl);
             }
 #pragma unroll
             for (int j = 0; j < R + 4; ++j) t[j] = down_u8(down_x(src[j][0]), down_x(src[j][1]), down_x(src[j][2]));
@@ -471,7 +563,18 @@ __device__ __forceinline__ void load_image(ImageRegs<MODE, R>& ir, const LevelJo
 #pragma unroll
             for (int j = 0; j < R + 4; ++j) {
                 const int yc = clampi(g.yw - 2 + j, 0, p.H - 1);
-                t[j] = ldb(rimg, yc * p.img_pitch + g.x);
+                t[j] = ldb(rimg, yc * p.img_pitch + g.xl);
+            }
+        }
+        if constexpr (MODE == IMG_U8_DOWN || MODE == IMG_U8_DOWN2) {
+            // lanes beyond the last column: copies of it (their own reduction
+            // would combine a clamped centre with an unclamped left column)
+            if (g.xlast < LANES - 1) {
+#pragma unroll
+                for (int j = 0; j < R + 4; ++j) {
+                    const unsigned e = (unsigned)__builtin_amdgcn_readlane((int)t[j], g.xlast);
+                    t[j] = g.lane > g.xlast ? e : t[j];
+                }
             }
         }
 #pragma unroll
@@ -512,11 +615,25 @@ __device__ __forceinline__ void level_tile(const LevelJob& p, int tile, float (&
     const int x0 = clampi(ox0 - p.h0, 0, p.W - LANES);
     const int y0 = clampi(oy0 - p.h0, 0, p.H - NW * R);
     g.x = x0 + g.lane;
+    g.xl = min(g.x, p.W - 1);
     g.yw = y0 + g.wave * R;  // image row of this thread's first row (wave-uniform)
     const int b = p.border;
     g.xin = g.x >= b && g.x < p.W - b;
     g.wave_in = (x0 >= b) && (x0 + LANES - 1 < p.W - b) && (g.yw >= b) && (g.yw + R - 1 < p.H - b);
-    const bool xedge = (x0 == 0) || (x0 + LANES == p.W);
+    // an image edge at or inside the buffer's left/right edge
+    const bool xedge = (x0 == 0) || (x0 + LANES >= p.W);
+    // Images narrower than the buffer (a single tile across): the lanes
+    // beyond the last column hold copies of it, loaded by clamped reads, so
+    // the image model's taps and the propagation (which keeps pixels within
+    // the border) see the reference's clamp-to-edge reads; the smoothing
+    // passes re-copy the last column before every X pass and clamp their row
+    // neighbours at the last row (rlast: the thread's last row inside the
+    // image, relative to its first row).
+    const int xlast = p.W - 1 - x0;  // lane of the last column
+    const bool xinside = xlast < LANES - 1;
+    g.xlast = xlast;
+    const int rlast = p.H - 1 - g.yw;
+    const bool yinside = y0 + NW * R > p.H;
     const int lane = g.lane, wave = g.wave;
     const bool out_x = g.x >= ox0 && g.x < ox1;
 
@@ -530,13 +647,13 @@ __device__ __forceinline__ void level_tile(const LevelJob& p, int tile, float (&
     // ------------------------------------- load the state and the image rows
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-        const int y = g.yw + r;
-        const float2 f = ld2(rfi, y * p.flow_in_pitch + g.x * 8);
+        const int y = min(g.yw + r, p.H - 1);
+        const float2 f = ld2(rfi, y * p.flow_in_pitch + g.xl * 8);
         s.fu[r] = f.x;
         s.fv[r] = f.y;
         if constexpr (DELTA) {
-            s.sa[r] = ld1(rai, y * p.a0_in_pitch + g.x * 4);
-            const float2 d = ld2(rsrc(p.dflow_in, p.dflow_in_pitch * p.H), y * p.dflow_in_pitch + g.x * 8);
+            s.sa[r] = ld1(rai, y * p.a0_in_pitch + g.xl * 4);
+            const float2 d = ld2(rsrc(p.dflow_in, p.dflow_in_pitch * p.H), y * p.dflow_in_pitch + g.xl * 8);
             s.du[r] = d.x;
             s.dv[r] = d.y;
         }
@@ -609,20 +726,21 @@ __device__ __forceinline__ void level_tile(const LevelJob& p, int tile, float (&
     int offa[R], offf[R];
     const Rsrc rdf = rsrc(p.dflow_out, p.dflow_out_pitch * p.H);
     const Rsrc rfu = rsrc(p.flow_upd, p.flow_upd_pitch * p.H);
-    const int xmx = DELTA ? p.xmap[g.x] : 0;
+    const int xmx = DELTA ? p.xmap[g.xl] : 0;
     const Rsrc rup = rsrc(p.upper, p.upper_pitch * p.Hu);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int y = g.yw + r;
         const bool out = out_x && y >= oy0 && y < oy1;
+        const int yl = min(y, p.H - 1);
         offa[r] = out ? y * p.a0_out_pitch + g.x * 4 : OOB;
         if constexpr (DELTA) {
             offf[r] = out ? y * p.dflow_out_pitch + g.x * 8 : OOB;
-            up[r] = ld2(rup, p.ymap[y] * p.upper_pitch + xmx * 8);
+            up[r] = ld2(rup, p.ymap[yl] * p.upper_pitch + xmx * 8);
             aold[r] = 0.f;
         } else {
             offf[r] = out ? y * p.flow_upd_pitch + g.x * 8 : OOB;
-            aold[r] = ld1(rai, y * p.a0_in_pitch + g.x * 4);
+            aold[r] = ld1(rai, yl * p.a0_in_pitch + g.xl * 4);
             up[r] = make_float2(0.f, 0.f);
         }
     }
@@ -634,6 +752,7 @@ __device__ __forceinline__ void level_tile(const LevelJob& p, int tile, float (&
     // ------------------------------------------------------------ smoothing
     // flowSmoothX_k / flowSmoothY_k (flowsmoothing_k.cu:21-87), K times.
     for (int it = 0; it < p.K; ++it) {
+        if (xinside) copy_last_column<R>(s.fu, s.fv, g.lane, xlast);
         if (xedge) smooth_x<true, R>(s.fu, s.fv);
         else smooth_x<false, R>(s.fu, s.fv);
         {
@@ -666,31 +785,8 @@ __device__ __forceinline__ void level_tile(const LevelJob& p, int tile, float (&
             b2v = o[3 * LANES + lane];
         }
         slot ^= 1;
-        float m2u = a2u, m2v = a2v, m1u = a1u, m1v = a1v;  // old rows r-2, r-1
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const float zu = s.fu[r], zv = s.fv[r];
-            const float p1u = r + 1 < R ? s.fu[r + 1] : b1u, p1v = r + 1 < R ? s.fv[r + 1] : b1v;
-            const float p2u = r + 2 < R ? s.fu[r + 2] : (r + 2 == R ? b1u : b2u);
-            const float p2v = r + 2 < R ? s.fv[r + 2] : (r + 2 == R ? b1v : b2v);
-            float su = 0.0f, sv = 0.0f;
-            su = fmaf(kBox, m2u, su);
-            sv = fmaf(kBox, m2v, sv);
-            su = fmaf(kBox, m1u, su);
-            sv = fmaf(kBox, m1v, sv);
-            su = fmaf(kBox, zu, su);
-            sv = fmaf(kBox, zv, sv);
-            su = fmaf(kBox, p1u, su);
-            sv = fmaf(kBox, p1v, sv);
-            su = fmaf(kBox, p2u, su);
-            sv = fmaf(kBox, p2v, sv);
-            m2u = m1u;
-            m2v = m1v;
-            m1u = zu;
-            m1v = zv;
-            s.fu[r] = su;
-            s.fv[r] = sv;
-        }
+        if (yinside) smooth_y<true, R>(s.fu, s.fv, a2u, a2v, a1u, a1v, b1u, b1v, b2u, b2v, rlast);
+        else smooth_y<false, R>(s.fu, s.fv, a2u, a2v, a1u, a1v, b1u, b1v, b2u, b2v, rlast);
     }
 
     // ------------------------------------------------- write the new state
@@ -703,25 +799,32 @@ __device__ __forceinline__ void level_tile(const LevelJob& p, int tile, float (&
     if (!DELTA && p.signal_idx >= 0) signal_done(p, counters);
 }
 
-// Blocks are grouped by tile body: top levels with float, u8 and reduced u8
-// images, then delta levels with float and u8 images; ends[i] is the first
-// block after group i. (Selecting the body from the block index rather than
-// from a job field also keeps the compiler's register allocation of each
-// body as tight as in a kernel of its own.)
+// Blocks are grouped by tile body: top levels with float, u8, reduced u8
+// and twice-reduced u8 images, then delta levels with float, u8 and reduced
+// u8 images; ends[i] is the first block after group i. (Selecting the body
+// from the block index rather than from a job field also keeps the
+// compiler's register allocation of each body as tight as in a kernel of
+// its own.)
+struct GroupEnds {
+    int e[6];
+};
+
 template <int R, int NW>
 __global__ void __launch_bounds__(NW* LANES)
     k_fused_frame(const LevelJob* __restrict__ jobs, const int* __restrict__ block_job, unsigned* counters,
-                  int timeout_idx, int end0, int end1, int end2, int end3) {
+                  int timeout_idx, GroupEnds ends) {
     // row exchange between the waves of the workgroup, double buffered
     __shared__ float xch[2][NW][10][LANES];
     const LevelJob& p = jobs[block_job[blockIdx.x]];
     const int tile = (int)blockIdx.x - p.first_block;
     const unsigned b = blockIdx.x;
-    if (b < (unsigned)end0) level_tile<false, IMG_F32, R, NW>(p, tile, xch, counters, timeout_idx);
-    else if (b < (unsigned)end1) level_tile<false, IMG_U8, R, NW>(p, tile, xch, counters, timeout_idx);
-    else if (b < (unsigned)end2) level_tile<false, IMG_U8_DOWN, R, NW>(p, tile, xch, counters, timeout_idx);
-    else if (b < (unsigned)end3) level_tile<true, IMG_F32, R, NW>(p, tile, xch, counters, timeout_idx);
-    else level_tile<true, IMG_U8, R, NW>(p, tile, xch, counters, timeout_idx);
+    if (b < (unsigned)ends.e[0]) level_tile<false, IMG_F32, R, NW>(p, tile, xch, counters, timeout_idx);
+    else if (b < (unsigned)ends.e[1]) level_tile<false, IMG_U8, R, NW>(p, tile, xch, counters, timeout_idx);
+    else if (b < (unsigned)ends.e[2]) level_tile<false, IMG_U8_DOWN, R, NW>(p, tile, xch, counters, timeout_idx);
+    else if (b < (unsigned)ends.e[3]) level_tile<false, IMG_U8_DOWN2, R, NW>(p, tile, xch, counters, timeout_idx);
+    else if (b < (unsigned)ends.e[4]) level_tile<true, IMG_F32, R, NW>(p, tile, xch, counters, timeout_idx);
+    else if (b < (unsigned)ends.e[5]) level_tile<true, IMG_U8, R, NW>(p, tile, xch, counters, timeout_idx);
+    else level_tile<true, IMG_U8_DOWN, R, NW>(p, tile, xch, counters, timeout_idx);
 }
 
 #ifndef FF_FUSED_R
@@ -740,7 +843,7 @@ void level_size(const LevelArgs& a, int& H, int& W) {
 }
 
 int halo_of(const LevelArgs& a) {
-    const int lo = a.img_mode == IMG_U8_DOWN ? 3 : 2;
+    const int lo = (a.img_mode == IMG_U8_DOWN || a.img_mode == IMG_U8_DOWN2) ? 3 : 2;
     return (a.prop_iters > lo ? a.prop_iters : lo) + 2 * a.smooth_iters;
 }
 
@@ -753,10 +856,10 @@ bool fits_i32(const DevImg& d) {
 bool fused_level_supported(const LevelArgs& a) {
     const int h0 = halo_of(a);
     if (a.border < 1 || a.prop_iters < 1 || a.smooth_iters < 1) return false;
-    if (2 * h0 >= LANES || 2 * h0 >= kNW * kR) return false;
     int H, W;
     level_size(a, H, W);
-    if (W < LANES || H < kNW * kR) return false;
+    if (H < 1 || W < 1) return false;
+    if ((W > LANES && 2 * h0 >= LANES) || (H > kNW * kR && 2 * h0 >= kNW * kR)) return false;
     if (!a.img.data || !a.flow_s_in.data || !a.a0_in.data || !a.flow_s_out.data || !a.a0_out.data) return false;
     switch (a.img_mode) {
     case IMG_F32:
@@ -764,7 +867,10 @@ bool fused_level_supported(const LevelArgs& a) {
         if (a.img.h != H || a.img.w != W) return false;
         break;
     case IMG_U8_DOWN:
-        if (a.img.h / 2 != H || a.img.w / 2 != W || a.delta) return false;
+        if (a.img.h / 2 != H || a.img.w / 2 != W) return false;
+        break;
+    case IMG_U8_DOWN2:
+        if ((a.img.h / 2) / 2 != H || (a.img.w / 2) / 2 != W || a.delta) return false;
         break;
     default:
         return false;
@@ -793,7 +899,7 @@ struct FusedFrame::Impl {
         std::vector<char> key;  // job records + block map, as uploaded
         DeviceArray dev;
         int njobs = 0, blocks = 0;
-        int ends[4] = {0, 0, 0, 0};
+        GroupEnds ends{};
         size_t map_offset = 0;
         unsigned long long token = 0;
     };
@@ -814,9 +920,10 @@ FusedFrame& FusedFrame::operator=(FusedFrame&& o) noexcept {
 
 namespace {
 // Tile body group of a job (the order of the kernel's dispatch).
+constexpr int kGroups = 7;
 int body_group(const LevelArgs& a) {
-    if (!a.delta) return a.img_mode == IMG_F32 ? 0 : (a.img_mode == IMG_U8 ? 1 : 2);
-    return a.img_mode == IMG_F32 ? 3 : 4;
+    if (!a.delta) return a.img_mode;  // IMG_F32 .. IMG_U8_DOWN2: groups 0 .. 3
+    return 4 + a.img_mode;            // IMG_F32 .. IMG_U8_DOWN: groups 4 .. 6
 }
 }  // namespace
 
@@ -831,7 +938,7 @@ int FusedFrame::prepare(const std::vector<LevelArgs>& jobs) {
     for (size_t i = 0; i < order.size(); ++i) pos[order[i]] = (int)i;
     std::vector<LevelJob> rec(jobs.size());
     int blocks = 0;
-    int ends[4] = {0, 0, 0, 0};
+    int ends[kGroups - 1] = {0, 0, 0, 0, 0, 0};
     for (size_t i = 0; i < jobs.size(); ++i) {
         const LevelArgs& a = jobs[order[i]];
         if (!fused_level_supported(a)) throw std::logic_error("fused frame: unsupported level parameters");
@@ -864,8 +971,9 @@ int FusedFrame::prepare(const std::vector<LevelArgs>& jobs) {
         p.K = a.smooth_iters;
         p.border = a.border;
         p.h0 = halo_of(a);
-        p.TW = LANES - 2 * p.h0;
-        p.TH = kNW * kR - 2 * p.h0;
+        // a level no wider (taller) than the buffer is one tile across (down)
+        p.TW = p.W <= LANES ? p.W : LANES - 2 * p.h0;
+        p.TH = p.H <= kNW * kR ? p.H : kNW * kR - 2 * p.h0;
         p.tiles_x = (p.W + p.TW - 1) / p.TW;
         p.tiles_y = (p.H + p.TH - 1) / p.TH;
         p.delta = a.delta;
@@ -877,7 +985,7 @@ int FusedFrame::prepare(const std::vector<LevelArgs>& jobs) {
         p.gamma = a.gamma;
         p.maxflow = a.maxflow;
         blocks += p.tiles_x * p.tiles_y;
-        for (int g = body_group(a); g < 4; ++g) ends[g] = blocks;
+        for (int g = body_group(a); g < kGroups - 1; ++g) ends[g] = blocks;
     }
     for (size_t i = 0; i < jobs.size(); ++i) {
         const int k = jobs[i].wait_on;
@@ -907,7 +1015,7 @@ int FusedFrame::prepare(const std::vector<LevelArgs>& jobs) {
     std::memcpy(key.data(), rec.data(), jobs_bytes);
     std::memcpy(key.data() + map_offset, map.data(), map.size() * sizeof(int));
     // the group ends are kernel arguments, part of the table's identity
-    for (int g = 0; g < 4; ++g) {
+    for (int g = 0; g < kGroups - 1; ++g) {
         const char* e = reinterpret_cast<const char*>(&ends[g]);
         key.insert(key.end(), e, e + sizeof(int));
     }
@@ -933,7 +1041,7 @@ int FusedFrame::prepare(const std::vector<LevelArgs>& jobs) {
     t.njobs = (int)rec.size();
     t.blocks = blocks;
     t.map_offset = map_offset;
-    for (int g = 0; g < 4; ++g) t.ends[g] = ends[g];
+    for (int g = 0; g < kGroups - 1; ++g) t.ends.e[g] = ends[g];
     t.token = ++impl_->uploads;
     impl_->next ^= 1;
     return s;
@@ -950,7 +1058,7 @@ void FusedFrame::launch(int table, hipStream_t s) const {
     const int timeout_idx = 2 * (impl_->ncounters - 1);
     hipLaunchKernelGGL((k_fused_frame<kR, kNW>), dim3(t.blocks), dim3(kNW * LANES), 0, s,
                        (const LevelJob*)base, (const int*)(base + t.map_offset), (unsigned*)impl_->counters.data(),
-                       timeout_idx, t.ends[0], t.ends[1], t.ends[2], t.ends[3]);
+                       timeout_idx, t.ends);
 }
 
 unsigned long long FusedFrame::timeouts() const {

@@ -76,6 +76,8 @@ enum ImgMode : int {
     IMG_U8 = 1,       // u8 image of the level's size (normalized read)
     IMG_U8_DOWN = 2,  // u8 image of twice the level's size, reduced by one
                       // pyramid step (imageDown{X,Y}_uint8_k) while loading
+    IMG_U8_DOWN2 = 3, // u8 image of four times the level's size, reduced by
+                      // two pyramid steps while loading
 };
 
 // Parameters of one level of one frame.

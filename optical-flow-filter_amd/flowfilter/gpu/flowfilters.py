@@ -6,8 +6,9 @@ getFlowDevice and the gamma/maxflow/smoothIterations/propagationBorder
 properties. Also exposes DeltaFlowFilter (C++ only in the reference).
 
 Additions without a reference counterpart: ``loadImageDevice`` (input from
-device memory, e.g. another GPUImage's data pointer) and the ``fused`` property
-selecting fused per-level kernels (default) or one launch per stage kernel.
+device memory, e.g. another GPUImage's data pointer), the ``fused`` property
+selecting the fused frame kernel (default) or one launch per stage kernel,
+and ``BatchedFlowFilter`` (many filters advanced by one launch per frame).
 """
 import ctypes
 
@@ -304,6 +305,57 @@ class PyramidalFlowFilter(Stage):
 
     def waitTimeouts(self):
         """Waits of a fused frame that ended at the spin limit (0 when every
+        level saw the level above it complete). Synchronizes the device."""
+        v = ctypes.c_ulonglong(0)
+        N.call("ff_filter_get_wait_timeouts", self.handle, ctypes.byref(v))
+        return v.value
+
+
+class BatchedFlowFilter(Stage):
+    """Many PyramidalFlowFilters advanced by one compute() (no reference
+    counterpart; replaces a loop of compute() calls over camera streams).
+
+    Every filter added keeps its own parameters, input and outputs (load
+    frames with its loadImage/loadImageDevice, read results with its
+    getFlow); compute() moves all of them one frame forward with a single
+    fused frame launch, with results identical to each filter's own
+    compute(). elapsedTime() is the time of the whole step.
+    """
+
+    def __init__(self, filters=()):
+        self._set_handle(N.get_handle("ff_batch_create"))
+        self._filters = []
+        for f in filters:
+            self.add(f)
+
+    def add(self, f):
+        if not isinstance(f, PyramidalFlowFilter):
+            raise TypeError("BatchedFlowFilter.add() takes a PyramidalFlowFilter")
+        N.call("ff_batch_add", self.handle, f.handle)
+        self._filters.append(f)
+        return len(self._filters) - 1
+
+    def __len__(self):
+        return N.get_int("ff_batch_size", self.handle)
+
+    def __getitem__(self, i):
+        return self._filters[i]
+
+    @property
+    def graphs(self):
+        """Replay each fused frame step as a captured HIP graph (default True)."""
+        return bool(N.get_int("ff_filter_get_graphs", self.handle))
+
+    @graphs.setter
+    def graphs(self, v):
+        N.call("ff_filter_set_graphs", self.handle, int(bool(v)))
+
+    def lastBlocks(self):
+        """Workgroups of the last fused launch."""
+        return N.get_int("ff_batch_get_blocks", self.handle)
+
+    def waitTimeouts(self):
+        """Waits of a fused step that ended at the spin limit (0 when every
         level saw the level above it complete). Synchronizes the device."""
         v = ctypes.c_ulonglong(0)
         N.call("ff_filter_get_wait_timeouts", self.handle, ctypes.byref(v))

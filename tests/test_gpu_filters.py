@@ -276,9 +276,11 @@ def test_cpp_api_demo(tmp_path):
 
 
 # ------------------------------------------------ fused kernel coverage
-# The fused per-level kernel needs W >= 64 and H >= 128 at every level; these
-# cases exercise it (tile clamping at the right/bottom edges, odd sizes,
-# several propagation/smoothing counts and borders) against the oracle.
+# One fused frame launch per frame after the first. These cases exercise it
+# against the oracle: tile clamping at the right/bottom edges, odd sizes,
+# levels narrower than 64 columns or shorter than 128 rows (a single tile
+# whose buffer extends past the image), several propagation/smoothing counts
+# and borders, 1-3 levels.
 
 @pytest.mark.parametrize("shape,levels,maxflow,smooth,border", [
     ((272, 512), 2, 4.0, [2, 2], 3),
@@ -288,6 +290,12 @@ def test_cpp_api_demo(tmp_path):
     ((480, 640), 1, 4.0, [2], 3),
     ((256, 130), 2, 1.0, [2, 2], 1),
     ((260, 400), 2, 6.0, [3, 2], 5),
+    ((60, 50), 1, 4.0, [2], 3),
+    ((130, 63), 1, 2.0, [1], 1),
+    ((129, 65), 1, 4.0, [2], 3),
+    ((240, 320), 3, 4.0, [2, 2, 2], 3),
+    ((37, 200), 2, 3.0, [2, 1], 2),
+    ((5, 7), 1, 1.0, [1], 1),
 ])
 def test_fused_matches_oracle(ffo, shape, levels, maxflow, smooth, border):
     from flowfilter import _native as N
