@@ -70,6 +70,76 @@ def cpu_baseline(frames, budget_s=12.0, threads=None):
                       "(oracle/ff_oracle.hpp, OpenMP over rows), %.1f s" % (n, el)}
 
 
+def batched_bench(args, N, gf, GPUImage):
+    """64 camera streams (synthetic.camera_streams, seed 3: 320x240 to
+    1920x1080, 1-3 levels each) resident in HBM. One step loads every
+    stream's next frame from device memory and advances all of them:
+    batched = one BatchedFlowFilter.compute() (one fused launch per step),
+    plain = each filter's own compute() in a loop. Returns aggregate level-0
+    Mpix/s of both and their ratio."""
+    import numpy as np
+    from flowfilter import synthetic
+    t0 = time.perf_counter()
+    streams = synthetic.camera_streams(64, 3)
+    print("generated 64 streams in %.1f s" % (time.perf_counter() - t0), file=sys.stderr, flush=True)
+    dev = []
+    for sp in streams:
+        F = len(sp["frames"])
+        g = GPUImage((F * sp["h"], sp["w"]), 1)
+        g.upload(np.concatenate(sp["frames"], axis=0))
+        dev.append(g)
+
+    def build():
+        fs = []
+        for sp in streams:
+            f = gf.PyramidalFlowFilter(sp["h"], sp["w"], sp["levels"])
+            f.gamma = sp["gamma"] if sp["levels"] > 1 else sp["gamma"][0]
+            f.maxflow = sp["maxflow"]
+            f.smoothIterations = sp["smooth"] if sp["levels"] > 1 else sp["smooth"][0]
+            fs.append(f)
+        return fs
+
+    def load(fs, i):
+        for f, sp, g in zip(fs, streams, dev):
+            F = len(sp["frames"])
+            f.loadImageDevice(g.data + (i % F) * sp["h"] * g.pitch, g.pitch)
+
+    px = sum(sp["h"] * sp["w"] for sp in streams)
+    res = {}
+    for mode in ("batched", "plain"):
+        fs = build()
+        batch = gf.BatchedFlowFilter(fs) if mode == "batched" else None
+
+        def step(i):
+            load(fs, i)
+            if batch is not None:
+                batch.compute()
+            else:
+                for f in fs:
+                    f.compute()
+
+        for i in range(5):
+            step(i)
+        N.synchronize()
+        m0 = N.Marker()
+        for i in range(args.batched_steps):
+            step(5 + i)
+        m1 = N.Marker()
+        ms = m0.elapsed_ms(m1)
+        res[mode] = px * args.batched_steps / (ms / 1e3) / 1e6
+        if batch is not None:
+            res["blocks"] = batch.lastBlocks()
+            res["timeouts"] = batch.waitTimeouts()
+        print("%s: %.3f ms/step, %.1f Mpix/s" % (mode, ms / args.batched_steps, res[mode]), file=sys.stderr,
+              flush=True)
+    return {"streams": len(streams), "mpix_per_step": round(px / 1e6, 3), "steps": args.batched_steps,
+            "value": round(res["batched"], 3), "plain_value": round(res["plain"], 3), "unit": "Mpix/s",
+            "speedup": round(res["batched"] / res["plain"], 3), "launch_blocks": res["blocks"],
+            "wait_timeouts": res["timeouts"],
+            "workload": "64 camera streams (seed 3): 320x240..1920x1080, 1-3 levels, gamma U[5,50], "
+                        "maxflow 4, smooth 1-2, affine motion; step = 64 device loadImage + compute"}
+
+
 def copy_bench(args, N, GPUImage):
     """GPUImage.copyFrom of one float2 flow image, K times: a kernel whose HBM
     traffic is known exactly (H*W*8 bytes read, the same written)."""
@@ -100,6 +170,8 @@ def main():
     ap.add_argument("--no-graphs", action="store_true", help="launch the fused kernels one by one")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-batched", action="store_true", help="skip the 64-stream batched measurement")
+    ap.add_argument("--batched-steps", type=int, default=60)
     ap.add_argument("--copy", action="store_true",
                     help="instead of the filter, time a plain device copy of one 2048x1088 flow image "
                          "(17.8 MB read + 17.8 MB written per launch; the reference kernel for HBM counters)")
@@ -192,6 +264,8 @@ def main():
                      "algorithmic_bytes_per_launch": traffic, "avg_launch_ms": round(avg_ms, 5),
                      "launches": launches},
     }
+    if not args.no_batched:
+        out["batched"] = batched_bench(args, N, gf, GPUImage)
     if not args.no_cpu:
         print("cpu baseline ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(frames[: min(F, 8)], args.cpu_seconds)

@@ -78,6 +78,17 @@ void new_image(const Image& img, ff_gpuimage** out) {
     *out = new ff_gpuimage{img};
 }
 
+template <class F>
+void on_fused_filter(ff_stage* s, F f) {
+    if (!s || !s->obj) throw null_handle("ff_stage");
+    Stage* o = s->obj.get();
+    if (auto* a = dynamic_cast<FlowFilter*>(o)) f(*a);
+    else if (auto* b = dynamic_cast<DeltaFlowFilter*>(o)) f(*b);
+    else if (auto* c = dynamic_cast<PyramidalFlowFilter*>(o)) f(*c);
+    else if (auto* d = dynamic_cast<BatchedFlowFilter*>(o)) f(*d);
+    else throw Error(FF_ERR_WRONG_TYPE, std::string("stage is a ") + o->name() + ", expected a filter");
+}
+
 template <class T, class... A>
 void new_stage(ff_stage** out, A&&... args) {
     out_ptr(out, "ff_stage**");
@@ -541,6 +552,16 @@ ff_status ff_filter_get_wait_timeouts(ff_stage* s, unsigned long long* out) {
         if (!s || !s->obj) throw null_handle("ff_stage");
         if (auto* b = dynamic_cast<BatchedFlowFilter*>(s->obj.get())) *out = b->waitTimeouts();
         else *out = as<PyramidalFlowFilter>(s, "PyramidalFlowFilter").waitTimeouts();
+    });
+}
+
+ff_status ff_filter_set_tile_waves(ff_stage* s, int waves) {
+    return guard([&] { on_fused_filter(s, [&](auto& f) { f.setTileWaves(waves); }); });
+}
+ff_status ff_filter_get_tile_waves(ff_stage* s, int* out) {
+    return guard([&] {
+        out_ptr(out, "waves");
+        on_fused_filter(s, [&](auto& f) { *out = f.tileWaves(); });
     });
 }
 

@@ -123,7 +123,7 @@ void FlowFilter::compute() {
         throw logic_error("FlowFilter::compute(): input image shape changed since configure()");
     startTiming();
     if (fused_ && fusedSupported()) {
-        const int t = frame_.prepare({fusedArgs()});
+        const int t = frame_.prepare({fusedArgs()}, stream());
         frame_.launch(t, stream());
         check_launch("FlowFilter::compute (fused)");
         fusedCommit();
@@ -303,7 +303,7 @@ void DeltaFlowFilter::compute() {
     requireConfigured("compute()");
     startTiming();
     if (fused_ && fusedSupported()) {
-        const int t = frame_.prepare({fusedArgs()});
+        const int t = frame_.prepare({fusedArgs()}, stream());
         frame_.launch(t, stream());
         check_launch("DeltaFlowFilter::compute (fused)");
         fusedCommit();
@@ -573,8 +573,8 @@ void run_maybe_graphed(bool graphs_on, GraphCache& cache, const std::vector<Leve
 }  // namespace
 
 void PyramidalFlowFilter::computeFused(const std::vector<LevelArgs>& args) {
-    const int table = frame_.prepare(args);
     hipStream_t s = stream();
+    const int table = frame_.prepare(args, s);
     run_maybe_graphed(graphs_on_, graphs_, args, frame_.token(table), s, [&] { enqueueFused(table, s); });
     commitFused();
 }
@@ -707,9 +707,9 @@ void BatchedFlowFilter::compute() {
         }
     }
     if (!all.empty()) {
-        const int table = frame_.prepare(all);
-        last_blocks_ = frame_.blocks(table);
         hipStream_t s = ffamd::stream();
+        const int table = frame_.prepare(all, s);
+        last_blocks_ = frame_.blocks(table);
         run_maybe_graphed(graphs_on_, graphs_, all, frame_.token(table), s, [&] {
             for (PyramidalFlowFilter* f : fused) f->enqueuePreFused();
             frame_.launch(table, s);

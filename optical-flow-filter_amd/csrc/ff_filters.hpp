@@ -48,6 +48,8 @@ public:
     int width() const { return w_; }
     void setFused(bool f) { fused_ = f; }
     bool fused() const { return fused_; }
+    void setTileWaves(int nw) { frame_.setWaves(nw); }
+    int tileWaves() const { return frame_.waves(); }
 
     // Used by PyramidalFlowFilter: the same work without the timer.
     void runImageModel();
@@ -99,6 +101,8 @@ public:
     int height() const { return input_.height(); }
     int width() const { return input_.width(); }
     void setFused(bool f) { fused_ = f; }
+    void setTileWaves(int nw) { frame_.setWaves(nw); }
+    int tileWaves() const { return frame_.waves(); }
 
     void runImageModel();
     void runPropagation();
@@ -183,6 +187,9 @@ public:
     }
     bool graphs() const { return graphs_on_; }
     unsigned long long waitTimeouts() const { return frame_.timeouts(); }
+    // Tile height of the fused frame kernel in waves: 0 = autotune.
+    void setTileWaves(int nw) { frame_.setWaves(nw); }
+    int tileWaves() const { return frame_.waves(); }
 
 private:
     friend class BatchedFlowFilter;
@@ -227,6 +234,8 @@ public:
     }
     bool graphs() const { return graphs_on_; }
     unsigned long long waitTimeouts() const { return frame_.timeouts(); }
+    void setTileWaves(int nw) { frame_.setWaves(nw); }
+    int tileWaves() const { return frame_.waves(); }
     // Workgroups of the last fused launch (0 before the first).
     int lastBlocks() const { return last_blocks_; }
 

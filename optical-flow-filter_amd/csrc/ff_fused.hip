@@ -19,9 +19,10 @@
 // spin limit. Lower levels read the previous frame's flow of the level
 // above (the reference's ascending update order) and do not wait.
 //
-// Tile geometry: a workgroup is NW waves stacked vertically; lane l of every
-// wave owns buffer column l (64 columns), and each thread owns R consecutive
-// rows of its wave's strip, so the buffer is 64 x (NW*R) pixels. The output
+// Tile geometry: a workgroup is nw waves stacked vertically (nw is a launch
+// parameter, picked by timing the candidates); lane l of every wave owns
+// buffer column l (64 columns), and each thread owns R consecutive rows of
+// its wave's strip, so the buffer is 64 x (nw*R) pixels. The output
 // tile is the buffer minus a halo h0 = max(N, 2 + e) + 2K on every side (the
 // distance an error at the buffer edge travels inward: one pixel per
 // propagation pass, two per smoothing pass, two for the image model's x
@@ -601,9 +602,16 @@ __device__ __forceinline__ void finish_image(const ImageRegs<MODE, R>& ir, float
     }
 }
 
-template <bool DELTA, int MODE, int R, int NW>
-__device__ __forceinline__ void level_tile(const LevelJob& p, int tile, float (&xch)[2][NW][10][LANES],
-                                           unsigned* counters, int timeout_idx) {
+// Row exchange buffer of the workgroup: [2][nw][10][LANES] floats (double
+// buffered; 10 rows of each wave: its first and last rows of up to five
+// fields).
+__device__ __forceinline__ float* xrow(float* xch, int nw, int slot, int wave) {
+    return xch + ((slot * nw + wave) * 10) * LANES;
+}
+
+template <bool DELTA, int MODE, int R>
+__device__ __forceinline__ void level_tile(const LevelJob& p, int tile, float* xch, int nw, unsigned* counters,
+                                           int timeout_idx) {
     Geo g;
     g.lane = threadIdx.x & (LANES - 1);
     g.wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -613,7 +621,7 @@ __device__ __forceinline__ void level_tile(const LevelJob& p, int tile, float (&
     const int ox0 = txi * p.TW, ox1 = min(ox0 + p.TW, p.W);
     const int oy0 = tyi * p.TH, oy1 = min(oy0 + p.TH, p.H);
     const int x0 = clampi(ox0 - p.h0, 0, p.W - LANES);
-    const int y0 = clampi(oy0 - p.h0, 0, p.H - NW * R);
+    const int y0 = clampi(oy0 - p.h0, 0, p.H - nw * R);
     g.x = x0 + g.lane;
     g.xl = min(g.x, p.W - 1);
     g.yw = y0 + g.wave * R;  // image row of this thread's first row (wave-uniform)
@@ -633,7 +641,7 @@ __device__ __forceinline__ void level_tile(const LevelJob& p, int tile, float (&
     const bool xinside = xlast < LANES - 1;
     g.xlast = xlast;
     const int rlast = p.H - 1 - g.yw;
-    const bool yinside = y0 + NW * R > p.H;
+    const bool yinside = y0 + nw * R > p.H;
     const int lane = g.lane, wave = g.wave;
     const bool out_x = g.x >= ox0 && g.x < ox1;
 
@@ -668,7 +676,7 @@ __device__ __forceinline__ void level_tile(const LevelJob& p, int tile, float (&
         if (g.wave_in) prop_x<DELTA, false, R>(s, g, p);
         else prop_x<DELTA, true, R>(s, g, p);
         {
-            float* my = &xch[slot][wave][0][0];
+            float* my = xrow(xch, nw, slot, wave);
             my[0 * LANES + lane] = s.fu[0];
             my[1 * LANES + lane] = s.fv[0];
             my[5 * LANES + lane] = s.fu[R - 1];
@@ -695,7 +703,7 @@ __device__ __forceinline__ void level_tile(const LevelJob& p, int tile, float (&
             eb.dv = s.dv[R - 1];
         }
         if (wave > 0) {
-            const float* o = &xch[slot][wave - 1][0][0];
+            const float* o = xrow(xch, nw, slot, wave - 1);
             ea.u = o[5 * LANES + lane];
             ea.v = o[6 * LANES + lane];
             if constexpr (DELTA) {
@@ -704,8 +712,8 @@ __device__ __forceinline__ void level_tile(const LevelJob& p, int tile, float (&
                 ea.dv = o[9 * LANES + lane];
             }
         }
-        if (wave < NW - 1) {
-            const float* o = &xch[slot][wave + 1][0][0];
+        if (wave < nw - 1) {
+            const float* o = xrow(xch, nw, slot, wave + 1);
             eb.u = o[0 * LANES + lane];
             eb.v = o[1 * LANES + lane];
             if constexpr (DELTA) {
@@ -756,7 +764,7 @@ __device__ __forceinline__ void level_tile(const LevelJob& p, int tile, float (&
         if (xedge) smooth_x<true, R>(s.fu, s.fv);
         else smooth_x<false, R>(s.fu, s.fv);
         {
-            float* my = &xch[slot][wave][0][0];
+            float* my = xrow(xch, nw, slot, wave);
             my[0 * LANES + lane] = s.fu[0];
             my[1 * LANES + lane] = s.fv[0];
             my[2 * LANES + lane] = s.fu[1];
@@ -771,14 +779,14 @@ __device__ __forceinline__ void level_tile(const LevelJob& p, int tile, float (&
         float a2u = s.fu[0], a2v = s.fv[0], a1u = s.fu[0], a1v = s.fv[0];
         float b1u = s.fu[R - 1], b1v = s.fv[R - 1], b2u = s.fu[R - 1], b2v = s.fv[R - 1];
         if (wave > 0) {
-            const float* o = &xch[slot][wave - 1][0][0];
+            const float* o = xrow(xch, nw, slot, wave - 1);
             a2u = o[4 * LANES + lane];
             a2v = o[5 * LANES + lane];
             a1u = o[6 * LANES + lane];
             a1v = o[7 * LANES + lane];
         }
-        if (wave < NW - 1) {
-            const float* o = &xch[slot][wave + 1][0][0];
+        if (wave < nw - 1) {
+            const float* o = xrow(xch, nw, slot, wave + 1);
             b1u = o[0 * LANES + lane];
             b1v = o[1 * LANES + lane];
             b2u = o[2 * LANES + lane];
@@ -809,34 +817,33 @@ struct GroupEnds {
     int e[6];
 };
 
-template <int R, int NW>
-__global__ void __launch_bounds__(NW* LANES)
+template <int R>
+__global__ void __launch_bounds__(1024)
     k_fused_frame(const LevelJob* __restrict__ jobs, const int* __restrict__ block_job, unsigned* counters,
-                  int timeout_idx, GroupEnds ends) {
-    // row exchange between the waves of the workgroup, double buffered
-    __shared__ float xch[2][NW][10][LANES];
+                  int timeout_idx, GroupEnds ends, int nw) {
+    // row exchange between the waves of the workgroup (2 * nw * 10 * LANES floats)
+    extern __shared__ float xch[];
     const LevelJob& p = jobs[block_job[blockIdx.x]];
     const int tile = (int)blockIdx.x - p.first_block;
     const unsigned b = blockIdx.x;
-    if (b < (unsigned)ends.e[0]) level_tile<false, IMG_F32, R, NW>(p, tile, xch, counters, timeout_idx);
-    else if (b < (unsigned)ends.e[1]) level_tile<false, IMG_U8, R, NW>(p, tile, xch, counters, timeout_idx);
-    else if (b < (unsigned)ends.e[2]) level_tile<false, IMG_U8_DOWN, R, NW>(p, tile, xch, counters, timeout_idx);
-    else if (b < (unsigned)ends.e[3]) level_tile<false, IMG_U8_DOWN2, R, NW>(p, tile, xch, counters, timeout_idx);
-    else if (b < (unsigned)ends.e[4]) level_tile<true, IMG_F32, R, NW>(p, tile, xch, counters, timeout_idx);
-    else if (b < (unsigned)ends.e[5]) level_tile<true, IMG_U8, R, NW>(p, tile, xch, counters, timeout_idx);
-    else level_tile<true, IMG_U8_DOWN, R, NW>(p, tile, xch, counters, timeout_idx);
+    if (b < (unsigned)ends.e[0]) level_tile<false, IMG_F32, R>(p, tile, xch, nw, counters, timeout_idx);
+    else if (b < (unsigned)ends.e[1]) level_tile<false, IMG_U8, R>(p, tile, xch, nw, counters, timeout_idx);
+    else if (b < (unsigned)ends.e[2]) level_tile<false, IMG_U8_DOWN, R>(p, tile, xch, nw, counters, timeout_idx);
+    else if (b < (unsigned)ends.e[3]) level_tile<false, IMG_U8_DOWN2, R>(p, tile, xch, nw, counters, timeout_idx);
+    else if (b < (unsigned)ends.e[4]) level_tile<true, IMG_F32, R>(p, tile, xch, nw, counters, timeout_idx);
+    else if (b < (unsigned)ends.e[5]) level_tile<true, IMG_U8, R>(p, tile, xch, nw, counters, timeout_idx);
+    else level_tile<true, IMG_U8_DOWN, R>(p, tile, xch, nw, counters, timeout_idx);
 }
 
-#ifndef FF_FUSED_R
-#define FF_FUSED_R 8
-#endif
-#ifndef FF_FUSED_NW
-#define FF_FUSED_NW 16
-#endif
-constexpr int kR = FF_FUSED_R;
-constexpr int kNW = FF_FUSED_NW;
+constexpr int kR = 8;        // rows per thread
+constexpr int kMaxWaves = 16;  // waves per workgroup, at most
+// Wave counts the autotuner tries: 16 and 15 (one workgroup per CU), 8
+// (two), 5 (three) and 4 (four); the register budget allows 16 waves per CU.
+constexpr int kWaveChoices[] = {16, 15, 8, 5, 4};
 
-// Level size of a job (the image is twice as large when reduced on load).
+size_t lds_bytes(int nw) { return (size_t)2 * nw * 10 * LANES * sizeof(float); }
+
+// Level size of a job (the image is larger when reduced on load).
 void level_size(const LevelArgs& a, int& H, int& W) {
     H = a.flow_s_in.h;
     W = a.flow_s_in.w;
@@ -851,15 +858,23 @@ bool fits_i32(const DevImg& d) {
     return (unsigned long long)d.pitch * (unsigned long long)(d.h > 0 ? d.h : 0) < 0x7fffffffULL;
 }
 
+// The tiling of a level with nw waves per workgroup leaves a non-empty
+// output tile (or the level fits in one tile).
+bool tiles_with(const LevelArgs& a, int nw) {
+    int H, W;
+    level_size(a, H, W);
+    const int h0 = halo_of(a);
+    return !((W > LANES && 2 * h0 >= LANES) || (H > nw * kR && 2 * h0 >= nw * kR));
+}
+
 }  // namespace
 
 bool fused_level_supported(const LevelArgs& a) {
-    const int h0 = halo_of(a);
     if (a.border < 1 || a.prop_iters < 1 || a.smooth_iters < 1) return false;
     int H, W;
     level_size(a, H, W);
     if (H < 1 || W < 1) return false;
-    if ((W > LANES && 2 * h0 >= LANES) || (H > kNW * kR && 2 * h0 >= kNW * kR)) return false;
+    if (!tiles_with(a, kMaxWaves)) return false;
     if (!a.img.data || !a.flow_s_in.data || !a.a0_in.data || !a.flow_s_out.data || !a.a0_out.data) return false;
     switch (a.img_mode) {
     case IMG_F32:
@@ -894,11 +909,31 @@ bool fused_level_supported(const LevelArgs& a) {
 
 // ------------------------------------------------------------- FusedFrame
 
+namespace {
+constexpr int kGroups = 7;
+// Tile body group of a job (the order of the kernel's dispatch).
+int body_group(const LevelArgs& a) {
+    if (!a.delta) return a.img_mode;  // IMG_F32 .. IMG_U8_DOWN2: groups 0 .. 3
+    return 4 + a.img_mode;            // IMG_F32 .. IMG_U8_DOWN: groups 4 .. 6
+}
+
+// What the tiling choice depends on: the shape and halo of every level.
+std::vector<int> tune_key(const std::vector<LevelArgs>& jobs) {
+    std::vector<int> k;
+    for (const LevelArgs& a : jobs) {
+        int H, W;
+        level_size(a, H, W);
+        k.insert(k.end(), {H, W, halo_of(a), a.prop_iters, a.smooth_iters, a.delta, a.img_mode, a.wait_on});
+    }
+    return k;
+}
+}  // namespace
+
 struct FusedFrame::Impl {
     struct Table {
-        std::vector<char> key;  // job records + block map, as uploaded
+        std::vector<char> key;  // job records + block map + launch shape, as uploaded
         DeviceArray dev;
-        int njobs = 0, blocks = 0;
+        int njobs = 0, blocks = 0, nw = 0;
         GroupEnds ends{};
         size_t map_offset = 0;
         unsigned long long token = 0;
@@ -908,6 +943,13 @@ struct FusedFrame::Impl {
     unsigned long long uploads = 0;
     DeviceArray counters;
     int ncounters = 0;  // pairs
+    int waves = 0;      // fixed wave count, 0: autotune
+    std::vector<int> tuned_for;
+    int tuned_waves = 0;
+
+    int table(const std::vector<LevelArgs>& jobs, int nw);
+    void run(int t, hipStream_t s) const;
+    int tune(const std::vector<LevelArgs>& jobs, hipStream_t s);
 };
 
 FusedFrame::FusedFrame() : impl_(new Impl) {}
@@ -918,17 +960,10 @@ FusedFrame& FusedFrame::operator=(FusedFrame&& o) noexcept {
     return *this;
 }
 
-namespace {
-// Tile body group of a job (the order of the kernel's dispatch).
-constexpr int kGroups = 7;
-int body_group(const LevelArgs& a) {
-    if (!a.delta) return a.img_mode;  // IMG_F32 .. IMG_U8_DOWN2: groups 0 .. 3
-    return 4 + a.img_mode;            // IMG_F32 .. IMG_U8_DOWN: groups 4 .. 6
-}
-}  // namespace
-
-int FusedFrame::prepare(const std::vector<LevelArgs>& jobs) {
-    if (jobs.empty()) throw std::invalid_argument("fused frame: no levels");
+// Job table of `jobs` tiled with nw waves per workgroup: found among the two
+// resident tables or built and uploaded (after a device synchronization,
+// since the slot it replaces may be in use).
+int FusedFrame::Impl::table(const std::vector<LevelArgs>& jobs, int nw) {
     // launch order: grouped by tile body, stable within a group; every top
     // level precedes every delta level, so producers precede consumers
     std::vector<int> order(jobs.size()), pos(jobs.size());
@@ -941,7 +976,8 @@ int FusedFrame::prepare(const std::vector<LevelArgs>& jobs) {
     int ends[kGroups - 1] = {0, 0, 0, 0, 0, 0};
     for (size_t i = 0; i < jobs.size(); ++i) {
         const LevelArgs& a = jobs[order[i]];
-        if (!fused_level_supported(a)) throw std::logic_error("fused frame: unsupported level parameters");
+        if (!fused_level_supported(a) || !tiles_with(a, nw))
+            throw std::logic_error("fused frame: unsupported level parameters");
         LevelJob& p = rec[i];
         std::memset(&p, 0, sizeof(p));
         p.img = a.img.data;
@@ -973,7 +1009,7 @@ int FusedFrame::prepare(const std::vector<LevelArgs>& jobs) {
         p.h0 = halo_of(a);
         // a level no wider (taller) than the buffer is one tile across (down)
         p.TW = p.W <= LANES ? p.W : LANES - 2 * p.h0;
-        p.TH = p.H <= kNW * kR ? p.H : kNW * kR - 2 * p.h0;
+        p.TH = p.H <= nw * kR ? p.H : nw * kR - 2 * p.h0;
         p.tiles_x = (p.W + p.TW - 1) / p.TW;
         p.tiles_y = (p.H + p.TH - 1) / p.TH;
         p.delta = a.delta;
@@ -1014,52 +1050,125 @@ int FusedFrame::prepare(const std::vector<LevelArgs>& jobs) {
     std::vector<char> key(map_offset + map.size() * sizeof(int), 0);
     std::memcpy(key.data(), rec.data(), jobs_bytes);
     std::memcpy(key.data() + map_offset, map.data(), map.size() * sizeof(int));
-    // the group ends are kernel arguments, part of the table's identity
+    // launch shape: part of the table's identity
     for (int g = 0; g < kGroups - 1; ++g) {
         const char* e = reinterpret_cast<const char*>(&ends[g]);
         key.insert(key.end(), e, e + sizeof(int));
     }
+    key.insert(key.end(), reinterpret_cast<const char*>(&nw), reinterpret_cast<const char*>(&nw) + sizeof(int));
 
     for (int i = 0; i < 2; ++i)
-        if (impl_->slot[i].key == key) return i;
+        if (slot[i].key == key) return i;
 
     // a table (or the counters) may be in use by launches still in flight
     FF_HIP(hipDeviceSynchronize());
     const int need = (int)jobs.size() + 1;
-    if (impl_->ncounters < need) {
+    if (ncounters < need) {
         const int n = need < 64 ? 64 : need;
         std::vector<unsigned> zero(2 * (size_t)n, 0u);
-        impl_->counters.assign(zero.data(), zero.size() * sizeof(unsigned));
-        impl_->ncounters = n;
+        counters.assign(zero.data(), zero.size() * sizeof(unsigned));
+        ncounters = n;
         // launches recorded with the old counters must not be replayed
-        for (Impl::Table& o : impl_->slot) o.token = ++impl_->uploads;
+        for (Table& o : slot) o.token = ++uploads;
     }
-    const int s = impl_->next;
-    Impl::Table& t = impl_->slot[s];
+    const int si = next;
+    Table& t = slot[si];
     t.dev.assign(key.data(), key.size());
     t.key = std::move(key);
     t.njobs = (int)rec.size();
     t.blocks = blocks;
+    t.nw = nw;
     t.map_offset = map_offset;
     for (int g = 0; g < kGroups - 1; ++g) t.ends.e[g] = ends[g];
-    t.token = ++impl_->uploads;
-    impl_->next ^= 1;
-    return s;
+    t.token = ++uploads;
+    next ^= 1;
+    return si;
 }
+
+void FusedFrame::Impl::run(int ti, hipStream_t s) const {
+    const Table& t = slot[ti & 1];
+    if (!t.blocks) throw std::logic_error("fused frame: launch of an unprepared table");
+    static const hipError_t lds_ok = hipFuncSetAttribute((const void*)k_fused_frame<kR>,
+                                                         hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                         (int)lds_bytes(kMaxWaves));
+    if (lds_ok != hipSuccess) throw_hip(lds_ok, "hipFuncSetAttribute", __FILE__, __LINE__);
+    ProfileScope ps("fused_frame", s);
+    const char* base = (const char*)t.dev.data();
+    const int timeout_idx = 2 * (ncounters - 1);
+    hipLaunchKernelGGL((k_fused_frame<kR>), dim3(t.blocks), dim3(t.nw * LANES), lds_bytes(t.nw), s,
+                       (const LevelJob*)base, (const int*)(base + t.map_offset), (unsigned*)counters.data(),
+                       timeout_idx, t.ends, t.nw);
+}
+
+// Time one launch of every candidate wave count on the actual job table
+// and keep the fastest. A launch reads the current state and writes the
+// next one without committing it, so repeated launches recompute the same
+// output; every tiling gives bitwise the same results.
+int FusedFrame::Impl::tune(const std::vector<LevelArgs>& jobs, hipStream_t s) {
+    int best = 0;
+    float best_ms = 0.0f;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    FF_HIP(hipEventCreate(&e0));
+    FF_HIP(hipEventCreate(&e1));
+    try {
+        for (int nw : kWaveChoices) {
+            bool ok = true;
+            for (const LevelArgs& a : jobs) ok = ok && tiles_with(a, nw);
+            if (!ok) continue;
+            const int t = table(jobs, nw);
+            run(t, s);  // warm-up
+            constexpr int kReps = 3;
+            FF_HIP(hipEventRecord(e0, s));
+            for (int r = 0; r < kReps; ++r) run(t, s);
+            FF_HIP(hipEventRecord(e1, s));
+            FF_HIP(hipEventSynchronize(e1));
+            float ms = 0.0f;
+            FF_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (!best || ms < best_ms) {
+                best = nw;
+                best_ms = ms;
+            }
+        }
+    } catch (...) {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (!best) throw std::logic_error("fused frame: no tiling fits the levels");
+    return best;
+}
+
+int FusedFrame::prepare(const std::vector<LevelArgs>& jobs, hipStream_t s) {
+    if (jobs.empty()) throw std::invalid_argument("fused frame: no levels");
+    int nw = impl_->waves;
+    if (!nw) {
+        std::vector<int> k = tune_key(jobs);
+        if (k != impl_->tuned_for || !impl_->tuned_waves) {
+            impl_->tuned_waves = impl_->tune(jobs, s);
+            impl_->tuned_for = std::move(k);
+        }
+        nw = impl_->tuned_waves;
+    }
+    return impl_->table(jobs, nw);
+}
+
+void FusedFrame::launch(int table, hipStream_t s) const { impl_->run(table, s); }
 
 int FusedFrame::blocks(int table) const { return impl_->slot[table & 1].blocks; }
 unsigned long long FusedFrame::token(int table) const { return impl_->slot[table & 1].token; }
 
-void FusedFrame::launch(int table, hipStream_t s) const {
-    const Impl::Table& t = impl_->slot[table & 1];
-    if (!t.blocks) throw std::logic_error("fused frame: launch of an unprepared table");
-    ProfileScope ps("fused_frame", s);
-    const char* base = (const char*)t.dev.data();
-    const int timeout_idx = 2 * (impl_->ncounters - 1);
-    hipLaunchKernelGGL((k_fused_frame<kR, kNW>), dim3(t.blocks), dim3(kNW * LANES), 0, s,
-                       (const LevelJob*)base, (const int*)(base + t.map_offset), (unsigned*)impl_->counters.data(),
-                       timeout_idx, t.ends);
+void FusedFrame::setWaves(int nw) {
+    if (nw != 0) {
+        bool ok = false;
+        for (int c : kWaveChoices) ok = ok || c == nw;
+        if (!ok) throw std::invalid_argument("fused frame: waves per workgroup must be 0 (autotune), 16, 15, 8, 5 or 4");
+    }
+    impl_->waves = nw;
 }
+
+int FusedFrame::waves() const { return impl_->waves ? impl_->waves : impl_->tuned_waves; }
 
 unsigned long long FusedFrame::timeouts() const {
     if (!impl_->ncounters) return 0;

@@ -112,8 +112,10 @@ bool fused_level_supported(const LevelArgs& a);
 
 // Device tables and completion counters of fused frame launches. prepare()
 // builds the job table of a list of levels (uploading it, after a device
-// synchronization, when it matches none of the two most recent tables) and
-// must not run under stream capture; launch() only enqueues the kernel.
+// synchronization, when it matches none of the two most recent tables),
+// choosing the tile height by timing the candidates the first time it sees
+// a set of level shapes; it must not run under stream capture. launch()
+// only enqueues the kernel.
 class FusedFrame {
 public:
     FusedFrame();
@@ -123,8 +125,14 @@ public:
     FusedFrame(FusedFrame&&) noexcept;
     FusedFrame& operator=(FusedFrame&&) noexcept;
 
-    int prepare(const std::vector<LevelArgs>& jobs);
+    // The first call for a set of level shapes times every candidate tiling
+    // on stream s (unless a wave count was fixed with setWaves).
+    int prepare(const std::vector<LevelArgs>& jobs, hipStream_t s);
     void launch(int table, hipStream_t s) const;
+    // Waves per workgroup (tile height / 8): 0 = autotune (default), or
+    // 16, 15, 8, 5, 4. waves() is the fixed or tuned value (0 before tuning).
+    void setWaves(int nw);
+    int waves() const;
     // Waits that gave up after the spin limit (a scheduling fault; results
     // of such a frame are not valid). Synchronizes the device.
     unsigned long long timeouts() const;

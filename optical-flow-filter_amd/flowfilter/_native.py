@@ -193,6 +193,8 @@ SIGNATURES = {
     "ff_filter_set_graphs": [_S, _i],
     "ff_filter_get_graphs": [_S, _pi],
     "ff_filter_get_wait_timeouts": [_S, ctypes.POINTER(ctypes.c_ulonglong)],
+    "ff_filter_set_tile_waves": [_S, _i],
+    "ff_filter_get_tile_waves": [_S, _pi],
     "ff_batch_create": [_pS],
     "ff_batch_add": [_S, _S],
     "ff_batch_size": [_S, _pi],

@@ -51,3 +51,14 @@ class Stage(object):
 
 def _hnd(img):
     return img.handle if img is not None else None
+
+    @property
+    def tileWaves(self):
+        """Filters only: tile height of the fused frame kernel in waves per
+        workgroup (0 = autotune on the first fused frame, the default; or 16,
+        15, 8, 5, 4). Reads the fixed or tuned value."""
+        return N.get_int("ff_filter_get_tile_waves", self.handle)
+
+    @tileWaves.setter
+    def tileWaves(self, v):
+        N.call("ff_filter_set_tile_waves", self.handle, int(v))

@@ -13,6 +13,13 @@ renders), so parity tests and benchmarks run on these generated sequences:
   frame. Default: 2048x1088, 400 blobs (sigma 2-20 px, amplitude 20-80),
   0.5 deg rotation + 1% expansion per frame, cap 4 px, noise sigma 2, seed 2.
 
+* ``camera_streams``: many independent streams for the batched filter, each
+  with its own resolution (320x240, 640x480, 1280x720 or 1920x1080), pyramid
+  depth (1-3), gamma per level (uniform 5-50), smoothing iterations (1-2 per
+  level) and band-limited noise texture moved by a random affine motion
+  (translation up to 2 px plus rotation up to 0.5 deg and scale within 1%
+  about the centre, per frame, displacement capped at 4 px). Seed 3.
+
 All randomness comes from numpy.random.default_rng(seed) (PCG64), so a seed
 gives the same frames on every machine with this numpy.
 """
@@ -93,3 +100,39 @@ def blob_affine(h=1088, w=2048, frames=300, blobs=400, seed=2, rot_deg=0.5, scal
 
 def random_u8(h, w, seed=0):
     return np.random.default_rng(seed).integers(0, 256, (h, w), dtype=np.uint8)
+
+
+STREAM_SIZES = ((240, 320), (480, 640), (720, 1280), (1080, 1920))
+
+
+def camera_streams(n=64, frames=3, seed=3, maxflow=4.0):
+    """Specs and frames of ``n`` independent camera streams.
+
+    Returns a list of dicts with keys h, w, levels, gamma (list, one per
+    level), maxflow, smooth (list) and frames (list of uint8 arrays; frame
+    t+1 is frame t's texture moved by the stream's affine motion)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        h, w = STREAM_SIZES[int(rng.integers(len(STREAM_SIZES)))]
+        levels = int(rng.integers(1, 4))
+        gamma = [float(g) for g in rng.uniform(5.0, 50.0, size=levels)]
+        smooth = [int(k) for k in rng.integers(1, 3, size=levels)]
+        tx, ty = rng.uniform(-2.0, 2.0, size=2)
+        rot = np.deg2rad(rng.uniform(-0.5, 0.5))
+        scale = 1.0 + rng.uniform(-0.01, 0.01)
+        pad = int(np.ceil(4.0 * frames)) + 8
+        base = band_limited_noise(h, w, 3.0, int(rng.integers(1 << 30)), pad)
+        yy, xx = np.mgrid[0:h, 0:w].astype(np.float64)
+        cx, cy = (w - 1) / 2.0, (h - 1) / 2.0
+        c, s_ = np.cos(rot) * scale, np.sin(rot) * scale
+        dx = (c - 1.0) * (xx - cx) - s_ * (yy - cy) + tx
+        dy = s_ * (xx - cx) + (c - 1.0) * (yy - cy) + ty
+        mag = np.maximum(np.hypot(dx, dy) / maxflow, 1.0)
+        dx, dy = dx / mag, dy / mag
+        seq = []
+        for t in range(frames):
+            coords = np.stack([yy + pad - dy * t, xx + pad - dx * t])
+            seq.append(_quantize(ndimage.map_coordinates(base, coords, order=1, mode="nearest")))
+        out.append(dict(h=h, w=w, levels=levels, gamma=gamma, maxflow=maxflow, smooth=smooth, frames=seq))
+    return out

@@ -325,3 +325,36 @@ def test_fused_matches_oracle(ffo, shape, levels, maxflow, smooth, border):
         check_frame(t, s.getFlow(), want, "staged")
         check_frame(t, g.getImage(), o.downloadImage(), "fused image")
     assert g.waitTimeouts() == 0
+
+
+@pytest.mark.parametrize("waves", [16, 15, 8, 5, 4])
+def test_fused_tile_heights_match_oracle(ffo, waves):
+    """Every tile height of the fused frame kernel gives the oracle's flow."""
+    from flowfilter import _native as N
+    h, w = 300, 410
+    say("tile waves", waves)
+    g, o = _pyr_pair(ffo, h, w, 3, [10.0, 30.0, 60.0], 4.0, [2, 2, 1], fused=True)
+    g.tileWaves = waves
+    assert g.tileWaves == waves
+    for t, img in enumerate(seq_blobs(h, w, 5)):
+        g.loadImage(img)
+        N.profile_begin("fused_frame")
+        g.compute()
+        _, n = N.profile_end()
+        assert n == (0 if t == 0 else 1)
+        o.compute(img)
+        check_frame(t, g.getFlow(), o.downloadFlow(), "waves %d" % waves)
+
+
+def test_fused_autotune_picks_a_tiling(ffo):
+    g, o = _pyr_pair(ffo, 272, 512, 2, [10.0, 50.0], 4.0, [2, 2], fused=True)
+    assert g.tileWaves == 0
+    for t, img in enumerate(seq_blobs(272, 512, 3)):
+        g.loadImage(img)
+        g.compute()
+        o.compute(img)
+        check_frame(t, g.getFlow(), o.downloadFlow(), "autotuned")
+    assert g.tileWaves in (16, 15, 8, 5, 4)
+    from flowfilter import errors
+    with pytest.raises(errors.InvalidArgument):
+        g.tileWaves = 7
