@@ -295,9 +295,9 @@ FF_API ff_status ff_filter_get_wait_timeouts(ff_stage* s, unsigned long long* ou
 /* Tile height of the fused frame kernel, in waves per workgroup (no
  * reference counterpart): 0 (the default) times the candidates 16, 15, 8, 5
  * and 4 on the first fused frame of a set of level shapes and keeps the
- * fastest; a nonzero value fixes it. Results do not depend on it. get
- * returns the fixed or tuned value (0 before the first fused frame).
- * FlowFilter, DeltaFlowFilter, PyramidalFlowFilter and batches. */
+ * fastest; one of them fixes it. Results do not depend on it. get returns
+ * the fixed or tuned value (0 before the first fused frame). FlowFilter,
+ * DeltaFlowFilter, PyramidalFlowFilter and batches. */
 FF_API ff_status ff_filter_set_tile_waves(ff_stage* s, int waves);
 FF_API ff_status ff_filter_get_tile_waves(ff_stage* s, int* out);
 

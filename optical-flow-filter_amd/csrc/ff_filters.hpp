@@ -187,7 +187,7 @@ public:
     }
     bool graphs() const { return graphs_on_; }
     unsigned long long waitTimeouts() const { return frame_.timeouts(); }
-    // Tile height of the fused frame kernel in waves: 0 = autotune.
+    // Waves per workgroup of the fused frame kernel: 0 = autotune.
     void setTileWaves(int nw) { frame_.setWaves(nw); }
     int tileWaves() const { return frame_.waves(); }
 

@@ -22,7 +22,8 @@
 // Tile geometry: a workgroup is nw waves stacked vertically (nw is a launch
 // parameter, picked by timing the candidates); lane l of every wave owns
 // buffer column l (64 columns), and each thread owns R consecutive rows of
-// its wave's strip, so the buffer is 64 x (nw*R) pixels. The output
+// its wave's strip (R = 8, or 4 on a top level that the level below waits
+// for), so the buffer is 64 x (nw*R) pixels. The output
 // tile is the buffer minus a halo h0 = max(N, 2 + e) + 2K on every side (the
 // distance an error at the buffer edge travels inward: one pixel per
 // propagation pass, two per smoothing pass, two for the image model's x
@@ -817,7 +818,12 @@ struct GroupEnds {
     int e[6];
 };
 
-template <int R>
+// Rows per thread of a tile body: 4 for top levels reduced on load (the
+// top level of a pyramid, which the level below waits for: smaller tiles
+// finish sooner and spread over more CUs), 8 otherwise.
+constexpr int kRowsTop = 4;
+constexpr int kRows = 8;
+
 __global__ void __launch_bounds__(1024)
     k_fused_frame(const LevelJob* __restrict__ jobs, const int* __restrict__ block_job, unsigned* counters,
                   int timeout_idx, GroupEnds ends, int nw) {
@@ -826,20 +832,25 @@ __global__ void __launch_bounds__(1024)
     const LevelJob& p = jobs[block_job[blockIdx.x]];
     const int tile = (int)blockIdx.x - p.first_block;
     const unsigned b = blockIdx.x;
-    if (b < (unsigned)ends.e[0]) level_tile<false, IMG_F32, R>(p, tile, xch, nw, counters, timeout_idx);
-    else if (b < (unsigned)ends.e[1]) level_tile<false, IMG_U8, R>(p, tile, xch, nw, counters, timeout_idx);
-    else if (b < (unsigned)ends.e[2]) level_tile<false, IMG_U8_DOWN, R>(p, tile, xch, nw, counters, timeout_idx);
-    else if (b < (unsigned)ends.e[3]) level_tile<false, IMG_U8_DOWN2, R>(p, tile, xch, nw, counters, timeout_idx);
-    else if (b < (unsigned)ends.e[4]) level_tile<true, IMG_F32, R>(p, tile, xch, nw, counters, timeout_idx);
-    else if (b < (unsigned)ends.e[5]) level_tile<true, IMG_U8, R>(p, tile, xch, nw, counters, timeout_idx);
-    else level_tile<true, IMG_U8_DOWN, R>(p, tile, xch, nw, counters, timeout_idx);
+    if (b < (unsigned)ends.e[0]) level_tile<false, IMG_F32, kRows>(p, tile, xch, nw, counters, timeout_idx);
+    else if (b < (unsigned)ends.e[1]) level_tile<false, IMG_U8, kRows>(p, tile, xch, nw, counters, timeout_idx);
+    else if (b < (unsigned)ends.e[2]) level_tile<false, IMG_U8_DOWN, kRowsTop>(p, tile, xch, nw, counters, timeout_idx);
+    else if (b < (unsigned)ends.e[3]) level_tile<false, IMG_U8_DOWN2, kRowsTop>(p, tile, xch, nw, counters, timeout_idx);
+    else if (b < (unsigned)ends.e[4]) level_tile<true, IMG_F32, kRows>(p, tile, xch, nw, counters, timeout_idx);
+    else if (b < (unsigned)ends.e[5]) level_tile<true, IMG_U8, kRows>(p, tile, xch, nw, counters, timeout_idx);
+    else level_tile<true, IMG_U8_DOWN, kRows>(p, tile, xch, nw, counters, timeout_idx);
 }
 
-constexpr int kR = 8;        // rows per thread
 constexpr int kMaxWaves = 16;  // waves per workgroup, at most
-// Wave counts the autotuner tries: 16 and 15 (one workgroup per CU), 8
-// (two), 5 (three) and 4 (four); the register budget allows 16 waves per CU.
+// Waves per workgroup the autotuner tries: 16 and 15 (one workgroup per CU),
+// 8 (two), 5 (three) and 4 (four); the register budget allows 16 waves per
+// CU.
 constexpr int kWaveChoices[] = {16, 15, 8, 5, 4};
+
+// Rows per thread of a job's tile body (see k_fused_frame).
+int rows_per_thread(const LevelArgs& a) {
+    return (!a.delta && (a.img_mode == IMG_U8_DOWN || a.img_mode == IMG_U8_DOWN2)) ? kRowsTop : kRows;
+}
 
 size_t lds_bytes(int nw) { return (size_t)2 * nw * 10 * LANES * sizeof(float); }
 
@@ -858,13 +869,13 @@ bool fits_i32(const DevImg& d) {
     return (unsigned long long)d.pitch * (unsigned long long)(d.h > 0 ? d.h : 0) < 0x7fffffffULL;
 }
 
-// The tiling of a level with nw waves per workgroup leaves a non-empty
+// The tiling of a level with buffers of `rows` rows leaves a non-empty
 // output tile (or the level fits in one tile).
-bool tiles_with(const LevelArgs& a, int nw) {
+bool tiles_with(const LevelArgs& a, int rows) {
     int H, W;
     level_size(a, H, W);
     const int h0 = halo_of(a);
-    return !((W > LANES && 2 * h0 >= LANES) || (H > nw * kR && 2 * h0 >= nw * kR));
+    return !((W > LANES && 2 * h0 >= LANES) || (H > rows && 2 * h0 >= rows));
 }
 
 }  // namespace
@@ -874,7 +885,7 @@ bool fused_level_supported(const LevelArgs& a) {
     int H, W;
     level_size(a, H, W);
     if (H < 1 || W < 1) return false;
-    if (!tiles_with(a, kMaxWaves)) return false;
+    if (!tiles_with(a, kMaxWaves * rows_per_thread(a))) return false;
     if (!a.img.data || !a.flow_s_in.data || !a.a0_in.data || !a.flow_s_out.data || !a.a0_out.data) return false;
     switch (a.img_mode) {
     case IMG_F32:
@@ -943,9 +954,9 @@ struct FusedFrame::Impl {
     unsigned long long uploads = 0;
     DeviceArray counters;
     int ncounters = 0;  // pairs
-    int waves = 0;      // fixed wave count, 0: autotune
+    int fixed = 0;  // waves per workgroup, 0: autotune
     std::vector<int> tuned_for;
-    int tuned_waves = 0;
+    int tuned = 0;
 
     int table(const std::vector<LevelArgs>& jobs, int nw);
     void run(int t, hipStream_t s) const;
@@ -976,8 +987,9 @@ int FusedFrame::Impl::table(const std::vector<LevelArgs>& jobs, int nw) {
     int ends[kGroups - 1] = {0, 0, 0, 0, 0, 0};
     for (size_t i = 0; i < jobs.size(); ++i) {
         const LevelArgs& a = jobs[order[i]];
-        if (!fused_level_supported(a) || !tiles_with(a, nw))
-            throw std::logic_error("fused frame: unsupported level parameters");
+        const int rows = nw * rows_per_thread(a);
+        if (!fused_level_supported(a) || !tiles_with(a, rows))
+            throw logic_error("fused frame: unsupported level parameters");
         LevelJob& p = rec[i];
         std::memset(&p, 0, sizeof(p));
         p.img = a.img.data;
@@ -1009,7 +1021,7 @@ int FusedFrame::Impl::table(const std::vector<LevelArgs>& jobs, int nw) {
         p.h0 = halo_of(a);
         // a level no wider (taller) than the buffer is one tile across (down)
         p.TW = p.W <= LANES ? p.W : LANES - 2 * p.h0;
-        p.TH = p.H <= nw * kR ? p.H : nw * kR - 2 * p.h0;
+        p.TH = p.H <= rows ? p.H : rows - 2 * p.h0;
         p.tiles_x = (p.W + p.TW - 1) / p.TW;
         p.tiles_y = (p.H + p.TH - 1) / p.TH;
         p.delta = a.delta;
@@ -1026,8 +1038,8 @@ int FusedFrame::Impl::table(const std::vector<LevelArgs>& jobs, int nw) {
     for (size_t i = 0; i < jobs.size(); ++i) {
         const int k = jobs[i].wait_on;
         if (k < 0) continue;
-        if (k >= (int)i) throw std::logic_error("fused frame: a level waits on a later level");
-        if (!jobs[i].delta || jobs[k].delta) throw std::logic_error("fused frame: only delta levels wait on a top level");
+        if (k >= (int)i) throw logic_error("fused frame: a level waits on a later level");
+        if (!jobs[i].delta || jobs[k].delta) throw logic_error("fused frame: only delta levels wait on a top level");
         LevelJob& c = rec[pos[i]];
         LevelJob& q = rec[pos[k]];
         c.wait_idx = pos[k];
@@ -1087,20 +1099,18 @@ int FusedFrame::Impl::table(const std::vector<LevelArgs>& jobs, int nw) {
 
 void FusedFrame::Impl::run(int ti, hipStream_t s) const {
     const Table& t = slot[ti & 1];
-    if (!t.blocks) throw std::logic_error("fused frame: launch of an unprepared table");
-    static const hipError_t lds_ok = hipFuncSetAttribute((const void*)k_fused_frame<kR>,
-                                                         hipFuncAttributeMaxDynamicSharedMemorySize,
-                                                         (int)lds_bytes(kMaxWaves));
+    if (!t.blocks) throw logic_error("fused frame: launch of an unprepared table");
+    static const hipError_t lds_ok = hipFuncSetAttribute(
+        (const void*)k_fused_frame, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes(kMaxWaves));
     if (lds_ok != hipSuccess) throw_hip(lds_ok, "hipFuncSetAttribute", __FILE__, __LINE__);
     ProfileScope ps("fused_frame", s);
     const char* base = (const char*)t.dev.data();
     const int timeout_idx = 2 * (ncounters - 1);
-    hipLaunchKernelGGL((k_fused_frame<kR>), dim3(t.blocks), dim3(t.nw * LANES), lds_bytes(t.nw), s,
-                       (const LevelJob*)base, (const int*)(base + t.map_offset), (unsigned*)counters.data(),
-                       timeout_idx, t.ends, t.nw);
+    hipLaunchKernelGGL(k_fused_frame, dim3(t.blocks), dim3(t.nw * LANES), lds_bytes(t.nw), s, (const LevelJob*)base,
+                       (const int*)(base + t.map_offset), (unsigned*)counters.data(), timeout_idx, t.ends, t.nw);
 }
 
-// Time one launch of every candidate wave count on the actual job table
+// Time the launches of every candidate wave count on the actual job table
 // and keep the fastest. A launch reads the current state and writes the
 // next one without committing it, so repeated launches recompute the same
 // output; every tiling gives bitwise the same results.
@@ -1113,7 +1123,7 @@ int FusedFrame::Impl::tune(const std::vector<LevelArgs>& jobs, hipStream_t s) {
     try {
         for (int nw : kWaveChoices) {
             bool ok = true;
-            for (const LevelArgs& a : jobs) ok = ok && tiles_with(a, nw);
+            for (const LevelArgs& a : jobs) ok = ok && tiles_with(a, nw * rows_per_thread(a));
             if (!ok) continue;
             const int t = table(jobs, nw);
             run(t, s);  // warm-up
@@ -1136,20 +1146,20 @@ int FusedFrame::Impl::tune(const std::vector<LevelArgs>& jobs, hipStream_t s) {
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    if (!best) throw std::logic_error("fused frame: no tiling fits the levels");
+    if (!best) throw logic_error("fused frame: no tiling fits the levels");
     return best;
 }
 
 int FusedFrame::prepare(const std::vector<LevelArgs>& jobs, hipStream_t s) {
-    if (jobs.empty()) throw std::invalid_argument("fused frame: no levels");
-    int nw = impl_->waves;
+    if (jobs.empty()) throw invalid_argument("fused frame: no levels");
+    int nw = impl_->fixed;
     if (!nw) {
         std::vector<int> k = tune_key(jobs);
-        if (k != impl_->tuned_for || !impl_->tuned_waves) {
-            impl_->tuned_waves = impl_->tune(jobs, s);
+        if (k != impl_->tuned_for || !impl_->tuned) {
+            impl_->tuned = impl_->tune(jobs, s);
             impl_->tuned_for = std::move(k);
         }
-        nw = impl_->tuned_waves;
+        nw = impl_->tuned;
     }
     return impl_->table(jobs, nw);
 }
@@ -1160,15 +1170,15 @@ int FusedFrame::blocks(int table) const { return impl_->slot[table & 1].blocks; 
 unsigned long long FusedFrame::token(int table) const { return impl_->slot[table & 1].token; }
 
 void FusedFrame::setWaves(int nw) {
-    if (nw != 0) {
-        bool ok = false;
-        for (int c : kWaveChoices) ok = ok || c == nw;
-        if (!ok) throw std::invalid_argument("fused frame: waves per workgroup must be 0 (autotune), 16, 15, 8, 5 or 4");
-    }
-    impl_->waves = nw;
+    bool ok = nw == 0;
+    for (int c : kWaveChoices) ok = ok || c == nw;
+    if (!ok)
+        throw invalid_argument("fused frame: waves per workgroup must be 0 (autotune), 16, 15, 8, 5 or 4, got " +
+                               std::to_string(nw));
+    impl_->fixed = nw;
 }
 
-int FusedFrame::waves() const { return impl_->waves ? impl_->waves : impl_->tuned_waves; }
+int FusedFrame::waves() const { return impl_->fixed ? impl_->fixed : impl_->tuned; }
 
 unsigned long long FusedFrame::timeouts() const {
     if (!impl_->ncounters) return 0;

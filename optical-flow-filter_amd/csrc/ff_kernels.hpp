@@ -129,8 +129,9 @@ public:
     // on stream s (unless a wave count was fixed with setWaves).
     int prepare(const std::vector<LevelArgs>& jobs, hipStream_t s);
     void launch(int table, hipStream_t s) const;
-    // Waves per workgroup (tile height / 8): 0 = autotune (default), or
-    // 16, 15, 8, 5, 4. waves() is the fixed or tuned value (0 before tuning).
+    // Waves per workgroup (a tile's buffer is 64 columns by waves * rows
+    // per thread rows): 0 = autotune (default), or 16, 15, 8, 5, 4. waves()
+    // reports the fixed or tuned value (0 before tuning).
     void setWaves(int nw);
     int waves() const;
     // Waits that gave up after the spin limit (a scheduling fault; results

@@ -48,10 +48,6 @@ class Stage(object):
     def _getf(self, fn):
         return N.get_float(fn, self.handle)
 
-
-def _hnd(img):
-    return img.handle if img is not None else None
-
     @property
     def tileWaves(self):
         """Filters only: tile height of the fused frame kernel in waves per
@@ -62,3 +58,7 @@ def _hnd(img):
     @tileWaves.setter
     def tileWaves(self, v):
         N.call("ff_filter_set_tile_waves", self.handle, int(v))
+
+
+def _hnd(img):
+    return img.handle if img is not None else None

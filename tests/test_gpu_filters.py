@@ -316,9 +316,10 @@ def test_fused_matches_oracle(ffo, shape, levels, maxflow, smooth, border):
             f.compute()
             if f is g:
                 _, n = N.profile_end()
-                # the first frame runs the stage kernels, every later frame
+                # the first frame runs the stage kernels, the second also
+                # times the candidate tile shapes, every later frame is
                 # one fused frame launch
-                assert n == (0 if t == 0 else 1), (t, n)
+                assert (n == 0) if t == 0 else ((n >= 1) if t == 1 else (n == 1)), (t, n)
         o.compute(img)
         want = o.downloadFlow()
         check_frame(t, g.getFlow(), want, "fused")
