@@ -126,6 +126,19 @@ __device__ __forceinline__ void st2(Rsrc r, int off, float a, float b) {
     const u32x2 v = {__float_as_uint(a), __float_as_uint(b)};
     __builtin_amdgcn_raw_buffer_store_b64(v, r, off, 0, 0);
 }
+// Device-coherent forms (cache policy sc1, as for agent-scope atomics): the
+// store is written through to memory and the load does not hit a stale line
+// in this XCD's caches. Used for the one buffer that one level writes and
+// another reads within the same launch (the top level's unsmoothed flow).
+constexpr int kCoherent = 16;
+__device__ __forceinline__ float2 ld2_coherent(Rsrc r, int off) {
+    const u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, kCoherent);
+    return make_float2(__uint_as_float(v.x), __uint_as_float(v.y));
+}
+__device__ __forceinline__ void st2_coherent(Rsrc r, int off, float a, float b) {
+    const u32x2 v = {__float_as_uint(a), __float_as_uint(b)};
+    __builtin_amdgcn_raw_buffer_store_b64(v, r, off, 0, kCoherent);
+}
 
 // ---------------------------------------------------------- lane shifts
 // Lane i receives lane i-1 (shr) or lane i+1 (shl). The zero-filling forms
@@ -195,12 +208,11 @@ __device__ __forceinline__ int xcd_tile(int b, int nt) {
 
 // ------------------------------------------------- completion counters
 
-// Wait until the producer job's tiles have all signalled, then make their
-// stores visible to this workgroup. Polling uses relaxed loads (an acquire
-// per poll would invalidate the XCD's L2 each time); one acquire follows the
-// wait, by the polling thread, and the workgroup barrier orders every
-// wave's later loads after it. The last consumer resets the counters for
-// the next launch.
+// Wait until the producer job's tiles have all signalled. The data they
+// published (the unsmoothed flow) is read with device-coherent loads after
+// the wait, so no cache invalidation is needed; the workgroup barrier
+// orders every wave's loads after the polling thread saw the count. The
+// last consumer resets the counters for the next launch.
 __device__ __forceinline__ void wait_upper(const LevelJob& p, unsigned* counters, int timeout_idx) {
     if (threadIdx.x == 0) {
         unsigned* done = counters + 2 * p.wait_idx;
@@ -213,7 +225,6 @@ __device__ __forceinline__ void wait_upper(const LevelJob& p, unsigned* counters
                 break;
             }
         }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         const unsigned u = __hip_atomic_fetch_add(used, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (u + 1 == (unsigned)p.wait_consumers) {
             __hip_atomic_store(done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -223,15 +234,15 @@ __device__ __forceinline__ void wait_upper(const LevelJob& p, unsigned* counters
     __syncthreads();
 }
 
-// Publish this tile's stores, then count the tile as done: the barrier
-// waits for every wave's stores to complete, and one agent-scope release
-// (an L2 write-back) precedes the count.
+// Count this tile as done once its device-coherent stores have completed:
+// every wave waits for its outstanding stores (vmcnt 0; the immediate
+// leaves the other counters unwaited), the barrier joins the waves, and one
+// thread increments the counter.
 __device__ __forceinline__ void signal_done(const LevelJob& p, unsigned* counters) {
+    __builtin_amdgcn_s_waitcnt(0x0F70);
     __syncthreads();
-    if (threadIdx.x == 0) {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    if (threadIdx.x == 0)
         __hip_atomic_fetch_add(counters + 2 * p.signal_idx, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
 }
 
 // ------------------------------------------------------------ tile body
@@ -397,7 +408,7 @@ __device__ __forceinline__ void model_update(State<DELTA, R>& s, const float (&i
             ox = not_finite(ox) ? 0.0f : ox;
             oy = not_finite(oy) ? 0.0f : oy;
             st1(ra0, offa[r], a0);
-            if (p.flow_upd) st2(rfu, offf[r], ox, oy);
+            if (p.flow_upd) st2_coherent(rfu, offf[r], ox, oy);
         }
         s.fu[r] = ox;
         s.fv[r] = oy;
@@ -745,7 +756,7 @@ __device__ __forceinline__ void level_tile(const LevelJob& p, int tile, float* x
         offa[r] = out ? y * p.a0_out_pitch + g.x * 4 : OOB;
         if constexpr (DELTA) {
             offf[r] = out ? y * p.dflow_out_pitch + g.x * 8 : OOB;
-            up[r] = ld2(rup, p.ymap[yl] * p.upper_pitch + xmx * 8);
+            up[r] = ld2_coherent(rup, p.ymap[yl] * p.upper_pitch + xmx * 8);
             aold[r] = 0.f;
         } else {
             offf[r] = out ? y * p.flow_upd_pitch + g.x * 8 : OOB;
