@@ -74,8 +74,9 @@ def batched_bench(args, N, gf, GPUImage):
     """64 camera streams (synthetic.camera_streams, seed 3: 320x240 to
     1920x1080, 1-3 levels each) resident in HBM. One step loads every
     stream's next frame from device memory and advances all of them:
-    batched = one BatchedFlowFilter.compute() (one fused launch per step),
-    plain = each filter's own compute() in a loop. Returns aggregate level-0
+    batched = BatchedFlowFilter.loadImagesDevice (one copy launch) and
+    compute() (one fused launch), plain = each filter's loadImageDevice and
+    compute() in a loop. Returns aggregate level-0
     Mpix/s of both and their ratio."""
     import numpy as np
     from flowfilter import synthetic
@@ -99,10 +100,10 @@ def batched_bench(args, N, gf, GPUImage):
             fs.append(f)
         return fs
 
-    def load(fs, i):
-        for f, sp, g in zip(fs, streams, dev):
-            F = len(sp["frames"])
-            f.loadImageDevice(g.data + (i % F) * sp["h"] * g.pitch, g.pitch)
+    def frame_ptrs(i):
+        return [g.data + (i % len(sp["frames"])) * sp["h"] * g.pitch for sp, g in zip(streams, dev)]
+
+    pitches = [g.pitch for g in dev]
 
     px = sum(sp["h"] * sp["w"] for sp in streams)
     res = {}
@@ -111,11 +112,13 @@ def batched_bench(args, N, gf, GPUImage):
         batch = gf.BatchedFlowFilter(fs) if mode == "batched" else None
 
         def step(i):
-            load(fs, i)
+            ptrs = frame_ptrs(i)
             if batch is not None:
+                batch.loadImagesDevice(ptrs, pitches)
                 batch.compute()
             else:
-                for f in fs:
+                for f, p, q in zip(fs, ptrs, pitches):
+                    f.loadImageDevice(p, q)
                     f.compute()
 
         for i in range(5):
@@ -137,7 +140,7 @@ def batched_bench(args, N, gf, GPUImage):
             "speedup": round(res["batched"] / res["plain"], 3), "launch_blocks": res["blocks"],
             "wait_timeouts": res["timeouts"],
             "workload": "64 camera streams (seed 3): 320x240..1920x1080, 1-3 levels, gamma U[5,50], "
-                        "maxflow 4, smooth 1-2, affine motion; step = 64 device loadImage + compute"}
+                        "maxflow 4, smooth 1-2, affine motion; step = load 64 frames from HBM + advance 64 filters"}
 
 
 def copy_bench(args, N, GPUImage):

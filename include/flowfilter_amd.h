@@ -315,6 +315,12 @@ FF_API ff_status ff_filter_get_tile_waves(ff_stage* s, int* out);
 FF_API ff_status ff_batch_create(ff_stage** out);
 FF_API ff_status ff_batch_add(ff_stage* batch, ff_stage* pyramidal_filter);
 FF_API ff_status ff_batch_size(ff_stage* batch, int* out);
+/* Load the next frame of every filter of the batch (in the order added)
+ * from device memory: frames[i] (u8, row pitch pitches[i] bytes, the
+ * filter's height x width) into filter i's input image, with one copy
+ * launch for all of them; n must equal the batch size. Enqueued on the
+ * library stream, like ff_pyramidal_load_image_device. */
+FF_API ff_status ff_batch_load_images_device(ff_stage* batch, const void* const* frames, const size_t* pitches, int n);
 /* Workgroups of the last fused launch (diagnostic). */
 FF_API ff_status ff_batch_get_blocks(ff_stage* batch, int* out);
 

@@ -582,6 +582,9 @@ ff_status ff_batch_size(ff_stage* batch, int* out) {
         *out = as<BatchedFlowFilter>(batch, "BatchedFlowFilter").size();
     });
 }
+ff_status ff_batch_load_images_device(ff_stage* batch, const void* const* frames, const size_t* pitches, int n) {
+    return guard([&] { as<BatchedFlowFilter>(batch, "BatchedFlowFilter").loadImagesDevice(frames, pitches, n); });
+}
 ff_status ff_batch_get_blocks(ff_stage* batch, int* out) {
     return guard([&] {
         out_ptr(out, "blocks");

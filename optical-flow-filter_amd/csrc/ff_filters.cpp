@@ -680,6 +680,25 @@ int BatchedFlowFilter::add(const std::shared_ptr<PyramidalFlowFilter>& f) {
     return (int)streams_.size() - 1;
 }
 
+void BatchedFlowFilter::loadImagesDevice(const void* const* ptrs, const size_t* pitches, int n) {
+    if (n != (int)streams_.size())
+        throw invalid_argument("BatchedFlowFilter::loadImagesDevice(): " + num(n) + " frames for " +
+                               num((long)streams_.size()) + " streams");
+    if (n > 0 && (!ptrs || !pitches)) throw invalid_argument("BatchedFlowFilter::loadImagesDevice(): null array");
+    std::vector<CopyRegion> r;
+    r.reserve(n);
+    for (int i = 0; i < n; ++i) {
+        Image& in = streams_[i]->input_;
+        if (!ptrs[i]) throw invalid_argument("BatchedFlowFilter::loadImagesDevice(): null frame " + num(i));
+        if (pitches[i] < in.rowBytes())
+            throw invalid_argument("BatchedFlowFilter::loadImagesDevice(): pitch of frame " + num(i) +
+                                   " smaller than a row");
+        r.push_back(CopyRegion{ptrs[i], in.data(), pitches[i], in.pitch(), in.rowBytes(), in.height()});
+    }
+    launch_copy_regions(r.data(), n, ffamd::stream());
+    check_launch("BatchedFlowFilter::loadImagesDevice");
+}
+
 const std::shared_ptr<PyramidalFlowFilter>& BatchedFlowFilter::stream(int i) const {
     if (i < 0 || i >= (int)streams_.size())
         throw invalid_argument("BatchedFlowFilter::stream(): index out of range: " + num(i));

@@ -227,6 +227,9 @@ public:
     const char* name() const override { return "BatchedFlowFilter"; }
     int add(const std::shared_ptr<PyramidalFlowFilter>& f);
     int size() const { return (int)streams_.size(); }
+    // Load the next frame of every stream from device memory (u8, pitched)
+    // in one copy launch; n must equal size().
+    void loadImagesDevice(const void* const* ptrs, const size_t* pitches, int n);
     const std::shared_ptr<PyramidalFlowFilter>& stream(int i) const;
     void setGraphs(bool g) {
         graphs_on_ = g;

@@ -63,6 +63,16 @@ void launch_down_pass(int axis, bool u8, const DevImg& in, const DevImg& out, hi
 // unlike a memset/memcpy node). row_bytes bytes per row.
 void launch_copy2d(const DevImg& src, const DevImg& dst, size_t row_bytes, hipStream_t s);
 
+// Copies of several pitched 2-D byte regions in one launch (16-byte
+// accesses where every row of the region allows them).
+struct CopyRegion {
+    const void* src;
+    void* dst;
+    size_t src_pitch, dst_pitch, row_bytes;
+    int rows;
+};
+void launch_copy_regions(const CopyRegion* regions, int n, hipStream_t s);
+
 // ------------------------------------------------------------ fused frame
 // One kernel launch runs every level of a frame (and, for batched filters,
 // of many frames): each level is a job whose output tiles are workgroups of

@@ -1,6 +1,8 @@
 // ff_runtime.cpp -- see ff_runtime.hpp.
 #include "ff_runtime.hpp"
 
+#include "ff_kernels.hpp"
+
 #include <cstring>
 #include <mutex>
 #include <sstream>
@@ -124,7 +126,9 @@ void Image::upload_device(const void* dev, size_t pitch) {
     if (rowBytes() == 0 || h_ == 0) return;
     if (!dev) throw invalid_argument("GPUImage::upload_device(): null device pointer");
     if (pitch < rowBytes()) throw invalid_argument("GPUImage::upload_device(): pitch smaller than a row");
-    FF_HIP(hipMemcpy2DAsync(data(), pitch_, dev, pitch, rowBytes(), h_, hipMemcpyDeviceToDevice, stream()));
+    const CopyRegion r{dev, data(), pitch, pitch_, rowBytes(), h_};
+    launch_copy_regions(&r, 1, stream());
+    check_launch("GPUImage::upload_device");
 }
 
 void Image::download_device(void* dev, size_t pitch) const {

@@ -85,6 +85,35 @@ __global__ void __launch_bounds__(256) k_copy2d_u32(DevImg src, DevImg dst, size
     d[i] = s[i];
 }
 
+// Many 2-D copies in one launch: block b copies a 4 KiB chunk (256 threads
+// x 16 bytes, or x 4 / x 1 bytes when unaligned) of one row of one region.
+constexpr int kCopyMax = 64;
+struct CopyJob {
+    const char* src;
+    char* dst;
+    long long src_pitch, dst_pitch;
+    int row_bytes, rows, chunks, first_block, unit;  // unit: bytes per thread access
+};
+struct CopyBatch {
+    int n;
+    CopyJob job[kCopyMax];
+};
+
+__global__ void __launch_bounds__(256) k_copy_regions(const CopyBatch batch) {
+    int j = 0;
+    while (j + 1 < batch.n && (int)blockIdx.x >= batch.job[j + 1].first_block) ++j;
+    const CopyJob& c = batch.job[j];
+    const int b = (int)blockIdx.x - c.first_block;
+    const int row = b / c.chunks, chunk = b - row * c.chunks;
+    const char* s = c.src + row * c.src_pitch;
+    char* d = c.dst + row * c.dst_pitch;
+    const int off = (chunk * 256 + (int)threadIdx.x) * c.unit;
+    if (off >= c.row_bytes) return;
+    if (c.unit == 16) *reinterpret_cast<uint4*>(d + off) = *reinterpret_cast<const uint4*>(s + off);
+    else if (c.unit == 4) *reinterpret_cast<uint32_t*>(d + off) = *reinterpret_cast<const uint32_t*>(s + off);
+    else d[off] = s[off];
+}
+
 // ------------------------------------------------------------ propagation
 
 // flowPropagate{X,Y}_k and flowPropagatePayload{X,Y}_k
@@ -279,6 +308,32 @@ void launch_down_pass(int axis, bool u8, const DevImg& in, const DevImg& out, hi
     } else {
         if (u8) hipLaunchKernelGGL((k_down<1, true>), g, b, 0, s, in, out);
         else hipLaunchKernelGGL((k_down<1, false>), g, b, 0, s, in, out);
+    }
+}
+
+void launch_copy_regions(const CopyRegion* regions, int n, hipStream_t s) {
+    for (int first = 0; first < n; first += kCopyMax) {
+        CopyBatch b{};
+        int blocks = 0;
+        for (int i = first; i < n && b.n < kCopyMax; ++i) {
+            const CopyRegion& r = regions[i];
+            if (r.rows <= 0 || r.row_bytes == 0) continue;
+            const uintptr_t a = (uintptr_t)r.src | (uintptr_t)r.dst | r.src_pitch | r.dst_pitch | r.row_bytes;
+            CopyJob& c = b.job[b.n++];
+            c.src = (const char*)r.src;
+            c.dst = (char*)r.dst;
+            c.src_pitch = (long long)r.src_pitch;
+            c.dst_pitch = (long long)r.dst_pitch;
+            c.row_bytes = (int)r.row_bytes;
+            c.rows = r.rows;
+            c.unit = (a % 16 == 0) ? 16 : ((a % 4 == 0) ? 4 : 1);
+            c.chunks = (int)((r.row_bytes + 256 * c.unit - 1) / (256 * c.unit));
+            c.first_block = blocks;
+            blocks += c.chunks * c.rows;
+        }
+        if (!blocks) continue;
+        ProfileScope ps("copy_regions", s);
+        hipLaunchKernelGGL(k_copy_regions, dim3(blocks), dim3(256), 0, s, b);
     }
 }
 

@@ -198,6 +198,7 @@ SIGNATURES = {
     "ff_batch_create": [_pS],
     "ff_batch_add": [_S, _S],
     "ff_batch_size": [_S, _pi],
+    "ff_batch_load_images_device": [_S, ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_size_t), _i],
     "ff_batch_get_blocks": [_S, _pi],
     # profiling
     "ff_profile_begin": [ctypes.c_char_p],

@@ -338,6 +338,16 @@ class BatchedFlowFilter(Stage):
     def __len__(self):
         return N.get_int("ff_batch_size", self.handle)
 
+    def loadImagesDevice(self, ptrs, pitches):
+        """Next frame of every filter (in the order added) from device
+        memory, in one copy launch: ptrs[i] with row pitch pitches[i]."""
+        n = len(ptrs)
+        if len(pitches) != n:
+            raise ValueError("loadImagesDevice: %d pointers, %d pitches" % (n, len(pitches)))
+        p = (ctypes.c_void_p * max(n, 1))(*[int(v) for v in ptrs])
+        q = (ctypes.c_size_t * max(n, 1))(*[int(v) for v in pitches])
+        N.call("ff_batch_load_images_device", self.handle, p, q, n)
+
     def __getitem__(self, i):
         return self._filters[i]
 

@@ -125,3 +125,31 @@ def test_batch_errors():
         batch.add(f)
     with pytest.raises(TypeError):
         batch.add(gf.FlowFilter(64, 64))
+
+
+def test_batch_device_frames_match_host_loads(ffo):
+    """loadImagesDevice (one copy launch for all streams) gives the same
+    results as loading each filter's frame from the host."""
+    import flowfilter.gpu.flowfilters as gf
+    from flowfilter.gpu.image import GPUImage
+    n = 4
+    specs = STREAMS[:5]
+    a = gf.BatchedFlowFilter([make(*sp) for sp in specs])
+    b = gf.BatchedFlowFilter([make(*sp) for sp in specs])
+    seqs = [frames_for(20 + i, sp[0], sp[1], n) for i, sp in enumerate(specs)]
+    dev = []
+    for i, sp in enumerate(specs):
+        g = GPUImage((n * sp[0], sp[1]), 1)
+        g.upload(np.concatenate(seqs[i], axis=0))
+        dev.append(g)
+    for t in range(n):
+        a.loadImagesDevice([g.data + t * sp[0] * g.pitch for g, sp in zip(dev, specs)], [g.pitch for g in dev])
+        for i in range(len(specs)):
+            b[i].loadImage(seqs[i][t])
+        a.compute()
+        b.compute()
+        for i in range(len(specs)):
+            assert np.array_equal(a[i].getFlow(), b[i].getFlow()), (i, t)
+    from flowfilter import errors
+    with pytest.raises(errors.InvalidArgument):
+        a.loadImagesDevice([dev[0].data], [dev[0].pitch])
