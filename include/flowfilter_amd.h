@@ -282,8 +282,11 @@ FF_API ff_status ff_pyramidal_load_image_device(ff_stage* s, const void* dev, si
  * identical results. Applies to FlowFilter and PyramidalFlowFilter. */
 FF_API ff_status ff_filter_set_fused(ff_stage* s, int fused);
 FF_API ff_status ff_filter_get_fused(ff_stage* s, int* out);
-/* PyramidalFlowFilter only: replay each fused frame as a captured HIP graph
- * (1, the default) or launch its kernels one by one (0). */
+/* PyramidalFlowFilter and batches: replay each fused frame as a captured
+ * HIP graph (1, the default) or launch its kernels one by one (0). Frames
+ * that are a single launch (pyramids of up to three levels) are launched
+ * directly either way: replaying a one-node graph costs more than the
+ * launch. */
 FF_API ff_status ff_filter_set_graphs(ff_stage* s, int graphs);
 FF_API ff_status ff_filter_get_graphs(ff_stage* s, int* out);
 /* PyramidalFlowFilter only (diagnostic, no reference counterpart): number of

@@ -575,7 +575,11 @@ void run_maybe_graphed(bool graphs_on, GraphCache& cache, const std::vector<Leve
 void PyramidalFlowFilter::computeFused(const std::vector<LevelArgs>& args) {
     hipStream_t s = stream();
     const int table = frame_.prepare(args, s);
-    run_maybe_graphed(graphs_on_, graphs_, args, frame_.token(table), s, [&] { enqueueFused(table, s); });
+    // a graph pays off only when the frame is more than the one launch
+    // (pyramid kernels for more than three levels); replaying a one-node
+    // graph costs more than launching the kernel
+    run_maybe_graphed(graphs_on_ && levels_ > 3, graphs_, args, frame_.token(table), s,
+                      [&] { enqueueFused(table, s); });
     commitFused();
 }
 
@@ -729,7 +733,9 @@ void BatchedFlowFilter::compute() {
         hipStream_t s = ffamd::stream();
         const int table = frame_.prepare(all, s);
         last_blocks_ = frame_.blocks(table);
-        run_maybe_graphed(graphs_on_, graphs_, all, frame_.token(table), s, [&] {
+        bool multi = false;  // more than the one launch (see PyramidalFlowFilter::computeFused)
+        for (PyramidalFlowFilter* f : fused) multi = multi || f->levels() > 3;
+        run_maybe_graphed(graphs_on_ && multi, graphs_, all, frame_.token(table), s, [&] {
             for (PyramidalFlowFilter* f : fused) f->enqueuePreFused();
             frame_.launch(table, s);
             check_launch("BatchedFlowFilter::compute (fused frame)");

@@ -85,33 +85,53 @@ __global__ void __launch_bounds__(256) k_copy2d_u32(DevImg src, DevImg dst, size
     d[i] = s[i];
 }
 
-// Many 2-D copies in one launch: block b copies a 4 KiB chunk (256 threads
-// x 16 bytes, or x 4 / x 1 bytes when unaligned) of one row of one region.
+// Many 2-D copies in one launch. A region is viewed as rows * (row_bytes /
+// unit) accesses of `unit` bytes (16, or 4 / 1 when rows are not aligned
+// for more); block b of a region copies kCopyPerBlock consecutive accesses,
+// each thread kCopyUnroll of them with all loads issued before the stores.
 constexpr int kCopyMax = 64;
+constexpr int kCopyUnroll = 8;
+constexpr int kCopyPerBlock = 256 * kCopyUnroll;
 struct CopyJob {
     const char* src;
     char* dst;
     long long src_pitch, dst_pitch;
-    int row_bytes, rows, chunks, first_block, unit;  // unit: bytes per thread access
+    int per_row, total, first_block, unit;  // accesses per row and in all
 };
 struct CopyBatch {
     int n;
     CopyJob job[kCopyMax];
 };
 
+template <class T>
+__device__ __forceinline__ void copy_units(const CopyJob& c, int base) {
+    T v[kCopyUnroll];
+#pragma unroll
+    for (int k = 0; k < kCopyUnroll; ++k) {
+        const int i = base + k * 256;
+        if (i < c.total) {
+            const int row = i / c.per_row, col = i - row * c.per_row;
+            v[k] = reinterpret_cast<const T*>(c.src + row * c.src_pitch)[col];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < kCopyUnroll; ++k) {
+        const int i = base + k * 256;
+        if (i < c.total) {
+            const int row = i / c.per_row, col = i - row * c.per_row;
+            reinterpret_cast<T*>(c.dst + row * c.dst_pitch)[col] = v[k];
+        }
+    }
+}
+
 __global__ void __launch_bounds__(256) k_copy_regions(const CopyBatch batch) {
     int j = 0;
     while (j + 1 < batch.n && (int)blockIdx.x >= batch.job[j + 1].first_block) ++j;
     const CopyJob& c = batch.job[j];
-    const int b = (int)blockIdx.x - c.first_block;
-    const int row = b / c.chunks, chunk = b - row * c.chunks;
-    const char* s = c.src + row * c.src_pitch;
-    char* d = c.dst + row * c.dst_pitch;
-    const int off = (chunk * 256 + (int)threadIdx.x) * c.unit;
-    if (off >= c.row_bytes) return;
-    if (c.unit == 16) *reinterpret_cast<uint4*>(d + off) = *reinterpret_cast<const uint4*>(s + off);
-    else if (c.unit == 4) *reinterpret_cast<uint32_t*>(d + off) = *reinterpret_cast<const uint32_t*>(s + off);
-    else d[off] = s[off];
+    const int base = ((int)blockIdx.x - c.first_block) * kCopyPerBlock + (int)threadIdx.x;
+    if (c.unit == 16) copy_units<uint4>(c, base);
+    else if (c.unit == 4) copy_units<uint32_t>(c, base);
+    else copy_units<uint8_t>(c, base);
 }
 
 // ------------------------------------------------------------ propagation
@@ -324,12 +344,11 @@ void launch_copy_regions(const CopyRegion* regions, int n, hipStream_t s) {
             c.dst = (char*)r.dst;
             c.src_pitch = (long long)r.src_pitch;
             c.dst_pitch = (long long)r.dst_pitch;
-            c.row_bytes = (int)r.row_bytes;
-            c.rows = r.rows;
             c.unit = (a % 16 == 0) ? 16 : ((a % 4 == 0) ? 4 : 1);
-            c.chunks = (int)((r.row_bytes + 256 * c.unit - 1) / (256 * c.unit));
+            c.per_row = (int)(r.row_bytes / c.unit);
+            c.total = c.per_row * r.rows;
             c.first_block = blocks;
-            blocks += c.chunks * c.rows;
+            blocks += (c.total + kCopyPerBlock - 1) / kCopyPerBlock;
         }
         if (!blocks) continue;
         ProfileScope ps("copy_regions", s);
