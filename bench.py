@@ -15,8 +15,12 @@ synchronized before and after.
 
 Prints one JSON line: Mpix/s of level-0 input pixels, the roofline of the
 dominant kernel (algorithmic bytes per launch / its average duration from
-HIP events on its stream, over a second pass of the same K steps), and a
-CPU baseline (the FP32 CPU oracle on the host cores, bounded sample).
+HIP events on its stream, over a second pass of the same K steps; `traffic`
+is the HBM bytes per launch measured with rocprofv3 FETCH_SIZE/WRITE_SIZE
+passes, calibrated on a copy kernel, read from
+profiles/fused_frame_2048x1088_2lvl_traffic.json), the 64-stream batched
+measurement, and a CPU baseline (the FP32 CPU oracle on the host cores,
+bounded sample).
 """
 import argparse
 import json
@@ -240,10 +244,18 @@ def main():
     timed(args.warmup + args.steps)
     tot_ms, launches = N.profile_end()
     avg_ms = tot_ms / max(launches, 1)
-    traffic = algorithmic_bytes()[dominant]
-    achieved = traffic / (avg_ms / 1e3) / 1e9
+    alg_bytes = algorithmic_bytes()[dominant]
+    achieved = alg_bytes / (avg_ms / 1e3) / 1e9
     print("dominant kernel %s: %d launches, %.4f ms avg, %.1f GB/s" % (dominant, launches, avg_ms, achieved),
           file=sys.stderr, flush=True)
+
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "fused_frame_2048x1088_2lvl_traffic.json")
+    if dominant == "fused_frame" and os.path.exists(tf):
+        with open(tf) as fh:
+            t = json.load(fh)
+        if t.get("algorithmic_bytes_per_launch") == algorithmic_bytes()[dominant]:
+            traffic = t["traffic_bytes_per_launch"]
 
     out = {
         "metric": "Mpix/s at 2048x1088 2-lvl; HBM GB/s % of peak; batched/plain speedup",
@@ -263,8 +275,8 @@ def main():
                    "path": "staged" if args.staged else "fused",
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
-                     "algorithmic_bytes_per_launch": traffic, "avg_launch_ms": round(avg_ms, 5),
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
+                     "algorithmic_bytes_per_launch": alg_bytes, "avg_launch_ms": round(avg_ms, 5),
                      "launches": launches},
     }
     if not args.no_batched:

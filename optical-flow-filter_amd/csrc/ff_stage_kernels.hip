@@ -90,7 +90,7 @@ __global__ void __launch_bounds__(256) k_copy2d_u32(DevImg src, DevImg dst, size
 // for more); block b of a region copies kCopyPerBlock consecutive accesses,
 // each thread kCopyUnroll of them with all loads issued before the stores.
 constexpr int kCopyMax = 64;
-constexpr int kCopyUnroll = 8;
+constexpr int kCopyUnroll = 2;
 constexpr int kCopyPerBlock = 256 * kCopyUnroll;
 struct CopyJob {
     const char* src;
