@@ -327,6 +327,78 @@ FF_API ff_status ff_batch_load_images_device(ff_stage* batch, const void* const*
 /* Workgroups of the last fused launch (diagnostic). */
 FF_API ff_status ff_batch_get_blocks(ff_stage* batch, int* out);
 
+/* ---------------------------------------------------------------------
+ * Differentiable form (no reference counterpart). Batched stages of the
+ * filter on float images and their adjoints (vector-Jacobian products),
+ * for the autograd function of the Python package
+ * (flowfilter.autograd). All arrays are device memory, B images of H x W,
+ * contiguous: scalar fields float[B][H][W], vector fields (flow, image
+ * gradient) float2[B][H][W]. `stream` is a hipStream_t (NULL: the null
+ * stream). Forward stages compute what the corresponding stage kernels
+ * compute (same arithmetic); each *_backward takes the forward inputs and
+ * the gradient of the outputs and writes the gradient of the inputs
+ * (added to the destination when `accumulate` is nonzero). Upwind
+ * directions, clamps and the zeroing of non-finite values are held at
+ * their forward values.
+ * ------------------------------------------------------------------- */
+/* Number of per-block partial sums per image written by the update
+ * backwards' gamma gradient (g_gamma_partial holds B times this). */
+FF_API ff_status ff_ad_partial_count(int height, int width, int* out);
+/* Image model (imagemodel_k.cu:31-133): a0 and grad (gx, gy) of a float
+ * image. Backward: work holds 2*B*H*W floats. */
+FF_API ff_status ff_ad_image_model(const float* image, float* a0, float* grad, int B, int H, int W, void* stream);
+FF_API ff_status ff_ad_image_model_backward(const float* g_a0, const float* g_grad, float* g_image, float* work, int B,
+                                            int H, int W, int accumulate, void* stream);
+/* One float pyramid step (pyramid_k.cu:81-145): half is (H/2) x (W/2);
+ * work holds B*H*(W/2) floats. */
+FF_API ff_status ff_ad_down(const float* image, float* half, float* work, int B, int H, int W, void* stream);
+FF_API ff_status ff_ad_down_backward(const float* g_half, float* g_image, float* work, int B, int H, int W,
+                                     int accumulate, void* stream);
+/* One upwind pass along axis 0 (x) or 1 (y) (propagation_k.cu:16-334) of
+ * the flow and, when scalar/vector are given, of a scalar and a vector
+ * payload; border >= 1. */
+FF_API ff_status ff_ad_propagate_pass(int axis, const float* flow, const float* scalar, const float* vector,
+                                      float* flow_out, float* scalar_out, float* vector_out, int B, int H, int W,
+                                      float dt, int border, void* stream);
+FF_API ff_status ff_ad_propagate_pass_backward(int axis, const float* flow, const float* scalar, const float* vector,
+                                               const float* g_flow_out, const float* g_scalar_out,
+                                               const float* g_vector_out, float* g_flow, float* g_scalar,
+                                               float* g_vector, int B, int H, int W, float dt, int border,
+                                               void* stream);
+/* Top-level update (update_k.cu:15-79). Backward writes the gradients of
+ * a0, grad, a0_old and flow and B*ff_ad_partial_count partial sums of the
+ * gamma gradient. */
+FF_API ff_status ff_ad_update(const float* a0, const float* grad, const float* a0_old, const float* flow,
+                              float* flow_out, int B, int H, int W, float gamma, float maxflow, void* stream);
+FF_API ff_status ff_ad_update_backward(const float* a0, const float* grad, const float* a0_old, const float* flow,
+                                       const float* g_flow_out, float* g_a0, float* g_grad, float* g_a0_old,
+                                       float* g_flow, float* g_gamma_partial, int B, int H, int W, float gamma,
+                                       float maxflow, int accumulate, void* stream);
+/* Delta-level update (update_k.cu:82-173); upper is the upper level's flow
+ * (Hu x Wu), xmap/ymap the upper texel index of every column/row.
+ * Backward leaves the upper-flow gradient per pixel of this level in
+ * g_upper_pixel (float2[B][H][W]); ff_ad_upper_backward sums it onto the
+ * upper level using, for every upper column X (row Y), the range
+ * [xlo[X], xhi[X]) ([ylo[Y], yhi[Y])) of this level's columns (rows) that
+ * map to it. */
+FF_API ff_status ff_ad_delta_update(const float* a0, const float* grad, const float* a0_prop, const float* dflow_prop,
+                                    const float* upper, const int* xmap, const int* ymap, float* dflow_out,
+                                    float* flow_out, int B, int H, int W, int Hu, int Wu, float gamma, float maxflow,
+                                    void* stream);
+FF_API ff_status ff_ad_delta_update_backward(const float* a0, const float* grad, const float* a0_prop,
+                                             const float* dflow_prop, const float* upper, const int* xmap,
+                                             const int* ymap, const float* g_dflow_out, const float* g_flow_out,
+                                             float* g_a0, float* g_grad, float* g_a0_prop, float* g_dflow_prop,
+                                             float* g_upper_pixel, float* g_gamma_partial, int B, int H, int W, int Hu,
+                                             int Wu, float gamma, float maxflow, int accumulate, void* stream);
+FF_API ff_status ff_ad_upper_backward(const float* g_upper_pixel, const int* xlo, const int* xhi, const int* ylo,
+                                      const int* yhi, float* g_upper, int B, int H, int W, int Hu, int Wu,
+                                      int accumulate, void* stream);
+/* One box-smoothing pass along axis 0 (x) or 1 (y) (flowsmoothing_k.cu:18-87). */
+FF_API ff_status ff_ad_smooth_pass(int axis, const float* flow, float* flow_out, int B, int H, int W, void* stream);
+FF_API ff_status ff_ad_smooth_pass_backward(int axis, const float* g_flow_out, float* g_flow, int B, int H, int W,
+                                            int accumulate, void* stream);
+
 /* Kernel profiling (no reference counterpart; the reference times whole
  * stages only, src/gpu/pipeline.cu:63-89). Between begin and end, every
  * kernel launch whose name contains `filter` is bracketed by HIP events on

@@ -1,0 +1,583 @@
+// ff_adjoint.hip -- differentiable form of the filter (gfx950): batched
+// forward stage kernels on float images and their hand-written adjoints.
+//
+// Data: B images of H x W, contiguous (scalar fields [B][H][W] float,
+// vector fields [B][H][W] float2). Forward kernels use the same per-pixel
+// arithmetic as the stage kernels (ff_device.hpp), so a float-image filter
+// built from them gives the same flow as the filter itself. Each backward
+// kernel computes the transpose of its forward stage's Jacobian at the
+// forward inputs (vector-Jacobian product); piecewise choices (upwind
+// direction, clamps, the sanitising of non-finite values) are held at the
+// forward values, as any derivative of the filter must.
+//
+// The backward kernels gather (each output of the adjoint sums the
+// contributions that reach it), so they are deterministic and need no
+// atomics; outputs are written, or added to the destination with
+// `accumulate` set.
+#include <hip/hip_runtime.h>
+
+#include "ff_device.hpp"
+#include "ff_kernels.hpp"
+#include "ff_runtime.hpp"
+
+namespace ffamd {
+
+namespace {
+
+constexpr int BX = 64, BY = 4;
+
+struct Dims {
+    int H, W;
+};
+
+__device__ __forceinline__ size_t at(int b, int y, int x, const Dims& d) {
+    return ((size_t)b * d.H + y) * d.W + x;
+}
+
+inline dim3 grid3(int B, int H, int W) { return dim3((W + BX - 1) / BX, (H + BY - 1) / BY, B); }
+
+__device__ __forceinline__ void put(float* p, size_t i, float v, bool acc) { p[i] = acc ? p[i] + v : v; }
+__device__ __forceinline__ void put2(float2* p, size_t i, float2 v, bool acc) {
+    if (acc) {
+        const float2 o = p[i];
+        v = make_float2(o.x + v.x, o.y + v.y);
+    }
+    p[i] = v;
+}
+
+// Adjoint of a 5-tap correlation with clamped reads along one axis:
+// y[i] = sum_c w[c] v[clamp(i + c)], c = -2..2, i in [0, n). Returns
+// dv[j] = sum over (i, c) with clamp(i + c) == j of w[c] dy[i], where dy(i)
+// reads the output adjoint at position i.
+template <class Wf, class Dy>
+__device__ __forceinline__ float corr5_adjoint(int j, int n, Wf w, Dy dy) {
+    float acc = 0.0f;
+    for (int i = j - 2; i <= j + 2; ++i) {
+        if (i < 0 || i >= n) continue;
+#pragma unroll
+        for (int c = -2; c <= 2; ++c)
+            if (clampi(i + c, 0, n - 1) == j) acc = fmaf(w(c + 2), dy(i), acc);
+    }
+    // (an output i reaches j only if |i - j| <= 2, clamped or not)
+    return acc;
+}
+
+struct WS {
+    __device__ __forceinline__ float operator()(int i) const { return smooth_w(i); }
+};
+struct WD {
+    __device__ __forceinline__ float operator()(int i) const { return diff_w(i); }
+};
+struct WB {
+    __device__ __forceinline__ float operator()(int) const { return kBox; }
+};
+
+// ------------------------------------------------------------ image model
+
+// imagePrefilter_k + imageModel_k on a float image (imagemodel_k.cu:31-133).
+__global__ void __launch_bounds__(BX* BY) k_ad_image_model(const float* I, float* a0, float2* g, Dims d) {
+    const int x = blockIdx.x * BX + threadIdx.x, y = blockIdx.y * BY + threadIdx.y, b = blockIdx.z;
+    if (x >= d.W || y >= d.H) return;
+    float dx = 0.0f, dy = 0.0f, sm = 0.0f;
+#pragma unroll
+    for (int c = -2; c <= 2; ++c) {
+        const int xc = clampi(x + c, 0, d.W - 1);
+        float sy = 0.0f;
+#pragma unroll
+        for (int r = -2; r <= 2; ++r) sy = fmaf(smooth_w(r + 2), I[at(b, clampi(y + r, 0, d.H - 1), xc, d)], sy);
+        dx = fmaf(diff_w(c + 2), sy, dx);
+        sm = fmaf(smooth_w(c + 2), sy, sm);
+    }
+#pragma unroll
+    for (int r = -2; r <= 2; ++r) {
+        const int yr = clampi(y + r, 0, d.H - 1);
+        float sx = 0.0f;
+#pragma unroll
+        for (int c = -2; c <= 2; ++c) sx = fmaf(smooth_w(c + 2), I[at(b, yr, clampi(x + c, 0, d.W - 1), d)], sx);
+        dy = fmaf(diff_w(r + 2), sx, dy);
+    }
+    a0[at(b, y, x, d)] = sm;
+    g[at(b, y, x, d)] = make_float2(dx, dy);
+}
+
+// a0 = Sx(sy), gx = Dx(sy), gy = Dy(sx) with sy = Sy(I), sx = Sx(I): the
+// adjoint first forms g_sy = Sx^T g_a0 + Dx^T g_gx and g_sx = Dy^T g_gy.
+__global__ void __launch_bounds__(BX* BY)
+    k_ad_image_model_bwd1(const float* ga0, const float2* gg, float* gsy, float* gsx, Dims d) {
+    const int x = blockIdx.x * BX + threadIdx.x, y = blockIdx.y * BY + threadIdx.y, b = blockIdx.z;
+    if (x >= d.W || y >= d.H) return;
+    const float s1 = corr5_adjoint(x, d.W, WS{}, [&](int i) { return ga0[at(b, y, i, d)]; });
+    const float s2 = corr5_adjoint(x, d.W, WD{}, [&](int i) { return gg[at(b, y, i, d)].x; });
+    gsy[at(b, y, x, d)] = s1 + s2;
+    gsx[at(b, y, x, d)] = corr5_adjoint(y, d.H, WD{}, [&](int i) { return gg[at(b, i, x, d)].y; });
+}
+
+// g_I = Sy^T g_sy + Sx^T g_sx.
+__global__ void __launch_bounds__(BX* BY)
+    k_ad_image_model_bwd2(const float* gsy, const float* gsx, float* gI, Dims d, int acc) {
+    const int x = blockIdx.x * BX + threadIdx.x, y = blockIdx.y * BY + threadIdx.y, b = blockIdx.z;
+    if (x >= d.W || y >= d.H) return;
+    const float a = corr5_adjoint(y, d.H, WS{}, [&](int i) { return gsy[at(b, i, x, d)]; });
+    const float c = corr5_adjoint(x, d.W, WS{}, [&](int i) { return gsx[at(b, y, i, d)]; });
+    put(gI, at(b, y, x, d), a + c, acc);
+}
+
+// ----------------------------------------------------------- float pyramid
+
+// imageDownX_float_k / imageDownY_float_k (pyramid_k.cu:81-145): out of size
+// (H, W/2) from (H, W) along x, or (H/2, W) from (H, W) along y.
+template <int AXIS>
+__global__ void __launch_bounds__(BX* BY) k_ad_down(const float* in, float* out, Dims di, Dims dout) {
+    const int x = blockIdx.x * BX + threadIdx.x, y = blockIdx.y * BY + threadIdx.y, b = blockIdx.z;
+    if (x >= dout.W || y >= dout.H) return;
+    float tm, t0, tp;
+    if (AXIS == 0) {
+        tm = in[at(b, y, clampi(2 * x - 1, 0, di.W - 1), di)];
+        t0 = in[at(b, y, clampi(2 * x, 0, di.W - 1), di)];
+        tp = in[at(b, y, clampi(2 * x + 1, 0, di.W - 1), di)];
+    } else {
+        tm = in[at(b, clampi(2 * y - 1, 0, di.H - 1), x, di)];
+        t0 = in[at(b, clampi(2 * y, 0, di.H - 1), x, di)];
+        tp = in[at(b, clampi(2 * y + 1, 0, di.H - 1), x, di)];
+    }
+    const float s = tm + tp;
+    out[at(b, y, x, dout)] = (float)(0.5 * (double)t0 + 0.25 * (double)s);
+}
+
+// Adjoint of k_ad_down: input j receives 0.5 from the output whose centre
+// tap it is and 0.25 from outputs whose side taps (clamped) read it.
+template <int AXIS>
+__global__ void __launch_bounds__(BX* BY) k_ad_down_bwd(const float* gout, float* gin, Dims di, Dims dout, int acc) {
+    const int x = blockIdx.x * BX + threadIdx.x, y = blockIdx.y * BY + threadIdx.y, b = blockIdx.z;
+    if (x >= di.W || y >= di.H) return;
+    const int j = AXIS == 0 ? x : y, n = AXIS == 0 ? di.W : di.H, no = AXIS == 0 ? dout.W : dout.H;
+    float s = 0.0f;
+    for (int o = (j - 1) / 2 - 1; o <= j / 2 + 1; ++o) {
+        if (o < 0 || o >= no) continue;
+        const float go = AXIS == 0 ? gout[at(b, y, o, dout)] : gout[at(b, o, x, dout)];
+        float w = 0.0f;
+        if (clampi(2 * o, 0, n - 1) == j) w += 0.5f;
+        if (clampi(2 * o - 1, 0, n - 1) == j) w += 0.25f;
+        if (clampi(2 * o + 1, 0, n - 1) == j) w += 0.25f;
+        s = fmaf(w, go, s);
+    }
+    put(gin, at(b, y, x, di), s, acc);
+}
+
+// ------------------------------------------------------------ propagation
+
+// One upwind pass of the flow and, with payloads, of a scalar and a vector
+// field (flowPropagate{X,Y}_k / flowPropagatePayload{X,Y}_k,
+// propagation_k.cu:16-334).
+template <int AXIS, bool PAY>
+__global__ void __launch_bounds__(BX* BY) k_ad_prop(const float2* fin, const float* sin, const float2* vin,
+                                                    float2* fout, float* sout, float2* vout, Dims d, float dt,
+                                                    int border) {
+    const int x = blockIdx.x * BX + threadIdx.x, y = blockIdx.y * BY + threadIdx.y, b = blockIdx.z;
+    if (x >= d.W || y >= d.H) return;
+    const int ym = AXIS == 1 ? clampi(y - 1, 0, d.H - 1) : y, yp = AXIS == 1 ? clampi(y + 1, 0, d.H - 1) : y;
+    const int xm = AXIS == 0 ? clampi(x - 1, 0, d.W - 1) : x, xp = AXIS == 0 ? clampi(x + 1, 0, d.W - 1) : x;
+    const size_t im = at(b, ym, xm, d), i0 = at(b, y, x, d), ip = at(b, yp, xp, d);
+    const float2 fm = fin[im], f0 = fin[i0], fp = fin[ip];
+    const float um = AXIS == 0 ? fm.x : fm.y, up = AXIS == 0 ? fp.x : fp.y;
+    const float Ud = (fabsf(up) - fabsf(um)) > 0.0f ? up : um;
+    const float nd = -(dt * Ud);
+    const bool back = Ud >= 0.0f;
+    const bool in = (x >= border && x < d.W - border) && (y >= border && y < d.H - border);
+    fout[i0] = in ? make_float2(upwind(fm.x, f0.x, fp.x, back, nd), upwind(fm.y, f0.y, fp.y, back, nd)) : f0;
+    if constexpr (PAY) {
+        sout[i0] = in ? upwind(sin[im], sin[i0], sin[ip], back, nd) : sin[i0];
+        const float2 vm = vin[im], v0 = vin[i0], vp = vin[ip];
+        vout[i0] = in ? make_float2(upwind(vm.x, v0.x, vp.x, back, nd), upwind(vm.y, v0.y, vp.y, back, nd)) : v0;
+    }
+}
+
+// Adjoint of one upwind pass. Output o = z + nd * D with nd = -dt*Ud and
+// D = back ? z - m : p - z, for every field, where Ud = (|p_u| > |m_u|) ?
+// p_u : m_u is read from the flow component along the pass (u). Holding
+// `back` and the choice of Ud fixed:
+//   d o/d z = 1 + nd*(back ? 1 : -1), d o/d m = back ? -nd : 0,
+//   d o/d p = back ? 0 : nd, d o/d Ud = -dt * D (for each field),
+// and Ud's derivative goes to p_u or m_u. Pixels within the border copy z.
+// Input j gathers from the outputs at j-1, j, j+1 along the pass (for
+// interior outputs the neighbour reads are unclamped: border >= 1).
+template <int AXIS, bool PAY>
+__global__ void __launch_bounds__(BX* BY)
+    k_ad_prop_bwd(const float2* fin, const float* sin, const float2* vin, const float2* gfo, const float* gso,
+                  const float2* gvo, float2* gfi, float* gsi, float2* gvi, Dims d, float dt, int border) {
+    const int x = blockIdx.x * BX + threadIdx.x, y = blockIdx.y * BY + threadIdx.y, b = blockIdx.z;
+    if (x >= d.W || y >= d.H) return;
+    const int n = AXIS == 0 ? d.W : d.H, j = AXIS == 0 ? x : y;
+    auto idx = [&](int k) { return AXIS == 0 ? at(b, y, k, d) : at(b, k, x, d); };
+    auto interior = [&](int k) {
+        const int xx = AXIS == 0 ? k : x, yy = AXIS == 0 ? y : k;
+        return (xx >= border && xx < d.W - border) && (yy >= border && yy < d.H - border);
+    };
+    float2 gf = make_float2(0.f, 0.f), gv = make_float2(0.f, 0.f);
+    float gs = 0.f;
+    for (int o = j - 1; o <= j + 1; ++o) {
+        if (o < 0 || o >= n) continue;
+        const size_t io = idx(o);
+        const float2 g_f = gfo[io];
+        const float g_s = PAY ? gso[io] : 0.f;
+        const float2 g_v = PAY ? gvo[io] : make_float2(0.f, 0.f);
+        if (!interior(o)) {
+            if (o == j) {  // out = z
+                gf.x += g_f.x;
+                gf.y += g_f.y;
+                if (PAY) {
+                    gs += g_s;
+                    gv.x += g_v.x;
+                    gv.y += g_v.y;
+                }
+            }
+            continue;
+        }
+        const size_t im = idx(o - 1), i0 = io, ip = idx(o + 1);
+        const float2 fm = fin[im], f0 = fin[i0], fp = fin[ip];
+        const float um = AXIS == 0 ? fm.x : fm.y, up = AXIS == 0 ? fp.x : fp.y;
+        const bool selp = (fabsf(up) - fabsf(um)) > 0.0f;
+        const float Ud = selp ? up : um;
+        const float nd = -(dt * Ud);
+        const bool back = Ud >= 0.0f;
+        // coefficient of this input in output o, by position of j in {m, z, p}
+        const float c = (j == o) ? (1.0f + (back ? nd : -nd)) : ((j == o - 1) ? (back ? -nd : 0.f) : (back ? 0.f : nd));
+        gf.x = fmaf(c, g_f.x, gf.x);
+        gf.y = fmaf(c, g_f.y, gf.y);
+        // adjoint of Ud: sum over fields of g_out * (-dt * D)
+        auto D = [&](float m, float z, float p) { return back ? (z - m) : (p - z); };
+        float gUd = g_f.x * D(fm.x, f0.x, fp.x) + g_f.y * D(fm.y, f0.y, fp.y);
+        if (PAY) {
+            const float sm = sin[im], s0 = sin[i0], sp = sin[ip];
+            const float2 vm = vin[im], v0 = vin[i0], vp = vin[ip];
+            gs = fmaf(c, g_s, gs);
+            gv.x = fmaf(c, g_v.x, gv.x);
+            gv.y = fmaf(c, g_v.y, gv.y);
+            gUd += g_s * D(sm, s0, sp) + g_v.x * D(vm.x, v0.x, vp.x) + g_v.y * D(vm.y, v0.y, vp.y);
+        }
+        gUd *= -dt;
+        // Ud is the u component of the neighbour on the chosen side
+        const bool from = selp ? (j == o + 1) : (j == o - 1);
+        if (from) {
+            if (AXIS == 0) gf.x += gUd;
+            else gf.y += gUd;
+        }
+    }
+    gfi[at(b, y, x, d)] = gf;
+    if constexpr (PAY) {
+        gsi[at(b, y, x, d)] = gs;
+        gvi[at(b, y, x, d)] = gv;
+    }
+}
+
+// ----------------------------------------------------------------- update
+
+// Adjoint of the closed-form solve (update_k.cu:43-63) at (a, b) = (gx, gy):
+// given g_ox, g_oy, adds the gradient of (a, b, Yt, fx, fy, gamma).
+struct SolveGrad {
+    float a, b, Y, fx, fy, gamma;
+};
+__device__ __forceinline__ SolveGrad solve2x2_adjoint(float a, float b, float Y, float fx, float fy, float gamma,
+                                                      float gox, float goy) {
+    const float ax2 = a * a, ay2 = b * b;
+    const float N00 = gamma + ay2, N01 = -a * b, N11 = gamma + ax2;
+    const float den = gamma * ((gamma + ax2) + ay2);
+    const float r = 1.0f / den;
+    const float qx = fmaf(gamma, fx, a * Y), qy = fmaf(gamma, fy, b * Y);
+    const float ux = N00 * qx + N01 * qy, uy = N01 * qx + N11 * qy;
+    const float gux = gox * r, guy = goy * r;
+    const float gr = gox * ux + goy * uy;
+    const float gN00 = gux * qx, gN01 = gux * qy + guy * qx, gN11 = guy * qy;
+    const float gqx = gux * N00 + guy * N01, gqy = gux * N01 + guy * N11;
+    const float gden = -gr * r * r;
+    SolveGrad g;
+    g.gamma = gden * (2.0f * gamma + ax2 + ay2) + gN00 + gN11 + gqx * fx + gqy * fy;
+    g.a = gden * gamma * 2.0f * a + gN11 * 2.0f * a - gN01 * b + gqx * Y;
+    g.b = gden * gamma * 2.0f * b + gN00 * 2.0f * b - gN01 * a + gqy * Y;
+    g.Y = gqx * a + gqy * b;
+    g.fx = gqx * gamma;
+    g.fy = gqy * gamma;
+    return g;
+}
+
+__device__ __forceinline__ bool inside(float v, float lim) { return v > -lim && v < lim; }
+
+// flowUpdate_k (update_k.cu:15-79) on a batch.
+__global__ void __launch_bounds__(BX* BY) k_ad_update(const float* a0, const float2* g, const float* aold,
+                                                      const float2* f, float2* fout, Dims d, float gamma,
+                                                      float maxflow) {
+    const int x = blockIdx.x * BX + threadIdx.x, y = blockIdx.y * BY + threadIdx.y, b = blockIdx.z;
+    if (x >= d.W || y >= d.H) return;
+    const size_t i = at(b, y, x, d);
+    const float2 gg = g[i], ff = f[i];
+    float ox, oy;
+    solve2x2(gg.x, gg.y, aold[i] - a0[i], ff.x, ff.y, gamma, ox, oy);
+    ox = fmaxf(-maxflow, fminf(ox, maxflow));
+    oy = fmaxf(-maxflow, fminf(oy, maxflow));
+    ox = not_finite(ox) ? 0.0f : ox;
+    oy = not_finite(oy) ? 0.0f : oy;
+    fout[i] = make_float2(ox, oy);
+}
+
+// Block sum of v into partial[blockid] (one value per workgroup).
+__device__ __forceinline__ void block_sum(float v, float* partial) {
+    __shared__ float red[BX * BY];
+    const int t = threadIdx.y * BX + threadIdx.x;
+    red[t] = v;
+    __syncthreads();
+    for (int s = (BX * BY) / 2; s > 0; s >>= 1) {
+        if (t < s) red[t] += red[t + s];
+        __syncthreads();
+    }
+    if (t == 0) partial[((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = red[0];
+}
+
+__global__ void __launch_bounds__(BX* BY)
+    k_ad_update_bwd(const float* a0, const float2* g, const float* aold, const float2* f, const float2* gout,
+                    float* ga0, float2* gg, float* gaold, float2* gf, float* gamma_partial, Dims d, float gamma,
+                    float maxflow, int acc) {
+    const int x = blockIdx.x * BX + threadIdx.x, y = blockIdx.y * BY + threadIdx.y, b = blockIdx.z;
+    float ggam = 0.0f;
+    if (x < d.W && y < d.H) {
+        const size_t i = at(b, y, x, d);
+        const float2 gr = g[i], ff = f[i];
+        const float Y = aold[i] - a0[i];
+        float ox, oy;
+        solve2x2(gr.x, gr.y, Y, ff.x, ff.y, gamma, ox, oy);
+        // clamp, then sanitize: the derivative passes where the value was
+        // finite and strictly inside the clamp
+        const float2 go = gout[i];
+        const float gox = (inside(ox, maxflow) && !not_finite(ox)) ? go.x : 0.0f;
+        const float goy = (inside(oy, maxflow) && !not_finite(oy)) ? go.y : 0.0f;
+        const SolveGrad s = solve2x2_adjoint(gr.x, gr.y, Y, ff.x, ff.y, gamma, gox, goy);
+        put(ga0, i, -s.Y, acc);
+        put(gaold, i, s.Y, acc);
+        put2(gg, i, make_float2(s.a, s.b), acc);
+        put2(gf, i, make_float2(s.fx, s.fy), acc);
+        ggam = s.gamma;
+    }
+    block_sum(ggam, gamma_partial);
+}
+
+// deltaFlowUpdate_k (update_k.cu:82-173) on a batch: writes the new delta
+// flow and the level's (unsmoothed) flow.
+__global__ void __launch_bounds__(BX* BY)
+    k_ad_delta_update(const float* a0, const float2* g, const float* sprop, const float2* dprop, const float2* upper,
+                      const int* xmap, const int* ymap, float2* dout, float2* fout, Dims d, Dims du, float gamma,
+                      float maxflow) {
+    const int x = blockIdx.x * BX + threadIdx.x, y = blockIdx.y * BY + threadIdx.y, b = blockIdx.z;
+    if (x >= d.W || y >= d.H) return;
+    const size_t i = at(b, y, x, d);
+    const float2 gg = g[i], dd = dprop[i];
+    float ox, oy;
+    solve2x2(gg.x, gg.y, sprop[i] - a0[i], dd.x, dd.y, gamma, ox, oy);
+    ox = not_finite(ox) ? 0.0f : ox;
+    oy = not_finite(oy) ? 0.0f : oy;
+    const float half = 0.5f * maxflow;
+    ox = fmaxf(-half, fminf(ox, half));
+    oy = fmaxf(-half, fminf(oy, half));
+    const float2 u = upper[at(b, ymap[y], xmap[x], du)];
+    float fx = ox + 2.0f * u.x, fy = oy + 2.0f * u.y;
+    fx = fmaxf(-maxflow, fminf(fx, maxflow));
+    fy = fmaxf(-maxflow, fminf(fy, maxflow));
+    dout[i] = make_float2(ox, oy);
+    fout[i] = make_float2(fx, fy);
+}
+
+// Adjoint of the delta update for the gradients of its new delta flow
+// (gdout) and flow (gfout). The upper-flow gradient is left per pixel of
+// this level (gup, 2 * the passed flow gradient); k_ad_upper_gather sums it
+// onto the upper level.
+__global__ void __launch_bounds__(BX* BY)
+    k_ad_delta_update_bwd(const float* a0, const float2* g, const float* sprop, const float2* dprop,
+                          const float2* upper, const int* xmap, const int* ymap, const float2* gdout,
+                          const float2* gfout, float* ga0, float2* gg, float* gsprop, float2* gdprop, float2* gup,
+                          float* gamma_partial, Dims d, Dims du, float gamma, float maxflow, int acc) {
+    const int x = blockIdx.x * BX + threadIdx.x, y = blockIdx.y * BY + threadIdx.y, b = blockIdx.z;
+    float ggam = 0.0f;
+    if (x < d.W && y < d.H) {
+        const size_t i = at(b, y, x, d);
+        const float2 gr = g[i], dd = dprop[i];
+        const float Y = sprop[i] - a0[i];
+        float sx, sy;
+        solve2x2(gr.x, gr.y, Y, dd.x, dd.y, gamma, sx, sy);
+        const bool fx_ok = !not_finite(sx), fy_ok = !not_finite(sy);
+        const float zx = fx_ok ? sx : 0.0f, zy = fy_ok ? sy : 0.0f;
+        const float half = 0.5f * maxflow;
+        const float ox = fmaxf(-half, fminf(zx, half)), oy = fmaxf(-half, fminf(zy, half));
+        const float2 u = upper[at(b, ymap[y], xmap[x], du)];
+        const float px_ = ox + 2.0f * u.x, py_ = oy + 2.0f * u.y;
+        const float2 gfo = gfout[i], gdo = gdout[i];
+        // through the final clamp
+        const float gpx = inside(px_, maxflow) ? gfo.x : 0.0f, gpy = inside(py_, maxflow) ? gfo.y : 0.0f;
+        put2(gup, i, make_float2(2.0f * gpx, 2.0f * gpy), false);
+        // o = clamp(z, half) feeds both outputs
+        const float gox = gpx + gdo.x, goy = gpy + gdo.y;
+        const float gzx = (fx_ok && inside(zx, half)) ? gox : 0.0f;
+        const float gzy = (fy_ok && inside(zy, half)) ? goy : 0.0f;
+        const SolveGrad s = solve2x2_adjoint(gr.x, gr.y, Y, dd.x, dd.y, gamma, gzx, gzy);
+        put(ga0, i, -s.Y, acc);
+        put(gsprop, i, s.Y, acc);
+        put2(gg, i, make_float2(s.a, s.b), acc);
+        put2(gdprop, i, make_float2(s.fx, s.fy), acc);
+        ggam = s.gamma;
+    }
+    block_sum(ggam, gamma_partial);
+}
+
+// Upper-level gradient: upper pixel (X, Y) sums the per-pixel gradients of
+// the pixels (x, y) of this level with xmap[x] == X and ymap[y] == Y; the
+// maps are non-decreasing, so those are the ranges [xlo[X], xhi[X]) and
+// [ylo[Y], yhi[Y]).
+__global__ void __launch_bounds__(BX* BY)
+    k_ad_upper_gather(const float2* gup, const int* xlo, const int* xhi, const int* ylo, const int* yhi, float2* gupper,
+                      Dims d, Dims du, int acc) {
+    const int X = blockIdx.x * BX + threadIdx.x, Y = blockIdx.y * BY + threadIdx.y, b = blockIdx.z;
+    if (X >= du.W || Y >= du.H) return;
+    float2 s = make_float2(0.f, 0.f);
+    for (int y = ylo[Y]; y < yhi[Y]; ++y)
+        for (int x = xlo[X]; x < xhi[X]; ++x) {
+            const float2 v = gup[at(b, y, x, d)];
+            s.x += v.x;
+            s.y += v.y;
+        }
+    put2(gupper, at(b, Y, X, du), s, acc);
+}
+
+// -------------------------------------------------------------- smoothing
+
+// flowSmoothX_k / flowSmoothY_k (flowsmoothing_k.cu:18-87) on a batch.
+template <int AXIS>
+__global__ void __launch_bounds__(BX* BY) k_ad_smooth(const float2* in, float2* out, Dims d) {
+    const int x = blockIdx.x * BX + threadIdx.x, y = blockIdx.y * BY + threadIdx.y, b = blockIdx.z;
+    if (x >= d.W || y >= d.H) return;
+    float sx = 0.0f, sy = 0.0f;
+#pragma unroll
+    for (int c = -2; c <= 2; ++c) {
+        const float2 v = AXIS == 0 ? in[at(b, y, clampi(x + c, 0, d.W - 1), d)] : in[at(b, clampi(y + c, 0, d.H - 1), x, d)];
+        sx = fmaf(kBox, v.x, sx);
+        sy = fmaf(kBox, v.y, sy);
+    }
+    out[at(b, y, x, d)] = make_float2(sx, sy);
+}
+
+template <int AXIS>
+__global__ void __launch_bounds__(BX* BY) k_ad_smooth_bwd(const float2* gout, float2* gin, Dims d, int acc) {
+    const int x = blockIdx.x * BX + threadIdx.x, y = blockIdx.y * BY + threadIdx.y, b = blockIdx.z;
+    if (x >= d.W || y >= d.H) return;
+    float2 s;
+    if (AXIS == 0) {
+        s.x = corr5_adjoint(x, d.W, WB{}, [&](int i) { return gout[at(b, y, i, d)].x; });
+        s.y = corr5_adjoint(x, d.W, WB{}, [&](int i) { return gout[at(b, y, i, d)].y; });
+    } else {
+        s.x = corr5_adjoint(y, d.H, WB{}, [&](int i) { return gout[at(b, i, x, d)].x; });
+        s.y = corr5_adjoint(y, d.H, WB{}, [&](int i) { return gout[at(b, i, x, d)].y; });
+    }
+    put2(gin, at(b, y, x, d), s, acc);
+}
+
+}  // namespace
+
+// ------------------------------------------------------------- launchers
+
+namespace ad {
+
+void image_model(const float* I, float* a0, float* g, int B, int H, int W, hipStream_t s) {
+    hipLaunchKernelGGL(k_ad_image_model, grid3(B, H, W), dim3(BX, BY), 0, s, I, a0, (float2*)g, Dims{H, W});
+}
+void image_model_bwd(const float* ga0, const float* gg, float* gI, float* work, int B, int H, int W, bool acc,
+                     hipStream_t s) {
+    const size_t n = (size_t)B * H * W;
+    hipLaunchKernelGGL(k_ad_image_model_bwd1, grid3(B, H, W), dim3(BX, BY), 0, s, ga0, (const float2*)gg, work,
+                       work + n, Dims{H, W});
+    hipLaunchKernelGGL(k_ad_image_model_bwd2, grid3(B, H, W), dim3(BX, BY), 0, s, work, work + n, gI, Dims{H, W},
+                       (int)acc);
+}
+void down(const float* I, float* O, float* work, int B, int H, int W, hipStream_t s) {
+    const Dims d0{H, W}, d1{H, W / 2}, d2{H / 2, W / 2};
+    hipLaunchKernelGGL((k_ad_down<0>), grid3(B, d1.H, d1.W), dim3(BX, BY), 0, s, I, work, d0, d1);
+    hipLaunchKernelGGL((k_ad_down<1>), grid3(B, d2.H, d2.W), dim3(BX, BY), 0, s, work, O, d1, d2);
+}
+void down_bwd(const float* gO, float* gI, float* work, int B, int H, int W, bool acc, hipStream_t s) {
+    const Dims d0{H, W}, d1{H, W / 2}, d2{H / 2, W / 2};
+    hipLaunchKernelGGL((k_ad_down_bwd<1>), grid3(B, d1.H, d1.W), dim3(BX, BY), 0, s, gO, work, d1, d2, 0);
+    hipLaunchKernelGGL((k_ad_down_bwd<0>), grid3(B, d0.H, d0.W), dim3(BX, BY), 0, s, work, gI, d0, d1, (int)acc);
+}
+void prop(int axis, const float* fin, const float* sin, const float* vin, float* fout, float* sout, float* vout, int B,
+          int H, int W, float dt, int border, hipStream_t s) {
+    const bool pay = sin != nullptr;
+    const auto g = grid3(B, H, W);
+    const dim3 blk(BX, BY);
+    const Dims d{H, W};
+    auto F = [](const float* p) { return (const float2*)p; };
+    if (axis == 0) {
+        if (pay) hipLaunchKernelGGL((k_ad_prop<0, true>), g, blk, 0, s, F(fin), sin, F(vin), (float2*)fout, sout, (float2*)vout, d, dt, border);
+        else hipLaunchKernelGGL((k_ad_prop<0, false>), g, blk, 0, s, F(fin), sin, F(vin), (float2*)fout, sout, (float2*)vout, d, dt, border);
+    } else {
+        if (pay) hipLaunchKernelGGL((k_ad_prop<1, true>), g, blk, 0, s, F(fin), sin, F(vin), (float2*)fout, sout, (float2*)vout, d, dt, border);
+        else hipLaunchKernelGGL((k_ad_prop<1, false>), g, blk, 0, s, F(fin), sin, F(vin), (float2*)fout, sout, (float2*)vout, d, dt, border);
+    }
+}
+void prop_bwd(int axis, const float* fin, const float* sin, const float* vin, const float* gfo, const float* gso,
+              const float* gvo, float* gfi, float* gsi, float* gvi, int B, int H, int W, float dt, int border,
+              hipStream_t s) {
+    const bool pay = sin != nullptr;
+    const auto g = grid3(B, H, W);
+    const dim3 blk(BX, BY);
+    const Dims d{H, W};
+    auto F = [](const float* p) { return (const float2*)p; };
+    auto O = [](float* p) { return (float2*)p; };
+    if (axis == 0) {
+        if (pay) hipLaunchKernelGGL((k_ad_prop_bwd<0, true>), g, blk, 0, s, F(fin), sin, F(vin), F(gfo), gso, F(gvo), O(gfi), gsi, O(gvi), d, dt, border);
+        else hipLaunchKernelGGL((k_ad_prop_bwd<0, false>), g, blk, 0, s, F(fin), sin, F(vin), F(gfo), gso, F(gvo), O(gfi), gsi, O(gvi), d, dt, border);
+    } else {
+        if (pay) hipLaunchKernelGGL((k_ad_prop_bwd<1, true>), g, blk, 0, s, F(fin), sin, F(vin), F(gfo), gso, F(gvo), O(gfi), gsi, O(gvi), d, dt, border);
+        else hipLaunchKernelGGL((k_ad_prop_bwd<1, false>), g, blk, 0, s, F(fin), sin, F(vin), F(gfo), gso, F(gvo), O(gfi), gsi, O(gvi), d, dt, border);
+    }
+}
+int blocks(int H, int W) { return ((W + BX - 1) / BX) * ((H + BY - 1) / BY); }
+void update(const float* a0, const float* g, const float* aold, const float* f, float* fout, int B, int H, int W,
+            float gamma, float maxflow, hipStream_t s) {
+    hipLaunchKernelGGL(k_ad_update, grid3(B, H, W), dim3(BX, BY), 0, s, a0, (const float2*)g, aold, (const float2*)f,
+                       (float2*)fout, Dims{H, W}, gamma, maxflow);
+}
+void update_bwd(const float* a0, const float* g, const float* aold, const float* f, const float* gout, float* ga0,
+                float* gg, float* gaold, float* gf, float* gamma_partial, int B, int H, int W, float gamma,
+                float maxflow, bool acc, hipStream_t s) {
+    hipLaunchKernelGGL(k_ad_update_bwd, grid3(B, H, W), dim3(BX, BY), 0, s, a0, (const float2*)g, aold,
+                       (const float2*)f, (const float2*)gout, ga0, (float2*)gg, gaold, (float2*)gf, gamma_partial,
+                       Dims{H, W}, gamma, maxflow, (int)acc);
+}
+void delta_update(const float* a0, const float* g, const float* sprop, const float* dprop, const float* upper,
+                  const int* xmap, const int* ymap, float* dout, float* fout, int B, int H, int W, int Hu, int Wu,
+                  float gamma, float maxflow, hipStream_t s) {
+    hipLaunchKernelGGL(k_ad_delta_update, grid3(B, H, W), dim3(BX, BY), 0, s, a0, (const float2*)g, sprop,
+                       (const float2*)dprop, (const float2*)upper, xmap, ymap, (float2*)dout, (float2*)fout,
+                       Dims{H, W}, Dims{Hu, Wu}, gamma, maxflow);
+}
+void delta_update_bwd(const float* a0, const float* g, const float* sprop, const float* dprop, const float* upper,
+                      const int* xmap, const int* ymap, const float* gdout, const float* gfout, float* ga0, float* gg,
+                      float* gsprop, float* gdprop, float* gup, float* gamma_partial, int B, int H, int W, int Hu,
+                      int Wu, float gamma, float maxflow, bool acc, hipStream_t s) {
+    hipLaunchKernelGGL(k_ad_delta_update_bwd, grid3(B, H, W), dim3(BX, BY), 0, s, a0, (const float2*)g, sprop,
+                       (const float2*)dprop, (const float2*)upper, xmap, ymap, (const float2*)gdout,
+                       (const float2*)gfout, ga0, (float2*)gg, gsprop, (float2*)gdprop, (float2*)gup, gamma_partial,
+                       Dims{H, W}, Dims{Hu, Wu}, gamma, maxflow, (int)acc);
+}
+void upper_gather(const float* gup, const int* xlo, const int* xhi, const int* ylo, const int* yhi, float* gupper,
+                  int B, int H, int W, int Hu, int Wu, bool acc, hipStream_t s) {
+    hipLaunchKernelGGL(k_ad_upper_gather, grid3(B, Hu, Wu), dim3(BX, BY), 0, s, (const float2*)gup, xlo, xhi, ylo, yhi,
+                       (float2*)gupper, Dims{H, W}, Dims{Hu, Wu}, (int)acc);
+}
+void smooth(int axis, const float* in, float* out, int B, int H, int W, hipStream_t s) {
+    if (axis == 0) hipLaunchKernelGGL((k_ad_smooth<0>), grid3(B, H, W), dim3(BX, BY), 0, s, (const float2*)in, (float2*)out, Dims{H, W});
+    else hipLaunchKernelGGL((k_ad_smooth<1>), grid3(B, H, W), dim3(BX, BY), 0, s, (const float2*)in, (float2*)out, Dims{H, W});
+}
+void smooth_bwd(int axis, const float* gout, float* gin, int B, int H, int W, bool acc, hipStream_t s) {
+    if (axis == 0) hipLaunchKernelGGL((k_ad_smooth_bwd<0>), grid3(B, H, W), dim3(BX, BY), 0, s, (const float2*)gout, (float2*)gin, Dims{H, W}, (int)acc);
+    else hipLaunchKernelGGL((k_ad_smooth_bwd<1>), grid3(B, H, W), dim3(BX, BY), 0, s, (const float2*)gout, (float2*)gin, Dims{H, W}, (int)acc);
+}
+
+}  // namespace ad
+
+}  // namespace ffamd

@@ -592,6 +592,185 @@ ff_status ff_batch_get_blocks(ff_stage* batch, int* out) {
     });
 }
 
+// ------------------------------------------------------ differentiable form
+namespace {
+void need(const void* p, const char* what) {
+    if (!p) throw invalid_argument(std::string("null pointer: ") + what);
+}
+void dims(int B, int H, int W) {
+    if (B <= 0 || H <= 0 || W <= 0)
+        throw invalid_argument("batch, height and width must be positive, got " + std::to_string(B) + ", " +
+                               std::to_string(H) + ", " + std::to_string(W));
+}
+hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+}  // namespace
+
+ff_status ff_ad_partial_count(int height, int width, int* out) {
+    return guard([&] {
+        out_ptr(out, "count");
+        dims(1, height, width);
+        *out = ad::blocks(height, width);
+    });
+}
+ff_status ff_ad_image_model(const float* image, float* a0, float* grad, int B, int H, int W, void* stream) {
+    return guard([&] {
+        dims(B, H, W);
+        need(image, "image"), need(a0, "a0"), need(grad, "grad");
+        ad::image_model(image, a0, grad, B, H, W, as_stream(stream));
+        check_launch("ff_ad_image_model");
+    });
+}
+ff_status ff_ad_image_model_backward(const float* g_a0, const float* g_grad, float* g_image, float* work, int B, int H,
+                                     int W, int accumulate, void* stream) {
+    return guard([&] {
+        dims(B, H, W);
+        need(g_a0, "g_a0"), need(g_grad, "g_grad"), need(g_image, "g_image"), need(work, "work");
+        ad::image_model_bwd(g_a0, g_grad, g_image, work, B, H, W, accumulate != 0, as_stream(stream));
+        check_launch("ff_ad_image_model_backward");
+    });
+}
+ff_status ff_ad_down(const float* image, float* half, float* work, int B, int H, int W, void* stream) {
+    return guard([&] {
+        dims(B, H / 2 > 0 ? H : 0, W / 2 > 0 ? W : 0);
+        need(image, "image"), need(half, "half"), need(work, "work");
+        ad::down(image, half, work, B, H, W, as_stream(stream));
+        check_launch("ff_ad_down");
+    });
+}
+ff_status ff_ad_down_backward(const float* g_half, float* g_image, float* work, int B, int H, int W, int accumulate,
+                              void* stream) {
+    return guard([&] {
+        dims(B, H / 2 > 0 ? H : 0, W / 2 > 0 ? W : 0);
+        need(g_half, "g_half"), need(g_image, "g_image"), need(work, "work");
+        ad::down_bwd(g_half, g_image, work, B, H, W, accumulate != 0, as_stream(stream));
+        check_launch("ff_ad_down_backward");
+    });
+}
+ff_status ff_ad_propagate_pass(int axis, const float* flow, const float* scalar, const float* vector, float* flow_out,
+                               float* scalar_out, float* vector_out, int B, int H, int W, float dt, int border,
+                               void* stream) {
+    return guard([&] {
+        dims(B, H, W);
+        if (axis != 0 && axis != 1) throw invalid_argument("axis must be 0 (x) or 1 (y)");
+        if (border < 1) throw invalid_argument("border must be at least 1");
+        need(flow, "flow"), need(flow_out, "flow_out");
+        if ((scalar == nullptr) != (vector == nullptr) || (scalar == nullptr) != (scalar_out == nullptr) ||
+            (vector == nullptr) != (vector_out == nullptr))
+            throw invalid_argument("payloads: pass all of scalar, vector and their outputs, or none");
+        ad::prop(axis, flow, scalar, vector, flow_out, scalar_out, vector_out, B, H, W, dt, border, as_stream(stream));
+        check_launch("ff_ad_propagate_pass");
+    });
+}
+ff_status ff_ad_propagate_pass_backward(int axis, const float* flow, const float* scalar, const float* vector,
+                                        const float* g_flow_out, const float* g_scalar_out, const float* g_vector_out,
+                                        float* g_flow, float* g_scalar, float* g_vector, int B, int H, int W, float dt,
+                                        int border, void* stream) {
+    return guard([&] {
+        dims(B, H, W);
+        if (axis != 0 && axis != 1) throw invalid_argument("axis must be 0 (x) or 1 (y)");
+        if (border < 1) throw invalid_argument("border must be at least 1");
+        need(flow, "flow"), need(g_flow_out, "g_flow_out"), need(g_flow, "g_flow");
+        const bool pay = scalar != nullptr;
+        if (pay && (!vector || !g_scalar_out || !g_vector_out || !g_scalar || !g_vector))
+            throw invalid_argument("payloads: pass all of scalar, vector, their output gradients and gradients");
+        ad::prop_bwd(axis, flow, scalar, vector, g_flow_out, g_scalar_out, g_vector_out, g_flow, g_scalar, g_vector, B,
+                     H, W, dt, border, as_stream(stream));
+        check_launch("ff_ad_propagate_pass_backward");
+    });
+}
+ff_status ff_ad_update(const float* a0, const float* grad, const float* a0_old, const float* flow, float* flow_out,
+                       int B, int H, int W, float gamma, float maxflow, void* stream) {
+    return guard([&] {
+        dims(B, H, W);
+        need(a0, "a0"), need(grad, "grad"), need(a0_old, "a0_old"), need(flow, "flow"), need(flow_out, "flow_out");
+        ad::update(a0, grad, a0_old, flow, flow_out, B, H, W, gamma, maxflow, as_stream(stream));
+        check_launch("ff_ad_update");
+    });
+}
+ff_status ff_ad_update_backward(const float* a0, const float* grad, const float* a0_old, const float* flow,
+                                const float* g_flow_out, float* g_a0, float* g_grad, float* g_a0_old, float* g_flow,
+                                float* g_gamma_partial, int B, int H, int W, float gamma, float maxflow,
+                                int accumulate, void* stream) {
+    return guard([&] {
+        dims(B, H, W);
+        for (const void* p : {(const void*)a0, (const void*)grad, (const void*)a0_old, (const void*)flow,
+                              (const void*)g_flow_out, (const void*)g_a0, (const void*)g_grad, (const void*)g_a0_old,
+                              (const void*)g_flow, (const void*)g_gamma_partial})
+            need(p, "argument");
+        ad::update_bwd(a0, grad, a0_old, flow, g_flow_out, g_a0, g_grad, g_a0_old, g_flow, g_gamma_partial, B, H, W,
+                       gamma, maxflow, accumulate != 0, as_stream(stream));
+        check_launch("ff_ad_update_backward");
+    });
+}
+ff_status ff_ad_delta_update(const float* a0, const float* grad, const float* a0_prop, const float* dflow_prop,
+                             const float* upper, const int* xmap, const int* ymap, float* dflow_out, float* flow_out,
+                             int B, int H, int W, int Hu, int Wu, float gamma, float maxflow, void* stream) {
+    return guard([&] {
+        dims(B, H, W);
+        dims(B, Hu, Wu);
+        for (const void* p : {(const void*)a0, (const void*)grad, (const void*)a0_prop, (const void*)dflow_prop,
+                              (const void*)upper, (const void*)xmap, (const void*)ymap, (const void*)dflow_out,
+                              (const void*)flow_out})
+            need(p, "argument");
+        ad::delta_update(a0, grad, a0_prop, dflow_prop, upper, xmap, ymap, dflow_out, flow_out, B, H, W, Hu, Wu, gamma,
+                         maxflow, as_stream(stream));
+        check_launch("ff_ad_delta_update");
+    });
+}
+ff_status ff_ad_delta_update_backward(const float* a0, const float* grad, const float* a0_prop,
+                                      const float* dflow_prop, const float* upper, const int* xmap, const int* ymap,
+                                      const float* g_dflow_out, const float* g_flow_out, float* g_a0, float* g_grad,
+                                      float* g_a0_prop, float* g_dflow_prop, float* g_upper_pixel,
+                                      float* g_gamma_partial, int B, int H, int W, int Hu, int Wu, float gamma,
+                                      float maxflow, int accumulate, void* stream) {
+    return guard([&] {
+        dims(B, H, W);
+        dims(B, Hu, Wu);
+        for (const void* p : {(const void*)a0, (const void*)grad, (const void*)a0_prop, (const void*)dflow_prop,
+                              (const void*)upper, (const void*)xmap, (const void*)ymap, (const void*)g_dflow_out,
+                              (const void*)g_flow_out, (const void*)g_a0, (const void*)g_grad, (const void*)g_a0_prop,
+                              (const void*)g_dflow_prop, (const void*)g_upper_pixel, (const void*)g_gamma_partial})
+            need(p, "argument");
+        ad::delta_update_bwd(a0, grad, a0_prop, dflow_prop, upper, xmap, ymap, g_dflow_out, g_flow_out, g_a0, g_grad,
+                             g_a0_prop, g_dflow_prop, g_upper_pixel, g_gamma_partial, B, H, W, Hu, Wu, gamma, maxflow,
+                             accumulate != 0, as_stream(stream));
+        check_launch("ff_ad_delta_update_backward");
+    });
+}
+ff_status ff_ad_upper_backward(const float* g_upper_pixel, const int* xlo, const int* xhi, const int* ylo,
+                               const int* yhi, float* g_upper, int B, int H, int W, int Hu, int Wu, int accumulate,
+                               void* stream) {
+    return guard([&] {
+        dims(B, H, W);
+        dims(B, Hu, Wu);
+        for (const void* p : {(const void*)g_upper_pixel, (const void*)xlo, (const void*)xhi, (const void*)ylo,
+                              (const void*)yhi, (const void*)g_upper})
+            need(p, "argument");
+        ad::upper_gather(g_upper_pixel, xlo, xhi, ylo, yhi, g_upper, B, H, W, Hu, Wu, accumulate != 0,
+                         as_stream(stream));
+        check_launch("ff_ad_upper_backward");
+    });
+}
+ff_status ff_ad_smooth_pass(int axis, const float* flow, float* flow_out, int B, int H, int W, void* stream) {
+    return guard([&] {
+        dims(B, H, W);
+        if (axis != 0 && axis != 1) throw invalid_argument("axis must be 0 (x) or 1 (y)");
+        need(flow, "flow"), need(flow_out, "flow_out");
+        ad::smooth(axis, flow, flow_out, B, H, W, as_stream(stream));
+        check_launch("ff_ad_smooth_pass");
+    });
+}
+ff_status ff_ad_smooth_pass_backward(int axis, const float* g_flow_out, float* g_flow, int B, int H, int W,
+                                     int accumulate, void* stream) {
+    return guard([&] {
+        dims(B, H, W);
+        if (axis != 0 && axis != 1) throw invalid_argument("axis must be 0 (x) or 1 (y)");
+        need(g_flow_out, "g_flow_out"), need(g_flow, "g_flow");
+        ad::smooth_bwd(axis, g_flow_out, g_flow, B, H, W, accumulate != 0, as_stream(stream));
+        check_launch("ff_ad_smooth_pass_backward");
+    });
+}
+
 // --------------------------------------------------------------- profiling
 ff_status ff_profile_begin(const char* filter) {
     return guard([&] { profile_begin(filter ? filter : ""); });

@@ -15,6 +15,12 @@ for p in (ROOT, PKG, os.path.join(ROOT, "oracle")):
     if p not in sys.path:
         sys.path.insert(0, p)
 
+# PyTorch before the library: torch carries its own copy of the HIP runtime
+# under the system runtime's soname, and whichever is loaded first is the
+# one the library binds to. Loading torch first gives the process a single
+# runtime, so the differentiable form can pass torch tensors to the library.
+import torch  # noqa: E402,F401
+
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an AMD GPU (MI355X) and the built HIP library")

@@ -1,0 +1,371 @@
+"""Differentiable form (flowfilter.autograd): forward stages against the
+CPU oracle, hand-written adjoints against torch autograd through the
+pure-PyTorch composition (tests/torch_composition.py, itself pinned to the
+oracle on the CPU), whole-window gradients against that composition and
+against float64 central finite differences.
+
+Bars:
+  * forward stages: bitwise equal to the oracle's float32 stages; the
+    single-level window bitwise equal to the oracle's u8 filter; deeper
+    windows within 1e-4 px of the float64 composition;
+  * stage adjoints: normwise relative error <= 1e-5 against float64 autograd;
+  * window gradients (frames and per-level gamma): normwise relative error
+    <= 1e-4 against float64 autograd of the composition;
+  * finite differences (float64 composition, central, step 1e-4) at 32
+    random frame coordinates and every gamma: |ad - fd| <= 1e-3 x
+    max(|fd|, 1e-2 x max |fd|) per coordinate (a relative bound, floored at
+    1 % of the largest derivative checked so that near-zero derivatives are
+    judged on an absolute scale);
+  * the checkpoint interval changes nothing (bitwise).
+"""
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+import torch_composition as tc
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ad():
+    import flowfilter.autograd as mod
+    assert torch.cuda.is_available()
+    return mod
+
+
+def cuda(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def rel(a, b):
+    a = a.detach().double().cpu()
+    b = b.detach().double().cpu()
+    return float((a - b).norm() / max(float(b.norm()), 1e-300))
+
+
+def rng(seed):
+    return np.random.default_rng(seed)
+
+
+# ---------------------------------------------------------------- stages
+
+def _call(ad, name, *args):
+    from flowfilter import _native as N
+    conv = [ad._p(a) if isinstance(a, torch.Tensor) else a for a in args]
+    N.call(name, *conv, ad._stream())
+
+
+def test_forward_stages_match_oracle(ad, ffo):
+    from flowfilter import _native as N
+    r = rng(1)
+    B, H, W = 2, 45, 70
+    img = r.uniform(0, 1, size=(B, H, W)).astype(np.float32)
+    I = cuda(img)
+    a0, g = torch.empty(B, H, W, device="cuda"), torch.empty(B, H, W, 2, device="cuda")
+    _call(ad, "ff_ad_image_model", I, a0, g, B, H, W)
+    half, work = torch.empty(B, H // 2, W // 2, device="cuda"), torch.empty(B, H, W // 2, device="cuda")
+    _call(ad, "ff_ad_down", I, half, work, B, H, W)
+    f = r.uniform(-3, 3, size=(B, H, W, 2)).astype(np.float32)
+    s = r.uniform(0, 1, size=(B, H, W)).astype(np.float32)
+    v = r.uniform(-1, 1, size=(B, H, W, 2)).astype(np.float32)
+    dt = float(np.float32(1) / np.float32(3))
+    props = {}
+    for axis in (0, 1):
+        fo, so, vo = torch.empty_like(cuda(f)), torch.empty_like(cuda(s)), torch.empty_like(cuda(v))
+        _call(ad, "ff_ad_propagate_pass", axis, cuda(f), cuda(s), cuda(v), fo, so, vo, B, H, W, ctypes_f(dt), 3)
+        props[axis] = (fo, so, vo)
+    gam = float(np.float32(10) / np.float32(65025))
+    a0o = r.uniform(0, 1, size=(B, H, W)).astype(np.float32)
+    gr = r.uniform(-0.2, 0.2, size=(B, H, W, 2)).astype(np.float32)
+    fu = torch.empty(B, H, W, 2, device="cuda")
+    _call(ad, "ff_ad_update", cuda(img), cuda(gr), cuda(a0o), cuda(f), fu, B, H, W, ctypes_f(gam), ctypes_f(4.0))
+    Hu, Wu = H // 2, W // 2
+    up = r.uniform(-2, 2, size=(B, Hu, Wu, 2)).astype(np.float32)
+    xm, ym = cuda(ad._index_map(W, Wu)), cuda(ad._index_map(H, Hu))
+    do, fl = torch.empty(B, H, W, 2, device="cuda"), torch.empty(B, H, W, 2, device="cuda")
+    _call(ad, "ff_ad_delta_update", cuda(img), cuda(gr), cuda(a0o), cuda(v), cuda(up), xm, ym, do, fl, B, H, W, Hu, Wu,
+          ctypes_f(gam), ctypes_f(4.0))
+    sm1, sm2 = torch.empty_like(cuda(f)), torch.empty_like(cuda(f))
+    _call(ad, "ff_ad_smooth_pass", 0, cuda(f), sm1, B, H, W)
+    _call(ad, "ff_ad_smooth_pass", 1, sm1, sm2, B, H, W)
+    torch.cuda.synchronize()
+    for b in range(B):
+        ra0, rg = ffo.image_model(img[b])
+        assert np.array_equal(a0[b].cpu().numpy(), ra0) and np.array_equal(g[b].cpu().numpy(), rg), b
+        assert np.array_equal(half[b].cpu().numpy(), ffo.pyramid(img[b], 2)[1]), b
+        for axis in (0, 1):
+            rf, rs, rv = ffo.propagate_pass(axis, f[b], s[b], v[b], dt=dt, border=3)
+            fo, so, vo = props[axis]
+            assert np.array_equal(fo[b].cpu().numpy(), rf), (axis, b)
+            assert np.array_equal(so[b].cpu().numpy(), rs), (axis, b)
+            assert np.array_equal(vo[b].cpu().numpy(), rv), (axis, b)
+        rfu, _ = ffo.update(img[b], gr[b], a0o[b], f[b], gam, 4.0)
+        assert np.array_equal(fu[b].cpu().numpy(), rfu), b
+        rd, rfl, _ = ffo.delta_update(img[b], gr[b], a0o[b], v[b], up[b], gam, 4.0)
+        assert np.array_equal(do[b].cpu().numpy(), rd), b
+        assert np.array_equal(fl[b].cpu().numpy(), rfl), b
+        assert np.array_equal(sm2[b].cpu().numpy(), ffo.smooth(f[b], 1)), b
+    N.synchronize()
+
+
+def ctypes_f(x):
+    import ctypes
+    return ctypes.c_float(x)
+
+
+def _vjp64(fn, inputs, cot):
+    """Gradients of sum(cot_i * out_i) w.r.t. inputs, float64 autograd."""
+    xs = [x.detach().double().cpu().requires_grad_(True) for x in inputs]
+    outs = fn(*xs)
+    outs = outs if isinstance(outs, (tuple, list)) else (outs,)
+    total = sum((o * c.double().cpu()).sum() for o, c in zip(outs, cot))
+    return torch.autograd.grad(total, xs, allow_unused=True)
+
+
+def test_stage_adjoints_match_autograd(ad):
+    r = rng(2)
+    B, H, W = 2, 38, 52
+    rnd = lambda *shape, lo=-1.0, hi=1.0: cuda(r.uniform(lo, hi, size=shape).astype(np.float32))
+    # image model
+    I = rnd(B, H, W, lo=0.0)
+    ga0, gg = rnd(B, H, W), rnd(B, H, W, 2)
+    gI, work = torch.empty(B, H, W, device="cuda"), torch.empty(B, 2 * H, W, device="cuda")
+    _call(ad, "ff_ad_image_model_backward", ga0, gg, gI, work, B, H, W, 0)
+    (ref,) = _vjp64(lambda x: tc.image_model(x), [I], [ga0, gg])
+    assert rel(gI, ref) <= 1e-5, ("image_model", rel(gI, ref))
+    # pyramid step
+    gh = rnd(B, H // 2, W // 2)
+    gI2, work2 = torch.empty(B, H, W, device="cuda"), torch.empty(B, H, W // 2, device="cuda")
+    _call(ad, "ff_ad_down_backward", gh, gI2, work2, B, H, W, 0)
+    (ref,) = _vjp64(tc.down, [I], [gh])
+    assert rel(gI2, ref) <= 1e-5, ("down", rel(gI2, ref))
+    # propagation passes, plain and with payloads, near ties and away
+    dt = float(np.float32(1) / np.float32(4))
+    for axis in (0, 1):
+        for pay in (False, True):
+            f, s, v = rnd(B, H, W, 2, lo=-4, hi=4), rnd(B, H, W, lo=0.0), rnd(B, H, W, 2)
+            gfo, gso, gvo = rnd(B, H, W, 2), rnd(B, H, W), rnd(B, H, W, 2)
+            gf, gs, gv = torch.empty_like(f), torch.empty_like(s), torch.empty_like(v)
+            if pay:
+                _call(ad, "ff_ad_propagate_pass_backward", axis, f, s, v, gfo, gso, gvo, gf, gs, gv, B, H, W,
+                      ctypes_f(dt), 3)
+                refs = _vjp64(lambda a, b_, c: tc.prop_pass(axis, a, b_, c, dt, 3), [f, s, v], [gfo, gso, gvo])
+                for mine, ref, nm in zip((gf, gs, gv), refs, "fsv"):
+                    assert rel(mine, ref) <= 1e-5, ("prop", axis, nm, rel(mine, ref))
+            else:
+                _call(ad, "ff_ad_propagate_pass_backward", axis, f, None, None, gfo, None, None, gf, None, None, B, H,
+                      W, ctypes_f(dt), 3)
+                (ref,) = _vjp64(lambda a: tc.prop_pass(axis, a, None, None, dt, 3)[0], [f], [gfo])
+                assert rel(gf, ref) <= 1e-5, ("prop", axis, rel(gf, ref))
+    # top update (with saturated pixels) and its gamma gradient
+    a0, a0o = rnd(B, H, W, lo=0.0), rnd(B, H, W, lo=0.0)
+    g = rnd(B, H, W, 2, lo=-0.2, hi=0.2)
+    f = rnd(B, H, W, 2, lo=-5, hi=5)
+    gam = float(np.float32(20) / np.float32(65025))
+    gout = rnd(B, H, W, 2)
+    ga0, gg, gao, gf = (torch.empty(B, H, W, device="cuda"), torch.empty(B, H, W, 2, device="cuda"),
+                        torch.empty(B, H, W, device="cuda"), torch.empty(B, H, W, 2, device="cuda"))
+    from flowfilter import _native as N
+    nb = N.get_int("ff_ad_partial_count", H, W)
+    part = torch.empty(B * nb, device="cuda")
+    _call(ad, "ff_ad_update_backward", a0, g, a0o, f, gout, ga0, gg, gao, gf, part, B, H, W, ctypes_f(gam),
+          ctypes_f(4.0), 0)
+    gam64 = torch.tensor(gam, dtype=torch.float64)
+    refs = _vjp64(lambda a, b_, c, d, e: tc.update(a, b_, c, d, e, 4.0), [a0, g, a0o, f, gam64], [gout])
+    for mine, ref, nm in zip((ga0, gg, gao, gf), refs, ("a0", "grad", "a0_old", "flow")):
+        assert rel(mine, ref) <= 1e-5, ("update", nm, rel(mine, ref))
+    assert abs(float(part.double().sum()) - float(refs[4])) <= 1e-4 * abs(float(refs[4])), "update gamma"
+    # delta update + the upper gather
+    Hu, Wu = H // 2, W // 2
+    sp, dp = rnd(B, H, W, lo=0.0), rnd(B, H, W, 2, lo=-3, hi=3)
+    up = rnd(B, Hu, Wu, 2, lo=-2, hi=2)
+    xm, ym = ad._index_map(W, Wu), ad._index_map(H, Hu)
+    xlo, xhi = ad._ranges(xm, Wu)
+    ylo, yhi = ad._ranges(ym, Hu)
+    xm, ym, xlo, xhi, ylo, yhi = (cuda(x) for x in (xm, ym, xlo, xhi, ylo, yhi))
+    gdo, gfo = rnd(B, H, W, 2), rnd(B, H, W, 2)
+    ga0, gg, gsp, gdp, gup = (torch.empty(B, H, W, device="cuda"), torch.empty(B, H, W, 2, device="cuda"),
+                              torch.empty(B, H, W, device="cuda"), torch.empty(B, H, W, 2, device="cuda"),
+                              torch.empty(B, H, W, 2, device="cuda"))
+    _call(ad, "ff_ad_delta_update_backward", a0, g, sp, dp, up, xm, ym, gdo, gfo, ga0, gg, gsp, gdp, gup, part, B, H,
+          W, Hu, Wu, ctypes_f(gam), ctypes_f(4.0), 0)
+    gupper = torch.empty(B, Hu, Wu, 2, device="cuda")
+    _call(ad, "ff_ad_upper_backward", gup, xlo, xhi, ylo, yhi, gupper, B, H, W, Hu, Wu, 0)
+    refs = _vjp64(lambda a, b_, c, d, e, gm: tc.delta_update(a, b_, c, d, e, xm.cpu(), ym.cpu(), gm, 4.0),
+                  [a0, g, sp, dp, up, gam64], [gdo, gfo])
+    for mine, ref, nm in zip((ga0, gg, gsp, gdp, gupper), refs, ("a0", "grad", "a0_prop", "dflow_prop", "upper")):
+        assert rel(mine, ref) <= 1e-5, ("delta_update", nm, rel(mine, ref))
+    assert abs(float(part.double().sum()) - float(refs[5])) <= 1e-4 * abs(float(refs[5])), "delta gamma"
+    # smoothing passes
+    for axis in (0, 1):
+        gsm = rnd(B, H, W, 2)
+        gin = torch.empty_like(gsm)
+        _call(ad, "ff_ad_smooth_pass_backward", axis, gsm, gin, B, H, W, 0)
+        dim = 2 if axis == 0 else 1
+        (ref,) = _vjp64(lambda x: sum(tc.BOX * tc.sh(x, dim, k) for k in range(-2, 3)), [gsm], [gsm])
+        assert rel(gin, ref) <= 1e-5, ("smooth", axis, rel(gin, ref))
+
+
+# ---------------------------------------------------------------- windows
+
+def _frames(B, T, H, W, seed, speed=2.0):
+    from flowfilter import synthetic
+    r = rng(seed)
+    out, vel = [], []
+    for b in range(B):
+        v = tuple(r.uniform(-speed, speed, size=2))
+        vel.append(v)
+        out.append(np.stack(list(synthetic.translating_noise(H, W, T, v, seed=seed * 100 + b))))
+    return (torch.from_numpy(np.stack(out).astype(np.float32) / np.float32(255.0)).cuda(),
+            torch.tensor(vel, dtype=torch.float32).cuda())
+
+
+def test_single_level_window_matches_oracle(ad, ffo):
+    """L = 1 on u8-valued frames is the plain filter: bitwise equal."""
+    B, T, H, W = 2, 5, 60, 84
+    frames, _ = _frames(B, T, H, W, 3)
+    spec = ad.FilterSpec(H, W, 1, maxflow=3.5, smooth=2)
+    gamma = torch.tensor([12.0], device="cuda")
+    with torch.no_grad():
+        flows = ad.flow_window(frames, gamma, spec, checkpoint=2)
+    torch.cuda.synchronize()
+    u8 = np.rint(frames.cpu().numpy() * 255.0).astype(np.uint8)
+    for b in range(B):
+        o = ffo.FlowFilter(H, W, smoothIterations=2, maxflow=3.5, gamma=12.0)
+        for t in range(T):
+            o.compute(u8[b, t])
+            assert np.array_equal(flows[b, t].cpu().numpy(), o.downloadFlow()), (b, t)
+
+
+@pytest.mark.parametrize("levels", [2, 3])
+def test_window_forward_matches_composition(ad, levels):
+    B, T, H, W = 2, 5, 96, 128
+    frames, _ = _frames(B, T, H, W, 4)
+    spec = ad.FilterSpec(H, W, levels, maxflow=4.0, smooth=[2, 1, 1][:levels])
+    gamma = torch.tensor([10.0, 30.0, 50.0][:levels], device="cuda")
+    with torch.no_grad():
+        mine = ad.flow_window(frames, gamma, spec)
+        ref = tc.window(spec, frames.double(), gamma.double())
+    err = float((mine.double() - ref).abs().max())
+    say("levels", levels, "max |flow - float64 composition|", err)
+    assert err <= 1e-4
+
+
+def _gradients(ad, spec, frames, gamma, weights, checkpoint):
+    x = frames.clone().requires_grad_(True)
+    gm = gamma.clone().requires_grad_(True)
+    flows = ad.flow_window(x, gm, spec, checkpoint=checkpoint)
+    (flows * weights).sum().backward()
+    return x.grad, gm.grad
+
+
+@pytest.mark.parametrize("levels", [1, 2, 3])
+def test_window_gradients_match_composition(ad, levels):
+    B, T, H, W = 2, 5, 64, 96
+    frames, _ = _frames(B, T, H, W, 5 + levels)
+    spec = ad.FilterSpec(H, W, levels, maxflow=3.0, smooth=[2, 1, 1][:levels])
+    gamma = torch.tensor([10.0, 30.0, 50.0][:levels], device="cuda")
+    weights = torch.from_numpy(rng(6).standard_normal((B, T, H, W, 2)).astype(np.float32)).cuda()
+    gx, gg = _gradients(ad, spec, frames, gamma, weights, checkpoint=2)
+    x64 = frames.double().requires_grad_(True)
+    g64 = gamma.double().requires_grad_(True)
+    (tc.window(spec, x64, g64) * weights.double()).sum().backward()
+    say("levels", levels, "frames rel", rel(gx, x64.grad), "gamma", gg.tolist(), g64.grad.tolist())
+    assert rel(gx, x64.grad) <= 1e-4
+    assert rel(gg, g64.grad) <= 1e-4
+
+
+def test_checkpoint_interval_changes_nothing(ad):
+    B, T, H, W = 1, 6, 48, 64
+    frames, _ = _frames(B, T, H, W, 9)
+    spec = ad.FilterSpec(H, W, 2, maxflow=2.0, smooth=1)
+    gamma = torch.tensor([10.0, 40.0], device="cuda")
+    weights = torch.from_numpy(rng(7).standard_normal((B, T, H, W, 2)).astype(np.float32)).cuda()
+    base = _gradients(ad, spec, frames, gamma, weights, checkpoint=T)
+    for c in (1, 4):
+        other = _gradients(ad, spec, frames, gamma, weights, checkpoint=c)
+        assert torch.equal(base[0], other[0]) and torch.equal(base[1], other[1]), c
+
+
+def _fd_check(ad, spec, frames, gamma, truth, seed, coords=32, h=1e-4):
+    """Gradient of flow_loss by the adjoint (float32) against float64
+    central differences of the composition."""
+    x = frames.clone().requires_grad_(True)
+    gm = gamma.clone().requires_grad_(True)
+    loss = ad.flow_loss(x, ad.flow_window(x, gm, spec, checkpoint=4), truth)
+    loss.backward()
+    B, T, H, W = frames.shape
+    r = rng(seed)
+    pts = [(int(r.integers(B)), int(r.integers(T)), int(r.integers(H)), int(r.integers(W))) for _ in range(coords)]
+    f64 = frames.double()
+    g64 = gamma.double()
+
+    def loss_b(b, fr, gam):
+        # sequences are independent and every term is a mean over them:
+        # the full loss is the mean over b of the per-sequence losses
+        fl = tc.window(spec, fr[b:b + 1], gam)
+        return float(ad.flow_loss(fr[b:b + 1], fl, truth[b:b + 1].double()))
+
+    fd, mine = [], []
+    with torch.no_grad():
+        for (b, t, yy, xx) in pts:
+            p, m = f64.clone(), f64.clone()
+            p[b, t, yy, xx] += h
+            m[b, t, yy, xx] -= h
+            fd.append((loss_b(b, p, g64) - loss_b(b, m, g64)) / (2 * h) / B)
+            mine.append(float(x.grad[b, t, yy, xx]))
+        for l in range(spec.L):
+            p, m = g64.clone(), g64.clone()
+            p[l] += h
+            m[l] -= h
+            fd.append(sum(loss_b(b, f64, p) - loss_b(b, f64, m) for b in range(B)) / (2 * h) / B)
+            mine.append(float(gm.grad[l]))
+    fd, mine = np.array(fd), np.array(mine)
+    nf = len(pts)
+    scale_x = np.maximum(np.abs(fd[:nf]), 1e-2 * np.abs(fd[:nf]).max())
+    err_x = np.abs(mine[:nf] - fd[:nf]) / scale_x
+    err_g = np.abs(mine[nf:] - fd[nf:]) / np.abs(fd[nf:])
+    say("fd: frames max rel", err_x.max(), "gamma rel", err_g.tolist())
+    return err_x, err_g
+
+
+def test_gradients_match_finite_differences(ad):
+    B, T, H, W = 2, 5, 64, 80
+    frames, vel = _frames(B, T, H, W, 11, speed=3.0)
+    spec = ad.FilterSpec(H, W, 2, maxflow=4.0, smooth=2)
+    gamma = torch.tensor([10.0, 50.0], device="cuda")
+    err_x, err_g = _fd_check(ad, spec, frames, gamma, vel, seed=12)
+    assert err_x.max() <= 1e-3
+    assert err_g.max() <= 1e-3
+
+
+def test_sixteen_sequence_workload_finite_differences(ad):
+    """16 sequences of 512 x 512 noise translating at U[-3, 3]^2 px/frame,
+    8 frames, 2 levels; loss: L2 to the true flow + 0.1 x photometric."""
+    B, T, H, W = 16, 8, 512, 512
+    frames, vel = _frames(B, T, H, W, 13, speed=3.0)
+    spec = ad.FilterSpec(H, W, 2, maxflow=4.0, smooth=2)
+    gamma = torch.tensor([10.0, 50.0], device="cuda")
+    err_x, err_g = _fd_check(ad, spec, frames, gamma, vel, seed=14)
+    assert err_x.max() <= 1e-3
+    assert err_g.max() <= 1e-3
+
+
+def test_errors(ad):
+    spec = ad.FilterSpec(32, 32, 2)
+    with pytest.raises(ValueError):
+        ad.flow_window(torch.zeros(1, 2, 32, 32, device="cuda"), torch.ones(3, device="cuda"), spec)
+    with pytest.raises(ValueError):
+        ad.flow_window(torch.zeros(1, 2, 16, 32, device="cuda"), torch.ones(2, device="cuda"), spec)
+    with pytest.raises(ValueError):
+        ad.FilterSpec(32, 32, 2, border=0)
+    with pytest.raises(ValueError):
+        ad.flow_window(torch.zeros(1, 2, 32, 32, device="cuda"), torch.ones(2, device="cuda"), spec, checkpoint=0)

@@ -1,0 +1,134 @@
+"""The pure-PyTorch composition used to check the differentiable form,
+pinned against the CPU oracle (float64 builds of the oracle's stages and
+of its pyramidal filter), on the CPU.
+
+Bar: agreement to 1e-9 (absolute, float64 values of order 1) for every
+stage and for whole pyramidal-filter sequences fed the oracle's own u8
+pyramid; the float pyramid step to float32 rounding (2e-7)."""
+import numpy as np
+import pytest
+import torch
+
+import torch_composition as tc
+
+
+def _spec(h, w, levels, maxflow=4.0, smooth=2):
+    from flowfilter.autograd import FilterSpec
+    return FilterSpec(h, w, levels, maxflow=maxflow, smooth=smooth, device="cpu")
+
+
+def _rng(seed):
+    return np.random.default_rng(seed)
+
+
+def t64(a):
+    return torch.from_numpy(np.asarray(a, dtype=np.float64))[None]
+
+
+def close(a, b, tol=1e-9):
+    a = a.detach().numpy()[0] if isinstance(a, torch.Tensor) else a
+    assert a.shape == b.shape, (a.shape, b.shape)
+    err = np.abs(a - b).max()
+    assert err <= tol, err
+
+
+def test_image_model_matches_oracle(ffo):
+    img = _rng(1).uniform(0, 1, size=(37, 53)).astype(np.float32)
+    a0, g = tc.image_model(t64(img))
+    ra0, rg = ffo.image_model(img, prec="f64")
+    close(a0, ra0)
+    close(g, rg)
+
+
+def test_down_matches_oracle(ffo):
+    img = _rng(2).uniform(0, 1, size=(41, 66)).astype(np.float32)
+    p = ffo.pyramid(img, 3, prec="f64")
+    x = t64(img)
+    for l in (1, 2):
+        x = tc.down(x)
+        close(x, p[l], 2e-7)
+        x = t64(p[l])
+
+
+@pytest.mark.parametrize("axis", [0, 1])
+@pytest.mark.parametrize("payload", [False, True])
+def test_propagation_pass_matches_oracle(ffo, axis, payload):
+    r = _rng(3 + axis + 2 * payload)
+    h, w = 29, 41
+    f = r.uniform(-3, 3, size=(h, w, 2)).astype(np.float32)
+    s = r.uniform(0, 1, size=(h, w)).astype(np.float32) if payload else None
+    v = r.uniform(-1, 1, size=(h, w, 2)).astype(np.float32) if payload else None
+    dt = float(np.float32(1) / np.float32(3))
+    fo, so, vo = tc.prop_pass(axis, t64(f), t64(s) if payload else None, t64(v) if payload else None, dt, 3)
+    rf, rs, rv = ffo.propagate_pass(axis, f, s, v, dt=dt, border=3, prec="f64")
+    close(fo, rf)
+    if payload:
+        close(so, rs)
+        close(vo, rv)
+
+
+def test_update_matches_oracle(ffo):
+    r = _rng(7)
+    h, w = 23, 31
+    a0 = r.uniform(0, 1, size=(h, w)).astype(np.float32)
+    a0o = r.uniform(0, 1, size=(h, w)).astype(np.float32)
+    g = r.uniform(-0.2, 0.2, size=(h, w, 2)).astype(np.float32)
+    f = r.uniform(-5, 5, size=(h, w, 2)).astype(np.float32)
+    gamma = float(np.float32(10.0) / np.float32(65025.0))
+    out = tc.update(t64(a0), t64(g), t64(a0o), t64(f), gamma, 4.0)
+    ref, _ = ffo.update(a0, g, a0o, f, gamma, 4.0, prec="f64")
+    close(out, ref)
+
+
+def test_delta_update_matches_oracle(ffo):
+    from flowfilter.autograd import _index_map
+    r = _rng(8)
+    h, w, hu, wu = 24, 34, 12, 17
+    # the float32 index rule and the oracle's exact one agree at these sizes
+    assert all(_index_map(w, wu)[x] == ffo.upper_index(x, w, wu, prec="f64") for x in range(w))
+    assert all(_index_map(h, hu)[y] == ffo.upper_index(y, h, hu, prec="f64") for y in range(h))
+    a0 = r.uniform(0, 1, size=(h, w)).astype(np.float32)
+    a0p = r.uniform(0, 1, size=(h, w)).astype(np.float32)
+    g = r.uniform(-0.2, 0.2, size=(h, w, 2)).astype(np.float32)
+    d = r.uniform(-3, 3, size=(h, w, 2)).astype(np.float32)
+    up = r.uniform(-2, 2, size=(hu, wu, 2)).astype(np.float32)
+    gamma = float(np.float32(50.0) / np.float32(65025.0))
+    xm = torch.from_numpy(_index_map(w, wu))
+    ym = torch.from_numpy(_index_map(h, hu))
+    do, fo = tc.delta_update(t64(a0), t64(g), t64(a0p), t64(d), t64(up), xm, ym, gamma, 4.0)
+    rd, rf, _ = ffo.delta_update(a0, g, a0p, d, up, gamma, 4.0, prec="f64")
+    close(do, rd)
+    close(fo, rf)
+
+
+def test_smooth_matches_oracle(ffo):
+    f = _rng(9).uniform(-4, 4, size=(19, 27, 2)).astype(np.float32)
+    close(tc.smooth(t64(f), 3), ffo.smooth(f, 3, prec="f64"))
+
+
+@pytest.mark.parametrize("levels", [1, 2, 3])
+def test_pyramidal_sequence_matches_oracle(ffo, levels):
+    """Whole filter over a sequence (level order, first-frame rules, the
+    unsmoothed top flow read by level L-2) with the oracle's u8 pyramid."""
+    from flowfilter import synthetic
+    from flowfilter.autograd import _index_map
+    h, w, n = 64, 96, 5
+    gamma, maxflow, smooth = [10.0, 30.0, 50.0][:levels], 4.0, [2, 1, 1][:levels]
+    spec = _spec(h, w, levels, maxflow, smooth)
+    for l in range(levels - 1):
+        hh, ww = spec.sizes[l]
+        hu, wu = spec.sizes[l + 1]
+        assert all(_index_map(ww, wu)[x] == ffo.upper_index(x, ww, wu, prec="f64") for x in range(ww))
+        assert all(_index_map(hh, hu)[y] == ffo.upper_index(y, hh, hu, prec="f64") for y in range(hh))
+    frames = list(synthetic.translating_noise(h, w, n, (1.3, -0.6), seed=5))
+    o = ffo.PyramidalFlowFilter(h, w, levels, prec="f64")
+    o.configure_like(gamma, maxflow, smooth)
+    gam = torch.tensor(gamma, dtype=torch.float64)
+    gammas = [float(np.float32(g) / np.float32(65025.0)) for g in gamma]
+    state = None
+    for t in range(n):
+        pyr = [t64(p.astype(np.float64) / 255.0) for p in ffo.pyramid(frames[t], levels)]
+        state = tc.frame(spec, state, pyr[0], gammas, levels=pyr)
+        o.compute(frames[t])
+        close(state[0]["f"], o.downloadFlow().astype(np.float64), 1e-9)
+    assert gam.numel() == levels
