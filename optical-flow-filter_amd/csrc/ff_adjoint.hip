@@ -272,31 +272,31 @@ __global__ void __launch_bounds__(BX* BY)
 
 // ----------------------------------------------------------------- update
 
-// Adjoint of the closed-form solve (update_k.cu:43-63) at (a, b) = (gx, gy):
-// given g_ox, g_oy, adds the gradient of (a, b, Yt, fx, fy, gamma).
+// Adjoint of the closed-form solve (update_k.cu:43-63). The solve is
+// o = A^-1 q with A = g g^T + gamma I (g = (a, b)) and q = g Y + gamma f,
+// which simplifies (Sherman-Morrison) to
+//   o = f + c g,   c = (Y - g.f) / (gamma + |g|^2).
+// With k = 1 / (gamma + |g|^2) and h = (g_o . g) k, its vector-Jacobian
+// product is
+//   g_g = c g_o - h (f + 2 c g),   g_Y = h,   g_f = g_o - h g,
+//   g_gamma = -c h.
+// No term carries the 1/gamma of A^-1, so nothing cancels when gamma is
+// much smaller than |g|^2 (the usual case: gamma is scaled by 1/255^2).
 struct SolveGrad {
     float a, b, Y, fx, fy, gamma;
 };
 __device__ __forceinline__ SolveGrad solve2x2_adjoint(float a, float b, float Y, float fx, float fy, float gamma,
                                                       float gox, float goy) {
-    const float ax2 = a * a, ay2 = b * b;
-    const float N00 = gamma + ay2, N01 = -a * b, N11 = gamma + ax2;
-    const float den = gamma * ((gamma + ax2) + ay2);
-    const float r = 1.0f / den;
-    const float qx = fmaf(gamma, fx, a * Y), qy = fmaf(gamma, fy, b * Y);
-    const float ux = N00 * qx + N01 * qy, uy = N01 * qx + N11 * qy;
-    const float gux = gox * r, guy = goy * r;
-    const float gr = gox * ux + goy * uy;
-    const float gN00 = gux * qx, gN01 = gux * qy + guy * qx, gN11 = guy * qy;
-    const float gqx = gux * N00 + guy * N01, gqy = gux * N01 + guy * N11;
-    const float gden = -gr * r * r;
+    const float k = 1.0f / ((gamma + a * a) + b * b);
+    const float c = (Y - fmaf(a, fx, b * fy)) * k;
+    const float h = fmaf(gox, a, goy * b) * k;
     SolveGrad g;
-    g.gamma = gden * (2.0f * gamma + ax2 + ay2) + gN00 + gN11 + gqx * fx + gqy * fy;
-    g.a = gden * gamma * 2.0f * a + gN11 * 2.0f * a - gN01 * b + gqx * Y;
-    g.b = gden * gamma * 2.0f * b + gN00 * 2.0f * b - gN01 * a + gqy * Y;
-    g.Y = gqx * a + gqy * b;
-    g.fx = gqx * gamma;
-    g.fy = gqy * gamma;
+    g.a = fmaf(c, gox, -h * fmaf(2.0f * c, a, fx));
+    g.b = fmaf(c, goy, -h * fmaf(2.0f * c, b, fy));
+    g.Y = h;
+    g.fx = fmaf(-h, a, gox);
+    g.fy = fmaf(-h, b, goy);
+    g.gamma = -c * h;
     return g;
 }
 

@@ -11,11 +11,12 @@ Bars:
   * stage adjoints: normwise relative error <= 1e-5 against float64 autograd;
   * window gradients (frames and per-level gamma): normwise relative error
     <= 1e-4 against float64 autograd of the composition;
-  * finite differences (float64 composition, central, step 1e-4) at 32
-    random frame coordinates and every gamma: |ad - fd| <= 1e-3 x
-    max(|fd|, 1e-2 x max |fd|) per coordinate (a relative bound, floored at
-    1 % of the largest derivative checked so that near-zero derivatives are
-    judged on an absolute scale);
+  * finite differences (float64 composition, central, step 1e-4, its
+    piecewise choices frozen at the base point) at 32 random frame
+    coordinates and every gamma: |ad - fd| <= 1e-3 x max(|fd|, 1e-2 x
+    max |fd|) per coordinate (a relative bound, floored at 1 % of the
+    largest derivative checked so that near-zero derivatives are judged on
+    an absolute scale);
   * the checkpoint interval changes nothing (bitwise).
 """
 import sys
@@ -295,9 +296,14 @@ def test_checkpoint_interval_changes_nothing(ad):
         assert torch.equal(base[0], other[0]) and torch.equal(base[1], other[1]), c
 
 
-def _fd_check(ad, spec, frames, gamma, truth, seed, coords=32, h=1e-4):
+def _fd_check(ad, spec, frames, gamma, truth, seed, coords=32, h=1e-4, plain=False):
     """Gradient of flow_loss by the adjoint (float32) against float64
-    central differences of the composition."""
+    central differences of the composition. The differences are taken with
+    the composition's piecewise choices frozen at the base point (see
+    torch_composition.Branches); with `plain`, also without freezing, which
+    is reported only: a step of 1e-4 in one pixel moves flows by up to
+    ~1e-2 px, enough to cross some of the dense kinks (upwind side, bilinear
+    cell, clamps) that the secant then averages over."""
     x = frames.clone().requires_grad_(True)
     gm = gamma.clone().requires_grad_(True)
     loss = ad.flow_loss(x, ad.flow_window(x, gm, spec, checkpoint=4), truth)
@@ -307,20 +313,35 @@ def _fd_check(ad, spec, frames, gamma, truth, seed, coords=32, h=1e-4):
     pts = [(int(r.integers(B)), int(r.integers(T)), int(r.integers(H)), int(r.integers(W))) for _ in range(coords)]
     f64 = frames.double()
     g64 = gamma.double()
+    t64 = truth.double()
+    recs = {}
 
-    def loss_b(b, fr, gam):
+    def loss_b(b, fr, gam, freeze=True):
         # sequences are independent and every term is a mean over them:
         # the full loss is the mean over b of the per-sequence losses
-        fl = tc.window(spec, fr[b:b + 1], gam)
-        return float(ad.flow_loss(fr[b:b + 1], fl, truth[b:b + 1].double()))
+        if not freeze:
+            return float(tc.flow_loss(fr[b:b + 1], tc.window(spec, fr[b:b + 1], gam), t64[b:b + 1]))
+        first = b not in recs
+        rec = recs.setdefault(b, tc.Branches())
+        with tc.frozen(rec):
+            fl = tc.window(spec, fr[b:b + 1], gam)
+            val = tc.flow_loss(fr[b:b + 1], fl, t64[b:b + 1])
+        if first:  # the base point; the composition's loss is the product's
+            ref = ad.flow_loss(fr[b:b + 1], fl, t64[b:b + 1])
+            assert abs(float(val) - float(ref)) <= 1e-12 * abs(float(ref)), (float(val), float(ref))
+        return float(val)
 
-    fd, mine = [], []
+    fd, fdp, mine = [], [], []
     with torch.no_grad():
+        for b in sorted(set(range(B))):
+            loss_b(b, f64, g64)
         for (b, t, yy, xx) in pts:
             p, m = f64.clone(), f64.clone()
             p[b, t, yy, xx] += h
             m[b, t, yy, xx] -= h
             fd.append((loss_b(b, p, g64) - loss_b(b, m, g64)) / (2 * h) / B)
+            if plain:
+                fdp.append((loss_b(b, p, g64, False) - loss_b(b, m, g64, False)) / (2 * h) / B)
             mine.append(float(x.grad[b, t, yy, xx]))
         for l in range(spec.L):
             p, m = g64.clone(), g64.clone()
@@ -333,16 +354,24 @@ def _fd_check(ad, spec, frames, gamma, truth, seed, coords=32, h=1e-4):
     scale_x = np.maximum(np.abs(fd[:nf]), 1e-2 * np.abs(fd[:nf]).max())
     err_x = np.abs(mine[:nf] - fd[:nf]) / scale_x
     err_g = np.abs(mine[nf:] - fd[nf:]) / np.abs(fd[nf:])
-    say("fd: frames max rel", err_x.max(), "gamma rel", err_g.tolist())
+    say("fd (frozen choices): frames max rel", err_x.max(), "median", np.median(err_x), "gamma rel", err_g.tolist())
+    if plain:
+        fdp = np.array(fdp)
+        ep = np.abs(mine[:nf] - fdp) / np.maximum(np.abs(fdp), 1e-2 * np.abs(fdp).max())
+        say("fd (plain secant): frames max rel", ep.max(), "median", np.median(ep),
+            "within 1e-3:", int((ep <= 1e-3).sum()), "of", nf)
+    k = int(np.argmax(err_x))
+    say("worst", pts[k], "fd", fd[k], "adjoint", mine[k], "max |fd|", np.abs(fd[:nf]).max())
     return err_x, err_g
 
 
-def test_gradients_match_finite_differences(ad):
+@pytest.mark.parametrize("levels", [2, 3])
+def test_gradients_match_finite_differences(ad, levels):
     B, T, H, W = 2, 5, 64, 80
     frames, vel = _frames(B, T, H, W, 11, speed=3.0)
-    spec = ad.FilterSpec(H, W, 2, maxflow=4.0, smooth=2)
-    gamma = torch.tensor([10.0, 50.0], device="cuda")
-    err_x, err_g = _fd_check(ad, spec, frames, gamma, vel, seed=12)
+    spec = ad.FilterSpec(H, W, levels, maxflow=4.0, smooth=2)
+    gamma = torch.tensor([10.0, 50.0, 30.0][:levels], device="cuda")
+    err_x, err_g = _fd_check(ad, spec, frames, gamma, vel, seed=12, plain=True)
     assert err_x.max() <= 1e-3
     assert err_g.max() <= 1e-3
 

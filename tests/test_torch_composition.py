@@ -132,3 +132,47 @@ def test_pyramidal_sequence_matches_oracle(ffo, levels):
         o.compute(frames[t])
         close(state[0]["f"], o.downloadFlow().astype(np.float64), 1e-9)
     assert gam.numel() == levels
+
+
+def test_warp_matches_grid_sample():
+    from flowfilter.autograd import photometric_error
+    r = _rng(10)
+    frames = torch.from_numpy(r.uniform(0, 1, size=(2, 3, 20, 30)))
+    flows = torch.from_numpy(r.uniform(-4, 4, size=(2, 3, 20, 30, 2))).requires_grad_(True)
+    a = photometric_error(frames, flows)
+    ga, = torch.autograd.grad(a, flows)
+    flows2 = flows.detach().clone().requires_grad_(True)
+    b = sum(((frames[:, t] - tc.warp_previous(frames[:, t - 1], flows2[:, t])) ** 2).mean() for t in (1, 2)) / 2
+    gb, = torch.autograd.grad(b, flows2)
+    assert abs(float(a) - float(b)) <= 1e-13
+    assert float((ga - gb).abs().max()) <= 1e-12
+
+
+def test_frozen_finite_differences_match_autograd():
+    """Central differences with the piecewise choices frozen at the base
+    point reproduce float64 autograd of the composition."""
+    from flowfilter import synthetic
+    h, w, n = 32, 40, 4
+    spec = _spec(h, w, 2, maxflow=3.0, smooth=1)
+    frames = torch.from_numpy(np.stack(list(synthetic.translating_noise(h, w, n, (1.2, -0.7), seed=3)))
+                              .astype(np.float64)[None] / 255.0)
+    gamma = torch.tensor([10.0, 40.0], dtype=torch.float64)
+    truth = torch.tensor([[1.2, -0.7]], dtype=torch.float64)
+    x = frames.clone().requires_grad_(True)
+    tc.flow_loss(x, tc.window(spec, x, gamma), truth).backward()
+    rec = tc.Branches()
+    with tc.frozen(rec):
+        tc.flow_loss(frames, tc.window(spec, frames, gamma), truth)
+    r = _rng(11)
+    step = 1e-4
+    for _ in range(8):
+        t, yy, xx = int(r.integers(n)), int(r.integers(h)), int(r.integers(w))
+        vals = []
+        for sgn in (1, -1):
+            p = frames.clone()
+            p[0, t, yy, xx] += sgn * step
+            with tc.frozen(rec):
+                vals.append(float(tc.flow_loss(p, tc.window(spec, p, gamma), truth)))
+        fd = (vals[0] - vals[1]) / (2 * step)
+        g = float(x.grad[0, t, yy, xx])
+        assert abs(fd - g) <= 1e-5 * max(abs(g), 1e-3 * float(x.grad.abs().max())), (t, yy, xx, fd, g)

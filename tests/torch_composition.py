@@ -15,6 +15,60 @@ DIFF = (-0.125, -0.25, 0.0, 0.25, 0.125)       # imagemodel_k.cu:22
 BOX = float(np.float32(0.2))                   # 0.2f, flowsmoothing_k.cu:19
 
 
+class Branches(object):
+    """Piecewise choices of one evaluation (upwind side and direction,
+    clamps, non-finite masks, bilinear cells), recorded on the first
+    evaluation under `frozen` and replayed on later ones: finite differences
+    then measure the derivative of the smooth piece the base point lies on,
+    which is what an adjoint computes, instead of a secant across the kinks
+    that the perturbation crosses."""
+
+    def __init__(self):
+        self.log = []
+        self.pos = None
+
+    def decide(self, value):
+        if self.pos is None:
+            self.log.append(value)
+            return value
+        v = self.log[self.pos]
+        self.pos += 1
+        return v
+
+    def replay(self):
+        self.pos = 0
+
+
+_BRANCHES = [None]
+
+
+class frozen(object):
+    """Context: route the composition's piecewise choices through `rec`."""
+
+    def __init__(self, rec):
+        self.rec = rec
+
+    def __enter__(self):
+        _BRANCHES[0] = self.rec
+        if self.rec.log:
+            self.rec.replay()
+        return self.rec
+
+    def __exit__(self, *exc):
+        _BRANCHES[0] = None
+        return False
+
+
+def _decide(value):
+    rec = _BRANCHES[0]
+    return value if rec is None else rec.decide(value)
+
+
+def _clamp(t, lo, hi):
+    below, above = _decide(t < lo), _decide(t > hi)
+    return torch.where(above, torch.full_like(t, hi), torch.where(below, torch.full_like(t, lo), t))
+
+
 def sh(t, dim, k):
     """t shifted by k along dim with clamped (edge-replicating) reads:
     out[i] = t[clamp(i + k, 0, n - 1)]."""
@@ -57,8 +111,8 @@ def prop_pass(axis, f, s, v, dt, border):
     dim, c = (2, 0) if axis == 0 else (1, 1)
     fm, fp = sh(f, dim, -1), sh(f, dim, 1)
     um, up = fm[..., c], fp[..., c]
-    Ud = torch.where(up.abs() - um.abs() > 0, up, um)
-    back = Ud >= 0
+    Ud = torch.where(_decide(up.abs() - um.abs() > 0), up, um)
+    back = _decide(Ud >= 0)
     nd = -(dt * Ud)
     mask = _interior(f.shape[1], f.shape[2], border, f.device)
 
@@ -84,14 +138,14 @@ def solve2x2(g, Y, f, gamma):
 
 
 def _finite(t):
-    return torch.where(torch.isfinite(t), t, torch.zeros_like(t))
+    return torch.where(_decide(torch.isfinite(t)), t, torch.zeros_like(t))
 
 
 def update(a0, g, a0_old, f, gamma, maxflow):
     """flowUpdate_k (update_k.cu:15-79): solve, clamp to +-maxflow, zero
     non-finite values."""
     ox, oy = solve2x2(g, a0_old - a0, f, gamma)
-    return torch.stack([_finite(torch.clamp(ox, -maxflow, maxflow)), _finite(torch.clamp(oy, -maxflow, maxflow))], -1)
+    return torch.stack([_finite(_clamp(ox, -maxflow, maxflow)), _finite(_clamp(oy, -maxflow, maxflow))], -1)
 
 
 def delta_update(a0, g, a0_prop, dflow_prop, upper, xmap, ymap, gamma, maxflow):
@@ -100,9 +154,9 @@ def delta_update(a0, g, a0_prop, dflow_prop, upper, xmap, ymap, gamma, maxflow):
     upper flow point-sampled at (xmap, ymap), clamped to +-maxflow."""
     ox, oy = solve2x2(g, a0_prop - a0, dflow_prop, gamma)
     half = 0.5 * maxflow
-    d = torch.stack([torch.clamp(_finite(ox), -half, half), torch.clamp(_finite(oy), -half, half)], -1)
+    d = torch.stack([_clamp(_finite(ox), -half, half), _clamp(_finite(oy), -half, half)], -1)
     u = upper.index_select(1, ymap.long()).index_select(2, xmap.long())
-    return d, torch.clamp(d + 2.0 * u, -maxflow, maxflow)
+    return d, _clamp(d + 2.0 * u, -maxflow, maxflow)
 
 
 def smooth(f, K):
@@ -170,3 +224,33 @@ def window(spec, frames, gamma, pyramids=None):
         state = frame(spec, state, frames[:, t], gammas, None if pyramids is None else pyramids[t])
         outs.append(state[0]["f"])
     return torch.stack(outs, 1)
+
+
+def warp_previous(prev, flow):
+    """prev [B,H,W] sampled bilinearly at x - flow(x), coordinates clamped
+    to the image (the warp of grid_sample with align_corners=True and
+    border padding), with its cell choices routed through `frozen`."""
+    B, H, W = prev.shape
+    dev, dt = prev.device, prev.dtype
+    xs = torch.arange(W, device=dev, dtype=dt)[None, None, :]
+    ys = torch.arange(H, device=dev, dtype=dt)[None, :, None]
+    sx = _clamp(xs - flow[..., 0], 0.0, W - 1.0)
+    sy = _clamp(ys - flow[..., 1], 0.0, H - 1.0)
+    x0, y0 = _decide(torch.floor(sx)), _decide(torch.floor(sy))
+    wx, wy = sx - x0, sy - y0
+    x0, y0 = x0.long(), y0.long()
+    x1, y1 = torch.clamp(x0 + 1, max=W - 1), torch.clamp(y0 + 1, max=H - 1)
+    flat = prev.reshape(B, H * W)
+    at = lambda yy, xx: flat.gather(1, (yy * W + xx).reshape(B, -1)).reshape(B, H, W)
+    top = (1 - wx) * at(y0, x0) + wx * at(y0, x1)
+    bot = (1 - wx) * at(y1, x0) + wx * at(y1, x1)
+    return (1 - wy) * top + wy * bot
+
+
+def flow_loss(frames, flows, truth, photometric_weight=0.1):
+    """The loss of flowfilter.autograd.flow_loss (truth [B,2]) with
+    warp_previous in place of grid_sample."""
+    T = frames.shape[1]
+    l2 = ((flows[:, 1:] - truth[:, None, None, None, :]) ** 2).mean()
+    ph = sum(((frames[:, t] - warp_previous(frames[:, t - 1], flows[:, t])) ** 2).mean() for t in range(1, T))
+    return l2 + photometric_weight * ph / (T - 1)
