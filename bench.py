@@ -167,6 +167,71 @@ def copy_bench(args, N, GPUImage):
                       "launches": args.steps}), flush=True)
 
 
+def differentiable_bench(args):
+    """Forward + backward of the differentiable filter over an unrolled
+    window: 3 levels at 3840x2160, 32 frames, maxflow 8, checkpoint every 4
+    frames, loss = L2 to the true flow field + 0.1 x photometric error;
+    and the 16-sequence 512x512 8-frame 2-level window."""
+    import torch
+    import numpy as np
+    from flowfilter import autograd as ad, synthetic
+
+    def measure(frames, gamma, spec, truth, reps, checkpoint):
+        def once(timing):
+            x = frames.detach().requires_grad_(True)
+            gm = gamma.detach().clone().requires_grad_(True)
+            e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+            e0.record()
+            flows = ad.flow_window(x, gm, spec, checkpoint=checkpoint)
+            e1.record()
+            ad.flow_loss(x, flows, truth).backward()
+            e2.record()
+            torch.cuda.synchronize()
+            if timing is not None:
+                timing.append((e0.elapsed_time(e1), e0.elapsed_time(e2)))
+            return x.grad, gm.grad
+        once(None)
+        torch.cuda.synchronize()
+        torch.cuda.reset_peak_memory_stats()
+        times = []
+        for _ in range(reps):
+            g, gg = once(times)
+        assert bool(torch.isfinite(g).all()) and bool(torch.isfinite(gg).all())
+        fwd = min(t[0] for t in times)
+        tot = min(t[1] for t in times)
+        pix = frames.shape[0] * frames.shape[1] * frames.shape[2] * frames.shape[3]
+        return {"ms_forward": round(fwd, 2), "ms_forward_backward": round(tot, 2),
+                "forward_mpix_s": round(pix / (fwd / 1e3) / 1e6, 1),
+                "forward_backward_mpix_s": round(pix / (tot / 1e3) / 1e6, 1),
+                "peak_hbm_gib": round(torch.cuda.max_memory_allocated() / 2 ** 30, 3), "reps": reps}
+
+    t0 = time.perf_counter()
+    frames, v = synthetic.smooth_flow_sequence(2160, 3840, 32, maxflow=8.0, flow_sigma=64.0, seed=7)
+    torch.cuda.synchronize()
+    print("generated the 3840x2160 window in %.1f s" % (time.perf_counter() - t0), file=sys.stderr, flush=True)
+    spec = ad.FilterSpec(2160, 3840, 3, maxflow=8.0, smooth=2)
+    big = measure(frames[None], torch.tensor([10.0, 30.0, 50.0], device="cuda"), spec, v[None], args.ad_reps, 4)
+    big["workload"] = ("3-level filter, 3840x2160 u8-valued frames, 32-frame window, maxflow 8, smooth 2, gamma "
+                       "[10,30,50], checkpoint every 4 frames; loss: L2 to the flow field + 0.1 x photometric; "
+                       "gradients w.r.t. frames and gamma; Mpix/s = level-0 pixels x frames / time")
+    del frames, v
+    print("differentiable 4K window: %s" % big, file=sys.stderr, flush=True)
+    rng = np.random.default_rng(13)
+    seqs, vel = [], []
+    for b in range(16):
+        vv = tuple(rng.uniform(-3.0, 3.0, size=2))
+        vel.append(vv)
+        seqs.append(np.stack(list(synthetic.translating_noise(512, 512, 8, vv, seed=1300 + b))))
+    frames = torch.from_numpy(np.stack(seqs).astype(np.float32) / np.float32(255.0)).cuda()
+    spec = ad.FilterSpec(512, 512, 2, maxflow=4.0, smooth=2)
+    small = measure(frames, torch.tensor([10.0, 50.0], device="cuda"), spec,
+                    torch.tensor(vel, dtype=torch.float32, device="cuda"), args.ad_reps, 4)
+    small["workload"] = ("16 sequences of 512x512 translating noise (U[-3,3]^2 px/frame), 8 frames, 2 levels, "
+                         "maxflow 4; loss: L2 to the true flow + 0.1 x photometric")
+    print("differentiable 16x512^2 window: %s" % small, file=sys.stderr, flush=True)
+    return {"window_3840x2160": big, "batch16_512x512": small}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -179,6 +244,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-batched", action="store_true", help="skip the 64-stream batched measurement")
     ap.add_argument("--batched-steps", type=int, default=60)
+    ap.add_argument("--no-differentiable", action="store_true",
+                    help="skip the differentiable-window measurements (and the torch import they need)")
+    ap.add_argument("--ad-reps", type=int, default=2)
+    ap.add_argument("--ad-only", action="store_true", help="only the differentiable-window measurements")
     ap.add_argument("--copy", action="store_true",
                     help="instead of the filter, time a plain device copy of one 2048x1088 flow image "
                          "(17.8 MB read + 17.8 MB written per launch; the reference kernel for HBM counters)")
@@ -187,6 +256,14 @@ def main():
         print("bench.py measures one GPU only: this path has no multi-GPU form "
               "(--gpus %d requested)" % args.gpus, file=sys.stderr)
         sys.exit(2)
+
+    if not args.no_differentiable:
+        # torch before the library: one HIP runtime in the process (torch
+        # ships its own under the system runtime's soname)
+        import torch  # noqa: F401
+        if args.ad_only:
+            print(json.dumps({"differentiable": differentiable_bench(args)}), flush=True)
+            return
 
     import numpy as np
     import flowfilter.gpu.flowfilters as gf
@@ -281,6 +358,8 @@ def main():
     }
     if not args.no_batched:
         out["batched"] = batched_bench(args, N, gf, GPUImage)
+    if not args.no_differentiable:
+        out["differentiable"] = differentiable_bench(args)
     if not args.no_cpu:
         print("cpu baseline ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(frames[: min(F, 8)], args.cpu_seconds)

@@ -336,8 +336,11 @@ FF_API ff_status ff_batch_get_blocks(ff_stage* batch, int* out);
  * gradient) float2[B][H][W]. `stream` is a hipStream_t (NULL: the null
  * stream). Forward stages compute what the corresponding stage kernels
  * compute (same arithmetic); each *_backward takes the forward inputs and
- * the gradient of the outputs and writes the gradient of the inputs
- * (added to the destination when `accumulate` is nonzero). Upwind
+ * the gradient of the outputs and writes the gradient of the inputs.
+ * `accumulate`: for a backward with one output, nonzero adds it to the
+ * destination; with several outputs it is a bit mask, bit k adding output
+ * k (in argument order, gamma partial sums and g_upper_pixel excluded).
+ * An output gradient passed as NULL where noted reads as zero. Upwind
  * directions, clamps and the zeroing of non-finite values are held at
  * their forward values.
  * ------------------------------------------------------------------- */
@@ -356,7 +359,8 @@ FF_API ff_status ff_ad_down_backward(const float* g_half, float* g_image, float*
                                      int accumulate, void* stream);
 /* One upwind pass along axis 0 (x) or 1 (y) (propagation_k.cu:16-334) of
  * the flow and, when scalar/vector are given, of a scalar and a vector
- * payload; border >= 1. */
+ * payload; border >= 1. Backward: g_flow_out, g_scalar_out and g_vector_out
+ * may be NULL (zero). */
 FF_API ff_status ff_ad_propagate_pass(int axis, const float* flow, const float* scalar, const float* vector,
                                       float* flow_out, float* scalar_out, float* vector_out, int B, int H, int W,
                                       float dt, int border, void* stream);
@@ -364,7 +368,7 @@ FF_API ff_status ff_ad_propagate_pass_backward(int axis, const float* flow, cons
                                                const float* g_flow_out, const float* g_scalar_out,
                                                const float* g_vector_out, float* g_flow, float* g_scalar,
                                                float* g_vector, int B, int H, int W, float dt, int border,
-                                               void* stream);
+                                               int accumulate, void* stream);
 /* Top-level update (update_k.cu:15-79). Backward writes the gradients of
  * a0, grad, a0_old and flow and B*ff_ad_partial_count partial sums of the
  * gamma gradient. */
@@ -380,7 +384,7 @@ FF_API ff_status ff_ad_update_backward(const float* a0, const float* grad, const
  * g_upper_pixel (float2[B][H][W]); ff_ad_upper_backward sums it onto the
  * upper level using, for every upper column X (row Y), the range
  * [xlo[X], xhi[X]) ([ylo[Y], yhi[Y])) of this level's columns (rows) that
- * map to it. */
+ * map to it. g_dflow_out may be NULL (zero). */
 FF_API ff_status ff_ad_delta_update(const float* a0, const float* grad, const float* a0_prop, const float* dflow_prop,
                                     const float* upper, const int* xmap, const int* ymap, float* dflow_out,
                                     float* flow_out, int B, int H, int W, int Hu, int Wu, float gamma, float maxflow,
@@ -394,10 +398,12 @@ FF_API ff_status ff_ad_delta_update_backward(const float* a0, const float* grad,
 FF_API ff_status ff_ad_upper_backward(const float* g_upper_pixel, const int* xlo, const int* xhi, const int* ylo,
                                       const int* yhi, float* g_upper, int B, int H, int W, int Hu, int Wu,
                                       int accumulate, void* stream);
-/* One box-smoothing pass along axis 0 (x) or 1 (y) (flowsmoothing_k.cu:18-87). */
+/* One box-smoothing pass along axis 0 (x) or 1 (y) (flowsmoothing_k.cu:18-87).
+ * Backward of the pass for the output gradient g_flow_out + g_flow_out2
+ * (g_flow_out2 may be NULL). */
 FF_API ff_status ff_ad_smooth_pass(int axis, const float* flow, float* flow_out, int B, int H, int W, void* stream);
-FF_API ff_status ff_ad_smooth_pass_backward(int axis, const float* g_flow_out, float* g_flow, int B, int H, int W,
-                                            int accumulate, void* stream);
+FF_API ff_status ff_ad_smooth_pass_backward(int axis, const float* g_flow_out, const float* g_flow_out2, float* g_flow,
+                                            int B, int H, int W, int accumulate, void* stream);
 
 /* Kernel profiling (no reference counterpart; the reference times whole
  * stages only, src/gpu/pipeline.cu:63-89). Between begin and end, every

@@ -12,8 +12,10 @@
 //
 // The backward kernels gather (each output of the adjoint sums the
 // contributions that reach it), so they are deterministic and need no
-// atomics; outputs are written, or added to the destination with
-// `accumulate` set.
+// atomics. `acc` is a bit mask over a kernel's outputs (bit k: output k, in
+// argument order, is added to the destination instead of written), so the
+// sums of a chain rule land in place without extra passes over HBM. A null
+// output-gradient input reads as zero.
 #include <hip/hip_runtime.h>
 
 #include "ff_device.hpp"
@@ -36,8 +38,8 @@ __device__ __forceinline__ size_t at(int b, int y, int x, const Dims& d) {
 
 inline dim3 grid3(int B, int H, int W) { return dim3((W + BX - 1) / BX, (H + BY - 1) / BY, B); }
 
-__device__ __forceinline__ void put(float* p, size_t i, float v, bool acc) { p[i] = acc ? p[i] + v : v; }
-__device__ __forceinline__ void put2(float2* p, size_t i, float2 v, bool acc) {
+__device__ __forceinline__ void put(float* p, size_t i, float v, int acc) { p[i] = acc ? p[i] + v : v; }
+__device__ __forceinline__ void put2(float2* p, size_t i, float2 v, int acc) {
     if (acc) {
         const float2 o = p[i];
         v = make_float2(o.x + v.x, o.y + v.y);
@@ -204,7 +206,7 @@ __global__ void __launch_bounds__(BX* BY) k_ad_prop(const float2* fin, const flo
 template <int AXIS, bool PAY>
 __global__ void __launch_bounds__(BX* BY)
     k_ad_prop_bwd(const float2* fin, const float* sin, const float2* vin, const float2* gfo, const float* gso,
-                  const float2* gvo, float2* gfi, float* gsi, float2* gvi, Dims d, float dt, int border) {
+                  const float2* gvo, float2* gfi, float* gsi, float2* gvi, Dims d, float dt, int border, int acc) {
     const int x = blockIdx.x * BX + threadIdx.x, y = blockIdx.y * BY + threadIdx.y, b = blockIdx.z;
     if (x >= d.W || y >= d.H) return;
     const int n = AXIS == 0 ? d.W : d.H, j = AXIS == 0 ? x : y;
@@ -218,9 +220,9 @@ __global__ void __launch_bounds__(BX* BY)
     for (int o = j - 1; o <= j + 1; ++o) {
         if (o < 0 || o >= n) continue;
         const size_t io = idx(o);
-        const float2 g_f = gfo[io];
-        const float g_s = PAY ? gso[io] : 0.f;
-        const float2 g_v = PAY ? gvo[io] : make_float2(0.f, 0.f);
+        const float2 g_f = gfo ? gfo[io] : make_float2(0.f, 0.f);
+        const float g_s = (PAY && gso) ? gso[io] : 0.f;
+        const float2 g_v = (PAY && gvo) ? gvo[io] : make_float2(0.f, 0.f);
         if (!interior(o)) {
             if (o == j) {  // out = z
                 gf.x += g_f.x;
@@ -263,10 +265,10 @@ __global__ void __launch_bounds__(BX* BY)
             else gf.y += gUd;
         }
     }
-    gfi[at(b, y, x, d)] = gf;
+    put2(gfi, at(b, y, x, d), gf, acc & 1);
     if constexpr (PAY) {
-        gsi[at(b, y, x, d)] = gs;
-        gvi[at(b, y, x, d)] = gv;
+        put(gsi, at(b, y, x, d), gs, acc & 2);
+        put2(gvi, at(b, y, x, d), gv, acc & 4);
     }
 }
 
@@ -350,10 +352,10 @@ __global__ void __launch_bounds__(BX* BY)
         const float gox = (inside(ox, maxflow) && !not_finite(ox)) ? go.x : 0.0f;
         const float goy = (inside(oy, maxflow) && !not_finite(oy)) ? go.y : 0.0f;
         const SolveGrad s = solve2x2_adjoint(gr.x, gr.y, Y, ff.x, ff.y, gamma, gox, goy);
-        put(ga0, i, -s.Y, acc);
-        put(gaold, i, s.Y, acc);
-        put2(gg, i, make_float2(s.a, s.b), acc);
-        put2(gf, i, make_float2(s.fx, s.fy), acc);
+        put(ga0, i, -s.Y, acc & 1);
+        put2(gg, i, make_float2(s.a, s.b), acc & 2);
+        put(gaold, i, s.Y, acc & 4);
+        put2(gf, i, make_float2(s.fx, s.fy), acc & 8);
         ggam = s.gamma;
     }
     block_sum(ggam, gamma_partial);
@@ -407,7 +409,7 @@ __global__ void __launch_bounds__(BX* BY)
         const float ox = fmaxf(-half, fminf(zx, half)), oy = fmaxf(-half, fminf(zy, half));
         const float2 u = upper[at(b, ymap[y], xmap[x], du)];
         const float px_ = ox + 2.0f * u.x, py_ = oy + 2.0f * u.y;
-        const float2 gfo = gfout[i], gdo = gdout[i];
+        const float2 gfo = gfout[i], gdo = gdout ? gdout[i] : make_float2(0.f, 0.f);
         // through the final clamp
         const float gpx = inside(px_, maxflow) ? gfo.x : 0.0f, gpy = inside(py_, maxflow) ? gfo.y : 0.0f;
         put2(gup, i, make_float2(2.0f * gpx, 2.0f * gpy), false);
@@ -416,10 +418,10 @@ __global__ void __launch_bounds__(BX* BY)
         const float gzx = (fx_ok && inside(zx, half)) ? gox : 0.0f;
         const float gzy = (fy_ok && inside(zy, half)) ? goy : 0.0f;
         const SolveGrad s = solve2x2_adjoint(gr.x, gr.y, Y, dd.x, dd.y, gamma, gzx, gzy);
-        put(ga0, i, -s.Y, acc);
-        put(gsprop, i, s.Y, acc);
-        put2(gg, i, make_float2(s.a, s.b), acc);
-        put2(gdprop, i, make_float2(s.fx, s.fy), acc);
+        put(ga0, i, -s.Y, acc & 1);
+        put2(gg, i, make_float2(s.a, s.b), acc & 2);
+        put(gsprop, i, s.Y, acc & 4);
+        put2(gdprop, i, make_float2(s.fx, s.fy), acc & 8);
         ggam = s.gamma;
     }
     block_sum(ggam, gamma_partial);
@@ -461,19 +463,30 @@ __global__ void __launch_bounds__(BX* BY) k_ad_smooth(const float2* in, float2* 
     out[at(b, y, x, d)] = make_float2(sx, sy);
 }
 
+// Adjoint of one smoothing pass of the sum gout + gout2 (gout2 may be null).
 template <int AXIS>
-__global__ void __launch_bounds__(BX* BY) k_ad_smooth_bwd(const float2* gout, float2* gin, Dims d, int acc) {
+__global__ void __launch_bounds__(BX* BY)
+    k_ad_smooth_bwd(const float2* gout, const float2* gout2, float2* gin, Dims d, int acc) {
     const int x = blockIdx.x * BX + threadIdx.x, y = blockIdx.y * BY + threadIdx.y, b = blockIdx.z;
     if (x >= d.W || y >= d.H) return;
+    auto g = [&](size_t i) {
+        float2 v = gout[i];
+        if (gout2) {
+            const float2 u = gout2[i];
+            v.x += u.x;
+            v.y += u.y;
+        }
+        return v;
+    };
     float2 s;
     if (AXIS == 0) {
-        s.x = corr5_adjoint(x, d.W, WB{}, [&](int i) { return gout[at(b, y, i, d)].x; });
-        s.y = corr5_adjoint(x, d.W, WB{}, [&](int i) { return gout[at(b, y, i, d)].y; });
+        s.x = corr5_adjoint(x, d.W, WB{}, [&](int i) { return g(at(b, y, i, d)).x; });
+        s.y = corr5_adjoint(x, d.W, WB{}, [&](int i) { return g(at(b, y, i, d)).y; });
     } else {
-        s.x = corr5_adjoint(y, d.H, WB{}, [&](int i) { return gout[at(b, i, x, d)].x; });
-        s.y = corr5_adjoint(y, d.H, WB{}, [&](int i) { return gout[at(b, i, x, d)].y; });
+        s.x = corr5_adjoint(y, d.H, WB{}, [&](int i) { return g(at(b, i, x, d)).x; });
+        s.y = corr5_adjoint(y, d.H, WB{}, [&](int i) { return g(at(b, i, x, d)).y; });
     }
-    put2(gin, at(b, y, x, d), s, acc);
+    put2(gin, at(b, y, x, d), s, acc & 1);
 }
 
 }  // namespace
@@ -485,23 +498,23 @@ namespace ad {
 void image_model(const float* I, float* a0, float* g, int B, int H, int W, hipStream_t s) {
     hipLaunchKernelGGL(k_ad_image_model, grid3(B, H, W), dim3(BX, BY), 0, s, I, a0, (float2*)g, Dims{H, W});
 }
-void image_model_bwd(const float* ga0, const float* gg, float* gI, float* work, int B, int H, int W, bool acc,
+void image_model_bwd(const float* ga0, const float* gg, float* gI, float* work, int B, int H, int W, int acc,
                      hipStream_t s) {
     const size_t n = (size_t)B * H * W;
     hipLaunchKernelGGL(k_ad_image_model_bwd1, grid3(B, H, W), dim3(BX, BY), 0, s, ga0, (const float2*)gg, work,
                        work + n, Dims{H, W});
     hipLaunchKernelGGL(k_ad_image_model_bwd2, grid3(B, H, W), dim3(BX, BY), 0, s, work, work + n, gI, Dims{H, W},
-                       (int)acc);
+                       acc);
 }
 void down(const float* I, float* O, float* work, int B, int H, int W, hipStream_t s) {
     const Dims d0{H, W}, d1{H, W / 2}, d2{H / 2, W / 2};
     hipLaunchKernelGGL((k_ad_down<0>), grid3(B, d1.H, d1.W), dim3(BX, BY), 0, s, I, work, d0, d1);
     hipLaunchKernelGGL((k_ad_down<1>), grid3(B, d2.H, d2.W), dim3(BX, BY), 0, s, work, O, d1, d2);
 }
-void down_bwd(const float* gO, float* gI, float* work, int B, int H, int W, bool acc, hipStream_t s) {
+void down_bwd(const float* gO, float* gI, float* work, int B, int H, int W, int acc, hipStream_t s) {
     const Dims d0{H, W}, d1{H, W / 2}, d2{H / 2, W / 2};
     hipLaunchKernelGGL((k_ad_down_bwd<1>), grid3(B, d1.H, d1.W), dim3(BX, BY), 0, s, gO, work, d1, d2, 0);
-    hipLaunchKernelGGL((k_ad_down_bwd<0>), grid3(B, d0.H, d0.W), dim3(BX, BY), 0, s, work, gI, d0, d1, (int)acc);
+    hipLaunchKernelGGL((k_ad_down_bwd<0>), grid3(B, d0.H, d0.W), dim3(BX, BY), 0, s, work, gI, d0, d1, acc);
 }
 void prop(int axis, const float* fin, const float* sin, const float* vin, float* fout, float* sout, float* vout, int B,
           int H, int W, float dt, int border, hipStream_t s) {
@@ -520,7 +533,7 @@ void prop(int axis, const float* fin, const float* sin, const float* vin, float*
 }
 void prop_bwd(int axis, const float* fin, const float* sin, const float* vin, const float* gfo, const float* gso,
               const float* gvo, float* gfi, float* gsi, float* gvi, int B, int H, int W, float dt, int border,
-              hipStream_t s) {
+              int acc, hipStream_t s) {
     const bool pay = sin != nullptr;
     const auto g = grid3(B, H, W);
     const dim3 blk(BX, BY);
@@ -528,11 +541,11 @@ void prop_bwd(int axis, const float* fin, const float* sin, const float* vin, co
     auto F = [](const float* p) { return (const float2*)p; };
     auto O = [](float* p) { return (float2*)p; };
     if (axis == 0) {
-        if (pay) hipLaunchKernelGGL((k_ad_prop_bwd<0, true>), g, blk, 0, s, F(fin), sin, F(vin), F(gfo), gso, F(gvo), O(gfi), gsi, O(gvi), d, dt, border);
-        else hipLaunchKernelGGL((k_ad_prop_bwd<0, false>), g, blk, 0, s, F(fin), sin, F(vin), F(gfo), gso, F(gvo), O(gfi), gsi, O(gvi), d, dt, border);
+        if (pay) hipLaunchKernelGGL((k_ad_prop_bwd<0, true>), g, blk, 0, s, F(fin), sin, F(vin), F(gfo), gso, F(gvo), O(gfi), gsi, O(gvi), d, dt, border, acc);
+        else hipLaunchKernelGGL((k_ad_prop_bwd<0, false>), g, blk, 0, s, F(fin), sin, F(vin), F(gfo), gso, F(gvo), O(gfi), gsi, O(gvi), d, dt, border, acc);
     } else {
-        if (pay) hipLaunchKernelGGL((k_ad_prop_bwd<1, true>), g, blk, 0, s, F(fin), sin, F(vin), F(gfo), gso, F(gvo), O(gfi), gsi, O(gvi), d, dt, border);
-        else hipLaunchKernelGGL((k_ad_prop_bwd<1, false>), g, blk, 0, s, F(fin), sin, F(vin), F(gfo), gso, F(gvo), O(gfi), gsi, O(gvi), d, dt, border);
+        if (pay) hipLaunchKernelGGL((k_ad_prop_bwd<1, true>), g, blk, 0, s, F(fin), sin, F(vin), F(gfo), gso, F(gvo), O(gfi), gsi, O(gvi), d, dt, border, acc);
+        else hipLaunchKernelGGL((k_ad_prop_bwd<1, false>), g, blk, 0, s, F(fin), sin, F(vin), F(gfo), gso, F(gvo), O(gfi), gsi, O(gvi), d, dt, border, acc);
     }
 }
 int blocks(int H, int W) { return ((W + BX - 1) / BX) * ((H + BY - 1) / BY); }
@@ -543,10 +556,10 @@ void update(const float* a0, const float* g, const float* aold, const float* f, 
 }
 void update_bwd(const float* a0, const float* g, const float* aold, const float* f, const float* gout, float* ga0,
                 float* gg, float* gaold, float* gf, float* gamma_partial, int B, int H, int W, float gamma,
-                float maxflow, bool acc, hipStream_t s) {
+                float maxflow, int acc, hipStream_t s) {
     hipLaunchKernelGGL(k_ad_update_bwd, grid3(B, H, W), dim3(BX, BY), 0, s, a0, (const float2*)g, aold,
                        (const float2*)f, (const float2*)gout, ga0, (float2*)gg, gaold, (float2*)gf, gamma_partial,
-                       Dims{H, W}, gamma, maxflow, (int)acc);
+                       Dims{H, W}, gamma, maxflow, acc);
 }
 void delta_update(const float* a0, const float* g, const float* sprop, const float* dprop, const float* upper,
                   const int* xmap, const int* ymap, float* dout, float* fout, int B, int H, int W, int Hu, int Wu,
@@ -558,24 +571,26 @@ void delta_update(const float* a0, const float* g, const float* sprop, const flo
 void delta_update_bwd(const float* a0, const float* g, const float* sprop, const float* dprop, const float* upper,
                       const int* xmap, const int* ymap, const float* gdout, const float* gfout, float* ga0, float* gg,
                       float* gsprop, float* gdprop, float* gup, float* gamma_partial, int B, int H, int W, int Hu,
-                      int Wu, float gamma, float maxflow, bool acc, hipStream_t s) {
+                      int Wu, float gamma, float maxflow, int acc, hipStream_t s) {
     hipLaunchKernelGGL(k_ad_delta_update_bwd, grid3(B, H, W), dim3(BX, BY), 0, s, a0, (const float2*)g, sprop,
                        (const float2*)dprop, (const float2*)upper, xmap, ymap, (const float2*)gdout,
                        (const float2*)gfout, ga0, (float2*)gg, gsprop, (float2*)gdprop, (float2*)gup, gamma_partial,
-                       Dims{H, W}, Dims{Hu, Wu}, gamma, maxflow, (int)acc);
+                       Dims{H, W}, Dims{Hu, Wu}, gamma, maxflow, acc);
 }
 void upper_gather(const float* gup, const int* xlo, const int* xhi, const int* ylo, const int* yhi, float* gupper,
-                  int B, int H, int W, int Hu, int Wu, bool acc, hipStream_t s) {
+                  int B, int H, int W, int Hu, int Wu, int acc, hipStream_t s) {
     hipLaunchKernelGGL(k_ad_upper_gather, grid3(B, Hu, Wu), dim3(BX, BY), 0, s, (const float2*)gup, xlo, xhi, ylo, yhi,
-                       (float2*)gupper, Dims{H, W}, Dims{Hu, Wu}, (int)acc);
+                       (float2*)gupper, Dims{H, W}, Dims{Hu, Wu}, acc);
 }
 void smooth(int axis, const float* in, float* out, int B, int H, int W, hipStream_t s) {
     if (axis == 0) hipLaunchKernelGGL((k_ad_smooth<0>), grid3(B, H, W), dim3(BX, BY), 0, s, (const float2*)in, (float2*)out, Dims{H, W});
     else hipLaunchKernelGGL((k_ad_smooth<1>), grid3(B, H, W), dim3(BX, BY), 0, s, (const float2*)in, (float2*)out, Dims{H, W});
 }
-void smooth_bwd(int axis, const float* gout, float* gin, int B, int H, int W, bool acc, hipStream_t s) {
-    if (axis == 0) hipLaunchKernelGGL((k_ad_smooth_bwd<0>), grid3(B, H, W), dim3(BX, BY), 0, s, (const float2*)gout, (float2*)gin, Dims{H, W}, (int)acc);
-    else hipLaunchKernelGGL((k_ad_smooth_bwd<1>), grid3(B, H, W), dim3(BX, BY), 0, s, (const float2*)gout, (float2*)gin, Dims{H, W}, (int)acc);
+void smooth_bwd(int axis, const float* gout, const float* gout2, float* gin, int B, int H, int W, int acc,
+                hipStream_t s) {
+    auto F = [](const float* p) { return (const float2*)p; };
+    if (axis == 0) hipLaunchKernelGGL((k_ad_smooth_bwd<0>), grid3(B, H, W), dim3(BX, BY), 0, s, F(gout), F(gout2), (float2*)gin, Dims{H, W}, acc);
+    else hipLaunchKernelGGL((k_ad_smooth_bwd<1>), grid3(B, H, W), dim3(BX, BY), 0, s, F(gout), F(gout2), (float2*)gin, Dims{H, W}, acc);
 }
 
 }  // namespace ad

@@ -625,7 +625,7 @@ ff_status ff_ad_image_model_backward(const float* g_a0, const float* g_grad, flo
     return guard([&] {
         dims(B, H, W);
         need(g_a0, "g_a0"), need(g_grad, "g_grad"), need(g_image, "g_image"), need(work, "work");
-        ad::image_model_bwd(g_a0, g_grad, g_image, work, B, H, W, accumulate != 0, as_stream(stream));
+        ad::image_model_bwd(g_a0, g_grad, g_image, work, B, H, W, accumulate != 0 ? 1 : 0, as_stream(stream));
         check_launch("ff_ad_image_model_backward");
     });
 }
@@ -642,7 +642,7 @@ ff_status ff_ad_down_backward(const float* g_half, float* g_image, float* work, 
     return guard([&] {
         dims(B, H / 2 > 0 ? H : 0, W / 2 > 0 ? W : 0);
         need(g_half, "g_half"), need(g_image, "g_image"), need(work, "work");
-        ad::down_bwd(g_half, g_image, work, B, H, W, accumulate != 0, as_stream(stream));
+        ad::down_bwd(g_half, g_image, work, B, H, W, accumulate != 0 ? 1 : 0, as_stream(stream));
         check_launch("ff_ad_down_backward");
     });
 }
@@ -664,17 +664,18 @@ ff_status ff_ad_propagate_pass(int axis, const float* flow, const float* scalar,
 ff_status ff_ad_propagate_pass_backward(int axis, const float* flow, const float* scalar, const float* vector,
                                         const float* g_flow_out, const float* g_scalar_out, const float* g_vector_out,
                                         float* g_flow, float* g_scalar, float* g_vector, int B, int H, int W, float dt,
-                                        int border, void* stream) {
+                                        int border, int accumulate, void* stream) {
     return guard([&] {
         dims(B, H, W);
         if (axis != 0 && axis != 1) throw invalid_argument("axis must be 0 (x) or 1 (y)");
         if (border < 1) throw invalid_argument("border must be at least 1");
-        need(flow, "flow"), need(g_flow_out, "g_flow_out"), need(g_flow, "g_flow");
+        need(flow, "flow"), need(g_flow, "g_flow");
         const bool pay = scalar != nullptr;
-        if (pay && (!vector || !g_scalar_out || !g_vector_out || !g_scalar || !g_vector))
-            throw invalid_argument("payloads: pass all of scalar, vector, their output gradients and gradients");
-        ad::prop_bwd(axis, flow, scalar, vector, g_flow_out, g_scalar_out, g_vector_out, g_flow, g_scalar, g_vector, B,
-                     H, W, dt, border, as_stream(stream));
+        if (pay && (!vector || !g_scalar || !g_vector))
+            throw invalid_argument("payloads: pass scalar, vector and both of their gradients");
+        ad::prop_bwd(axis, flow, scalar, vector, g_flow_out, pay ? g_scalar_out : nullptr,
+                     pay ? g_vector_out : nullptr, g_flow, g_scalar, g_vector, B, H, W, dt, border, accumulate,
+                     as_stream(stream));
         check_launch("ff_ad_propagate_pass_backward");
     });
 }
@@ -698,7 +699,7 @@ ff_status ff_ad_update_backward(const float* a0, const float* grad, const float*
                               (const void*)g_flow, (const void*)g_gamma_partial})
             need(p, "argument");
         ad::update_bwd(a0, grad, a0_old, flow, g_flow_out, g_a0, g_grad, g_a0_old, g_flow, g_gamma_partial, B, H, W,
-                       gamma, maxflow, accumulate != 0, as_stream(stream));
+                       gamma, maxflow, accumulate, as_stream(stream));
         check_launch("ff_ad_update_backward");
     });
 }
@@ -727,13 +728,13 @@ ff_status ff_ad_delta_update_backward(const float* a0, const float* grad, const 
         dims(B, H, W);
         dims(B, Hu, Wu);
         for (const void* p : {(const void*)a0, (const void*)grad, (const void*)a0_prop, (const void*)dflow_prop,
-                              (const void*)upper, (const void*)xmap, (const void*)ymap, (const void*)g_dflow_out,
+                              (const void*)upper, (const void*)xmap, (const void*)ymap,
                               (const void*)g_flow_out, (const void*)g_a0, (const void*)g_grad, (const void*)g_a0_prop,
                               (const void*)g_dflow_prop, (const void*)g_upper_pixel, (const void*)g_gamma_partial})
             need(p, "argument");
         ad::delta_update_bwd(a0, grad, a0_prop, dflow_prop, upper, xmap, ymap, g_dflow_out, g_flow_out, g_a0, g_grad,
                              g_a0_prop, g_dflow_prop, g_upper_pixel, g_gamma_partial, B, H, W, Hu, Wu, gamma, maxflow,
-                             accumulate != 0, as_stream(stream));
+                             accumulate, as_stream(stream));
         check_launch("ff_ad_delta_update_backward");
     });
 }
@@ -746,7 +747,7 @@ ff_status ff_ad_upper_backward(const float* g_upper_pixel, const int* xlo, const
         for (const void* p : {(const void*)g_upper_pixel, (const void*)xlo, (const void*)xhi, (const void*)ylo,
                               (const void*)yhi, (const void*)g_upper})
             need(p, "argument");
-        ad::upper_gather(g_upper_pixel, xlo, xhi, ylo, yhi, g_upper, B, H, W, Hu, Wu, accumulate != 0,
+        ad::upper_gather(g_upper_pixel, xlo, xhi, ylo, yhi, g_upper, B, H, W, Hu, Wu, accumulate != 0 ? 1 : 0,
                          as_stream(stream));
         check_launch("ff_ad_upper_backward");
     });
@@ -760,13 +761,13 @@ ff_status ff_ad_smooth_pass(int axis, const float* flow, float* flow_out, int B,
         check_launch("ff_ad_smooth_pass");
     });
 }
-ff_status ff_ad_smooth_pass_backward(int axis, const float* g_flow_out, float* g_flow, int B, int H, int W,
-                                     int accumulate, void* stream) {
+ff_status ff_ad_smooth_pass_backward(int axis, const float* g_flow_out, const float* g_flow_out2, float* g_flow, int B,
+                                     int H, int W, int accumulate, void* stream) {
     return guard([&] {
         dims(B, H, W);
         if (axis != 0 && axis != 1) throw invalid_argument("axis must be 0 (x) or 1 (y)");
         need(g_flow_out, "g_flow_out"), need(g_flow, "g_flow");
-        ad::smooth_bwd(axis, g_flow_out, g_flow, B, H, W, accumulate != 0, as_stream(stream));
+        ad::smooth_bwd(axis, g_flow_out, g_flow_out2, g_flow, B, H, W, accumulate != 0 ? 1 : 0, as_stream(stream));
         check_launch("ff_ad_smooth_pass_backward");
     });
 }

@@ -164,32 +164,33 @@ private:
 // (ff_adjoint.hip). B contiguous images of H x W; vector fields are float2.
 namespace ad {
 void image_model(const float* I, float* a0, float* g, int B, int H, int W, hipStream_t s);
-void image_model_bwd(const float* ga0, const float* gg, float* gI, float* work, int B, int H, int W, bool acc,
+void image_model_bwd(const float* ga0, const float* gg, float* gI, float* work, int B, int H, int W, int acc,
                      hipStream_t s);
 void down(const float* I, float* O, float* work, int B, int H, int W, hipStream_t s);
-void down_bwd(const float* gO, float* gI, float* work, int B, int H, int W, bool acc, hipStream_t s);
+void down_bwd(const float* gO, float* gI, float* work, int B, int H, int W, int acc, hipStream_t s);
 void prop(int axis, const float* fin, const float* sin, const float* vin, float* fout, float* sout, float* vout, int B,
           int H, int W, float dt, int border, hipStream_t s);
 void prop_bwd(int axis, const float* fin, const float* sin, const float* vin, const float* gfo, const float* gso,
               const float* gvo, float* gfi, float* gsi, float* gvi, int B, int H, int W, float dt, int border,
-              hipStream_t s);
+              int acc, hipStream_t s);
 int blocks(int H, int W);
 void update(const float* a0, const float* g, const float* aold, const float* f, float* fout, int B, int H, int W,
             float gamma, float maxflow, hipStream_t s);
 void update_bwd(const float* a0, const float* g, const float* aold, const float* f, const float* gout, float* ga0,
                 float* gg, float* gaold, float* gf, float* gamma_partial, int B, int H, int W, float gamma,
-                float maxflow, bool acc, hipStream_t s);
+                float maxflow, int acc, hipStream_t s);
 void delta_update(const float* a0, const float* g, const float* sprop, const float* dprop, const float* upper,
                   const int* xmap, const int* ymap, float* dout, float* fout, int B, int H, int W, int Hu, int Wu,
                   float gamma, float maxflow, hipStream_t s);
 void delta_update_bwd(const float* a0, const float* g, const float* sprop, const float* dprop, const float* upper,
                       const int* xmap, const int* ymap, const float* gdout, const float* gfout, float* ga0, float* gg,
                       float* gsprop, float* gdprop, float* gup, float* gamma_partial, int B, int H, int W, int Hu,
-                      int Wu, float gamma, float maxflow, bool acc, hipStream_t s);
+                      int Wu, float gamma, float maxflow, int acc, hipStream_t s);
 void upper_gather(const float* gup, const int* xlo, const int* xhi, const int* ylo, const int* yhi, float* gupper,
-                  int B, int H, int W, int Hu, int Wu, bool acc, hipStream_t s);
+                  int B, int H, int W, int Hu, int Wu, int acc, hipStream_t s);
 void smooth(int axis, const float* in, float* out, int B, int H, int W, hipStream_t s);
-void smooth_bwd(int axis, const float* gout, float* gin, int B, int H, int W, bool acc, hipStream_t s);
+void smooth_bwd(int axis, const float* gout, const float* gout2, float* gin, int B, int H, int W, int acc,
+                hipStream_t s);
 }  // namespace ad
 
 }  // namespace ffamd

@@ -123,6 +123,10 @@ def _stream():
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
+def _stream_device():
+    return torch.device("cuda", torch.cuda.current_device())
+
+
 def _empty(B, h, w, c=None, like=None):
     shape = (B, h, w) if c is None else (B, h, w, c)
     return torch.empty(shape, dtype=torch.float32, device=like.device)
@@ -230,120 +234,139 @@ def _smooth(f, K, B, h, w, s):
     return f
 
 
-def _smooth_bwd(gf, K, B, h, w, s):
-    for _ in range(K):
-        for axis in (1, 0):
-            o = torch.empty_like(gf)
-            N.call("ff_ad_smooth_pass_backward", axis, _p(gf), _p(o), B, h, w, 0, s)
-            gf = o
-    return gf
+def _smooth_bwd(parts, K, B, h, w, s, out=None):
+    """Adjoint of K smoothing iterations for the output gradient sum(parts)
+    (one or two tensors, summed inside the first pass). The result is
+    written to a new tensor, or added to `out`."""
+    parts = [p for p in parts if p is not None]
+    if not parts:  # no gradient reaches the smoothed flow
+        return out if out is not None else torch.zeros(B, h, w, 2, device=_stream_device())
+    if K == 0:
+        total = parts[0] if len(parts) == 1 else parts[0] + parts[1]
+        if out is None:
+            return total
+        out.add_(total)
+        return out
+    n = 2 * K
+    g, g2 = parts[0], (parts[1] if len(parts) > 1 else None)
+    for i, axis in enumerate([1, 0] * K):
+        last = i == n - 1
+        o = out if (last and out is not None) else torch.empty_like(g)
+        N.call("ff_ad_smooth_pass_backward", axis, _p(g), _p(g2), _p(o), B, h, w, 1 if (last and out is not None) else 0,
+               s)
+        g, g2 = o, None
+    return g
 
 
-def _frame_backward(spec, saved, g_new, g_out, gammas):
+def _frame_backward(spec, saved, g_new, g_out, gammas, g_image_out):
     """Adjoint of one frame. g_new: per level dict of gradients of the new
-    state (None entries are zero); g_out: gradient of the output flow.
-    Returns (g_old_state, g_image, g_gamma_eff [L])."""
+    state (missing entries are zero; the tensors are consumed: the adjoint
+    adds into them in place); g_out: gradient of the output flow (level 0's
+    smoothed flow); g_image_out: where the frame's image gradient goes.
+    Returns (g_old_state or None for the first frame, g_gamma_eff [L])."""
     L = spec.L
     top = L - 1
     a0, g = saved["a0"], saved["g"]
     B = a0[0].shape[0]
     s = _stream()
     dev = a0[0].device
-    zeros = lambda *shape: torch.zeros(*shape, device=dev)
     g_old = [dict() for _ in range(L)]
     g_gamma = torch.zeros(L, dtype=torch.float64, device=dev)
     g_a0 = [None] * L
     g_g = [None] * L
-    g_upd_top = zeros(B, spec.sizes[top][0], spec.sizes[top][1], 2)
-
-    def gnew(l, k, shape):
-        v = g_new[l].get(k) if g_new is not None else None
-        return v if v is not None else zeros(*shape)
+    g_upd_top = None
+    new = (lambda l, k: g_new[l].get(k)) if g_new is not None else (lambda l, k: None)
+    f32 = ctypes.c_float
 
     for l in range(L - 1):
         h, w = spec.sizes[l]
         hu, wu = spec.sizes[l + 1]
         lv = spec.levels[l]
-        gfs = gnew(l, "f", (B, h, w, 2))
-        if l == 0 and g_out is not None:
-            gfs = gfs + g_out
-        gfl = _smooth_bwd(gfs.contiguous(), lv["K"], B, h, w, s)
-        gdn = gnew(l, "d", (B, h, w, 2)).contiguous()
-        ga, gg, gsp, gvp, gup = (_empty(B, h, w, like=gfl), _empty(B, h, w, 2, like=gfl), _empty(B, h, w, like=gfl),
-                                 _empty(B, h, w, 2, like=gfl), _empty(B, h, w, 2, like=gfl))
-        nb = N.get_int("ff_ad_partial_count", h, w)
-        part = torch.empty(B * nb, device=dev)
+        gfl = _smooth_bwd([new(l, "f"), g_out if l == 0 else None], lv["K"], B, h, w, s)
+        ga = new(l, "a0")
+        mask = 1 if ga is not None else 0
+        if ga is None:
+            ga = _empty(B, h, w, like=gfl)
+        gg, gsp, gvp, gup = (_empty(B, h, w, 2, like=gfl), _empty(B, h, w, like=gfl), _empty(B, h, w, 2, like=gfl),
+                             _empty(B, h, w, 2, like=gfl))
+        part = torch.empty(B * N.get_int("ff_ad_partial_count", h, w), device=dev)
         f, sc, vc = saved["prop"][l]
         N.call("ff_ad_delta_update_backward", _p(a0[l]), _p(g[l]), _p(sc), _p(vc), _p(saved["upper"][l]),
-               _p(lv["xmap"]), _p(lv["ymap"]), _p(gdn), _p(gfl), _p(ga), _p(gg), _p(gsp), _p(gvp), _p(gup), _p(part),
-               B, h, w, hu, wu, ctypes.c_float(gammas[l]), ctypes.c_float(lv["maxflow"]), 0, s)
+               _p(lv["xmap"]), _p(lv["ymap"]), _p(new(l, "d")), _p(gfl), _p(ga), _p(gg), _p(gsp), _p(gvp), _p(gup),
+               _p(part), B, h, w, hu, wu, f32(gammas[l]), f32(lv["maxflow"]), mask, s)
         g_gamma[l] += part.double().sum()
-        # the upper flow: this frame's top update (l = L-2) or the old state
+        # the upper flow: this frame's top update (l = L-2) or the old
+        # smoothed flow of level l+1 (whose own adjoint comes later and
+        # adds into it)
         gupper = _empty(B, hu, wu, 2, like=gfl)
         N.call("ff_ad_upper_backward", _p(gup), _p(lv["xlo"]), _p(lv["xhi"]), _p(lv["ylo"]), _p(lv["yhi"]),
                _p(gupper), B, h, w, hu, wu, 0, s)
         if l == L - 2:
-            g_upd_top += gupper
+            g_upd_top = gupper
         else:
-            g_old[l + 1]["f"] = g_old[l + 1].get("f", 0) + gupper
-        g_a0[l] = ga + gnew(l, "a0", (B, h, w))
+            g_old[l + 1]["f"] = gupper
+        g_a0[l] = ga
         g_g[l] = gg
-        # propagation passes in reverse
-        gf, gs, gv = zeros(B, h, w, 2), gsp, gvp
-        gf = gf.contiguous()
-        for (fi, si, vi), axis in zip(reversed(saved["passes"][l]), [1, 0] * lv["N"]):
-            nf, ns, nv = _empty(B, h, w, 2, like=gfl), _empty(B, h, w, like=gfl), _empty(B, h, w, 2, like=gfl)
+        # propagation passes in reverse; the propagated flow itself is not
+        # read by the delta update, so its gradient enters as zero (None)
+        gf, gs, gv = None, gsp, gvp
+        passes = saved["passes"][l]
+        for i, ((fi, si, vi), axis) in enumerate(zip(reversed(passes), [1, 0] * lv["N"])):
+            last = i == len(passes) - 1
+            acc_f = last and "f" in g_old[l]
+            nf = g_old[l]["f"] if acc_f else _empty(B, h, w, 2, like=gfl)
+            ns, nv = _empty(B, h, w, like=gfl), _empty(B, h, w, 2, like=gfl)
             N.call("ff_ad_propagate_pass_backward", axis, _p(fi), _p(si), _p(vi), _p(gf), _p(gs), _p(gv), _p(nf),
-                   _p(ns), _p(nv), B, h, w, ctypes.c_float(lv["dt"]), spec.border, s)
+                   _p(ns), _p(nv), B, h, w, f32(lv["dt"]), spec.border, 1 if acc_f else 0, s)
             gf, gs, gv = nf, ns, nv
-        g_old[l]["f"] = g_old[l].get("f", 0) + gf
+        g_old[l]["f"] = gf
         g_old[l]["a0"] = gs
         g_old[l]["d"] = gv
     # top level
     h, w = spec.sizes[top]
     lv = spec.levels[top]
-    gfs = gnew(top, "f", (B, h, w, 2))
-    if L == 1 and g_out is not None:
-        gfs = gfs + g_out
-    g_upd_top = g_upd_top + _smooth_bwd(gfs.contiguous(), lv["K"], B, h, w, s)
-    ga, gg, gao, gfp = (_empty(B, h, w, like=g_upd_top), _empty(B, h, w, 2, like=g_upd_top),
-                        _empty(B, h, w, like=g_upd_top), _empty(B, h, w, 2, like=g_upd_top))
-    nb = N.get_int("ff_ad_partial_count", h, w)
-    part = torch.empty(B * nb, device=dev)
+    g_upd_top = _smooth_bwd([new(top, "f"), g_out if L == 1 else None], lv["K"], B, h, w, s, out=g_upd_top)
+    ga = new(top, "a0")
+    mask = 1 if ga is not None else 0
+    if ga is None:
+        ga = _empty(B, h, w, like=g_upd_top)
+    gg, gao, gfp = (_empty(B, h, w, 2, like=g_upd_top), _empty(B, h, w, like=g_upd_top),
+                    _empty(B, h, w, 2, like=g_upd_top))
+    part = torch.empty(B * N.get_int("ff_ad_partial_count", h, w), device=dev)
     st = saved["state"][top]
     N.call("ff_ad_update_backward", _p(a0[top]), _p(g[top]), _p(st["a0"]), _p(saved["prop"][top][0]),
-           _p(g_upd_top.contiguous()), _p(ga), _p(gg), _p(gao), _p(gfp), _p(part), B, h, w,
-           ctypes.c_float(gammas[top]), ctypes.c_float(lv["maxflow"]), 0, s)
+           _p(g_upd_top), _p(ga), _p(gg), _p(gao), _p(gfp), _p(part), B, h, w, f32(gammas[top]),
+           f32(lv["maxflow"]), mask, s)
     g_gamma[top] += part.double().sum()
-    g_a0[top] = ga + gnew(top, "a0", (B, h, w))
+    g_a0[top] = ga
     g_g[top] = gg
     g_old[top]["a0"] = gao
     gf = gfp
     for (fi, _, _), axis in zip(reversed(saved["passes"][top]), [1, 0] * lv["N"]):
         nf = _empty(B, h, w, 2, like=gf)
         N.call("ff_ad_propagate_pass_backward", axis, _p(fi), None, None, _p(gf), None, None, _p(nf), None, None,
-               B, h, w, ctypes.c_float(lv["dt"]), spec.border, s)
+               B, h, w, f32(lv["dt"]), spec.border, 0, s)
         gf = nf
-    g_old[top]["f"] = g_old[top].get("f", 0) + gf
+    g_old[top]["f"] = gf
     if saved["first"]:
         # first frame: the old brightness (and the delta levels' scalar
         # payload) was this frame's a0; the flow and delta flow were zero
         for l in range(L):
-            g_a0[l] = g_a0[l] + g_old[l]["a0"]
+            g_a0[l].add_(g_old[l]["a0"])
         g_old = None
     # image models, then the pyramid from the top down
     g_img = [None] * L
     for l in range(L):
         h, w = spec.sizes[l]
-        gi = _empty(B, h, w, like=g_a0[l])
+        gi = g_image_out if l == 0 else _empty(B, h, w, like=g_a0[l])
         work = _empty(B, 2 * h, w, like=g_a0[l])
-        N.call("ff_ad_image_model_backward", _p(g_a0[l].contiguous()), _p(g_g[l]), _p(gi), _p(work), B, h, w, 0, s)
+        N.call("ff_ad_image_model_backward", _p(g_a0[l]), _p(g_g[l]), _p(gi), _p(work), B, h, w, 0, s)
         g_img[l] = gi
     for l in range(L - 1, 0, -1):
         h, w = spec.sizes[l - 1]
         work = _empty(B, h, w // 2, like=g_img[l])
         N.call("ff_ad_down_backward", _p(g_img[l]), _p(g_img[l - 1]), _p(work), B, h, w, 1, s)
-    return g_old, g_img[0], g_gamma
+    return g_old, g_gamma
 
 
 # ---------------------------------------------------------- the function
@@ -376,7 +399,8 @@ class _FlowWindow(torch.autograd.Function):
         (frames,) = ctx.saved_tensors
         spec, C = ctx.spec, ctx.checkpoint
         B, T = frames.shape[:2]
-        g_frames = torch.zeros_like(frames)
+        # image gradients frame-major, so each frame's is one contiguous block
+        g_tb = torch.empty((T, B) + tuple(frames.shape[2:]), dtype=frames.dtype, device=frames.device)
         g_gamma = torch.zeros(spec.L, dtype=torch.float64, device=frames.device)
         g_state = None
         for t0 in reversed(range(0, T, C)):
@@ -386,15 +410,15 @@ class _FlowWindow(torch.autograd.Function):
                 state, _, sv = _frame_forward(spec, state, frames[:, t].contiguous(), ctx.gammas, True)
                 saved.append(sv)
             for t in reversed(range(t0, t1)):
-                g_state, g_img, g_gam = _frame_backward(spec, saved[t - t0], g_state, g_flows[:, t].contiguous(),
-                                                         ctx.gammas)
-                g_frames[:, t] = g_img
+                g_state, g_gam = _frame_backward(spec, saved[t - t0], g_state, g_flows[:, t].contiguous(),
+                                                 ctx.gammas, g_tb[t])
                 g_gamma += g_gam
+                saved[t - t0] = None
             del saved
         ctx.ckpts = None
         dtype, device, shape = ctx.gamma_like
         g_gamma = (g_gamma / _U8_SCALE).to(dtype=dtype, device=device).reshape(shape)
-        return g_frames, g_gamma, None, None
+        return g_tb.transpose(0, 1), g_gamma, None, None
 
 
 def flow_window(frames, gamma, spec, checkpoint=4):
@@ -414,21 +438,23 @@ def flow_window(frames, gamma, spec, checkpoint=4):
 def photometric_error(frames, flows):
     """Mean over frames t >= 1 of the mean squared difference between frame
     t and frame t-1 sampled bilinearly at x - flow_t(x) (edge-clamped).
-    frames [B,T,H,W], flows [B,T,H,W,2] (pixels per frame)."""
+    frames [B,T,H,W], flows [B,T,H,W,2] (pixels per frame). All frame
+    pairs go through one grid_sample call, so autograd handles two slices
+    of the frames instead of two per frame."""
     B, T, H, W = frames.shape
     if T < 2:
         return frames.sum() * 0.0
     dev, dt = frames.device, frames.dtype
-    ys, xs = torch.meshgrid(torch.arange(H, device=dev, dtype=dt), torch.arange(W, device=dev, dtype=dt),
-                            indexing="ij")
-    total = 0.0
-    for t in range(1, T):
-        gx = 2.0 * (xs - flows[:, t, ..., 0]) / max(W - 1, 1) - 1.0
-        gy = 2.0 * (ys - flows[:, t, ..., 1]) / max(H - 1, 1) - 1.0
-        prev = torch.nn.functional.grid_sample(frames[:, t - 1, None], torch.stack([gx, gy], -1), mode="bilinear",
-                                               padding_mode="border", align_corners=True)[:, 0]
-        total = total + ((frames[:, t] - prev) ** 2).mean()
-    return total / (T - 1)
+    xs = torch.arange(W, device=dev, dtype=dt)
+    ys = torch.arange(H, device=dev, dtype=dt)[:, None]
+    fl = flows[:, 1:]
+    gx = (xs - fl[..., 0]) * (2.0 / max(W - 1, 1)) - 1.0
+    gy = (ys - fl[..., 1]) * (2.0 / max(H - 1, 1)) - 1.0
+    grid = torch.stack([gx, gy], -1).reshape(B * (T - 1), H, W, 2)
+    prev = frames[:, :-1].reshape(B * (T - 1), 1, H, W)
+    warped = torch.nn.functional.grid_sample(prev, grid, mode="bilinear", padding_mode="border",
+                                             align_corners=True).reshape(B, T - 1, H, W)
+    return ((frames[:, 1:] - warped) ** 2).mean()
 
 
 def flow_loss(frames, flows, truth, photometric_weight=0.1):

@@ -20,8 +20,16 @@ renders), so parity tests and benchmarks run on these generated sequences:
   (translation up to 2 px plus rotation up to 0.5 deg and scale within 1%
   about the centre, per frame, displacement capped at 4 px). Seed 3.
 
+* ``smooth_flow_sequence``: band-limited noise texture moved by a smooth
+  random flow field (seeded Gaussian noise low-passed with sigma 64 px,
+  scaled to a maximum of 8 px per frame): frame t(x) = texture(x - t v(x)),
+  quantized to u8 levels. Built with torch on a device (FFT low-pass and
+  bilinear warp), so 4K windows take well under a second on a GPU.
+
 All randomness comes from numpy.random.default_rng(seed) (PCG64), so a seed
-gives the same frames on every machine with this numpy.
+gives the same frames on every machine with this numpy (for
+``smooth_flow_sequence``, up to the device FFT's rounding before the u8
+quantization).
 """
 import numpy as np
 from scipy import ndimage
@@ -136,3 +144,39 @@ def camera_streams(n=64, frames=3, seed=3, maxflow=4.0):
             seq.append(_quantize(ndimage.map_coordinates(base, coords, order=1, mode="nearest")))
         out.append(dict(h=h, w=w, levels=levels, gamma=gamma, maxflow=maxflow, smooth=smooth, frames=seq))
     return out
+
+
+def smooth_flow_sequence(h, w, frames, maxflow=8.0, flow_sigma=64.0, texture_sigma=3.0, seed=7, device="cuda"):
+    """Frames [frames, h, w] (float32 torch tensor, u8 levels / 255) of a
+    band-limited noise texture advected by a smooth random flow field, and
+    that field v [h, w, 2] (px/frame): frame t(x) = texture(x - t v(x)).
+    The field is Gaussian white noise low-passed (sigma flow_sigma px, via
+    FFT) and scaled so that max |v| = maxflow."""
+    import torch
+
+    rng = np.random.default_rng(seed)
+    pad = int(np.ceil(maxflow * frames)) + 16
+
+    def lowpass(x, sigma):
+        H, W = x.shape[-2:]
+        ky = torch.fft.fftfreq(H, device=x.device, dtype=torch.float64)[:, None]
+        kx = torch.fft.rfftfreq(W, device=x.device, dtype=torch.float64)[None, :]
+        g = torch.exp(-2.0 * (np.pi * sigma) ** 2 * (kx * kx + ky * ky))
+        return torch.fft.irfft2(torch.fft.rfft2(x) * g, s=(H, W))
+
+    v = lowpass(torch.from_numpy(rng.standard_normal((2, h, w))).to(device), flow_sigma)
+    v = v * (maxflow / float(torch.sqrt(v[0] ** 2 + v[1] ** 2).max()))
+    tex = lowpass(torch.from_numpy(rng.standard_normal((h + 2 * pad, w + 2 * pad))).to(device), texture_sigma)
+    tex = (tex - tex.min()) * (255.0 / float(tex.max() - tex.min()))
+    ys = torch.arange(h, device=device, dtype=torch.float64)[:, None] + pad
+    xs = torch.arange(w, device=device, dtype=torch.float64)[None, :] + pad
+    Hp, Wp = tex.shape
+    out = torch.empty(frames, h, w, dtype=torch.float32, device=device)
+    for t in range(frames):
+        gx = 2.0 * (xs - t * v[0]) / (Wp - 1) - 1.0
+        gy = 2.0 * (ys - t * v[1]) / (Hp - 1) - 1.0
+        grid = torch.stack([gx.expand(h, w), gy.expand(h, w)], -1)[None]
+        f = torch.nn.functional.grid_sample(tex[None, None], grid, mode="bilinear", padding_mode="border",
+                                            align_corners=True)[0, 0]
+        out[t] = (torch.clamp(torch.round(f), 0, 255) / 255.0).float()
+    return out, v.permute(1, 2, 0).float().contiguous()

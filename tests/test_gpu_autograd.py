@@ -157,13 +157,13 @@ def test_stage_adjoints_match_autograd(ad):
             gf, gs, gv = torch.empty_like(f), torch.empty_like(s), torch.empty_like(v)
             if pay:
                 _call(ad, "ff_ad_propagate_pass_backward", axis, f, s, v, gfo, gso, gvo, gf, gs, gv, B, H, W,
-                      ctypes_f(dt), 3)
+                      ctypes_f(dt), 3, 0)
                 refs = _vjp64(lambda a, b_, c: tc.prop_pass(axis, a, b_, c, dt, 3), [f, s, v], [gfo, gso, gvo])
                 for mine, ref, nm in zip((gf, gs, gv), refs, "fsv"):
                     assert rel(mine, ref) <= 1e-5, ("prop", axis, nm, rel(mine, ref))
             else:
                 _call(ad, "ff_ad_propagate_pass_backward", axis, f, None, None, gfo, None, None, gf, None, None, B, H,
-                      W, ctypes_f(dt), 3)
+                      W, ctypes_f(dt), 3, 0)
                 (ref,) = _vjp64(lambda a: tc.prop_pass(axis, a, None, None, dt, 3)[0], [f], [gfo])
                 assert rel(gf, ref) <= 1e-5, ("prop", axis, rel(gf, ref))
     # top update (with saturated pixels) and its gamma gradient
@@ -209,10 +209,59 @@ def test_stage_adjoints_match_autograd(ad):
     for axis in (0, 1):
         gsm = rnd(B, H, W, 2)
         gin = torch.empty_like(gsm)
-        _call(ad, "ff_ad_smooth_pass_backward", axis, gsm, gin, B, H, W, 0)
+        _call(ad, "ff_ad_smooth_pass_backward", axis, gsm, None, gin, B, H, W, 0)
         dim = 2 if axis == 0 else 1
         (ref,) = _vjp64(lambda x: sum(tc.BOX * tc.sh(x, dim, k) for k in range(-2, 3)), [gsm], [gsm])
         assert rel(gin, ref) <= 1e-5, ("smooth", axis, rel(gin, ref))
+
+
+def test_adjoint_accumulate_masks_and_zero_inputs(ad):
+    """Bit k of `accumulate` adds output k in place; NULL output gradients
+    read as zero; the smoothing adjoint of g1 + g2 equals that of the sum."""
+    from flowfilter import _native as N
+    r = rng(3)
+    B, H, W = 2, 30, 44
+    rnd = lambda *shape, lo=-1.0, hi=1.0: cuda(r.uniform(lo, hi, size=shape).astype(np.float32))
+    a0, a0o, g, f, gout = (rnd(B, H, W, lo=0.0), rnd(B, H, W, lo=0.0), rnd(B, H, W, 2, lo=-0.2, hi=0.2),
+                           rnd(B, H, W, 2, lo=-3, hi=3), rnd(B, H, W, 2))
+    nb = N.get_int("ff_ad_partial_count", H, W)
+    part = torch.empty(B * nb, device="cuda")
+    gam = ctypes_f(float(np.float32(20) / np.float32(65025)))
+    fresh = [torch.empty(B, H, W, device="cuda"), torch.empty(B, H, W, 2, device="cuda"),
+             torch.empty(B, H, W, device="cuda"), torch.empty(B, H, W, 2, device="cuda")]
+    _call(ad, "ff_ad_update_backward", a0, g, a0o, f, gout, *fresh, part, B, H, W, gam, ctypes_f(4.0), 0)
+    for mask in (1, 2, 4, 8, 5, 15):
+        pre = [rnd(*t.shape) for t in fresh]
+        outs = [t.clone() for t in pre]
+        _call(ad, "ff_ad_update_backward", a0, g, a0o, f, gout, *outs, part, B, H, W, gam, ctypes_f(4.0), mask)
+        for k in range(4):
+            want = pre[k] + fresh[k] if mask & (1 << k) else fresh[k]
+            assert torch.equal(outs[k], want), (mask, k)
+    # propagation: NULL output gradients are zeros
+    s, v = rnd(B, H, W, lo=0.0), rnd(B, H, W, 2)
+    zeros2, zeros1 = torch.zeros(B, H, W, 2, device="cuda"), torch.zeros(B, H, W, device="cuda")
+    gso, gvo = rnd(B, H, W), rnd(B, H, W, 2)
+    dt = ctypes_f(0.25)
+    res = []
+    for gfo in (None, zeros2):
+        o = [torch.empty(B, H, W, 2, device="cuda"), torch.empty(B, H, W, device="cuda"),
+             torch.empty(B, H, W, 2, device="cuda")]
+        _call(ad, "ff_ad_propagate_pass_backward", 0, f, s, v, gfo, gso, gvo, *o, B, H, W, dt, 3, 0)
+        res.append(o)
+    assert all(torch.equal(x, y) for x, y in zip(*res))
+    pre = rnd(B, H, W, 2)
+    acc = pre.clone()
+    _call(ad, "ff_ad_propagate_pass_backward", 0, f, s, v, None, gso, gvo, acc, res[0][1].clone(),
+          res[0][2].clone(), B, H, W, dt, 3, 1)
+    assert torch.equal(acc, pre + res[0][0])
+    # smoothing with two inputs
+    g1, g2 = rnd(B, H, W, 2), rnd(B, H, W, 2)
+    a, b = torch.empty_like(g1), torch.empty_like(g1)
+    _call(ad, "ff_ad_smooth_pass_backward", 1, g1, g2, a, B, H, W, 0)
+    _call(ad, "ff_ad_smooth_pass_backward", 1, g1 + g2, None, b, B, H, W, 0)
+    assert torch.equal(a, b)
+    torch.cuda.synchronize()
+    assert zeros1.abs().sum() == 0
 
 
 # ---------------------------------------------------------------- windows

@@ -144,7 +144,7 @@ def test_warp_matches_grid_sample():
     flows2 = flows.detach().clone().requires_grad_(True)
     b = sum(((frames[:, t] - tc.warp_previous(frames[:, t - 1], flows2[:, t])) ** 2).mean() for t in (1, 2)) / 2
     gb, = torch.autograd.grad(b, flows2)
-    assert abs(float(a) - float(b)) <= 1e-13
+    assert abs(float(a.detach()) - float(b.detach())) <= 1e-13
     assert float((ga - gb).abs().max()) <= 1e-12
 
 
