@@ -306,14 +306,15 @@ def main():
         for i in range(args.steps):
             step(first + i)
         m1 = N.Marker()
+        enq = time.perf_counter() - w0
         ms = m0.elapsed_ms(m1)
-        return ms, time.perf_counter() - w0
+        return ms, time.perf_counter() - w0, enq
 
-    ms, wall = timed(args.warmup)
+    ms, wall, enq = timed(args.warmup)
     ms_per_step = ms / args.steps
     value = H * W * args.steps / (ms / 1e3) / 1e6
-    print("timed %d steps: %.3f ms/step (wall %.3f s)" % (args.steps, ms_per_step, wall), file=sys.stderr,
-          flush=True)
+    print("timed %d steps: %.3f ms/step (wall %.3f s; host enqueue %.3f ms/step)"
+          % (args.steps, ms_per_step, wall, enq * 1e3 / args.steps), file=sys.stderr, flush=True)
 
     # roofline of the dominant kernel: second pass, HIP events per launch
     dominant = "propagate_payload" if args.staged else "fused_frame"
