@@ -153,3 +153,41 @@ def test_batch_device_frames_match_host_loads(ffo):
     from flowfilter import errors
     with pytest.raises(errors.InvalidArgument):
         a.loadImagesDevice([dev[0].data], [dev[0].pitch])
+
+
+def test_batch_sixty_four_streams_hundred_twenty_steps():
+    """The 64-stream workload (seed 3: 320x240 to 1920x1080, 1-3 levels,
+    gamma U[5,50], smooth 1-2, affine motion), 120 steps over each stream's
+    frames (cycled), frames loaded from device memory in one launch: every
+    stream's flow and brightness bitwise equal to its filter computed alone
+    (loop of loadImageDevice + compute) at steps 1, 60 and 119."""
+    import flowfilter.gpu.flowfilters as gf
+    from flowfilter import synthetic
+    from flowfilter.gpu.image import GPUImage
+    streams = synthetic.camera_streams(64, 3)
+    dev = []
+    for sp in streams:
+        g = GPUImage((3 * sp["h"], sp["w"]), 1)
+        g.upload(np.concatenate(sp["frames"], axis=0))
+        dev.append(g)
+    pitches = [g.pitch for g in dev]
+
+    def build():
+        return [make(sp["h"], sp["w"], sp["levels"], sp["gamma"], sp["maxflow"], sp["smooth"]) for sp in streams]
+
+    batched, alone = build(), build()
+    batch = gf.BatchedFlowFilter(batched)
+    for t in range(120):
+        ptrs = [g.data + (t % 3) * sp["h"] * g.pitch for sp, g in zip(streams, dev)]
+        batch.loadImagesDevice(ptrs, pitches)
+        batch.compute()
+        for f, p, q in zip(alone, ptrs, pitches):
+            f.loadImageDevice(p, q)
+            f.compute()
+        if t in (1, 60, 119):
+            say("64 streams, step", t)
+            for i in range(len(streams)):
+                a, b = batched[i].getFlow(), alone[i].getFlow()
+                assert np.array_equal(a, b), "stream %d step %d: max |diff| %g" % (i, t, np.abs(a - b).max())
+                assert np.array_equal(batched[i].getImage(), alone[i].getImage()), (i, t)
+    assert batch.waitTimeouts() == 0
