@@ -275,6 +275,14 @@ FF_API ff_status ff_pyramidal_width(ff_stage* s, int* out);                     
 FF_API ff_status ff_pyramidal_levels(ff_stage* s, int* out);                         /* :895-897 */
 /* loadImage from device memory (no reference counterpart; enqueued, no wait) */
 FF_API ff_status ff_pyramidal_load_image_device(ff_stage* s, const void* dev, size_t pitch);
+/* Frame in place (no reference counterpart): the next compute() reads its
+ * frame from `dev` (u8, the filter's height x width, row pitch `pitch`
+ * bytes) where it lies, without the copy into the input image; the memory
+ * must hold the frame until that compute() has run on the library stream.
+ * The filter copies it in instead on its first frame, for pyramids deeper
+ * than three levels, on the staged path, inside a batch, or when `dev` or
+ * `pitch` is not a multiple of 4. A later load_image* replaces it. */
+FF_API ff_status ff_pyramidal_set_image_device(ff_stage* s, const void* dev, size_t pitch);
 
 /* ----------------------------------------- execution controls (new) */
 /* Select how compute() runs a filter: 1 = fused per-level kernels (the

@@ -411,6 +411,10 @@ public:
     void loadImageDevice(const void* dev, std::size_t pitch) {
         check(ff_pyramidal_load_image_device(s_.get(), dev, pitch));
     }
+    // the next compute() reads the frame in place (ff_pyramidal_set_image_device)
+    void setImageDevice(const void* dev, std::size_t pitch) {
+        check(ff_pyramidal_set_image_device(s_.get(), dev, pitch));
+    }
 };
 
 // Many PyramidalFlowFilters advanced by one compute() in a single fused

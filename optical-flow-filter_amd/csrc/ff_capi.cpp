@@ -509,6 +509,9 @@ GETTER(ff_pyramidal_levels, PyramidalFlowFilter, int, levels)
 ff_status ff_pyramidal_load_image_device(ff_stage* s, const void* dev, size_t pitch) {
     return guard([&] { as<PyramidalFlowFilter>(s, "PyramidalFlowFilter").loadImageDevice(dev, pitch); });
 }
+ff_status ff_pyramidal_set_image_device(ff_stage* s, const void* dev, size_t pitch) {
+    return guard([&] { as<PyramidalFlowFilter>(s, "PyramidalFlowFilter").setImageDevice(dev, pitch); });
+}
 
 // ------------------------------------------------------- execution controls
 ff_status ff_filter_set_fused(ff_stage* s, int fused) {

@@ -444,7 +444,7 @@ bool same_args(const LevelArgs& a, const LevelArgs& b) {
            same_img(a.flow_upd_out, b.flow_upd_out) && same_img(a.upper, b.upper) && a.xmap == b.xmap &&
            a.ymap == b.ymap && a.gamma == b.gamma && a.maxflow == b.maxflow && a.dt == b.dt &&
            a.prop_iters == b.prop_iters && a.smooth_iters == b.smooth_iters && a.border == b.border &&
-           a.delta == b.delta;
+           a.delta == b.delta && a.input_frame == b.input_frame;
 }
 }  // namespace
 
@@ -520,8 +520,11 @@ bool PyramidalFlowFilter::fusedPlan(std::vector<LevelArgs>& args) {
         if (l == levels_ - 2) a.wait_on = 0;
         args.push_back(a);
     }
-    for (const LevelArgs& a : args)
+    for (LevelArgs& a : args) {
         if (!fused_level_supported(a)) return false;
+        // levels that read the input frame itself may take it in place
+        a.input_frame = reduce && a.img.data == input_.view().data ? 1 : 0;
+    }
     return true;
 }
 
@@ -534,22 +537,23 @@ void PyramidalFlowFilter::commitFused() {
     for (int l = 0; l < levels_ - 1; ++l) low_[l].fusedCommit();
 }
 
-void PyramidalFlowFilter::enqueueFused(int table, hipStream_t s) {
+void PyramidalFlowFilter::enqueueFused(int table, hipStream_t s, const void* frame, size_t frame_pitch) {
     enqueuePreFused();
-    frame_.launch(table, s);
+    frame_.launch(table, s, frame, frame_pitch);
     check_launch("PyramidalFlowFilter::compute (fused frame)");
 }
 
 namespace {
 // Enqueue `work` on s, through a graph of `cache` keyed by (args, token)
 // when graphs are on and no kernel profile is open (profiling brackets each
-// launch with events, which a replay would not do).
+// launch with events, which a replay would not do). Returns whether a graph
+// was launched.
 template <class F>
-void run_maybe_graphed(bool graphs_on, GraphCache& cache, const std::vector<LevelArgs>& args,
+bool run_maybe_graphed(bool graphs_on, GraphCache& cache, const std::vector<LevelArgs>& args,
                        unsigned long long token, hipStream_t s, F work) {
     if (!graphs_on || profiling_active()) {
         work();
-        return;
+        return false;
     }
     hipGraphExec_t exec = cache.find(args, token);
     if (!exec) {
@@ -569,33 +573,59 @@ void run_maybe_graphed(bool graphs_on, GraphCache& cache, const std::vector<Leve
         cache.insert(args, token, exec);
     }
     FF_HIP(hipGraphLaunch(exec, s));
+    return true;
 }
 }  // namespace
 
-void PyramidalFlowFilter::computeFused(const std::vector<LevelArgs>& args) {
+void PyramidalFlowFilter::computeFused(const std::vector<LevelArgs>& args, const void* frame, size_t frame_pitch) {
     hipStream_t s = stream();
     const int table = frame_.prepare(args, s);
     // a graph pays off only when the frame is more than the one launch
     // (pyramid kernels for more than three levels); replaying a one-node
     // graph costs more than launching the kernel
-    run_maybe_graphed(graphs_on_ && levels_ > 3, graphs_, args, frame_.token(table), s,
-                      [&] { enqueueFused(table, s); });
+    if (run_maybe_graphed(graphs_on_ && levels_ > 3, graphs_, args, frame_.token(table), s,
+                          [&] { enqueueFused(table, s, frame, frame_pitch); }))
+        frame_.replayed();
     commitFused();
 }
 
 void PyramidalFlowFilter::compute() {
     if (!configured_) throw logic_error("PyramidalFlowFilter::compute(): filter not configured");
     startTiming();
+    const void* frame = frame_dev_;
+    const size_t pitch = frame_pitch_;
+    frame_dev_ = nullptr;
     std::vector<LevelArgs> args;
-    if (fused_ && fusedPlan(args)) computeFused(args);
+    const bool fused = fused_ && fusedPlan(args);
+    // a frame passed in place is read by the fused launch (up to three
+    // levels: every level reduces the input frame while loading it), else
+    // copied into the input image first
+    const bool in_place = frame && fused && levels_ <= 3 && ((uintptr_t)frame % 4 == 0) && (pitch % 4 == 0);
+    if (frame && !in_place) input_.upload_device(frame, pitch);
+    if (fused) computeFused(args, in_place ? frame : nullptr, in_place ? pitch : 0);
     else computeStaged();
     stopTiming();
 }
 
 Image PyramidalFlowFilter::getFlow() const { return levels_ == 1 ? top_.getFlow() : low_[0].getFlow(); }
 
-void PyramidalFlowFilter::loadImage(const ff_image_t& img) { input_.upload(img); }
-void PyramidalFlowFilter::loadImageDevice(const void* dev, size_t pitch) { input_.upload_device(dev, pitch); }
+void PyramidalFlowFilter::loadImage(const ff_image_t& img) {
+    input_.upload(img);
+    frame_dev_ = nullptr;
+}
+void PyramidalFlowFilter::loadImageDevice(const void* dev, size_t pitch) {
+    input_.upload_device(dev, pitch);
+    frame_dev_ = nullptr;
+}
+void PyramidalFlowFilter::setImageDevice(const void* dev, size_t pitch) {
+    if (!dev) throw invalid_argument("PyramidalFlowFilter::setImageDevice(): null frame");
+    if (!configured_) throw logic_error("PyramidalFlowFilter::setImageDevice(): filter not configured");
+    if (pitch < (size_t)w_)
+        throw invalid_argument("PyramidalFlowFilter::setImageDevice(): pitch " + std::to_string(pitch) +
+                               " is less than the width " + std::to_string(w_));
+    frame_dev_ = dev;
+    frame_pitch_ = pitch;
+}
 
 void PyramidalFlowFilter::downloadFlow(ff_image_t& flow) {
     if (levels_ == 1) top_.downloadFlow(flow);
@@ -718,6 +748,12 @@ void BatchedFlowFilter::compute() {
     std::vector<LevelArgs> all, one;
     std::vector<PyramidalFlowFilter*> fused;
     for (const auto& f : streams_) {
+        // a frame a stream was given in place is copied in: the batched
+        // launch reads every stream's input image
+        if (f->frame_dev_) {
+            f->input_.upload_device(f->frame_dev_, f->frame_pitch_);
+            f->frame_dev_ = nullptr;
+        }
         if (f->fused() && f->fusedPlan(one)) {
             const int off = (int)all.size();
             for (LevelArgs a : one) {
@@ -735,11 +771,12 @@ void BatchedFlowFilter::compute() {
         last_blocks_ = frame_.blocks(table);
         bool multi = false;  // more than the one launch (see PyramidalFlowFilter::computeFused)
         for (PyramidalFlowFilter* f : fused) multi = multi || f->levels() > 3;
-        run_maybe_graphed(graphs_on_ && multi, graphs_, all, frame_.token(table), s, [&] {
-            for (PyramidalFlowFilter* f : fused) f->enqueuePreFused();
-            frame_.launch(table, s);
-            check_launch("BatchedFlowFilter::compute (fused frame)");
-        });
+        if (run_maybe_graphed(graphs_on_ && multi, graphs_, all, frame_.token(table), s, [&] {
+                for (PyramidalFlowFilter* f : fused) f->enqueuePreFused();
+                frame_.launch(table, s);
+                check_launch("BatchedFlowFilter::compute (fused frame)");
+            }))
+            frame_.replayed();
         for (PyramidalFlowFilter* f : fused) f->commitFused();
     }
     stopTiming();

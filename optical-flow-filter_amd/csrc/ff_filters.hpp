@@ -163,6 +163,14 @@ public:
     Image getFlow() const;
     void loadImage(const ff_image_t& img);
     void loadImageDevice(const void* dev, size_t pitch);
+    // The next compute() reads its frame where the caller keeps it (device
+    // u8, the filter's height x width, row pitch `pitch` bytes): the fused
+    // frame kernel loads it directly instead of a copy into the input image.
+    // The memory must hold the frame until that compute() has run on the
+    // stream. Falls back to a copy (loadImageDevice) for the first frame,
+    // pyramids deeper than three levels, the staged path, or a pointer or
+    // pitch that is not a multiple of 4 bytes. No reference counterpart.
+    void setImageDevice(const void* dev, size_t pitch);
     void downloadFlow(ff_image_t& flow);
     void downloadImage(ff_image_t& img);
     float getGamma(int level) const;
@@ -202,9 +210,12 @@ private:
     // Level arguments of a fused frame; false when some level is not ready
     // (first frame) or not covered by the fused kernel.
     bool fusedPlan(std::vector<LevelArgs>& args);
-    void computeFused(const std::vector<LevelArgs>& args);
-    void enqueueFused(int table, hipStream_t s);
+    void computeFused(const std::vector<LevelArgs>& args, const void* frame = nullptr, size_t frame_pitch = 0);
+    void enqueueFused(int table, hipStream_t s, const void* frame = nullptr, size_t frame_pitch = 0);
     bool configured_ = false, fused_ = true, graphs_on_ = true;
+    // frame passed in place for the next compute() (setImageDevice)
+    const void* frame_dev_ = nullptr;
+    size_t frame_pitch_ = 0;
     GraphCache graphs_;
     FusedFrame frame_;
     int h_ = 0, w_ = 0, levels_ = 0;

@@ -94,9 +94,11 @@ struct LevelJob {
     int W, H, Hu, Hsrc;  // level size, upper level height, image height
     int N, K, border, h0, TW, TH, tiles_x, tiles_y;
     int delta, img_mode, first_block;
-    // completion counters: wait for counter wait_idx to reach wait_target
-    // (wait_consumers tiles wait on it in total); signal counter signal_idx
-    int wait_idx, wait_target, wait_consumers, signal_idx;
+    // completion counters: wait until counter wait_idx has counted
+    // wait_target more tiles in this launch than before it; signal counter
+    // signal_idx. img_arg: the level's image is the filter's input frame,
+    // which a launch may pass in place (read where the caller keeps it).
+    int wait_idx, wait_target, img_arg, signal_idx;
     float dt, gamma, maxflow;
     int pad_;
 };
@@ -208,27 +210,26 @@ __device__ __forceinline__ int xcd_tile(int b, int nt) {
 
 // ------------------------------------------------- completion counters
 
-// Wait until the producer job's tiles have all signalled. The data they
+// Wait until the producer job's tiles have all signalled. The counters
+// only grow: a launch with sequence number `epoch` (1, 2, ... since the host
+// last cleared them) waits for epoch * wait_target, compared modulo 2^32.
+// Nothing is reset in the kernel, so the wait costs one relaxed load when the
+// producer is done (a returning atomic here would put a device-scope round
+// trip on every consumer tile's critical path). The data the producers
 // published (the unsmoothed flow) is read with device-coherent loads after
 // the wait, so no cache invalidation is needed; the workgroup barrier
-// orders every wave's loads after the polling thread saw the count. The
-// last consumer resets the counters for the next launch.
-__device__ __forceinline__ void wait_upper(const LevelJob& p, unsigned* counters, int timeout_idx) {
+// orders every wave's loads after the polling thread saw the count.
+__device__ __forceinline__ void wait_upper(const LevelJob& p, unsigned* counters, int timeout_idx, unsigned epoch) {
     if (threadIdx.x == 0) {
-        unsigned* done = counters + 2 * p.wait_idx;
-        unsigned* used = done + 1;
+        const unsigned* done = counters + 2 * p.wait_idx;
+        const unsigned target = epoch * (unsigned)p.wait_target;
         int spins = 0;
-        while (__hip_atomic_load(done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < (unsigned)p.wait_target) {
+        while ((int)(__hip_atomic_load(done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - target) < 0) {
             __builtin_amdgcn_s_sleep(8);
             if (++spins > kSpinLimit) {
                 __hip_atomic_fetch_add(counters + timeout_idx, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 break;
             }
-        }
-        const unsigned u = __hip_atomic_fetch_add(used, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (u + 1 == (unsigned)p.wait_consumers) {
-            __hip_atomic_store(done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(used, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
     __syncthreads();
@@ -514,14 +515,21 @@ struct ImageRegs {
     unsigned v[kN];
 };
 
+// The level's source image: the job's, or the frame passed to the launch.
+struct ImgSrc {
+    const void* data;
+    int pitch;
+};
+
 template <int MODE, int R>
-__device__ __forceinline__ void load_image(ImageRegs<MODE, R>& ir, const LevelJob& p, const Geo& g) {
-    const Rsrc rimg = rsrc(p.img, p.img_pitch * p.Hsrc);
+__device__ __forceinline__ void load_image(ImageRegs<MODE, R>& ir, const LevelJob& p, const Geo& g, ImgSrc src) {
+    const Rsrc rimg = rsrc(src.data, src.pitch * p.Hsrc);
+    const int ipitch = src.pitch;
     if constexpr (MODE == IMG_F32) {
 #pragma unroll
         for (int j = 0; j < R + 4; ++j) {
             const int yc = clampi(g.yw - 2 + j, 0, p.H - 1);
-            ir.v[j] = __builtin_amdgcn_raw_buffer_load_b32(rimg, yc * p.img_pitch + g.xl * 4, 0, 0);
+            ir.v[j] = __builtin_amdgcn_raw_buffer_load_b32(rimg, yc * ipitch + g.xl * 4, 0, 0);
         }
         return;
     } else {
@@ -542,7 +550,7 @@ __device__ __forceinline__ void load_image(ImageRegs<MODE, R>& ir, const LevelJo
 #pragma unroll
                     for (int m = 0; m < 3; ++m) {
                         const int r0 = max(2 * r1 - 1 + m, 0);
-                        const unsigned wd = __builtin_amdgcn_raw_buffer_load_b32(rimg, r0 * p.img_pitch + 4 * g.xl, 0, 0);
+                        const unsigned wd = __builtin_amdgcn_raw_buffer_load_b32(rimg, r0 * ipitch + 4 * g.xl, 0, 0);
                         const unsigned b0 = wd & 0xffu, b1 = (wd >> 8) & 0xffu, b2 = (wd >> 16) & 0xffu, b3 = wd >> 24;
                         const unsigned left = (shr_or(wd, wd << 24) >> 24) & 0xffu;  // column 4x-1 (clamped)
                         xa[m] = down_u8(left, b0, b1);
@@ -566,9 +574,9 @@ __device__ __forceinline__ void load_image(ImageRegs<MODE, R>& ir, const LevelJo
             for (int j = 0; j < R + 4; ++j) {
                 const int r1 = 2 * clampi(g.yw - 2 + j, 0, p.H - 1);
                 const int r0 = r1 > 0 ? r1 - 1 : 0;
-                src[j][0] = ldh(rimg, r0 * p.img_pitch + 2 * g.xl);
-                src[j][1] = ldh(rimg, r1 * p.img_pitch + 2 * g.xl);
-                src[j][2] = ldh(rimg, (r1 + 1) * p.img_pitch + 2 * g.xl);
+                src[j][0] = ldh(rimg, r0 * ipitch + 2 * g.xl);
+                src[j][1] = ldh(rimg, r1 * ipitch + 2 * g.xl);
+                src[j][2] = ldh(rimg, (r1 + 1) * ipitch + 2 * g.xl);
             }
 #pragma unroll
             for (int j = 0; j < R + 4; ++j) t[j] = down_u8(down_x(src[j][0]), down_x(src[j][1]), down_x(src[j][2]));
@@ -576,7 +584,7 @@ __device__ __forceinline__ void load_image(ImageRegs<MODE, R>& ir, const LevelJo
 #pragma unroll
             for (int j = 0; j < R + 4; ++j) {
                 const int yc = clampi(g.yw - 2 + j, 0, p.H - 1);
-                t[j] = ldb(rimg, yc * p.img_pitch + g.xl);
+                t[j] = ldb(rimg, yc * ipitch + g.xl);
             }
         }
         if constexpr (MODE == IMG_U8_DOWN || MODE == IMG_U8_DOWN2) {
@@ -622,8 +630,8 @@ __device__ __forceinline__ float* xrow(float* xch, int nw, int slot, int wave) {
 }
 
 template <bool DELTA, int MODE, int R>
-__device__ __forceinline__ void level_tile(const LevelJob& p, int tile, float* xch, int nw, unsigned* counters,
-                                           int timeout_idx) {
+__device__ __forceinline__ void level_tile(const LevelJob& p, int tile, float* xch, int nw, unsigned* counters, unsigned epoch,
+                                           int timeout_idx, ImgSrc frame) {
     Geo g;
     g.lane = threadIdx.x & (LANES - 1);
     g.wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -679,7 +687,7 @@ __device__ __forceinline__ void level_tile(const LevelJob& p, int tile, float* x
         }
     }
     ImageRegs<MODE, R> ir;
-    load_image<MODE, R>(ir, p, g);
+    load_image<MODE, R>(ir, p, g, (p.img_arg && frame.data) ? frame : ImgSrc{p.img, p.img_pitch});
 
     int slot = 0;
 
@@ -740,7 +748,7 @@ __device__ __forceinline__ void level_tile(const LevelJob& p, int tile, float* x
     }
 
     // ------------------------------------------------ image model + update
-    if (DELTA && p.wait_idx >= 0) wait_upper(p, counters, timeout_idx);
+    if (DELTA && p.wait_idx >= 0) wait_upper(p, counters, timeout_idx, epoch);
     float aold[R];  // unused (and removed) on delta levels
     float2 up[R];
     int offa[R], offf[R];
@@ -837,19 +845,26 @@ constexpr int kRows = 8;
 
 __global__ void __launch_bounds__(1024)
     k_fused_frame(const LevelJob* __restrict__ jobs, const int* __restrict__ block_job, unsigned* counters,
-                  int timeout_idx, GroupEnds ends, int nw) {
+                  unsigned epoch, int timeout_idx, GroupEnds ends, int nw, const unsigned char* frame,
+                  int frame_pitch) {
     // row exchange between the waves of the workgroup (2 * nw * 10 * LANES floats)
     extern __shared__ float xch[];
     const LevelJob& p = jobs[block_job[blockIdx.x]];
     const int tile = (int)blockIdx.x - p.first_block;
     const unsigned b = blockIdx.x;
-    if (b < (unsigned)ends.e[0]) level_tile<false, IMG_F32, kRows>(p, tile, xch, nw, counters, timeout_idx);
-    else if (b < (unsigned)ends.e[1]) level_tile<false, IMG_U8, kRows>(p, tile, xch, nw, counters, timeout_idx);
-    else if (b < (unsigned)ends.e[2]) level_tile<false, IMG_U8_DOWN, kRowsTop>(p, tile, xch, nw, counters, timeout_idx);
-    else if (b < (unsigned)ends.e[3]) level_tile<false, IMG_U8_DOWN2, kRowsTop>(p, tile, xch, nw, counters, timeout_idx);
-    else if (b < (unsigned)ends.e[4]) level_tile<true, IMG_F32, kRows>(p, tile, xch, nw, counters, timeout_idx);
-    else if (b < (unsigned)ends.e[5]) level_tile<true, IMG_U8, kRows>(p, tile, xch, nw, counters, timeout_idx);
-    else level_tile<true, IMG_U8_DOWN, kRows>(p, tile, xch, nw, counters, timeout_idx);
+    const ImgSrc fr{frame, frame_pitch};
+    if (b < (unsigned)ends.e[0]) level_tile<false, IMG_F32, kRows>(p, tile, xch, nw, counters, epoch, timeout_idx, fr);
+    else if (b < (unsigned)ends.e[1]) level_tile<false, IMG_U8, kRows>(p, tile, xch, nw, counters, epoch, timeout_idx, fr);
+    else if (b < (unsigned)ends.e[2]) level_tile<false, IMG_U8_DOWN, kRowsTop>(p, tile, xch, nw, counters, epoch, timeout_idx, fr);
+    else if (b < (unsigned)ends.e[3]) level_tile<false, IMG_U8_DOWN2, kRowsTop>(p, tile, xch, nw, counters, epoch, timeout_idx, fr);
+    else if (b < (unsigned)ends.e[4]) level_tile<true, IMG_F32, kRows>(p, tile, xch, nw, counters, epoch, timeout_idx, fr);
+    else if (b < (unsigned)ends.e[5]) level_tile<true, IMG_U8, kRows>(p, tile, xch, nw, counters, epoch, timeout_idx, fr);
+    else level_tile<true, IMG_U8_DOWN, kRows>(p, tile, xch, nw, counters, epoch, timeout_idx, fr);
+}
+
+__global__ void k_clear_counters(unsigned* c, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) c[i] = 0u;
 }
 
 constexpr int kMaxWaves = 16;  // waves per workgroup, at most
@@ -963,14 +978,24 @@ struct FusedFrame::Impl {
     Table slot[2];
     int next = 0;
     unsigned long long uploads = 0;
+    // Completion counters: one region of `npairs` (done, spare) pairs per
+    // table slot (the two slots alternate when a filter's state buffers
+    // ping-pong between frames), then the timeout count.
     DeviceArray counters;
-    int ncounters = 0;  // pairs
+    int npairs = 0;
+    // per slot: launches counted since its region was last cleared, and the
+    // table they count for (0: clear before the next launch)
+    unsigned epoch[2] = {0, 0};
+    unsigned long long counted[2] = {0, 0};
     int fixed = 0;  // waves per workgroup, 0: autotune
     std::vector<int> tuned_for;
     int tuned = 0;
 
     int table(const std::vector<LevelArgs>& jobs, int nw);
-    void run(int t, hipStream_t s) const;
+    unsigned* region(int si) const { return (unsigned*)counters.data() + (size_t)si * 2 * npairs; }
+    int timeout_word() const { return 4 * npairs; }
+    void clear_counters(int si, hipStream_t s) const;
+    void run(int t, hipStream_t s, const void* frame = nullptr, int frame_pitch = 0);
     int tune(const std::vector<LevelArgs>& jobs, hipStream_t s);
 };
 
@@ -1040,6 +1065,7 @@ int FusedFrame::Impl::table(const std::vector<LevelArgs>& jobs, int nw) {
         p.first_block = blocks;
         p.wait_idx = -1;
         p.signal_idx = -1;
+        p.img_arg = a.input_frame ? 1 : 0;
         p.dt = a.dt;
         p.gamma = a.gamma;
         p.maxflow = a.maxflow;
@@ -1056,13 +1082,6 @@ int FusedFrame::Impl::table(const std::vector<LevelArgs>& jobs, int nw) {
         c.wait_idx = pos[k];
         c.wait_target = q.tiles_x * q.tiles_y;
         q.signal_idx = pos[k];
-    }
-    for (size_t i = 0; i < rec.size(); ++i) {
-        if (rec[i].wait_idx < 0) continue;
-        int consumers = 0;
-        for (size_t j = 0; j < rec.size(); ++j)
-            if (rec[j].wait_idx == rec[i].wait_idx) consumers += rec[j].tiles_x * rec[j].tiles_y;
-        rec[i].wait_consumers = consumers;
     }
     std::vector<int> map(blocks);
     for (size_t i = 0; i < rec.size(); ++i)
@@ -1085,12 +1104,12 @@ int FusedFrame::Impl::table(const std::vector<LevelArgs>& jobs, int nw) {
 
     // a table (or the counters) may be in use by launches still in flight
     FF_HIP(hipDeviceSynchronize());
-    const int need = (int)jobs.size() + 1;
-    if (ncounters < need) {
+    const int need = (int)jobs.size();
+    if (npairs < need) {
         const int n = need < 64 ? 64 : need;
-        std::vector<unsigned> zero(2 * (size_t)n, 0u);
+        std::vector<unsigned> zero(4 * (size_t)n + 2, 0u);
         counters.assign(zero.data(), zero.size() * sizeof(unsigned));
-        ncounters = n;
+        npairs = n;
         // launches recorded with the old counters must not be replayed
         for (Table& o : slot) o.token = ++uploads;
     }
@@ -1108,17 +1127,45 @@ int FusedFrame::Impl::table(const std::vector<LevelArgs>& jobs, int nw) {
     return si;
 }
 
-void FusedFrame::Impl::run(int ti, hipStream_t s) const {
-    const Table& t = slot[ti & 1];
+void FusedFrame::Impl::clear_counters(int si, hipStream_t s) const {
+    const int n = 2 * npairs;
+    hipLaunchKernelGGL(k_clear_counters, dim3((n + 255) / 256), dim3(256), 0, s, region(si), n);
+}
+
+void FusedFrame::Impl::run(int ti, hipStream_t s, const void* frame, int frame_pitch) {
+    const int si = ti & 1;
+    const Table& t = slot[si];
     if (!t.blocks) throw logic_error("fused frame: launch of an unprepared table");
     static const hipError_t lds_ok = hipFuncSetAttribute(
         (const void*)k_fused_frame, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes(kMaxWaves));
     if (lds_ok != hipSuccess) throw_hip(lds_ok, "hipFuncSetAttribute", __FILE__, __LINE__);
+    // Sequence number of this launch for its slot's completion counters. A
+    // launch recorded into a graph clears them itself and counts as the
+    // first, so every replay is self-contained; direct launches count up
+    // from the last clear, which happens whenever the region served another
+    // table or a replay (replayed()) in between.
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    FF_HIP(hipStreamIsCapturing(s, &cs));
+    unsigned e;
+    if (cs != hipStreamCaptureStatusNone) {
+        clear_counters(si, s);
+        e = 1;
+        counted[si] = 0;
+    } else {
+        if (counted[si] != t.token || epoch[si] == 0xffffffffu) {
+            clear_counters(si, s);
+            epoch[si] = 0;
+            counted[si] = t.token;
+        }
+        e = ++epoch[si];
+    }
     ProfileScope ps("fused_frame", s);
     const char* base = (const char*)t.dev.data();
-    const int timeout_idx = 2 * (ncounters - 1);
+    // the timeout word, relative to this slot's region
+    const int timeout_idx = timeout_word() - si * 2 * npairs;
     hipLaunchKernelGGL(k_fused_frame, dim3(t.blocks), dim3(t.nw * LANES), lds_bytes(t.nw), s, (const LevelJob*)base,
-                       (const int*)(base + t.map_offset), (unsigned*)counters.data(), timeout_idx, t.ends, t.nw);
+                       (const int*)(base + t.map_offset), region(si), e, timeout_idx, t.ends, t.nw,
+                       (const unsigned char*)frame, frame_pitch);
 }
 
 // Time the launches of every candidate wave count on the actual job table
@@ -1175,7 +1222,12 @@ int FusedFrame::prepare(const std::vector<LevelArgs>& jobs, hipStream_t s) {
     return impl_->table(jobs, nw);
 }
 
-void FusedFrame::launch(int table, hipStream_t s) const { impl_->run(table, s); }
+void FusedFrame::launch(int table, hipStream_t s, const void* frame, size_t frame_pitch) {
+    if (frame && frame_pitch > 0x7fffffffu) throw invalid_argument("fused frame: frame pitch too large");
+    impl_->run(table, s, frame, (int)frame_pitch);
+}
+
+void FusedFrame::replayed() { impl_->counted[0] = impl_->counted[1] = 0; }
 
 int FusedFrame::blocks(int table) const { return impl_->slot[table & 1].blocks; }
 unsigned long long FusedFrame::token(int table) const { return impl_->slot[table & 1].token; }
@@ -1192,11 +1244,11 @@ void FusedFrame::setWaves(int nw) {
 int FusedFrame::waves() const { return impl_->fixed ? impl_->fixed : impl_->tuned; }
 
 unsigned long long FusedFrame::timeouts() const {
-    if (!impl_->ncounters) return 0;
+    if (!impl_->npairs) return 0;
     FF_HIP(hipDeviceSynchronize());
     unsigned v = 0;
     const unsigned* c = (const unsigned*)impl_->counters.data();
-    FF_HIP(hipMemcpy(&v, c + 2 * (impl_->ncounters - 1), sizeof(v), hipMemcpyDeviceToHost));
+    FF_HIP(hipMemcpy(&v, c + impl_->timeout_word(), sizeof(v), hipMemcpyDeviceToHost));
     return v;
 }
 

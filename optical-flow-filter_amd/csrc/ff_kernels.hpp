@@ -114,6 +114,9 @@ struct LevelArgs {
     // reads as `upper`, or -1 when `upper` is not written in this launch.
     // The job must come earlier in the list.
     int wait_on = -1;
+    // 1: `img` is the filter's input frame, which a launch may instead pass
+    // in place (FusedFrame::launch's frame argument)
+    int input_frame = 0;
 };
 
 // Returns false when the level's parameters are outside what the fused
@@ -138,7 +141,13 @@ public:
     // The first call for a set of level shapes times every candidate tiling
     // on stream s (unless a wave count was fixed with setWaves).
     int prepare(const std::vector<LevelArgs>& jobs, hipStream_t s);
-    void launch(int table, hipStream_t s) const;
+    // frame (optional): device u8 image of the input frame's size, with row
+    // pitch frame_pitch (a multiple of 4), read in place by the jobs marked
+    // input_frame instead of their img.
+    void launch(int table, hipStream_t s, const void* frame = nullptr, size_t frame_pitch = 0);
+    // A graph holding launches of this frame was replayed on the stream:
+    // the next direct launch clears the completion counters first.
+    void replayed();
     // Waves per workgroup (a tile's buffer is 64 columns by waves * rows
     // per thread rows): 0 = autotune (default), or 16, 15, 8, 5, 4. waves()
     // reports the fixed or tuned value (0 before tuning).

@@ -98,9 +98,13 @@ struct CopyJob {
     long long src_pitch, dst_pitch;
     int per_row, total, first_block, unit;  // accesses per row and in all
 };
+// The job list travels as a kernel argument sized for the launch (one job,
+// up to 8, up to kCopyMax): a single frame's copy does not pay for setting
+// up 3 KB of arguments.
+template <int M>
 struct CopyBatch {
     int n;
-    CopyJob job[kCopyMax];
+    CopyJob job[M];
 };
 
 template <class T>
@@ -124,7 +128,8 @@ __device__ __forceinline__ void copy_units(const CopyJob& c, int base) {
     }
 }
 
-__global__ void __launch_bounds__(256) k_copy_regions(const CopyBatch batch) {
+template <int M>
+__global__ void __launch_bounds__(256) k_copy_regions(const CopyBatch<M> batch) {
     int j = 0;
     while (j + 1 < batch.n && (int)blockIdx.x >= batch.job[j + 1].first_block) ++j;
     const CopyJob& c = batch.job[j];
@@ -331,28 +336,38 @@ void launch_down_pass(int axis, bool u8, const DevImg& in, const DevImg& out, hi
     }
 }
 
+namespace {
+template <int M>
+void launch_copy_batch(const CopyRegion* regions, int n, hipStream_t s) {
+    CopyBatch<M> b{};
+    int blocks = 0;
+    for (int i = 0; i < n; ++i) {
+        const CopyRegion& r = regions[i];
+        if (r.rows <= 0 || r.row_bytes == 0) continue;
+        const uintptr_t a = (uintptr_t)r.src | (uintptr_t)r.dst | r.src_pitch | r.dst_pitch | r.row_bytes;
+        CopyJob& c = b.job[b.n++];
+        c.src = (const char*)r.src;
+        c.dst = (char*)r.dst;
+        c.src_pitch = (long long)r.src_pitch;
+        c.dst_pitch = (long long)r.dst_pitch;
+        c.unit = (a % 16 == 0) ? 16 : ((a % 4 == 0) ? 4 : 1);
+        c.per_row = (int)(r.row_bytes / c.unit);
+        c.total = c.per_row * r.rows;
+        c.first_block = blocks;
+        blocks += (c.total + kCopyPerBlock - 1) / kCopyPerBlock;
+    }
+    if (!blocks) return;
+    ProfileScope ps("copy_regions", s);
+    hipLaunchKernelGGL(k_copy_regions<M>, dim3(blocks), dim3(256), 0, s, b);
+}
+}  // namespace
+
 void launch_copy_regions(const CopyRegion* regions, int n, hipStream_t s) {
     for (int first = 0; first < n; first += kCopyMax) {
-        CopyBatch b{};
-        int blocks = 0;
-        for (int i = first; i < n && b.n < kCopyMax; ++i) {
-            const CopyRegion& r = regions[i];
-            if (r.rows <= 0 || r.row_bytes == 0) continue;
-            const uintptr_t a = (uintptr_t)r.src | (uintptr_t)r.dst | r.src_pitch | r.dst_pitch | r.row_bytes;
-            CopyJob& c = b.job[b.n++];
-            c.src = (const char*)r.src;
-            c.dst = (char*)r.dst;
-            c.src_pitch = (long long)r.src_pitch;
-            c.dst_pitch = (long long)r.dst_pitch;
-            c.unit = (a % 16 == 0) ? 16 : ((a % 4 == 0) ? 4 : 1);
-            c.per_row = (int)(r.row_bytes / c.unit);
-            c.total = c.per_row * r.rows;
-            c.first_block = blocks;
-            blocks += (c.total + kCopyPerBlock - 1) / kCopyPerBlock;
-        }
-        if (!blocks) continue;
-        ProfileScope ps("copy_regions", s);
-        hipLaunchKernelGGL(k_copy_regions, dim3(blocks), dim3(256), 0, s, b);
+        const int m = n - first < kCopyMax ? n - first : kCopyMax;
+        if (m == 1) launch_copy_batch<1>(regions + first, m, s);
+        else if (m <= 8) launch_copy_batch<8>(regions + first, m, s);
+        else launch_copy_batch<kCopyMax>(regions + first, m, s);
     }
 }
 

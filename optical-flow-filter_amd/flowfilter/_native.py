@@ -187,6 +187,7 @@ SIGNATURES = {
     "ff_pyramidal_width": [_S, _pi],
     "ff_pyramidal_levels": [_S, _pi],
     "ff_pyramidal_load_image_device": [_S, _v, _z],
+    "ff_pyramidal_set_image_device": [_S, _v, _z],
     # execution controls
     "ff_filter_set_fused": [_S, _i],
     "ff_filter_get_fused": [_S, _pi],

@@ -212,6 +212,12 @@ class PyramidalFlowFilter(Stage):
     def loadImageDevice(self, ptr, pitch):
         N.call("ff_pyramidal_load_image_device", self.handle, ctypes.c_void_p(ptr), ctypes.c_size_t(pitch))
 
+    def setImageDevice(self, ptr, pitch):
+        """The next compute() reads its u8 frame in place from device memory
+        (ptr, row pitch in bytes) instead of copying it into the input image;
+        the memory must hold the frame until that compute() has run."""
+        N.call("ff_pyramidal_set_image_device", self.handle, ctypes.c_void_p(ptr), ctypes.c_size_t(pitch))
+
     def getFlow(self, flow=None):
         if flow is None:
             flow = np.zeros((self.height, self.width, 2), dtype=np.float32)
