@@ -61,6 +61,7 @@
 #include <cstring>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 
 #include "ff_device.hpp"
 #include "ff_kernels.hpp"
@@ -266,9 +267,12 @@ struct State {
 };
 
 // One upwind X pass (flowPropagateX_k / flowPropagatePayloadX_k,
-// propagation_k.cu:16-72, 134-232).
-template <bool DELTA, bool BORDER, int R>
+// propagation_k.cu:16-72, 134-232). LAST: the final iteration of a delta
+// level, whose propagated flow is read only as the Y pass's speed (the
+// delta update consumes the payloads): the u component is not advanced.
+template <bool DELTA, bool BORDER, int R, bool LAST = false>
 __device__ __forceinline__ void prop_x(State<DELTA, R>& s, const Geo& g, const LevelJob& p) {
+    constexpr bool kSkipU = DELTA && LAST;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const float zu = s.fu[r], zv = s.fv[r];
@@ -276,7 +280,7 @@ __device__ __forceinline__ void prop_x(State<DELTA, R>& s, const Geo& g, const L
         const float Ud = (fabsf(pu) - fabsf(mu)) > 0.0f ? pu : mu;
         const float nd = -(p.dt * Ud);
         const bool back = Ud >= 0.0f;
-        float ou = upwind(mu, zu, pu, back, nd);
+        float ou = kSkipU ? zu : upwind(mu, zu, pu, back, nd);
         float ov = upwind(zshr(zv), zv, zshl(zv), back, nd);
         float os = 0.f, odu = 0.f, odv = 0.f;
         if constexpr (DELTA) {
@@ -287,7 +291,7 @@ __device__ __forceinline__ void prop_x(State<DELTA, R>& s, const Geo& g, const L
         if constexpr (BORDER) {
             const int y = g.yw + r;
             const bool in = g.xin && y >= p.border && y < p.H - p.border;
-            ou = in ? ou : zu;
+            if constexpr (!kSkipU) ou = in ? ou : zu;
             ov = in ? ov : zv;
             if constexpr (DELTA) {
                 os = in ? os : s.sa[r];
@@ -312,10 +316,13 @@ struct Edge5 {
 
 // One upwind Y pass (flowPropagateY_k / flowPropagatePayloadY_k,
 // propagation_k.cu:75-131, 235-334); rows top-down, m* keep the old values
-// of row r-1.
-template <bool DELTA, bool BORDER, int R>
+// of row r-1. LAST: the final pass of a delta level, whose propagated flow
+// is dead (only the payloads feed the delta update): the flow is not
+// advanced, only read for the speed Vd.
+template <bool DELTA, bool BORDER, int R, bool LAST = false>
 __device__ __forceinline__ void prop_y(State<DELTA, R>& s, const Geo& g, const LevelJob& p, const Edge5& a,
                                        const Edge5& b) {
+    constexpr bool kSkipF = DELTA && LAST;
     float mu = a.u, mv = a.v, ms = a.s, mdu = a.du, mdv = a.dv;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
@@ -325,16 +332,18 @@ __device__ __forceinline__ void prop_y(State<DELTA, R>& s, const Geo& g, const L
         const float Vd = (fabsf(pv) - fabsf(mv)) > 0.0f ? pv : mv;
         const float nd = -(p.dt * Vd);
         const bool back = Vd >= 0.0f;
-        float ou = upwind(mu, zu, pu, back, nd);
-        float ov = upwind(mv, zv, pv, back, nd);
+        float ou = kSkipF ? zu : upwind(mu, zu, pu, back, nd);
+        float ov = kSkipF ? zv : upwind(mv, zv, pv, back, nd);
         mu = zu;
         mv = zv;
         bool in = true;
         if constexpr (BORDER) {
             const int y = g.yw + r;
             in = g.xin && y >= p.border && y < p.H - p.border;
-            ou = in ? ou : zu;
-            ov = in ? ov : zv;
+            if constexpr (!kSkipF) {
+                ou = in ? ou : zu;
+                ov = in ? ov : zv;
+            }
         }
         s.fu[r] = ou;
         s.fv[r] = ov;
@@ -692,9 +701,12 @@ __device__ __forceinline__ void level_tile(const LevelJob& p, int tile, float* x
     int slot = 0;
 
     // ---------------------------------------------------------- propagation
-    for (int it = 0; it < p.N; ++it) {
-        if (g.wave_in) prop_x<DELTA, false, R>(s, g, p);
-        else prop_x<DELTA, true, R>(s, g, p);
+    // N iterations of an X and a Y pass; the last one separately, since on
+    // a delta level part of its flow output is dead
+    auto iteration = [&](auto last) {
+        constexpr bool LAST = decltype(last)::value;
+        if (g.wave_in) prop_x<DELTA, false, R, LAST>(s, g, p);
+        else prop_x<DELTA, true, R, LAST>(s, g, p);
         {
             float* my = xrow(xch, nw, slot, wave);
             my[0 * LANES + lane] = s.fu[0];
@@ -743,9 +755,11 @@ __device__ __forceinline__ void level_tile(const LevelJob& p, int tile, float* x
             }
         }
         slot ^= 1;
-        if (g.wave_in) prop_y<DELTA, false, R>(s, g, p, ea, eb);
-        else prop_y<DELTA, true, R>(s, g, p, ea, eb);
-    }
+        if (g.wave_in) prop_y<DELTA, false, R, LAST>(s, g, p, ea, eb);
+        else prop_y<DELTA, true, R, LAST>(s, g, p, ea, eb);
+    };
+    for (int it = 0; it + 1 < p.N; ++it) iteration(std::false_type{});
+    iteration(std::true_type{});
 
     // ------------------------------------------------ image model + update
     if (DELTA && p.wait_idx >= 0) wait_upper(p, counters, timeout_idx, epoch);
@@ -837,12 +851,11 @@ struct GroupEnds {
     int e[6];
 };
 
-// Rows per thread of a tile body: top levels reduced on load (the top level
-// of a pyramid, which the level below waits for) and the others. Both are
-// 8: smaller top tiles (4 rows) finished sooner but, with the waits costing
-// one relaxed load, spent more of their rows on halos (64-stream batch
-// 0.90 ms against 0.89 ms per step with 8).
-constexpr int kRowsTop = 8;
+// Rows per thread of a tile body: 4 for top levels reduced on load (the
+// top level of a pyramid, which the level below waits for: more, shorter
+// tiles finish sooner, which pyramids launched alone need; a 64-stream batch
+// is 1.5 % faster with 8), 8 otherwise.
+constexpr int kRowsTop = 4;
 constexpr int kRows = 8;
 
 __global__ void __launch_bounds__(1024)
