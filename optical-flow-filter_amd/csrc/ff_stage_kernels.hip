@@ -130,9 +130,16 @@ __device__ __forceinline__ void copy_units(const CopyJob& c, int base) {
 
 template <int M>
 __global__ void __launch_bounds__(256) k_copy_regions(const CopyBatch<M> batch) {
-    int j = 0;
-    while (j + 1 < batch.n && (int)blockIdx.x >= batch.job[j + 1].first_block) ++j;
-    const CopyJob& c = batch.job[j];
+    // the job whose block range holds this block: binary search over the
+    // ascending first blocks (a linear scan costs a dependent scalar load
+    // per job for the last regions' blocks)
+    int lo = 0, hi = batch.n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if ((int)blockIdx.x >= batch.job[mid].first_block) lo = mid;
+        else hi = mid - 1;
+    }
+    const CopyJob& c = batch.job[lo];
     const int base = ((int)blockIdx.x - c.first_block) * kCopyPerBlock + (int)threadIdx.x;
     if (c.unit == 16) copy_units<uint4>(c, base);
     else if (c.unit == 4) copy_units<uint32_t>(c, base);
