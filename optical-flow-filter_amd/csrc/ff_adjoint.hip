@@ -215,16 +215,41 @@ __global__ void __launch_bounds__(BX* BY)
         const int xx = AXIS == 0 ? k : x, yy = AXIS == 0 ? y : k;
         return (xx >= border && xx < d.W - border) && (yy >= border && yy < d.H - border);
     };
+    // Everything this input can reach, loaded once up front: the forward
+    // inputs at j-2 .. j+2 and the output gradients at j-1 .. j+1 (reads
+    // clamped to the line; a clamped value is only used where its position
+    // is inside, since an interior output's neighbours are: border >= 1).
+    float2 F[5], V[5];
+    float S[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        const size_t i = idx(clampi(j - 2 + k, 0, n - 1));
+        F[k] = fin[i];
+        if constexpr (PAY) {
+            S[k] = sin[i];
+            V[k] = vin[i];
+        }
+    }
+    float2 GF[3], GV[3];
+    float GS[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const size_t i = idx(clampi(j - 1 + k, 0, n - 1));
+        GF[k] = gfo ? gfo[i] : make_float2(0.f, 0.f);
+        GS[k] = (PAY && gso) ? gso[i] : 0.f;
+        GV[k] = (PAY && gvo) ? gvo[i] : make_float2(0.f, 0.f);
+    }
     float2 gf = make_float2(0.f, 0.f), gv = make_float2(0.f, 0.f);
     float gs = 0.f;
-    for (int o = j - 1; o <= j + 1; ++o) {
+#pragma unroll
+    for (int oi = 0; oi < 3; ++oi) {  // output o = j - 1 + oi
+        const int o = j - 1 + oi;
         if (o < 0 || o >= n) continue;
-        const size_t io = idx(o);
-        const float2 g_f = gfo ? gfo[io] : make_float2(0.f, 0.f);
-        const float g_s = (PAY && gso) ? gso[io] : 0.f;
-        const float2 g_v = (PAY && gvo) ? gvo[io] : make_float2(0.f, 0.f);
+        const float2 g_f = GF[oi];
+        const float g_s = GS[oi];
+        const float2 g_v = GV[oi];
         if (!interior(o)) {
-            if (o == j) {  // out = z
+            if (oi == 1) {  // out = z
                 gf.x += g_f.x;
                 gf.y += g_f.y;
                 if (PAY) {
@@ -235,31 +260,30 @@ __global__ void __launch_bounds__(BX* BY)
             }
             continue;
         }
-        const size_t im = idx(o - 1), i0 = io, ip = idx(o + 1);
-        const float2 fm = fin[im], f0 = fin[i0], fp = fin[ip];
+        // o - 1, o, o + 1 are window positions oi, oi + 1, oi + 2
+        const float2 fm = F[oi], f0 = F[oi + 1], fp = F[oi + 2];
         const float um = AXIS == 0 ? fm.x : fm.y, up = AXIS == 0 ? fp.x : fp.y;
         const bool selp = (fabsf(up) - fabsf(um)) > 0.0f;
         const float Ud = selp ? up : um;
         const float nd = -(dt * Ud);
         const bool back = Ud >= 0.0f;
         // coefficient of this input in output o, by position of j in {m, z, p}
-        const float c = (j == o) ? (1.0f + (back ? nd : -nd)) : ((j == o - 1) ? (back ? -nd : 0.f) : (back ? 0.f : nd));
+        const float c = (oi == 1) ? (1.0f + (back ? nd : -nd)) : ((oi == 2) ? (back ? -nd : 0.f) : (back ? 0.f : nd));
         gf.x = fmaf(c, g_f.x, gf.x);
         gf.y = fmaf(c, g_f.y, gf.y);
         // adjoint of Ud: sum over fields of g_out * (-dt * D)
         auto D = [&](float m, float z, float p) { return back ? (z - m) : (p - z); };
         float gUd = g_f.x * D(fm.x, f0.x, fp.x) + g_f.y * D(fm.y, f0.y, fp.y);
-        if (PAY) {
-            const float sm = sin[im], s0 = sin[i0], sp = sin[ip];
-            const float2 vm = vin[im], v0 = vin[i0], vp = vin[ip];
+        if constexpr (PAY) {
             gs = fmaf(c, g_s, gs);
             gv.x = fmaf(c, g_v.x, gv.x);
             gv.y = fmaf(c, g_v.y, gv.y);
-            gUd += g_s * D(sm, s0, sp) + g_v.x * D(vm.x, v0.x, vp.x) + g_v.y * D(vm.y, v0.y, vp.y);
+            gUd += g_s * D(S[oi], S[oi + 1], S[oi + 2]) + g_v.x * D(V[oi].x, V[oi + 1].x, V[oi + 2].x) +
+                   g_v.y * D(V[oi].y, V[oi + 1].y, V[oi + 2].y);
         }
         gUd *= -dt;
         // Ud is the u component of the neighbour on the chosen side
-        const bool from = selp ? (j == o + 1) : (j == o - 1);
+        const bool from = selp ? (oi == 0) : (oi == 2);
         if (from) {
             if (AXIS == 0) gf.x += gUd;
             else gf.y += gUd;
