@@ -372,6 +372,12 @@ FF_API ff_status ff_ad_down_backward(const float* g_half, float* g_image, float*
 FF_API ff_status ff_ad_propagate_pass(int axis, const float* flow, const float* scalar, const float* vector,
                                       float* flow_out, float* scalar_out, float* vector_out, int B, int H, int W,
                                       float dt, int border, void* stream);
+/* An X pass and then a Y pass in one launch (the same results as two
+ * ff_ad_propagate_pass calls). With flow_mid (and scalar_mid, vector_mid
+ * when there are payloads) the X pass's result is written too. */
+FF_API ff_status ff_ad_propagate_pair(const float* flow, const float* scalar, const float* vector, float* flow_mid,
+                                      float* scalar_mid, float* vector_mid, float* flow_out, float* scalar_out,
+                                      float* vector_out, int B, int H, int W, float dt, int border, void* stream);
 FF_API ff_status ff_ad_propagate_pass_backward(int axis, const float* flow, const float* scalar, const float* vector,
                                                const float* g_flow_out, const float* g_scalar_out,
                                                const float* g_vector_out, float* g_flow, float* g_scalar,

@@ -168,6 +168,53 @@ __global__ void __launch_bounds__(BX* BY) k_ad_down_bwd(const float* gout, float
 
 // ------------------------------------------------------------ propagation
 
+// The fields advected by one pass at one pixel: flow, scalar and vector
+// payloads.
+struct Px5 {
+    float2 f;
+    float s;
+    float2 v;
+};
+
+// One upwind update of a pixel from its neighbours m, z, p along the pass
+// (propagation_k.cu:45-53 and the payload kernels): the speed is the flow
+// component c (0: x, 1: y) of the neighbour with the larger magnitude;
+// pixels outside the border keep z.
+template <int C, bool PAY>
+__device__ __forceinline__ Px5 upwind_px(const Px5& m, const Px5& z, const Px5& p, float dt, bool in) {
+    const float um = C == 0 ? m.f.x : m.f.y, up = C == 0 ? p.f.x : p.f.y;
+    const float Ud = (fabsf(up) - fabsf(um)) > 0.0f ? up : um;
+    const float nd = -(dt * Ud);
+    const bool back = Ud >= 0.0f;
+    Px5 o;
+    o.f = in ? make_float2(upwind(m.f.x, z.f.x, p.f.x, back, nd), upwind(m.f.y, z.f.y, p.f.y, back, nd)) : z.f;
+    if constexpr (PAY) {
+        o.s = in ? upwind(m.s, z.s, p.s, back, nd) : z.s;
+        o.v = in ? make_float2(upwind(m.v.x, z.v.x, p.v.x, back, nd), upwind(m.v.y, z.v.y, p.v.y, back, nd)) : z.v;
+    }
+    return o;
+}
+
+template <bool PAY>
+__device__ __forceinline__ Px5 load_px(const float2* f, const float* s, const float2* v, size_t i) {
+    Px5 r;
+    r.f = f[i];
+    if constexpr (PAY) {
+        r.s = s[i];
+        r.v = v[i];
+    }
+    return r;
+}
+
+template <bool PAY>
+__device__ __forceinline__ void store_px(float2* f, float* s, float2* v, size_t i, const Px5& r) {
+    f[i] = r.f;
+    if constexpr (PAY) {
+        s[i] = r.s;
+        v[i] = r.v;
+    }
+}
+
 // One upwind pass of the flow and, with payloads, of a scalar and a vector
 // field (flowPropagate{X,Y}_k / flowPropagatePayload{X,Y}_k,
 // propagation_k.cu:16-334).
@@ -179,18 +226,45 @@ __global__ void __launch_bounds__(BX* BY) k_ad_prop(const float2* fin, const flo
     if (x >= d.W || y >= d.H) return;
     const int ym = AXIS == 1 ? clampi(y - 1, 0, d.H - 1) : y, yp = AXIS == 1 ? clampi(y + 1, 0, d.H - 1) : y;
     const int xm = AXIS == 0 ? clampi(x - 1, 0, d.W - 1) : x, xp = AXIS == 0 ? clampi(x + 1, 0, d.W - 1) : x;
-    const size_t im = at(b, ym, xm, d), i0 = at(b, y, x, d), ip = at(b, yp, xp, d);
-    const float2 fm = fin[im], f0 = fin[i0], fp = fin[ip];
-    const float um = AXIS == 0 ? fm.x : fm.y, up = AXIS == 0 ? fp.x : fp.y;
-    const float Ud = (fabsf(up) - fabsf(um)) > 0.0f ? up : um;
-    const float nd = -(dt * Ud);
-    const bool back = Ud >= 0.0f;
+    const size_t i0 = at(b, y, x, d);
     const bool in = (x >= border && x < d.W - border) && (y >= border && y < d.H - border);
-    fout[i0] = in ? make_float2(upwind(fm.x, f0.x, fp.x, back, nd), upwind(fm.y, f0.y, fp.y, back, nd)) : f0;
-    if constexpr (PAY) {
-        sout[i0] = in ? upwind(sin[im], sin[i0], sin[ip], back, nd) : sin[i0];
-        const float2 vm = vin[im], v0 = vin[i0], vp = vin[ip];
-        vout[i0] = in ? make_float2(upwind(vm.x, v0.x, vp.x, back, nd), upwind(vm.y, v0.y, vp.y, back, nd)) : v0;
+    const Px5 o = upwind_px<AXIS, PAY>(load_px<PAY>(fin, sin, vin, at(b, ym, xm, d)), load_px<PAY>(fin, sin, vin, i0),
+                                       load_px<PAY>(fin, sin, vin, at(b, yp, xp, d)), dt, in);
+    store_px<PAY>(fout, sout, vout, i0, o);
+}
+
+// An X pass followed by a Y pass in one launch: a workgroup of 64 x 16
+// threads computes the X pass on 16 rows (clamped into the image) into LDS
+// and the Y pass on the 14 inner ones, so the pair reads and writes each
+// field once instead of twice. With SAVE, the X pass's result (the Y pass's
+// input, which the adjoint needs) is written too. Same arithmetic as two
+// k_ad_prop launches.
+constexpr int kPairRows = 14;
+template <bool PAY, bool SAVE>
+__global__ void __launch_bounds__(64 * (kPairRows + 2))
+    k_ad_prop_pair(const float2* fin, const float* sin, const float2* vin, float2* fmid, float* smid, float2* vmid,
+                   float2* fout, float* sout, float2* vout, Dims d, float dt, int border) {
+    __shared__ Px5 L[kPairRows + 2][64];
+    const int tx = threadIdx.x, ty = threadIdx.y, b = blockIdx.z;
+    const int x = blockIdx.x * 64 + tx;
+    const int yr = (int)blockIdx.y * kPairRows - 1 + ty;  // row of this thread's X pass
+    const int yc = clampi(yr, 0, d.H - 1);
+    const bool xok = x < d.W;
+    const bool xin = x >= border && x < d.W - border;
+    if (xok) {
+        const int xm = clampi(x - 1, 0, d.W - 1), xp = clampi(x + 1, 0, d.W - 1);
+        const bool in = xin && yc >= border && yc < d.H - border;
+        const size_t i0 = at(b, yc, x, d);
+        const Px5 o = upwind_px<0, PAY>(load_px<PAY>(fin, sin, vin, at(b, yc, xm, d)), load_px<PAY>(fin, sin, vin, i0),
+                                        load_px<PAY>(fin, sin, vin, at(b, yc, xp, d)), dt, in);
+        L[ty][tx] = o;
+        if (SAVE && ty >= 1 && ty <= kPairRows && yr < d.H) store_px<PAY>(fmid, smid, vmid, i0, o);
+    }
+    __syncthreads();
+    if (xok && ty >= 1 && ty <= kPairRows && yr < d.H) {
+        const bool in = xin && yr >= border && yr < d.H - border;
+        const Px5 o = upwind_px<1, PAY>(L[ty - 1][tx], L[ty][tx], L[ty + 1][tx], dt, in);
+        store_px<PAY>(fout, sout, vout, at(b, yr, x, d), o);
     }
 }
 
@@ -554,6 +628,22 @@ void prop(int axis, const float* fin, const float* sin, const float* vin, float*
         if (pay) hipLaunchKernelGGL((k_ad_prop<1, true>), g, blk, 0, s, F(fin), sin, F(vin), (float2*)fout, sout, (float2*)vout, d, dt, border);
         else hipLaunchKernelGGL((k_ad_prop<1, false>), g, blk, 0, s, F(fin), sin, F(vin), (float2*)fout, sout, (float2*)vout, d, dt, border);
     }
+}
+void prop_pair(const float* fin, const float* sin, const float* vin, float* fmid, float* smid, float* vmid,
+               float* fout, float* sout, float* vout, int B, int H, int W, float dt, int border, hipStream_t s) {
+    const bool pay = sin != nullptr, save = fmid != nullptr;
+    const dim3 g((W + 63) / 64, (H + kPairRows - 1) / kPairRows, B), blk(64, kPairRows + 2);
+    const Dims d{H, W};
+    auto F = [](const float* p) { return (const float2*)p; };
+    auto O = [](float* p) { return (float2*)p; };
+    if (pay && save)
+        hipLaunchKernelGGL((k_ad_prop_pair<true, true>), g, blk, 0, s, F(fin), sin, F(vin), O(fmid), smid, O(vmid), O(fout), sout, O(vout), d, dt, border);
+    else if (pay)
+        hipLaunchKernelGGL((k_ad_prop_pair<true, false>), g, blk, 0, s, F(fin), sin, F(vin), O(fmid), smid, O(vmid), O(fout), sout, O(vout), d, dt, border);
+    else if (save)
+        hipLaunchKernelGGL((k_ad_prop_pair<false, true>), g, blk, 0, s, F(fin), sin, F(vin), O(fmid), smid, O(vmid), O(fout), sout, O(vout), d, dt, border);
+    else
+        hipLaunchKernelGGL((k_ad_prop_pair<false, false>), g, blk, 0, s, F(fin), sin, F(vin), O(fmid), smid, O(vmid), O(fout), sout, O(vout), d, dt, border);
 }
 void prop_bwd(int axis, const float* fin, const float* sin, const float* vin, const float* gfo, const float* gso,
               const float* gvo, float* gfi, float* gsi, float* gvi, int B, int H, int W, float dt, int border,

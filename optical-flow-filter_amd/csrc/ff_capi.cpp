@@ -664,6 +664,24 @@ ff_status ff_ad_propagate_pass(int axis, const float* flow, const float* scalar,
         check_launch("ff_ad_propagate_pass");
     });
 }
+ff_status ff_ad_propagate_pair(const float* flow, const float* scalar, const float* vector, float* flow_mid,
+                               float* scalar_mid, float* vector_mid, float* flow_out, float* scalar_out,
+                               float* vector_out, int B, int H, int W, float dt, int border, void* stream) {
+    return guard([&] {
+        dims(B, H, W);
+        if (border < 1) throw invalid_argument("border must be at least 1");
+        need(flow, "flow"), need(flow_out, "flow_out");
+        const bool pay = scalar != nullptr;
+        if (pay != (vector != nullptr) || pay != (scalar_out != nullptr) || pay != (vector_out != nullptr))
+            throw invalid_argument("payloads: pass all of scalar, vector and their outputs, or none");
+        const bool save = flow_mid != nullptr;
+        if (save && (pay ? (!scalar_mid || !vector_mid) : (scalar_mid || vector_mid)))
+            throw invalid_argument("intermediates: pass those of the payloads with the flow's, or none");
+        ad::prop_pair(flow, scalar, vector, flow_mid, save ? scalar_mid : nullptr, save ? vector_mid : nullptr, flow_out,
+                      scalar_out, vector_out, B, H, W, dt, border, as_stream(stream));
+        check_launch("ff_ad_propagate_pair");
+    });
+}
 ff_status ff_ad_propagate_pass_backward(int axis, const float* flow, const float* scalar, const float* vector,
                                         const float* g_flow_out, const float* g_scalar_out, const float* g_vector_out,
                                         float* g_flow, float* g_scalar, float* g_vector, int B, int H, int W, float dt,

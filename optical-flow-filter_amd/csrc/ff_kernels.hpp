@@ -179,6 +179,8 @@ void down(const float* I, float* O, float* work, int B, int H, int W, hipStream_
 void down_bwd(const float* gO, float* gI, float* work, int B, int H, int W, int acc, hipStream_t s);
 void prop(int axis, const float* fin, const float* sin, const float* vin, float* fout, float* sout, float* vout, int B,
           int H, int W, float dt, int border, hipStream_t s);
+void prop_pair(const float* fin, const float* sin, const float* vin, float* fmid, float* smid, float* vmid,
+               float* fout, float* sout, float* vout, int B, int H, int W, float dt, int border, hipStream_t s);
 void prop_bwd(int axis, const float* fin, const float* sin, const float* vin, const float* gfo, const float* gso,
               const float* gvo, float* gfi, float* gsi, float* gvi, int B, int H, int W, float dt, int border,
               int acc, hipStream_t s);

@@ -181,15 +181,22 @@ def _frame_forward(spec, state, image, gammas, save):
         sc = state[l]["a0"] if l < top else None
         vc = state[l]["d"] if l < top else None
         passes = []
+        pay = sc is not None
         for _ in range(lv["N"]):
-            for axis in (0, 1):
-                fo = _empty(B, h, w, 2, like=image)
-                so = _empty(B, h, w, like=image) if sc is not None else None
-                vo = _empty(B, h, w, 2, like=image) if vc is not None else None
-                N.call("ff_ad_propagate_pass", axis, _p(f), _p(sc), _p(vc), _p(fo), _p(so), _p(vo), B, h, w,
-                       ctypes.c_float(lv["dt"]), spec.border, s)
+            # an X and a Y pass in one launch; the X pass's result (the Y
+            # pass's input) is kept when the adjoint will need it
+            fm = _empty(B, h, w, 2, like=image) if save else None
+            sm = _empty(B, h, w, like=image) if save and pay else None
+            vm = _empty(B, h, w, 2, like=image) if save and pay else None
+            fo = _empty(B, h, w, 2, like=image)
+            so = _empty(B, h, w, like=image) if pay else None
+            vo = _empty(B, h, w, 2, like=image) if pay else None
+            N.call("ff_ad_propagate_pair", _p(f), _p(sc), _p(vc), _p(fm), _p(sm), _p(vm), _p(fo), _p(so), _p(vo),
+                   B, h, w, ctypes.c_float(lv["dt"]), spec.border, s)
+            if save:
                 passes.append((f, sc, vc))
-                f, sc, vc = fo, so, vo
+                passes.append((fm, sm, vm))
+            f, sc, vc = fo, so, vo
         prop[l] = (f, sc, vc)
         if save:
             saved["passes"][l] = passes

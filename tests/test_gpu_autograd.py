@@ -117,6 +117,33 @@ def test_forward_stages_match_oracle(ad, ffo):
     N.synchronize()
 
 
+@pytest.mark.parametrize("shape", [(2, 45, 70), (1, 9, 130), (3, 31, 64), (1, 1, 5)])
+@pytest.mark.parametrize("pay", [False, True])
+def test_pass_pair_equals_two_passes(ad, shape, pay):
+    """ff_ad_propagate_pair (X then Y in one launch, optionally keeping the
+    X pass's result) is bitwise the two single passes."""
+    B, H, W = shape
+    r = rng(21 + H)
+    f = cuda(r.uniform(-3, 3, size=(B, H, W, 2)).astype(np.float32))
+    sc = cuda(r.uniform(0, 1, size=(B, H, W)).astype(np.float32)) if pay else None
+    vc = cuda(r.uniform(-1, 1, size=(B, H, W, 2)).astype(np.float32)) if pay else None
+    dt = ctypes_f(float(np.float32(1) / np.float32(3)))
+    e2, e1 = (lambda: torch.empty(B, H, W, 2, device="cuda")), (lambda: torch.empty(B, H, W, device="cuda"))
+    m = [e2(), e1() if pay else None, e2() if pay else None]
+    o = [e2(), e1() if pay else None, e2() if pay else None]
+    _call(ad, "ff_ad_propagate_pass", 0, f, sc, vc, *m, B, H, W, dt, 1)
+    _call(ad, "ff_ad_propagate_pass", 1, *m, *o, B, H, W, dt, 1)
+    pm = [e2(), e1() if pay else None, e2() if pay else None]
+    po = [e2(), e1() if pay else None, e2() if pay else None]
+    _call(ad, "ff_ad_propagate_pair", f, sc, vc, *pm, *po, B, H, W, dt, 1)
+    qo = [e2(), e1() if pay else None, e2() if pay else None]
+    _call(ad, "ff_ad_propagate_pair", f, sc, vc, None, None, None, *qo, B, H, W, dt, 1)
+    torch.cuda.synchronize()
+    for a, b_ in zip(m + o + o, pm + po + qo):
+        if a is not None:
+            assert torch.equal(a, b_)
+
+
 def ctypes_f(x):
     import ctypes
     return ctypes.c_float(x)
