@@ -378,6 +378,15 @@ FF_API ff_status ff_ad_propagate_pass(int axis, const float* flow, const float* 
 FF_API ff_status ff_ad_propagate_pair(const float* flow, const float* scalar, const float* vector, float* flow_mid,
                                       float* scalar_mid, float* vector_mid, float* flow_out, float* scalar_out,
                                       float* vector_out, int B, int H, int W, float dt, int border, void* stream);
+/* Adjoint of ff_ad_propagate_pair, given the pair's input and the X pass's
+ * result (flow_mid ...); the same results as the two passes' adjoints.
+ * g_*_out may be NULL (zero); accumulate as for the pass backward. */
+FF_API ff_status ff_ad_propagate_pair_backward(const float* flow, const float* scalar, const float* vector,
+                                               const float* flow_mid, const float* scalar_mid,
+                                               const float* vector_mid, const float* g_flow_out,
+                                               const float* g_scalar_out, const float* g_vector_out, float* g_flow,
+                                               float* g_scalar, float* g_vector, int B, int H, int W, float dt,
+                                               int border, int accumulate, void* stream);
 FF_API ff_status ff_ad_propagate_pass_backward(int axis, const float* flow, const float* scalar, const float* vector,
                                                const float* g_flow_out, const float* g_scalar_out,
                                                const float* g_vector_out, float* g_flow, float* g_scalar,

@@ -277,51 +277,23 @@ __global__ void __launch_bounds__(64 * (kPairRows + 2))
 // and Ud's derivative goes to p_u or m_u. Pixels within the border copy z.
 // Input j gathers from the outputs at j-1, j, j+1 along the pass (for
 // interior outputs the neighbour reads are unclamped: border >= 1).
-template <int AXIS, bool PAY>
-__global__ void __launch_bounds__(BX* BY)
-    k_ad_prop_bwd(const float2* fin, const float* sin, const float2* vin, const float2* gfo, const float* gso,
-                  const float2* gvo, float2* gfi, float* gsi, float2* gvi, Dims d, float dt, int border, int acc) {
-    const int x = blockIdx.x * BX + threadIdx.x, y = blockIdx.y * BY + threadIdx.y, b = blockIdx.z;
-    if (x >= d.W || y >= d.H) return;
-    const int n = AXIS == 0 ? d.W : d.H, j = AXIS == 0 ? x : y;
-    auto idx = [&](int k) { return AXIS == 0 ? at(b, y, k, d) : at(b, k, x, d); };
-    auto interior = [&](int k) {
-        const int xx = AXIS == 0 ? k : x, yy = AXIS == 0 ? y : k;
-        return (xx >= border && xx < d.W - border) && (yy >= border && yy < d.H - border);
-    };
-    // Everything this input can reach, loaded once up front: the forward
-    // inputs at j-2 .. j+2 and the output gradients at j-1 .. j+1 (reads
-    // clamped to the line; a clamped value is only used where its position
-    // is inside, since an interior output's neighbours are: border >= 1).
-    float2 F[5], V[5];
-    float S[5];
-#pragma unroll
-    for (int k = 0; k < 5; ++k) {
-        const size_t i = idx(clampi(j - 2 + k, 0, n - 1));
-        F[k] = fin[i];
-        if constexpr (PAY) {
-            S[k] = sin[i];
-            V[k] = vin[i];
-        }
-    }
-    float2 GF[3], GV[3];
-    float GS[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        const size_t i = idx(clampi(j - 1 + k, 0, n - 1));
-        GF[k] = gfo ? gfo[i] : make_float2(0.f, 0.f);
-        GS[k] = (PAY && gso) ? gso[i] : 0.f;
-        GV[k] = (PAY && gvo) ? gvo[i] : make_float2(0.f, 0.f);
-    }
+//
+// One input position j of a line of n: X holds the forward inputs at
+// j-2 .. j+2 and G the output gradients at j-1 .. j+1 (values at positions
+// outside the line are never used); interior(o) tells whether output o is
+// inside the border. C: the flow component of the speed (0: X pass).
+template <int C, bool PAY, class Interior>
+__device__ __forceinline__ Px5 upwind_adjoint_px(const Px5 (&X)[5], const Px5 (&G)[3], int j, int n,
+                                                 Interior interior, float dt) {
     float2 gf = make_float2(0.f, 0.f), gv = make_float2(0.f, 0.f);
     float gs = 0.f;
 #pragma unroll
     for (int oi = 0; oi < 3; ++oi) {  // output o = j - 1 + oi
         const int o = j - 1 + oi;
         if (o < 0 || o >= n) continue;
-        const float2 g_f = GF[oi];
-        const float g_s = GS[oi];
-        const float2 g_v = GV[oi];
+        const float2 g_f = G[oi].f;
+        const float g_s = PAY ? G[oi].s : 0.f;
+        const float2 g_v = PAY ? G[oi].v : make_float2(0.f, 0.f);
         if (!interior(o)) {
             if (oi == 1) {  // out = z
                 gf.x += g_f.x;
@@ -335,8 +307,8 @@ __global__ void __launch_bounds__(BX* BY)
             continue;
         }
         // o - 1, o, o + 1 are window positions oi, oi + 1, oi + 2
-        const float2 fm = F[oi], f0 = F[oi + 1], fp = F[oi + 2];
-        const float um = AXIS == 0 ? fm.x : fm.y, up = AXIS == 0 ? fp.x : fp.y;
+        const float2 fm = X[oi].f, f0 = X[oi + 1].f, fp = X[oi + 2].f;
+        const float um = C == 0 ? fm.x : fm.y, up = C == 0 ? fp.x : fp.y;
         const bool selp = (fabsf(up) - fabsf(um)) > 0.0f;
         const float Ud = selp ? up : um;
         const float nd = -(dt * Ud);
@@ -352,21 +324,104 @@ __global__ void __launch_bounds__(BX* BY)
             gs = fmaf(c, g_s, gs);
             gv.x = fmaf(c, g_v.x, gv.x);
             gv.y = fmaf(c, g_v.y, gv.y);
-            gUd += g_s * D(S[oi], S[oi + 1], S[oi + 2]) + g_v.x * D(V[oi].x, V[oi + 1].x, V[oi + 2].x) +
-                   g_v.y * D(V[oi].y, V[oi + 1].y, V[oi + 2].y);
+            gUd += g_s * D(X[oi].s, X[oi + 1].s, X[oi + 2].s) + g_v.x * D(X[oi].v.x, X[oi + 1].v.x, X[oi + 2].v.x) +
+                   g_v.y * D(X[oi].v.y, X[oi + 1].v.y, X[oi + 2].v.y);
         }
         gUd *= -dt;
         // Ud is the u component of the neighbour on the chosen side
         const bool from = selp ? (oi == 0) : (oi == 2);
         if (from) {
-            if (AXIS == 0) gf.x += gUd;
+            if (C == 0) gf.x += gUd;
             else gf.y += gUd;
         }
     }
-    put2(gfi, at(b, y, x, d), gf, acc & 1);
+    Px5 r;
+    r.f = gf;
+    r.s = gs;
+    r.v = gv;
+    return r;
+}
+
+template <bool PAY>
+__device__ __forceinline__ Px5 load_grad_px(const float2* f, const float* s, const float2* v, size_t i) {
+    Px5 r;
+    r.f = f ? f[i] : make_float2(0.f, 0.f);
+    r.s = (PAY && s) ? s[i] : 0.f;
+    r.v = (PAY && v) ? v[i] : make_float2(0.f, 0.f);
+    return r;
+}
+
+template <bool PAY>
+__device__ __forceinline__ void put_px(float2* f, float* s, float2* v, size_t i, const Px5& r, int acc) {
+    put2(f, i, r.f, acc & 1);
     if constexpr (PAY) {
-        put(gsi, at(b, y, x, d), gs, acc & 2);
-        put2(gvi, at(b, y, x, d), gv, acc & 4);
+        put(s, i, r.s, acc & 2);
+        put2(v, i, r.v, acc & 4);
+    }
+}
+
+template <int AXIS, bool PAY>
+__global__ void __launch_bounds__(BX* BY)
+    k_ad_prop_bwd(const float2* fin, const float* sin, const float2* vin, const float2* gfo, const float* gso,
+                  const float2* gvo, float2* gfi, float* gsi, float2* gvi, Dims d, float dt, int border, int acc) {
+    const int x = blockIdx.x * BX + threadIdx.x, y = blockIdx.y * BY + threadIdx.y, b = blockIdx.z;
+    if (x >= d.W || y >= d.H) return;
+    const int n = AXIS == 0 ? d.W : d.H, j = AXIS == 0 ? x : y;
+    auto idx = [&](int k) { return AXIS == 0 ? at(b, y, k, d) : at(b, k, x, d); };
+    auto interior = [&](int k) {
+        const int xx = AXIS == 0 ? k : x, yy = AXIS == 0 ? y : k;
+        return (xx >= border && xx < d.W - border) && (yy >= border && yy < d.H - border);
+    };
+    // everything this input can reach, loaded once up front (reads clamped
+    // to the line)
+    Px5 X[5], G[3];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) X[k] = load_px<PAY>(fin, sin, vin, idx(clampi(j - 2 + k, 0, n - 1)));
+#pragma unroll
+    for (int k = 0; k < 3; ++k) G[k] = load_grad_px<PAY>(gfo, gso, gvo, idx(clampi(j - 1 + k, 0, n - 1)));
+    put_px<PAY>(gfi, gsi, gvi, at(b, y, x, d), upwind_adjoint_px<AXIS, PAY>(X, G, j, n, interior, dt), acc);
+}
+
+// Adjoint of an X pass followed by a Y pass (the pair of k_ad_prop_pair):
+// a workgroup of 64 x 16 threads computes the Y pass's adjoint at 64 columns
+// (the 62 of its output plus one on each side) of 16 rows into LDS, then
+// the X pass's adjoint at the 62 inner columns, reading the pair's input,
+// the X pass's result (mid) and the output gradients once. Same arithmetic
+// as two k_ad_prop_bwd launches.
+constexpr int kPairCols = 62, kPairBwdRows = 16;
+template <bool PAY>
+__global__ void __launch_bounds__(64 * kPairBwdRows)
+    k_ad_prop_pair_bwd(const float2* fin, const float* sin, const float2* vin, const float2* fmid, const float* smid,
+                       const float2* vmid, const float2* gfo, const float* gso, const float2* gvo, float2* gfi,
+                       float* gsi, float2* gvi, Dims d, float dt, int border, int acc) {
+    __shared__ Px5 LG[kPairBwdRows][64];
+    const int tx = threadIdx.x, ty = threadIdx.y, b = blockIdx.z;
+    const int x = (int)blockIdx.x * kPairCols - 1 + tx;
+    const int y = (int)blockIdx.y * kPairBwdRows + ty;
+    const bool xin = x >= border && x < d.W - border;
+    Px5 gm;
+    gm.f = make_float2(0.f, 0.f);
+    gm.s = 0.f;
+    gm.v = make_float2(0.f, 0.f);
+    if (x >= 0 && x < d.W && y < d.H) {
+        Px5 X[5], G[3];
+#pragma unroll
+        for (int k = 0; k < 5; ++k) X[k] = load_px<PAY>(fmid, smid, vmid, at(b, clampi(y - 2 + k, 0, d.H - 1), x, d));
+#pragma unroll
+        for (int k = 0; k < 3; ++k) G[k] = load_grad_px<PAY>(gfo, gso, gvo, at(b, clampi(y - 1 + k, 0, d.H - 1), x, d));
+        auto interior = [&](int o) { return xin && o >= border && o < d.H - border; };
+        gm = upwind_adjoint_px<1, PAY>(X, G, y, d.H, interior, dt);
+    }
+    LG[ty][tx] = gm;
+    __syncthreads();
+    if (tx >= 1 && tx <= kPairCols && x < d.W && y < d.H) {
+        Px5 X[5];
+        const Px5 G[3] = {LG[ty][tx - 1], LG[ty][tx], LG[ty][tx + 1]};
+#pragma unroll
+        for (int k = 0; k < 5; ++k) X[k] = load_px<PAY>(fin, sin, vin, at(b, y, clampi(x - 2 + k, 0, d.W - 1), d));
+        const bool yin = y >= border && y < d.H - border;
+        auto interior = [&](int o) { return yin && o >= border && o < d.W - border; };
+        put_px<PAY>(gfi, gsi, gvi, at(b, y, x, d), upwind_adjoint_px<0, PAY>(X, G, x, d.W, interior, dt), acc);
     }
 }
 
@@ -644,6 +699,18 @@ void prop_pair(const float* fin, const float* sin, const float* vin, float* fmid
         hipLaunchKernelGGL((k_ad_prop_pair<false, true>), g, blk, 0, s, F(fin), sin, F(vin), O(fmid), smid, O(vmid), O(fout), sout, O(vout), d, dt, border);
     else
         hipLaunchKernelGGL((k_ad_prop_pair<false, false>), g, blk, 0, s, F(fin), sin, F(vin), O(fmid), smid, O(vmid), O(fout), sout, O(vout), d, dt, border);
+}
+void prop_pair_bwd(const float* fin, const float* sin, const float* vin, const float* fmid, const float* smid,
+                   const float* vmid, const float* gfo, const float* gso, const float* gvo, float* gfi, float* gsi,
+                   float* gvi, int B, int H, int W, float dt, int border, int acc, hipStream_t s) {
+    const dim3 g((W + kPairCols - 1) / kPairCols, (H + kPairBwdRows - 1) / kPairBwdRows, B), blk(64, kPairBwdRows);
+    const Dims d{H, W};
+    auto F = [](const float* p) { return (const float2*)p; };
+    auto O = [](float* p) { return (float2*)p; };
+    if (sin)
+        hipLaunchKernelGGL((k_ad_prop_pair_bwd<true>), g, blk, 0, s, F(fin), sin, F(vin), F(fmid), smid, F(vmid), F(gfo), gso, F(gvo), O(gfi), gsi, O(gvi), d, dt, border, acc);
+    else
+        hipLaunchKernelGGL((k_ad_prop_pair_bwd<false>), g, blk, 0, s, F(fin), sin, F(vin), F(fmid), smid, F(vmid), F(gfo), gso, F(gvo), O(gfi), gsi, O(gvi), d, dt, border, acc);
 }
 void prop_bwd(int axis, const float* fin, const float* sin, const float* vin, const float* gfo, const float* gso,
               const float* gvo, float* gfi, float* gsi, float* gvi, int B, int H, int W, float dt, int border,

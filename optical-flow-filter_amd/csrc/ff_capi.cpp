@@ -682,6 +682,24 @@ ff_status ff_ad_propagate_pair(const float* flow, const float* scalar, const flo
         check_launch("ff_ad_propagate_pair");
     });
 }
+ff_status ff_ad_propagate_pair_backward(const float* flow, const float* scalar, const float* vector,
+                                        const float* flow_mid, const float* scalar_mid, const float* vector_mid,
+                                        const float* g_flow_out, const float* g_scalar_out, const float* g_vector_out,
+                                        float* g_flow, float* g_scalar, float* g_vector, int B, int H, int W, float dt,
+                                        int border, int accumulate, void* stream) {
+    return guard([&] {
+        dims(B, H, W);
+        if (border < 1) throw invalid_argument("border must be at least 1");
+        need(flow, "flow"), need(flow_mid, "flow_mid"), need(g_flow, "g_flow");
+        const bool pay = scalar != nullptr;
+        if (pay && (!vector || !scalar_mid || !vector_mid || !g_scalar || !g_vector))
+            throw invalid_argument("payloads: pass scalar, vector, their intermediates and both of their gradients");
+        ad::prop_pair_bwd(flow, scalar, vector, flow_mid, pay ? scalar_mid : nullptr, pay ? vector_mid : nullptr,
+                          g_flow_out, pay ? g_scalar_out : nullptr, pay ? g_vector_out : nullptr, g_flow, g_scalar,
+                          g_vector, B, H, W, dt, border, accumulate, as_stream(stream));
+        check_launch("ff_ad_propagate_pair_backward");
+    });
+}
 ff_status ff_ad_propagate_pass_backward(int axis, const float* flow, const float* scalar, const float* vector,
                                         const float* g_flow_out, const float* g_scalar_out, const float* g_vector_out,
                                         float* g_flow, float* g_scalar, float* g_vector, int B, int H, int W, float dt,

@@ -314,17 +314,18 @@ def _frame_backward(spec, saved, g_new, g_out, gammas, g_image_out):
             g_old[l + 1]["f"] = gupper
         g_a0[l] = ga
         g_g[l] = gg
-        # propagation passes in reverse; the propagated flow itself is not
-        # read by the delta update, so its gradient enters as zero (None)
+        # propagation pass pairs in reverse; the propagated flow itself is
+        # not read by the delta update, so its gradient enters as zero (None)
         gf, gs, gv = None, gsp, gvp
         passes = saved["passes"][l]
-        for i, ((fi, si, vi), axis) in enumerate(zip(reversed(passes), [1, 0] * lv["N"])):
-            last = i == len(passes) - 1
-            acc_f = last and "f" in g_old[l]
+        npairs = len(passes) // 2
+        for i in range(npairs):
+            (fi, si, vi), (fm, sm, vm) = passes[2 * (npairs - 1 - i)], passes[2 * (npairs - 1 - i) + 1]
+            acc_f = i == npairs - 1 and "f" in g_old[l]
             nf = g_old[l]["f"] if acc_f else _empty(B, h, w, 2, like=gfl)
             ns, nv = _empty(B, h, w, like=gfl), _empty(B, h, w, 2, like=gfl)
-            N.call("ff_ad_propagate_pass_backward", axis, _p(fi), _p(si), _p(vi), _p(gf), _p(gs), _p(gv), _p(nf),
-                   _p(ns), _p(nv), B, h, w, f32(lv["dt"]), spec.border, 1 if acc_f else 0, s)
+            N.call("ff_ad_propagate_pair_backward", _p(fi), _p(si), _p(vi), _p(fm), _p(sm), _p(vm), _p(gf), _p(gs),
+                   _p(gv), _p(nf), _p(ns), _p(nv), B, h, w, f32(lv["dt"]), spec.border, 1 if acc_f else 0, s)
             gf, gs, gv = nf, ns, nv
         g_old[l]["f"] = gf
         g_old[l]["a0"] = gs
@@ -349,10 +350,11 @@ def _frame_backward(spec, saved, g_new, g_out, gammas, g_image_out):
     g_g[top] = gg
     g_old[top]["a0"] = gao
     gf = gfp
-    for (fi, _, _), axis in zip(reversed(saved["passes"][top]), [1, 0] * lv["N"]):
+    passes = saved["passes"][top]
+    for i in reversed(range(len(passes) // 2)):
         nf = _empty(B, h, w, 2, like=gf)
-        N.call("ff_ad_propagate_pass_backward", axis, _p(fi), None, None, _p(gf), None, None, _p(nf), None, None,
-               B, h, w, f32(lv["dt"]), spec.border, 0, s)
+        N.call("ff_ad_propagate_pair_backward", _p(passes[2 * i][0]), None, None, _p(passes[2 * i + 1][0]), None, None,
+               _p(gf), None, None, _p(nf), None, None, B, h, w, f32(lv["dt"]), spec.border, 0, s)
         gf = nf
     g_old[top]["f"] = gf
     if saved["first"]:

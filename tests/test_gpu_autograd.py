@@ -144,6 +144,36 @@ def test_pass_pair_equals_two_passes(ad, shape, pay):
             assert torch.equal(a, b_)
 
 
+@pytest.mark.parametrize("shape", [(2, 45, 70), (1, 9, 130), (3, 33, 62), (1, 1, 5)])
+@pytest.mark.parametrize("pay", [False, True])
+def test_pass_pair_adjoint_equals_two_adjoints(ad, shape, pay):
+    """ff_ad_propagate_pair_backward is bitwise the Y pass's adjoint followed
+    by the X pass's (zero and accumulated output gradients included)."""
+    B, H, W = shape
+    r = rng(31 + H)
+    rnd = lambda *sh, lo=-1.0, hi=1.0: cuda(r.uniform(lo, hi, size=sh).astype(np.float32))
+    f = rnd(B, H, W, 2, lo=-3, hi=3)
+    sc, vc = (rnd(B, H, W, lo=0.0), rnd(B, H, W, 2)) if pay else (None, None)
+    dt = ctypes_f(float(np.float32(1) / np.float32(4)))
+    e2, e1 = (lambda: torch.empty(B, H, W, 2, device="cuda")), (lambda: torch.empty(B, H, W, device="cuda"))
+    m = [e2(), e1() if pay else None, e2() if pay else None]
+    o = [e2(), e1() if pay else None, e2() if pay else None]
+    _call(ad, "ff_ad_propagate_pair", f, sc, vc, *m, *o, B, H, W, dt, 2)
+    for zero_flow in (False, True):
+        g = [None if zero_flow else rnd(B, H, W, 2), rnd(B, H, W) if pay else None, rnd(B, H, W, 2) if pay else None]
+        mid = [e2(), e1() if pay else None, e2() if pay else None]
+        ref = [rnd(B, H, W, 2), e1() if pay else None, e2() if pay else None]
+        pre = ref[0].clone()
+        _call(ad, "ff_ad_propagate_pass_backward", 1, *m, *g, *mid, B, H, W, dt, 2, 0)
+        _call(ad, "ff_ad_propagate_pass_backward", 0, f, sc, vc, *mid, *ref, B, H, W, dt, 2, 1)
+        got = [pre.clone(), e1() if pay else None, e2() if pay else None]
+        _call(ad, "ff_ad_propagate_pair_backward", f, sc, vc, *m, *g, *got, B, H, W, dt, 2, 1)
+        torch.cuda.synchronize()
+        for a, b_ in zip(ref, got):
+            if a is not None:
+                assert torch.equal(a, b_), zero_flow
+
+
 def ctypes_f(x):
     import ctypes
     return ctypes.c_float(x)
