@@ -13,7 +13,9 @@ device address handed over with setImageDevice; the fused frame kernel
 loads it directly). W warm-up steps, then K timed steps between two HIP
 events on the library stream, with the stream synchronized before and
 after. The same K steps with a device loadImage (a copy into the filter's
-input image) before each compute() are reported as `value_with_copy`.
+input image) before each compute() are reported as `value_with_copy`, and
+with a host loadImage (the reference's boundary, PCIe included; wall clock)
+as `value_host_upload`.
 
 Prints one JSON line: Mpix/s of level-0 input pixels, the roofline of the
 dominant kernel (algorithmic bytes per launch / its average duration from
@@ -355,6 +357,20 @@ def main():
         value_copy = round(H * W * args.steps / (ms_copy / 1e3) / 1e6, 3)
         print("with a device loadImage copy per step: %.3f ms/step" % (ms_copy / args.steps), file=sys.stderr,
               flush=True)
+    # the reference's own boundary: loadImage of a host frame (PCIe
+    # included), then compute(); wall clock over the same number of steps
+    value_host = None
+    if not args.no_copy_variant:
+        N.synchronize()
+        w0 = time.perf_counter()
+        for i in range(args.steps):
+            f.loadImage(frames[i % F])
+            f.compute()
+        N.synchronize()
+        host_ms = (time.perf_counter() - w0) * 1e3
+        value_host = round(H * W * args.steps / (host_ms / 1e3) / 1e6, 3)
+        print("with a host loadImage per step (PCIe included): %.3f ms/step" % (host_ms / args.steps),
+              file=sys.stderr, flush=True)
 
     # roofline of the dominant kernel: second pass, HIP events per launch
     dominant = "propagate_payload" if args.staged else "fused_frame"
@@ -379,6 +395,7 @@ def main():
         "metric": "Mpix/s at 2048x1088 2-lvl; HBM GB/s % of peak; batched/plain speedup",
         "value": round(value, 3),
         "value_with_copy": value_copy,
+        "value_host_upload": value_host,
         "unit": "Mpix/s",
         "n_gpus": 1,
         "steps": args.steps,
