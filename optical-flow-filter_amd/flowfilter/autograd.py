@@ -110,10 +110,6 @@ class FilterSpec(object):
             for k, v in (("xmap", xm), ("ymap", ym), ("xlo", xlo), ("xhi", xhi), ("ylo", ylo), ("yhi", yhi)):
                 lv[k] = torch.from_numpy(v).to(self.device)
 
-    def pixels(self):
-        """Level-0 pixels of one frame of one sequence."""
-        return self.H * self.W
-
 
 def _p(t):
     return ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p(0)

@@ -1,7 +1,7 @@
 """GPU parity of the filters (FlowFilter, DeltaFlowFilter, PyramidalFlowFilter)
 against the CPU oracle over whole synthetic sequences, through the C ABI.
 
-Bar (BASELINE north star): max |flow_gpu - flow_oracle_fp32| <= 1e-4 px on
+Bar: max |flow_gpu - flow_oracle_fp32| <= 1e-4 px on
 every frame. The kernels follow the oracle's rounding points, so the
 observed difference is expected to be exactly 0; both are asserted. The
 GPU result is also compared with the FP64 oracle and the error printed.
