@@ -30,10 +30,11 @@ def make(h, w, levels):
 
 @pytest.mark.parametrize("levels", [1, 2, 3, 4])
 @pytest.mark.parametrize("offset", [0, 1])
-def test_frames_in_place_match_copies(levels, offset):
+@pytest.mark.parametrize("shape", [(272, 480), (301, 411)])
+def test_frames_in_place_match_copies(levels, offset, shape):
     from flowfilter import _native as N, synthetic
     from flowfilter.gpu.image import GPUImage
-    H, W, n = 272, 480, 6
+    (H, W), n = shape, 6
     frames = list(synthetic.translating_noise(H, W, n, (1.1, -0.7), seed=40 + levels))
     stack = np.concatenate(frames, axis=0)
     g = GPUImage((n * H, W + 4), 1)
