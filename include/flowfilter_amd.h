@@ -291,10 +291,11 @@ FF_API ff_status ff_pyramidal_set_image_device(ff_stage* s, const void* dev, siz
 FF_API ff_status ff_filter_set_fused(ff_stage* s, int fused);
 FF_API ff_status ff_filter_get_fused(ff_stage* s, int* out);
 /* PyramidalFlowFilter and batches: replay each fused frame as a captured
- * HIP graph (1, the default) or launch its kernels one by one (0). Frames
- * that are a single launch (pyramids of up to three levels) are launched
- * directly either way: replaying a one-node graph costs more than the
- * launch. */
+ * HIP graph (1, the default) or launch its kernels one by one (0). A single
+ * filter's frames that are one launch (pyramids of up to three levels) are
+ * launched directly either way: replaying the graph costs more than the
+ * launch. A batch's step is replayed as a graph whenever this is on (about
+ * 1 % of a 64-stream step). */
 FF_API ff_status ff_filter_set_graphs(ff_stage* s, int graphs);
 FF_API ff_status ff_filter_get_graphs(ff_stage* s, int* out);
 /* PyramidalFlowFilter only (diagnostic, no reference counterpart): number of
