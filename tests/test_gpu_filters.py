@@ -182,25 +182,80 @@ def test_pyramidal_sequence(ffo, shape, levels, frames, fused):
     check_frame(frames, dev, o.getFlow(), "pyramidal getFlowDevice")
 
 
-def test_pyramidal_full_size_cfg2(ffo):
-    """2048x1088, 2 levels, gamma [10, 50], maxflow 4, smooth [2, 2]: the benchmark workload."""
-    h, w, frames = 1088, 2048, 10
+def _prefetched(gen):
+    """Yield the items of ``gen``, producing the next one on a worker thread
+    while the caller works on the current one (the oracle's C calls release
+    the interpreter lock, the generator's numpy work holds it)."""
+    from concurrent.futures import ThreadPoolExecutor
+    it = iter(gen)
+    with ThreadPoolExecutor(1) as pool:
+        fut = pool.submit(next, it, None)
+        while True:
+            item = fut.result()
+            if item is None:
+                return
+            fut = pool.submit(next, it, None)
+            yield item
+
+
+def test_flowfilter_full_sequence_640x480(ffo):
+    """The single-level workload over its whole sequence: 640x480, 200 frames
+    of band-limited noise (sigma 3) translating by (0.7, -0.4) px/frame;
+    gamma 10, maxflow 4, 2 smoothing iterations. Every frame bitwise equal to
+    the FP32 oracle; the FP64 oracle's error reported for every frame."""
+    import flowfilter.gpu.flowfilters as gf
+    h, w, frames = 480, 640, 200
     ffo.set_threads(16)
-    say("pyramidal full size", h, w, frames)
+    say("flowfilter full sequence", h, w, frames)
+    g = gf.FlowFilter(h, w, 2, 4.0, 10.0)
+    o = ffo.FlowFilter(h, w, 2, 4.0, 10.0)
+    o64 = ffo.FlowFilter(h, w, 2, 4.0, 10.0, prec="f64")
+    worst64 = 0.0
+    for t, img in enumerate(_prefetched(seq_noise(h, w, frames))):
+        g.loadImage(img)
+        g.compute()
+        o.compute(img)
+        o64.compute(img)
+        got = g.getFlow()
+        check_frame(t, got, o.downloadFlow(), "640x480 flow")
+        worst64 = max(worst64, float(np.abs(got - o64.downloadFlow()).max()))
+        if t % 50 == 49:
+            say("  frame", t, "ok, max |gpu - fp64 oracle| so far %.3g px" % worst64)
+    say("  max |gpu - fp64 oracle| over %d frames = %.3g px" % (frames, worst64))
+    assert worst64 < 1e-3
+
+
+def test_pyramidal_full_sequence_2048x1088(ffo):
+    """The benchmark workload over its whole sequence: 2048x1088, 300 frames
+    of 400 Gaussian blobs (sigma 2-20 px, amplitude 20-80) on mid-grey under
+    the capped affine motion (0.5 deg/frame, 1 % expansion, at most 4 px),
+    N(0, 2) sensor noise; 2 levels, gamma [10, 50], maxflow 4, smooth [2, 2].
+    Every frame bitwise equal to the FP32 oracle; the FP64 oracle runs over
+    the first 30 frames (its cost is three times the FP32 oracle's) and its
+    error is reported."""
+    from flowfilter import synthetic
+    h, w, frames, f64_frames = 1088, 2048, 300, 30
+    ffo.set_threads(16)
+    say("pyramidal full sequence", h, w, frames)
     g, o = _pyr_pair(ffo, h, w, 2, [10.0, 50.0], 4.0, [2, 2])
     o64 = ffo.PyramidalFlowFilter(h, w, 2, prec="f64")
     o64.configure_like([10.0, 50.0], 4.0, [2, 2])
     worst64 = 0.0
-    for t, img in enumerate(seq_blobs(h, w, frames)):
+    seq = synthetic.blob_affine(h, w, frames, blobs=400)
+    for t, img in enumerate(_prefetched(seq)):
         g.loadImage(img)
         g.compute()
         o.compute(img)
         got = g.getFlow()
-        check_frame(t, got, o.downloadFlow(), "cfg2 flow")
-        o64.compute(img)
-        worst64 = max(worst64, float(np.abs(got - o64.downloadFlow()).max()))
-        say("  frame", t, "ok")
-    say("  max |gpu - fp64 oracle| = %.3g px" % worst64)
+        check_frame(t, got, o.downloadFlow(), "2048x1088 flow")
+        if t < f64_frames:
+            o64.compute(img)
+            worst64 = max(worst64, float(np.abs(got - o64.downloadFlow()).max()))
+        if t % 50 == 49:
+            say("  frame", t, "ok")
+    assert t == frames - 1
+    say("  max |gpu - fp64 oracle| over the first %d frames = %.3g px" % (f64_frames, worst64))
+    assert worst64 < 1e-3
 
 
 def test_pyramidal_parameters_and_errors(ffo):
