@@ -381,7 +381,10 @@ FF_API ff_status ff_ad_propagate_pair(const float* flow, const float* scalar, co
                                       float* vector_out, int B, int H, int W, float dt, int border, void* stream);
 /* Adjoint of ff_ad_propagate_pair, given the pair's input and the X pass's
  * result (flow_mid ...); the same results as the two passes' adjoints.
- * g_*_out may be NULL (zero); accumulate as for the pass backward. */
+ * With flow_mid, scalar_mid and vector_mid all NULL the X pass's result is
+ * recomputed from the pair's input inside the launch (bitwise the same
+ * gradients; the forward then need not keep it). g_*_out may be NULL
+ * (zero); accumulate as for the pass backward. */
 FF_API ff_status ff_ad_propagate_pair_backward(const float* flow, const float* scalar, const float* vector,
                                                const float* flow_mid, const float* scalar_mid,
                                                const float* vector_mid, const float* g_flow_out,

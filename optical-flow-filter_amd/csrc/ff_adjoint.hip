@@ -425,6 +425,69 @@ __global__ void __launch_bounds__(64 * kPairBwdRows)
     }
 }
 
+// The same adjoint without the X pass's result: the workgroup stages the
+// pair's input at 64 columns of its 16 rows plus two above and two below
+// (reads clamped into the image, as the forward's) in LDS, reruns the X
+// pass on the inner 62 columns of those 20 rows (the Y pass's inputs its
+// adjoint reads), then takes the Y pass's adjoint at those 62 columns and
+// the X pass's at the inner 60. Every value is computed by the same
+// per-pixel functions from the same operands as in the forward and in
+// k_ad_prop_pair_bwd, so the results are bitwise theirs; the saved X-pass
+// result (20 bytes per pixel written forward and read back) is not needed.
+constexpr int kPairRcCols = 60, kPairRcRows = kPairBwdRows + 4;
+template <bool PAY>
+__global__ void __launch_bounds__(64 * kPairBwdRows)
+    k_ad_prop_pair_bwd_rc(const float2* fin, const float* sin, const float2* vin, const float2* gfo,
+                          const float* gso, const float2* gvo, float2* gfi, float* gsi, float2* gvi, Dims d, float dt,
+                          int border, int acc) {
+    __shared__ Px5 LF[kPairRcRows][64];  // pair input, rows y0-2 .. y0+17
+    __shared__ Px5 LM[kPairRcRows][64];  // its X pass; later the Y pass's adjoint (rows y0 .. y0+15)
+    const int tx = threadIdx.x, ty = threadIdx.y, b = blockIdx.z;
+    const int x = (int)blockIdx.x * kPairRcCols - 2 + tx;  // column of this lane
+    const int y0 = (int)blockIdx.y * kPairBwdRows, y = y0 + ty;
+    const int xc = clampi(x, 0, d.W - 1);
+    const bool xin = x >= border && x < d.W - border;
+    for (int i = ty; i < kPairRcRows; i += kPairBwdRows)
+        LF[i][tx] = load_px<PAY>(fin, sin, vin, at(b, clampi(y0 - 2 + i, 0, d.H - 1), xc, d));
+    __syncthreads();
+    // X pass: lane tx-1 / tx+1 hold the clamped reads of columns x-1 / x+1
+    if (tx >= 1 && tx <= 62) {
+        for (int i = ty; i < kPairRcRows; i += kPairBwdRows) {
+            const int yc = clampi(y0 - 2 + i, 0, d.H - 1);
+            const bool in = xin && yc >= border && yc < d.H - border;
+            LM[i][tx] = upwind_px<0, PAY>(LF[i][tx - 1], LF[i][tx], LF[i][tx + 1], dt, in);
+        }
+    }
+    __syncthreads();
+    // adjoint of the Y pass at columns x0-1 .. x0+60 (zero outside the image)
+    Px5 gm;
+    gm.f = make_float2(0.f, 0.f);
+    gm.s = 0.f;
+    gm.v = make_float2(0.f, 0.f);
+    if (tx >= 1 && tx <= 62 && x >= 0 && x < d.W && y < d.H) {
+        Px5 X[5], G[3];
+#pragma unroll
+        for (int k = 0; k < 5; ++k) X[k] = LM[ty + k][tx];  // rows clamp(y-2+k)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) G[k] = load_grad_px<PAY>(gfo, gso, gvo, at(b, clampi(y - 1 + k, 0, d.H - 1), x, d));
+        auto interior = [&](int o) { return xin && o >= border && o < d.H - border; };
+        gm = upwind_adjoint_px<1, PAY>(X, G, y, d.H, interior, dt);
+    }
+    __syncthreads();  // every read of the X pass is done
+    LM[ty][tx] = gm;
+    __syncthreads();
+    // adjoint of the X pass at columns x0 .. x0+59
+    if (tx >= 2 && tx <= 61 && x < d.W && y < d.H) {
+        Px5 X[5];
+        const Px5 G[3] = {LM[ty][tx - 1], LM[ty][tx], LM[ty][tx + 1]};
+#pragma unroll
+        for (int k = 0; k < 5; ++k) X[k] = LF[ty + 2][tx - 2 + k];  // columns clamp(x-2+k) of row y
+        const bool yin = y >= border && y < d.H - border;
+        auto interior = [&](int o) { return yin && o >= border && o < d.W - border; };
+        put_px<PAY>(gfi, gsi, gvi, at(b, y, x, d), upwind_adjoint_px<0, PAY>(X, G, x, d.W, interior, dt), acc);
+    }
+}
+
 // ----------------------------------------------------------------- update
 
 // Adjoint of the closed-form solve (update_k.cu:43-63). The solve is
@@ -703,10 +766,19 @@ void prop_pair(const float* fin, const float* sin, const float* vin, float* fmid
 void prop_pair_bwd(const float* fin, const float* sin, const float* vin, const float* fmid, const float* smid,
                    const float* vmid, const float* gfo, const float* gso, const float* gvo, float* gfi, float* gsi,
                    float* gvi, int B, int H, int W, float dt, int border, int acc, hipStream_t s) {
-    const dim3 g((W + kPairCols - 1) / kPairCols, (H + kPairBwdRows - 1) / kPairBwdRows, B), blk(64, kPairBwdRows);
     const Dims d{H, W};
     auto F = [](const float* p) { return (const float2*)p; };
     auto O = [](float* p) { return (float2*)p; };
+    if (!fmid) {  // recompute the X pass's result
+        const dim3 g((W + kPairRcCols - 1) / kPairRcCols, (H + kPairBwdRows - 1) / kPairBwdRows, B),
+            blk(64, kPairBwdRows);
+        if (sin)
+            hipLaunchKernelGGL((k_ad_prop_pair_bwd_rc<true>), g, blk, 0, s, F(fin), sin, F(vin), F(gfo), gso, F(gvo), O(gfi), gsi, O(gvi), d, dt, border, acc);
+        else
+            hipLaunchKernelGGL((k_ad_prop_pair_bwd_rc<false>), g, blk, 0, s, F(fin), sin, F(vin), F(gfo), gso, F(gvo), O(gfi), gsi, O(gvi), d, dt, border, acc);
+        return;
+    }
+    const dim3 g((W + kPairCols - 1) / kPairCols, (H + kPairBwdRows - 1) / kPairBwdRows, B), blk(64, kPairBwdRows);
     if (sin)
         hipLaunchKernelGGL((k_ad_prop_pair_bwd<true>), g, blk, 0, s, F(fin), sin, F(vin), F(fmid), smid, F(vmid), F(gfo), gso, F(gvo), O(gfi), gsi, O(gvi), d, dt, border, acc);
     else

@@ -690,10 +690,14 @@ ff_status ff_ad_propagate_pair_backward(const float* flow, const float* scalar, 
     return guard([&] {
         dims(B, H, W);
         if (border < 1) throw invalid_argument("border must be at least 1");
-        need(flow, "flow"), need(flow_mid, "flow_mid"), need(g_flow, "g_flow");
-        const bool pay = scalar != nullptr;
-        if (pay && (!vector || !scalar_mid || !vector_mid || !g_scalar || !g_vector))
-            throw invalid_argument("payloads: pass scalar, vector, their intermediates and both of their gradients");
+        need(flow, "flow"), need(g_flow, "g_flow");
+        const bool pay = scalar != nullptr, recompute = flow_mid == nullptr;
+        if (recompute && (scalar_mid || vector_mid))
+            throw invalid_argument("flow_mid is NULL (the X pass is recomputed): scalar_mid and vector_mid must be NULL too");
+        if (pay && (!vector || !g_scalar || !g_vector))
+            throw invalid_argument("payloads: pass scalar, vector and both of their gradients");
+        if (pay && !recompute && (!scalar_mid || !vector_mid))
+            throw invalid_argument("payloads: pass the X pass's scalar_mid and vector_mid with flow_mid");
         ad::prop_pair_bwd(flow, scalar, vector, flow_mid, pay ? scalar_mid : nullptr, pay ? vector_mid : nullptr,
                           g_flow_out, pay ? g_scalar_out : nullptr, pay ? g_vector_out : nullptr, g_flow, g_scalar,
                           g_vector, B, H, W, dt, border, accumulate, as_stream(stream));

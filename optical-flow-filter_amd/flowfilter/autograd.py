@@ -179,19 +179,15 @@ def _frame_forward(spec, state, image, gammas, save):
         passes = []
         pay = sc is not None
         for _ in range(lv["N"]):
-            # an X and a Y pass in one launch; the X pass's result (the Y
-            # pass's input) is kept when the adjoint will need it
-            fm = _empty(B, h, w, 2, like=image) if save else None
-            sm = _empty(B, h, w, like=image) if save and pay else None
-            vm = _empty(B, h, w, 2, like=image) if save and pay else None
+            # an X and a Y pass in one launch; the adjoint keeps the pair's
+            # input and recomputes the X pass's result itself
             fo = _empty(B, h, w, 2, like=image)
             so = _empty(B, h, w, like=image) if pay else None
             vo = _empty(B, h, w, 2, like=image) if pay else None
-            N.call("ff_ad_propagate_pair", _p(f), _p(sc), _p(vc), _p(fm), _p(sm), _p(vm), _p(fo), _p(so), _p(vo),
+            N.call("ff_ad_propagate_pair", _p(f), _p(sc), _p(vc), None, None, None, _p(fo), _p(so), _p(vo),
                    B, h, w, ctypes.c_float(lv["dt"]), spec.border, s)
             if save:
                 passes.append((f, sc, vc))
-                passes.append((fm, sm, vm))
             f, sc, vc = fo, so, vo
         prop[l] = (f, sc, vc)
         if save:
@@ -314,13 +310,13 @@ def _frame_backward(spec, saved, g_new, g_out, gammas, g_image_out):
         # not read by the delta update, so its gradient enters as zero (None)
         gf, gs, gv = None, gsp, gvp
         passes = saved["passes"][l]
-        npairs = len(passes) // 2
+        npairs = len(passes)
         for i in range(npairs):
-            (fi, si, vi), (fm, sm, vm) = passes[2 * (npairs - 1 - i)], passes[2 * (npairs - 1 - i) + 1]
+            fi, si, vi = passes[npairs - 1 - i]
             acc_f = i == npairs - 1 and "f" in g_old[l]
             nf = g_old[l]["f"] if acc_f else _empty(B, h, w, 2, like=gfl)
             ns, nv = _empty(B, h, w, like=gfl), _empty(B, h, w, 2, like=gfl)
-            N.call("ff_ad_propagate_pair_backward", _p(fi), _p(si), _p(vi), _p(fm), _p(sm), _p(vm), _p(gf), _p(gs),
+            N.call("ff_ad_propagate_pair_backward", _p(fi), _p(si), _p(vi), None, None, None, _p(gf), _p(gs),
                    _p(gv), _p(nf), _p(ns), _p(nv), B, h, w, f32(lv["dt"]), spec.border, 1 if acc_f else 0, s)
             gf, gs, gv = nf, ns, nv
         g_old[l]["f"] = gf
@@ -347,9 +343,9 @@ def _frame_backward(spec, saved, g_new, g_out, gammas, g_image_out):
     g_old[top]["a0"] = gao
     gf = gfp
     passes = saved["passes"][top]
-    for i in reversed(range(len(passes) // 2)):
+    for i in reversed(range(len(passes))):
         nf = _empty(B, h, w, 2, like=gf)
-        N.call("ff_ad_propagate_pair_backward", _p(passes[2 * i][0]), None, None, _p(passes[2 * i + 1][0]), None, None,
+        N.call("ff_ad_propagate_pair_backward", _p(passes[i][0]), None, None, None, None, None,
                _p(gf), None, None, _p(nf), None, None, B, h, w, f32(lv["dt"]), spec.border, 0, s)
         gf = nf
     g_old[top]["f"] = gf

@@ -144,11 +144,12 @@ def test_pass_pair_equals_two_passes(ad, shape, pay):
             assert torch.equal(a, b_)
 
 
-@pytest.mark.parametrize("shape", [(2, 45, 70), (1, 9, 130), (3, 33, 62), (1, 1, 5)])
+@pytest.mark.parametrize("shape", [(2, 45, 70), (1, 9, 130), (3, 33, 62), (1, 1, 5), (2, 17, 121), (1, 40, 2)])
 @pytest.mark.parametrize("pay", [False, True])
 def test_pass_pair_adjoint_equals_two_adjoints(ad, shape, pay):
     """ff_ad_propagate_pair_backward is bitwise the Y pass's adjoint followed
-    by the X pass's (zero and accumulated output gradients included)."""
+    by the X pass's (zero and accumulated output gradients included), both
+    given the X pass's result and recomputing it (NULL intermediates)."""
     B, H, W = shape
     r = rng(31 + H)
     rnd = lambda *sh, lo=-1.0, hi=1.0: cuda(r.uniform(lo, hi, size=sh).astype(np.float32))
@@ -166,12 +167,13 @@ def test_pass_pair_adjoint_equals_two_adjoints(ad, shape, pay):
         pre = ref[0].clone()
         _call(ad, "ff_ad_propagate_pass_backward", 1, *m, *g, *mid, B, H, W, dt, 2, 0)
         _call(ad, "ff_ad_propagate_pass_backward", 0, f, sc, vc, *mid, *ref, B, H, W, dt, 2, 1)
-        got = [pre.clone(), e1() if pay else None, e2() if pay else None]
-        _call(ad, "ff_ad_propagate_pair_backward", f, sc, vc, *m, *g, *got, B, H, W, dt, 2, 1)
-        torch.cuda.synchronize()
-        for a, b_ in zip(ref, got):
-            if a is not None:
-                assert torch.equal(a, b_), zero_flow
+        for mids in (m, [None, None, None]):
+            got = [pre.clone(), e1() if pay else None, e2() if pay else None]
+            _call(ad, "ff_ad_propagate_pair_backward", f, sc, vc, *mids, *g, *got, B, H, W, dt, 2, 1)
+            torch.cuda.synchronize()
+            for a, b_ in zip(ref, got):
+                if a is not None:
+                    assert torch.equal(a, b_), (zero_flow, mids[0] is None)
 
 
 def ctypes_f(x):
@@ -504,3 +506,11 @@ def test_errors(ad):
         ad.FilterSpec(32, 32, 2, border=0)
     with pytest.raises(ValueError):
         ad.flow_window(torch.zeros(1, 2, 32, 32, device="cuda"), torch.ones(2, device="cuda"), spec, checkpoint=0)
+    # pair adjoint: intermediates all given or all NULL (recomputed)
+    f, gf = torch.zeros(1, 8, 8, 2, device="cuda"), torch.zeros(1, 8, 8, 2, device="cuda")
+    sc, gs = torch.zeros(1, 8, 8, device="cuda"), torch.zeros(1, 8, 8, device="cuda")
+    dt = ctypes_f(0.25)
+    with pytest.raises(ValueError):
+        _call(ad, "ff_ad_propagate_pair_backward", f, sc, f, None, sc, None, gf, gs, gf, gf, gs, gf, 1, 8, 8, dt, 1, 0)
+    with pytest.raises(ValueError):
+        _call(ad, "ff_ad_propagate_pair_backward", f, sc, f, f, None, None, gf, gs, gf, gf, gs, gf, 1, 8, 8, dt, 1, 0)
