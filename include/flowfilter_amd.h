@@ -431,6 +431,15 @@ FF_API ff_status ff_ad_upper_backward(const float* g_upper_pixel, const int* xlo
 FF_API ff_status ff_ad_smooth_pass(int axis, const float* flow, float* flow_out, int B, int H, int W, void* stream);
 FF_API ff_status ff_ad_smooth_pass_backward(int axis, const float* g_flow_out, const float* g_flow_out2, float* g_flow,
                                             int B, int H, int W, int accumulate, void* stream);
+/* One smoothing iteration, an X pass and then a Y pass, in one launch
+ * (flowsmoothing_k.cu:18-87 as FlowSmoother runs it, flowsmoothing.cu:83-104;
+ * the same results as two ff_ad_smooth_pass calls). flow_out must not be
+ * flow. Backward: the Y pass's adjoint and then the X pass's, for the
+ * output gradient g_flow_out + g_flow_out2 (g_flow_out2 may be NULL); g_flow
+ * must be neither of them. */
+FF_API ff_status ff_ad_smooth_pair(const float* flow, float* flow_out, int B, int H, int W, void* stream);
+FF_API ff_status ff_ad_smooth_pair_backward(const float* g_flow_out, const float* g_flow_out2, float* g_flow, int B,
+                                            int H, int W, int accumulate, void* stream);
 
 /* Kernel profiling (no reference counterpart; the reference times whole
  * stages only, src/gpu/pipeline.cu:63-89). Between begin and end, every

@@ -705,6 +705,80 @@ __global__ void __launch_bounds__(BX* BY)
     put2(gin, at(b, y, x, d), s, acc & 1);
 }
 
+// One smoothing iteration, an X and then a Y pass, in one launch: a
+// workgroup of 64 x 16 threads computes the X pass on 16 rows (clamped into
+// the image) into LDS and the Y pass on the inner 12, reading and writing
+// the flow once instead of twice. Same arithmetic as two k_ad_smooth
+// launches.
+constexpr int kSmoothRows = 12;
+__global__ void __launch_bounds__(64 * (kSmoothRows + 4)) k_ad_smooth_pair(const float2* in, float2* out, Dims d) {
+    __shared__ float2 L[kSmoothRows + 4][64];
+    const int tx = threadIdx.x, ty = threadIdx.y, b = blockIdx.z;
+    const int x = (int)blockIdx.x * 64 + tx;
+    const int yr = (int)blockIdx.y * kSmoothRows - 2 + ty;  // row of this thread's X pass
+    const int yc = clampi(yr, 0, d.H - 1);
+    if (x < d.W) {
+        float sx = 0.0f, sy = 0.0f;
+#pragma unroll
+        for (int c = -2; c <= 2; ++c) {
+            const float2 v = in[at(b, yc, clampi(x + c, 0, d.W - 1), d)];
+            sx = fmaf(kBox, v.x, sx);
+            sy = fmaf(kBox, v.y, sy);
+        }
+        L[ty][tx] = make_float2(sx, sy);
+    }
+    __syncthreads();
+    // row yr reads rows clamp(yr + c): staged rows ty + c
+    if (x < d.W && ty >= 2 && ty < kSmoothRows + 2 && yr < d.H) {
+        float sx = 0.0f, sy = 0.0f;
+#pragma unroll
+        for (int c = -2; c <= 2; ++c) {
+            const float2 v = L[ty + c][tx];
+            sx = fmaf(kBox, v.x, sx);
+            sy = fmaf(kBox, v.y, sy);
+        }
+        out[at(b, yr, x, d)] = make_float2(sx, sy);
+    }
+}
+
+// Adjoint of k_ad_smooth_pair for the output gradient gout + gout2 (gout2
+// may be null): a workgroup of 64 x 16 threads takes the Y pass's adjoint
+// at 64 columns (its 60 output columns and two on each side) of 16 rows
+// into LDS, then the X pass's adjoint at the inner 60. Same arithmetic as
+// k_ad_smooth_bwd<1> followed by k_ad_smooth_bwd<0>.
+constexpr int kSmoothBwdCols = 60, kSmoothBwdRows = 16;
+__global__ void __launch_bounds__(64 * kSmoothBwdRows)
+    k_ad_smooth_pair_bwd(const float2* gout, const float2* gout2, float2* gin, Dims d, int acc) {
+    __shared__ float2 L[kSmoothBwdRows][64];
+    const int tx = threadIdx.x, ty = threadIdx.y, b = blockIdx.z;
+    const int x = (int)blockIdx.x * kSmoothBwdCols - 2 + tx;
+    const int y = (int)blockIdx.y * kSmoothBwdRows + ty;
+    float2 s = make_float2(0.f, 0.f);
+    if (x >= 0 && x < d.W && y < d.H) {
+        auto g = [&](size_t i) {
+            float2 v = gout[i];
+            if (gout2) {
+                const float2 u = gout2[i];
+                v.x += u.x;
+                v.y += u.y;
+            }
+            return v;
+        };
+        s.x = corr5_adjoint(y, d.H, WB{}, [&](int i) { return g(at(b, i, x, d)).x; });
+        s.y = corr5_adjoint(y, d.H, WB{}, [&](int i) { return g(at(b, i, x, d)).y; });
+    }
+    L[ty][tx] = s;
+    __syncthreads();
+    // column i of the image is lane tx + (i - x); the adjoint reads
+    // columns x-2 .. x+2 inside the image only
+    if (tx >= 2 && tx < kSmoothBwdCols + 2 && x < d.W && y < d.H) {
+        float2 r;
+        r.x = corr5_adjoint(x, d.W, WB{}, [&](int i) { return L[ty][tx + (i - x)].x; });
+        r.y = corr5_adjoint(x, d.W, WB{}, [&](int i) { return L[ty][tx + (i - x)].y; });
+        put2(gin, at(b, y, x, d), r, acc & 1);
+    }
+}
+
 }  // namespace
 
 // ------------------------------------------------------------- launchers
@@ -838,6 +912,16 @@ void upper_gather(const float* gup, const int* xlo, const int* xhi, const int* y
 void smooth(int axis, const float* in, float* out, int B, int H, int W, hipStream_t s) {
     if (axis == 0) hipLaunchKernelGGL((k_ad_smooth<0>), grid3(B, H, W), dim3(BX, BY), 0, s, (const float2*)in, (float2*)out, Dims{H, W});
     else hipLaunchKernelGGL((k_ad_smooth<1>), grid3(B, H, W), dim3(BX, BY), 0, s, (const float2*)in, (float2*)out, Dims{H, W});
+}
+void smooth_pair(const float* in, float* out, int B, int H, int W, hipStream_t s) {
+    const dim3 g((W + 63) / 64, (H + kSmoothRows - 1) / kSmoothRows, B), blk(64, kSmoothRows + 4);
+    hipLaunchKernelGGL(k_ad_smooth_pair, g, blk, 0, s, (const float2*)in, (float2*)out, Dims{H, W});
+}
+void smooth_pair_bwd(const float* gout, const float* gout2, float* gin, int B, int H, int W, int acc, hipStream_t s) {
+    const dim3 g((W + kSmoothBwdCols - 1) / kSmoothBwdCols, (H + kSmoothBwdRows - 1) / kSmoothBwdRows, B),
+        blk(64, kSmoothBwdRows);
+    hipLaunchKernelGGL(k_ad_smooth_pair_bwd, g, blk, 0, s, (const float2*)gout, (const float2*)gout2, (float2*)gin,
+                       Dims{H, W}, acc);
 }
 void smooth_bwd(int axis, const float* gout, const float* gout2, float* gin, int B, int H, int W, int acc,
                 hipStream_t s) {

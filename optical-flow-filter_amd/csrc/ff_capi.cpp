@@ -815,6 +815,27 @@ ff_status ff_ad_smooth_pass_backward(int axis, const float* g_flow_out, const fl
     });
 }
 
+ff_status ff_ad_smooth_pair(const float* flow, float* flow_out, int B, int H, int W, void* stream) {
+    return guard([&] {
+        dims(B, H, W);
+        need(flow, "flow"), need(flow_out, "flow_out");
+        if (flow == flow_out) throw invalid_argument("flow_out must not be flow (the passes read neighbours)");
+        ad::smooth_pair(flow, flow_out, B, H, W, as_stream(stream));
+        check_launch("ff_ad_smooth_pair");
+    });
+}
+ff_status ff_ad_smooth_pair_backward(const float* g_flow_out, const float* g_flow_out2, float* g_flow, int B, int H,
+                                     int W, int accumulate, void* stream) {
+    return guard([&] {
+        dims(B, H, W);
+        need(g_flow_out, "g_flow_out"), need(g_flow, "g_flow");
+        if (g_flow == g_flow_out || g_flow == g_flow_out2)
+            throw invalid_argument("g_flow must not be an output gradient (the adjoint reads neighbours)");
+        ad::smooth_pair_bwd(g_flow_out, g_flow_out2, g_flow, B, H, W, accumulate != 0 ? 1 : 0, as_stream(stream));
+        check_launch("ff_ad_smooth_pair_backward");
+    });
+}
+
 // --------------------------------------------------------------- profiling
 ff_status ff_profile_begin(const char* filter) {
     return guard([&] { profile_begin(filter ? filter : ""); });

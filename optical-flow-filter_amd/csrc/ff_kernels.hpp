@@ -203,6 +203,9 @@ void delta_update_bwd(const float* a0, const float* g, const float* sprop, const
 void upper_gather(const float* gup, const int* xlo, const int* xhi, const int* ylo, const int* yhi, float* gupper,
                   int B, int H, int W, int Hu, int Wu, int acc, hipStream_t s);
 void smooth(int axis, const float* in, float* out, int B, int H, int W, hipStream_t s);
+void smooth_pair(const float* in, float* out, int B, int H, int W, hipStream_t s);
+void smooth_pair_bwd(const float* gout, const float* gout2, float* gin, int B, int H, int W, int acc,
+                     hipStream_t s);
 void smooth_bwd(int axis, const float* gout, const float* gout2, float* gin, int B, int H, int W, int acc,
                 hipStream_t s);
 }  // namespace ad

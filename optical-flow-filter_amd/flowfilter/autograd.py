@@ -225,11 +225,10 @@ def _frame_forward(spec, state, image, gammas, save):
 
 
 def _smooth(f, K, B, h, w, s):
-    for _ in range(K):
-        for axis in (0, 1):
-            o = torch.empty_like(f)
-            N.call("ff_ad_smooth_pass", axis, _p(f), _p(o), B, h, w, s)
-            f = o
+    for _ in range(K):  # an X and a Y pass per launch
+        o = torch.empty_like(f)
+        N.call("ff_ad_smooth_pair", _p(f), _p(o), B, h, w, s)
+        f = o
     return f
 
 
@@ -246,13 +245,11 @@ def _smooth_bwd(parts, K, B, h, w, s, out=None):
             return total
         out.add_(total)
         return out
-    n = 2 * K
     g, g2 = parts[0], (parts[1] if len(parts) > 1 else None)
-    for i, axis in enumerate([1, 0] * K):
-        last = i == n - 1
+    for i in range(K):  # the Y pass's and then the X pass's adjoint per launch
+        last = i == K - 1
         o = out if (last and out is not None) else torch.empty_like(g)
-        N.call("ff_ad_smooth_pass_backward", axis, _p(g), _p(g2), _p(o), B, h, w, 1 if (last and out is not None) else 0,
-               s)
+        N.call("ff_ad_smooth_pair_backward", _p(g), _p(g2), _p(o), B, h, w, 1 if (last and out is not None) else 0, s)
         g, g2 = o, None
     return g
 

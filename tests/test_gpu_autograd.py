@@ -176,6 +176,34 @@ def test_pass_pair_adjoint_equals_two_adjoints(ad, shape, pay):
                     assert torch.equal(a, b_), (zero_flow, mids[0] is None)
 
 
+@pytest.mark.parametrize("shape", [(2, 45, 70), (1, 13, 130), (3, 33, 61), (1, 1, 5), (2, 4, 3), (1, 40, 2)])
+def test_smooth_pair_equals_two_passes(ad, shape):
+    """ff_ad_smooth_pair is bitwise an X pass then a Y pass, and its adjoint
+    bitwise the Y pass's adjoint then the X pass's (with a second output
+    gradient, and accumulating)."""
+    B, H, W = shape
+    r = rng(41 + H + W)
+    rnd = lambda *sh: cuda(r.uniform(-2, 2, size=sh).astype(np.float32))
+    e2 = lambda: torch.empty(B, H, W, 2, device="cuda")
+    f = rnd(B, H, W, 2)
+    mid, ref, got = e2(), e2(), e2()
+    _call(ad, "ff_ad_smooth_pass", 0, f, mid, B, H, W)
+    _call(ad, "ff_ad_smooth_pass", 1, mid, ref, B, H, W)
+    _call(ad, "ff_ad_smooth_pair", f, got, B, H, W)
+    torch.cuda.synchronize()
+    assert torch.equal(ref, got)
+    for second in (False, True):
+        for acc in (0, 1):
+            g, g2 = rnd(B, H, W, 2), (rnd(B, H, W, 2) if second else None)
+            base = rnd(B, H, W, 2)
+            ref, got = base.clone(), base.clone()
+            _call(ad, "ff_ad_smooth_pass_backward", 1, g, g2, mid, B, H, W, 0)
+            _call(ad, "ff_ad_smooth_pass_backward", 0, mid, None, ref, B, H, W, acc)
+            _call(ad, "ff_ad_smooth_pair_backward", g, g2, got, B, H, W, acc)
+            torch.cuda.synchronize()
+            assert torch.equal(ref, got), (second, acc)
+
+
 def ctypes_f(x):
     import ctypes
     return ctypes.c_float(x)
@@ -514,3 +542,8 @@ def test_errors(ad):
         _call(ad, "ff_ad_propagate_pair_backward", f, sc, f, None, sc, None, gf, gs, gf, gf, gs, gf, 1, 8, 8, dt, 1, 0)
     with pytest.raises(ValueError):
         _call(ad, "ff_ad_propagate_pair_backward", f, sc, f, f, None, None, gf, gs, gf, gf, gs, gf, 1, 8, 8, dt, 1, 0)
+    # smoothing pairs read neighbours: no output may be an input
+    with pytest.raises(ValueError):
+        _call(ad, "ff_ad_smooth_pair", f, f, 1, 8, 8)
+    with pytest.raises(ValueError):
+        _call(ad, "ff_ad_smooth_pair_backward", gf, None, gf, 1, 8, 8, 0)
