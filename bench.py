@@ -166,12 +166,14 @@ def batched_bench(args, N, gf, GPUImage):
         res[mode] = px * args.batched_steps / (ms / 1e3) / 1e6
         if batch is not None:
             res["blocks"] = batch.lastBlocks()
+            res["tile"] = {"waves": batch.tileWaves, "top_rows": batch.tileTopRows}
             res["timeouts"] = batch.waitTimeouts()
         print("%s: %.3f ms/step, %.1f Mpix/s" % (mode, ms / args.batched_steps, res[mode]), file=sys.stderr,
               flush=True)
     return {"streams": len(streams), "mpix_per_step": round(px / 1e6, 3), "steps": args.batched_steps,
             "value": round(res["batched"], 3), "plain_value": round(res["plain"], 3), "unit": "Mpix/s",
             "speedup": round(res["batched"] / res["plain"], 3), "launch_blocks": res["blocks"],
+            "tile": res["tile"],
             "wait_timeouts": res["timeouts"],
             "workload": "64 camera streams (seed 3): 320x240..1920x1080, 1-3 levels, gamma U[5,50], "
                         "maxflow 4, smooth 1-2, affine motion; step = load 64 frames from HBM + advance 64 filters"}
@@ -410,6 +412,7 @@ def main():
                                "smooth [2,2], u8 frames, step = compute() of a frame read in place from HBM "
                                "(setImageDevice)",
                    "path": "staged" if args.staged else "fused",
+                   "tile": None if args.staged else {"waves": f.tileWaves, "top_rows": f.tileTopRows},
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,

@@ -312,6 +312,14 @@ FF_API ff_status ff_filter_get_wait_timeouts(ff_stage* s, unsigned long long* ou
  * DeltaFlowFilter, PyramidalFlowFilter and batches. */
 FF_API ff_status ff_filter_set_tile_waves(ff_stage* s, int waves);
 FF_API ff_status ff_filter_get_tile_waves(ff_stage* s, int* out);
+/* Rows per thread of the fused kernel's tiles on a pyramid's top level
+ * (which reads the input frame reduced while loading; no reference
+ * counterpart): 0 (the default) tunes it with the tile height, 4 (more,
+ * shorter tiles, which the level below waits for less on) or 8 (less halo)
+ * fixes it. Results do not depend on it. get returns the fixed or tuned
+ * value (0 before the first fused frame). Same filters as above. */
+FF_API ff_status ff_filter_set_tile_top_rows(ff_stage* s, int rows);
+FF_API ff_status ff_filter_get_tile_top_rows(ff_stage* s, int* out);
 
 /* Batched filters (no reference counterpart; replaces a caller's loop of
  * PyramidalFlowFilter::compute over camera streams, e.g. the per-camera

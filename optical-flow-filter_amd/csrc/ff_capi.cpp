@@ -567,6 +567,15 @@ ff_status ff_filter_get_tile_waves(ff_stage* s, int* out) {
         on_fused_filter(s, [&](auto& f) { *out = f.tileWaves(); });
     });
 }
+ff_status ff_filter_set_tile_top_rows(ff_stage* s, int rows) {
+    return guard([&] { on_fused_filter(s, [&](auto& f) { f.setTileTopRows(rows); }); });
+}
+ff_status ff_filter_get_tile_top_rows(ff_stage* s, int* out) {
+    return guard([&] {
+        out_ptr(out, "rows");
+        on_fused_filter(s, [&](auto& f) { *out = f.tileTopRows(); });
+    });
+}
 
 // ------------------------------------------------------------------ batches
 ff_status ff_batch_create(ff_stage** out) {

@@ -50,6 +50,8 @@ public:
     bool fused() const { return fused_; }
     void setTileWaves(int nw) { frame_.setWaves(nw); }
     int tileWaves() const { return frame_.waves(); }
+    void setTileTopRows(int rows) { frame_.setTopRows(rows); }
+    int tileTopRows() const { return frame_.topRows(); }
 
     // Used by PyramidalFlowFilter: the same work without the timer.
     void runImageModel();
@@ -103,6 +105,8 @@ public:
     void setFused(bool f) { fused_ = f; }
     void setTileWaves(int nw) { frame_.setWaves(nw); }
     int tileWaves() const { return frame_.waves(); }
+    void setTileTopRows(int rows) { frame_.setTopRows(rows); }
+    int tileTopRows() const { return frame_.topRows(); }
 
     void runImageModel();
     void runPropagation();
@@ -198,6 +202,8 @@ public:
     // Waves per workgroup of the fused frame kernel: 0 = autotune.
     void setTileWaves(int nw) { frame_.setWaves(nw); }
     int tileWaves() const { return frame_.waves(); }
+    void setTileTopRows(int rows) { frame_.setTopRows(rows); }
+    int tileTopRows() const { return frame_.topRows(); }
 
 private:
     friend class BatchedFlowFilter;
@@ -250,6 +256,8 @@ public:
     unsigned long long waitTimeouts() const { return frame_.timeouts(); }
     void setTileWaves(int nw) { frame_.setWaves(nw); }
     int tileWaves() const { return frame_.waves(); }
+    void setTileTopRows(int rows) { frame_.setTopRows(rows); }
+    int tileTopRows() const { return frame_.topRows(); }
     // Workgroups of the last fused launch (0 before the first).
     int lastBlocks() const { return last_blocks_; }
 

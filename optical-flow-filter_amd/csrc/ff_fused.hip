@@ -851,13 +851,16 @@ struct GroupEnds {
     int e[6];
 };
 
-// Rows per thread of a tile body: 4 for top levels reduced on load (the
-// top level of a pyramid, which the level below waits for: more, shorter
-// tiles finish sooner, which pyramids launched alone need; a 64-stream batch
-// is 1.5 % faster with 8), 8 otherwise.
-constexpr int kRowsTop = 4;
+// Rows per thread of a tile body: 8, except for top levels reduced on load
+// (the top level of a pyramid, which the level below waits for), which use
+// RTOP, a parameter of the kernel chosen per launch by the tuner among
+// kTopRowChoices: with 4 the tiles are more and shorter and finish sooner
+// (a pyramid launched alone is faster), with 8 they carry less halo (a
+// 64-stream batch is 1.4 % faster).
 constexpr int kRows = 8;
+constexpr int kTopRowChoices[] = {4, 8};
 
+template <int RTOP>
 __global__ void __launch_bounds__(1024)
     k_fused_frame(const LevelJob* __restrict__ jobs, const int* __restrict__ block_job, unsigned* counters,
                   unsigned epoch, int timeout_idx, GroupEnds ends, int nw, const unsigned char* frame,
@@ -870,8 +873,8 @@ __global__ void __launch_bounds__(1024)
     const ImgSrc fr{frame, frame_pitch};
     if (b < (unsigned)ends.e[0]) level_tile<false, IMG_F32, kRows>(p, tile, xch, nw, counters, epoch, timeout_idx, fr);
     else if (b < (unsigned)ends.e[1]) level_tile<false, IMG_U8, kRows>(p, tile, xch, nw, counters, epoch, timeout_idx, fr);
-    else if (b < (unsigned)ends.e[2]) level_tile<false, IMG_U8_DOWN, kRowsTop>(p, tile, xch, nw, counters, epoch, timeout_idx, fr);
-    else if (b < (unsigned)ends.e[3]) level_tile<false, IMG_U8_DOWN2, kRowsTop>(p, tile, xch, nw, counters, epoch, timeout_idx, fr);
+    else if (b < (unsigned)ends.e[2]) level_tile<false, IMG_U8_DOWN, RTOP>(p, tile, xch, nw, counters, epoch, timeout_idx, fr);
+    else if (b < (unsigned)ends.e[3]) level_tile<false, IMG_U8_DOWN2, RTOP>(p, tile, xch, nw, counters, epoch, timeout_idx, fr);
     else if (b < (unsigned)ends.e[4]) level_tile<true, IMG_F32, kRows>(p, tile, xch, nw, counters, epoch, timeout_idx, fr);
     else if (b < (unsigned)ends.e[5]) level_tile<true, IMG_U8, kRows>(p, tile, xch, nw, counters, epoch, timeout_idx, fr);
     else level_tile<true, IMG_U8_DOWN, kRows>(p, tile, xch, nw, counters, epoch, timeout_idx, fr);
@@ -888,10 +891,12 @@ constexpr int kMaxWaves = 16;  // waves per workgroup, at most
 // CU.
 constexpr int kWaveChoices[] = {16, 15, 8, 5, 4};
 
+// Whether a job runs a top-level body reduced on load (rows per thread
+// RTOP).
+bool top_rows_body(const LevelArgs& a) { return !a.delta && (a.img_mode == IMG_U8_DOWN || a.img_mode == IMG_U8_DOWN2); }
+
 // Rows per thread of a job's tile body (see k_fused_frame).
-int rows_per_thread(const LevelArgs& a) {
-    return (!a.delta && (a.img_mode == IMG_U8_DOWN || a.img_mode == IMG_U8_DOWN2)) ? kRowsTop : kRows;
-}
+int rows_per_thread(const LevelArgs& a, int rtop) { return top_rows_body(a) ? rtop : kRows; }
 
 size_t lds_bytes(int nw) { return (size_t)2 * nw * 10 * LANES * sizeof(float); }
 
@@ -926,7 +931,7 @@ bool fused_level_supported(const LevelArgs& a) {
     int H, W;
     level_size(a, H, W);
     if (H < 1 || W < 1) return false;
-    if (!tiles_with(a, kMaxWaves * rows_per_thread(a))) return false;
+    if (!tiles_with(a, kMaxWaves * rows_per_thread(a, kTopRowChoices[0]))) return false;
     if (!a.img.data || !a.flow_s_in.data || !a.a0_in.data || !a.flow_s_out.data || !a.a0_out.data) return false;
     switch (a.img_mode) {
     case IMG_F32:
@@ -987,6 +992,7 @@ struct FusedFrame::Impl {
         DeviceArray dev;
         int njobs = 0, blocks = 0, nw = 0;
         GroupEnds ends{};
+        int rtop = 0;
         size_t map_offset = 0;
         unsigned long long token = 0;
     };
@@ -1002,16 +1008,16 @@ struct FusedFrame::Impl {
     // table they count for (0: clear before the next launch)
     unsigned epoch[2] = {0, 0};
     unsigned long long counted[2] = {0, 0};
-    int fixed = 0;  // waves per workgroup, 0: autotune
+    int fixed = 0, fixed_rtop = 0;  // waves per workgroup, top-level rows per thread; 0: autotune
     std::vector<int> tuned_for;
-    int tuned = 0;
+    int tuned = 0, tuned_rtop = 0;
 
-    int table(const std::vector<LevelArgs>& jobs, int nw);
+    int table(const std::vector<LevelArgs>& jobs, int nw, int rtop);
     unsigned* region(int si) const { return (unsigned*)counters.data() + (size_t)si * 2 * npairs; }
     int timeout_word() const { return 4 * npairs; }
     void clear_counters(int si, hipStream_t s) const;
     void run(int t, hipStream_t s, const void* frame = nullptr, int frame_pitch = 0);
-    int tune(const std::vector<LevelArgs>& jobs, hipStream_t s);
+    void tune(const std::vector<LevelArgs>& jobs, hipStream_t s);
 };
 
 FusedFrame::FusedFrame() : impl_(new Impl) {}
@@ -1022,10 +1028,11 @@ FusedFrame& FusedFrame::operator=(FusedFrame&& o) noexcept {
     return *this;
 }
 
-// Job table of `jobs` tiled with nw waves per workgroup: found among the two
-// resident tables or built and uploaded (after a device synchronization,
-// since the slot it replaces may be in use).
-int FusedFrame::Impl::table(const std::vector<LevelArgs>& jobs, int nw) {
+// Job table of `jobs` tiled with nw waves per workgroup and rtop rows per
+// thread on reduced top levels: found among the two resident tables or
+// built and uploaded (after a device synchronization, since the slot it
+// replaces may be in use).
+int FusedFrame::Impl::table(const std::vector<LevelArgs>& jobs, int nw, int rtop) {
     // launch order: grouped by tile body, stable within a group; every top
     // level precedes every delta level, so producers precede consumers
     std::vector<int> order(jobs.size()), pos(jobs.size());
@@ -1038,7 +1045,7 @@ int FusedFrame::Impl::table(const std::vector<LevelArgs>& jobs, int nw) {
     int ends[kGroups - 1] = {0, 0, 0, 0, 0, 0};
     for (size_t i = 0; i < jobs.size(); ++i) {
         const LevelArgs& a = jobs[order[i]];
-        const int rows = nw * rows_per_thread(a);
+        const int rows = nw * rows_per_thread(a, rtop);
         if (!fused_level_supported(a) || !tiles_with(a, rows))
             throw logic_error("fused frame: unsupported level parameters");
         LevelJob& p = rec[i];
@@ -1113,6 +1120,7 @@ int FusedFrame::Impl::table(const std::vector<LevelArgs>& jobs, int nw) {
         key.insert(key.end(), e, e + sizeof(int));
     }
     key.insert(key.end(), reinterpret_cast<const char*>(&nw), reinterpret_cast<const char*>(&nw) + sizeof(int));
+    key.insert(key.end(), reinterpret_cast<const char*>(&rtop), reinterpret_cast<const char*>(&rtop) + sizeof(int));
 
     for (int i = 0; i < 2; ++i)
         if (slot[i].key == key) return i;
@@ -1135,6 +1143,7 @@ int FusedFrame::Impl::table(const std::vector<LevelArgs>& jobs, int nw) {
     t.njobs = (int)rec.size();
     t.blocks = blocks;
     t.nw = nw;
+    t.rtop = rtop;
     t.map_offset = map_offset;
     for (int g = 0; g < kGroups - 1; ++g) t.ends.e[g] = ends[g];
     t.token = ++uploads;
@@ -1151,9 +1160,12 @@ void FusedFrame::Impl::run(int ti, hipStream_t s, const void* frame, int frame_p
     const int si = ti & 1;
     const Table& t = slot[si];
     if (!t.blocks) throw logic_error("fused frame: launch of an unprepared table");
-    static const hipError_t lds_ok = hipFuncSetAttribute(
-        (const void*)k_fused_frame, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes(kMaxWaves));
-    if (lds_ok != hipSuccess) throw_hip(lds_ok, "hipFuncSetAttribute", __FILE__, __LINE__);
+    static const hipError_t lds_ok4 = hipFuncSetAttribute(
+        (const void*)k_fused_frame<4>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes(kMaxWaves));
+    static const hipError_t lds_ok8 = hipFuncSetAttribute(
+        (const void*)k_fused_frame<8>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes(kMaxWaves));
+    if (lds_ok4 != hipSuccess) throw_hip(lds_ok4, "hipFuncSetAttribute", __FILE__, __LINE__);
+    if (lds_ok8 != hipSuccess) throw_hip(lds_ok8, "hipFuncSetAttribute", __FILE__, __LINE__);
     // Sequence number of this launch for its slot's completion counters. A
     // launch recorded into a graph clears them itself and counts as the
     // first, so every replay is self-contained; direct launches count up
@@ -1178,38 +1190,52 @@ void FusedFrame::Impl::run(int ti, hipStream_t s, const void* frame, int frame_p
     const char* base = (const char*)t.dev.data();
     // the timeout word, relative to this slot's region
     const int timeout_idx = timeout_word() - si * 2 * npairs;
-    hipLaunchKernelGGL(k_fused_frame, dim3(t.blocks), dim3(t.nw * LANES), lds_bytes(t.nw), s, (const LevelJob*)base,
-                       (const int*)(base + t.map_offset), region(si), e, timeout_idx, t.ends, t.nw,
-                       (const unsigned char*)frame, frame_pitch);
+    if (t.rtop == 8)
+        hipLaunchKernelGGL(k_fused_frame<8>, dim3(t.blocks), dim3(t.nw * LANES), lds_bytes(t.nw), s,
+                           (const LevelJob*)base, (const int*)(base + t.map_offset), region(si), e, timeout_idx, t.ends,
+                           t.nw, (const unsigned char*)frame, frame_pitch);
+    else
+        hipLaunchKernelGGL(k_fused_frame<4>, dim3(t.blocks), dim3(t.nw * LANES), lds_bytes(t.nw), s,
+                           (const LevelJob*)base, (const int*)(base + t.map_offset), region(si), e, timeout_idx, t.ends,
+                           t.nw, (const unsigned char*)frame, frame_pitch);
 }
 
-// Time the launches of every candidate wave count on the actual job table
-// and keep the fastest. A launch reads the current state and writes the
-// next one without committing it, so repeated launches recompute the same
-// output; every tiling gives bitwise the same results.
-int FusedFrame::Impl::tune(const std::vector<LevelArgs>& jobs, hipStream_t s) {
-    int best = 0;
+// Time the launches of every candidate tiling (waves per workgroup and, when
+// a job runs a reduced top-level body, its rows per thread) on the actual
+// job table and keep the fastest; a fixed choice is not varied. A launch
+// reads the current state and writes the next one without committing it, so
+// repeated launches recompute the same output; every tiling gives bitwise
+// the same results.
+void FusedFrame::Impl::tune(const std::vector<LevelArgs>& jobs, hipStream_t s) {
+    bool reduced_top = false;
+    for (const LevelArgs& a : jobs) reduced_top = reduced_top || top_rows_body(a);
+    int best_nw = 0, best_rtop = 0;
     float best_ms = 0.0f;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     FF_HIP(hipEventCreate(&e0));
     FF_HIP(hipEventCreate(&e1));
     try {
-        for (int nw : kWaveChoices) {
-            bool ok = true;
-            for (const LevelArgs& a : jobs) ok = ok && tiles_with(a, nw * rows_per_thread(a));
-            if (!ok) continue;
-            const int t = table(jobs, nw);
-            run(t, s);  // warm-up
-            constexpr int kReps = 3;
-            FF_HIP(hipEventRecord(e0, s));
-            for (int r = 0; r < kReps; ++r) run(t, s);
-            FF_HIP(hipEventRecord(e1, s));
-            FF_HIP(hipEventSynchronize(e1));
-            float ms = 0.0f;
-            FF_HIP(hipEventElapsedTime(&ms, e0, e1));
-            if (!best || ms < best_ms) {
-                best = nw;
-                best_ms = ms;
+        for (int rtop : kTopRowChoices) {
+            if (fixed_rtop ? rtop != fixed_rtop : (!reduced_top && rtop != kTopRowChoices[0])) continue;
+            for (int nw : kWaveChoices) {
+                if (fixed && nw != fixed) continue;
+                bool ok = true;
+                for (const LevelArgs& a : jobs) ok = ok && tiles_with(a, nw * rows_per_thread(a, rtop));
+                if (!ok) continue;
+                const int t = table(jobs, nw, rtop);
+                run(t, s);  // warm-up
+                constexpr int kReps = 3;
+                FF_HIP(hipEventRecord(e0, s));
+                for (int r = 0; r < kReps; ++r) run(t, s);
+                FF_HIP(hipEventRecord(e1, s));
+                FF_HIP(hipEventSynchronize(e1));
+                float ms = 0.0f;
+                FF_HIP(hipEventElapsedTime(&ms, e0, e1));
+                if (!best_nw || ms < best_ms) {
+                    best_nw = nw;
+                    best_rtop = rtop;
+                    best_ms = ms;
+                }
             }
         }
     } catch (...) {
@@ -1219,22 +1245,27 @@ int FusedFrame::Impl::tune(const std::vector<LevelArgs>& jobs, hipStream_t s) {
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    if (!best) throw logic_error("fused frame: no tiling fits the levels");
-    return best;
+    if (!best_nw) throw logic_error("fused frame: no tiling fits the levels");
+    tuned = best_nw;
+    tuned_rtop = best_rtop;
 }
 
 int FusedFrame::prepare(const std::vector<LevelArgs>& jobs, hipStream_t s) {
     if (jobs.empty()) throw invalid_argument("fused frame: no levels");
-    int nw = impl_->fixed;
-    if (!nw) {
+    int nw = impl_->fixed, rtop = impl_->fixed_rtop;
+    if (!nw || !rtop) {
+        // tuned per set of level shapes and per fixed part of the choice
         std::vector<int> k = tune_key(jobs);
+        k.push_back(impl_->fixed);
+        k.push_back(impl_->fixed_rtop);
         if (k != impl_->tuned_for || !impl_->tuned) {
-            impl_->tuned = impl_->tune(jobs, s);
+            impl_->tune(jobs, s);
             impl_->tuned_for = std::move(k);
         }
         nw = impl_->tuned;
+        rtop = impl_->tuned_rtop;
     }
-    return impl_->table(jobs, nw);
+    return impl_->table(jobs, nw, rtop);
 }
 
 void FusedFrame::launch(int table, hipStream_t s, const void* frame, size_t frame_pitch) {
@@ -1257,6 +1288,17 @@ void FusedFrame::setWaves(int nw) {
 }
 
 int FusedFrame::waves() const { return impl_->fixed ? impl_->fixed : impl_->tuned; }
+
+void FusedFrame::setTopRows(int rows) {
+    bool ok = rows == 0;
+    for (int c : kTopRowChoices) ok = ok || c == rows;
+    if (!ok)
+        throw invalid_argument("fused frame: top-level rows per thread must be 0 (autotune), 4 or 8, got " +
+                               std::to_string(rows));
+    impl_->fixed_rtop = rows;
+}
+
+int FusedFrame::topRows() const { return impl_->fixed_rtop ? impl_->fixed_rtop : impl_->tuned_rtop; }
 
 unsigned long long FusedFrame::timeouts() const {
     if (!impl_->npairs) return 0;

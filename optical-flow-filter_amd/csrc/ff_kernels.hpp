@@ -139,7 +139,7 @@ public:
     FusedFrame& operator=(FusedFrame&&) noexcept;
 
     // The first call for a set of level shapes times every candidate tiling
-    // on stream s (unless a wave count was fixed with setWaves).
+    // on stream s (except what setWaves / setTopRows fixed).
     int prepare(const std::vector<LevelArgs>& jobs, hipStream_t s);
     // frame (optional): device u8 image of the input frame's size, with row
     // pitch frame_pitch (a multiple of 4), read in place by the jobs marked
@@ -153,6 +153,11 @@ public:
     // reports the fixed or tuned value (0 before tuning).
     void setWaves(int nw);
     int waves() const;
+    // Rows per thread of the tiles of top levels reduced on load (a
+    // pyramid's top level): 0 = autotune (default), 4 or 8. topRows()
+    // reports the fixed or tuned value (0 before tuning).
+    void setTopRows(int rows);
+    int topRows() const;
     // Waits that gave up after the spin limit (a scheduling fault; results
     // of such a frame are not valid). Synchronizes the device.
     unsigned long long timeouts() const;

@@ -59,6 +59,17 @@ class Stage(object):
     def tileWaves(self, v):
         N.call("ff_filter_set_tile_waves", self.handle, int(v))
 
+    @property
+    def tileTopRows(self):
+        """Filters only: rows per thread of the fused kernel's tiles on a
+        pyramid's top level (0 = tuned with the tile height, the default; or
+        4, 8). Reads the fixed or tuned value."""
+        return N.get_int("ff_filter_get_tile_top_rows", self.handle)
+
+    @tileTopRows.setter
+    def tileTopRows(self, v):
+        N.call("ff_filter_set_tile_top_rows", self.handle, int(v))
+
 
 def _hnd(img):
     return img.handle if img is not None else None

@@ -383,15 +383,19 @@ def test_fused_matches_oracle(ffo, shape, levels, maxflow, smooth, border):
     assert g.waitTimeouts() == 0
 
 
+@pytest.mark.parametrize("levels", [2, 3])
+@pytest.mark.parametrize("top_rows", [4, 8])
 @pytest.mark.parametrize("waves", [16, 15, 8, 5, 4])
-def test_fused_tile_heights_match_oracle(ffo, waves):
-    """Every tile height of the fused frame kernel gives the oracle's flow."""
+def test_fused_tile_heights_match_oracle(ffo, waves, top_rows, levels):
+    """Every tile height of the fused frame kernel, with either row count on
+    the reduced top level, gives the oracle's flow."""
     from flowfilter import _native as N
     h, w = 300, 410
-    say("tile waves", waves)
-    g, o = _pyr_pair(ffo, h, w, 3, [10.0, 30.0, 60.0], 4.0, [2, 2, 1], fused=True)
+    say("tile waves", waves, "top rows", top_rows, "levels", levels)
+    g, o = _pyr_pair(ffo, h, w, levels, [10.0, 30.0, 60.0][:levels], 4.0, [2, 2, 1][:levels], fused=True)
     g.tileWaves = waves
-    assert g.tileWaves == waves
+    g.tileTopRows = top_rows
+    assert (g.tileWaves, g.tileTopRows) == (waves, top_rows)
     for t, img in enumerate(seq_blobs(h, w, 5)):
         g.loadImage(img)
         N.profile_begin("fused_frame")
@@ -399,7 +403,7 @@ def test_fused_tile_heights_match_oracle(ffo, waves):
         _, n = N.profile_end()
         assert n == (0 if t == 0 else 1)
         o.compute(img)
-        check_frame(t, g.getFlow(), o.downloadFlow(), "waves %d" % waves)
+        check_frame(t, g.getFlow(), o.downloadFlow(), "waves %d top rows %d" % (waves, top_rows))
 
 
 def test_fused_autotune_picks_a_tiling(ffo):
@@ -411,6 +415,9 @@ def test_fused_autotune_picks_a_tiling(ffo):
         o.compute(img)
         check_frame(t, g.getFlow(), o.downloadFlow(), "autotuned")
     assert g.tileWaves in (16, 15, 8, 5, 4)
+    assert g.tileTopRows in (4, 8)
     from flowfilter import errors
     with pytest.raises(errors.InvalidArgument):
         g.tileWaves = 7
+    with pytest.raises(errors.InvalidArgument):
+        g.tileTopRows = 6
