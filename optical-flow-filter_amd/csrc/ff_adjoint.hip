@@ -206,6 +206,23 @@ __device__ __forceinline__ Px5 load_px(const float2* f, const float* s, const fl
     return r;
 }
 
+// Lane i of a wave receives lane i-1's value (PREV) or lane i+1's; the lane
+// without a source receives 0. Every lane of the wave must be active.
+template <bool PREV>
+__device__ __forceinline__ float lane_shift(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), PREV ? 0x138 : 0x130, 0xf, 0xf, true));
+}
+template <bool PREV, bool PAY>
+__device__ __forceinline__ Px5 lane_shift_px(const Px5& z) {
+    Px5 r;
+    r.f = make_float2(lane_shift<PREV>(z.f.x), lane_shift<PREV>(z.f.y));
+    if constexpr (PAY) {
+        r.s = lane_shift<PREV>(z.s);
+        r.v = make_float2(lane_shift<PREV>(z.v.x), lane_shift<PREV>(z.v.y));
+    }
+    return r;
+}
+
 template <bool PAY>
 __device__ __forceinline__ void store_px(float2* f, float* s, float2* v, size_t i, const Px5& r) {
     f[i] = r.f;
@@ -234,11 +251,13 @@ __global__ void __launch_bounds__(BX* BY) k_ad_prop(const float2* fin, const flo
 }
 
 // An X pass followed by a Y pass in one launch: a workgroup of 64 x 16
-// threads computes the X pass on 16 rows (clamped into the image) into LDS
-// and the Y pass on the 14 inner ones, so the pair reads and writes each
-// field once instead of twice. With SAVE, the X pass's result (the Y pass's
-// input, which the adjoint needs) is written too. Same arithmetic as two
-// k_ad_prop launches.
+// threads (a wave per row) computes the X pass on 16 rows (clamped into the
+// image) into LDS and the Y pass on the 14 inner ones, so the pair reads and
+// writes each field once instead of twice. Each lane loads its own pixel
+// (lanes past the last column load that column, the clamped read) and takes
+// its row neighbours from the adjacent lanes; only the first and last lane
+// load theirs. With SAVE, the X pass's result (the Y pass's input, which the
+// adjoint needs) is written too. Same arithmetic as two k_ad_prop launches.
 constexpr int kPairRows = 14;
 template <bool PAY, bool SAVE>
 __global__ void __launch_bounds__(64 * (kPairRows + 2))
@@ -251,14 +270,15 @@ __global__ void __launch_bounds__(64 * (kPairRows + 2))
     const int yc = clampi(yr, 0, d.H - 1);
     const bool xok = x < d.W;
     const bool xin = x >= border && x < d.W - border;
-    if (xok) {
-        const int xm = clampi(x - 1, 0, d.W - 1), xp = clampi(x + 1, 0, d.W - 1);
+    {
+        const Px5 z = load_px<PAY>(fin, sin, vin, at(b, yc, clampi(x, 0, d.W - 1), d));
+        Px5 m = lane_shift_px<true, PAY>(z), p = lane_shift_px<false, PAY>(z);
+        if (tx == 0) m = load_px<PAY>(fin, sin, vin, at(b, yc, clampi(x - 1, 0, d.W - 1), d));
+        if (tx == 63) p = load_px<PAY>(fin, sin, vin, at(b, yc, clampi(x + 1, 0, d.W - 1), d));
         const bool in = xin && yc >= border && yc < d.H - border;
-        const size_t i0 = at(b, yc, x, d);
-        const Px5 o = upwind_px<0, PAY>(load_px<PAY>(fin, sin, vin, at(b, yc, xm, d)), load_px<PAY>(fin, sin, vin, i0),
-                                        load_px<PAY>(fin, sin, vin, at(b, yc, xp, d)), dt, in);
+        const Px5 o = upwind_px<0, PAY>(m, z, p, dt, in);
         L[ty][tx] = o;
-        if (SAVE && ty >= 1 && ty <= kPairRows && yr < d.H) store_px<PAY>(fmid, smid, vmid, i0, o);
+        if (SAVE && xok && ty >= 1 && ty <= kPairRows && yr < d.H) store_px<PAY>(fmid, smid, vmid, at(b, yc, x, d), o);
     }
     __syncthreads();
     if (xok && ty >= 1 && ty <= kPairRows && yr < d.H) {
