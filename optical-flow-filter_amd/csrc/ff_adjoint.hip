@@ -445,14 +445,18 @@ __global__ void __launch_bounds__(64 * kPairBwdRows)
     }
 }
 
-// The same adjoint without the X pass's result: the workgroup stages the
-// pair's input at 64 columns of its 16 rows plus two above and two below
-// (reads clamped into the image, as the forward's) in LDS, reruns the X
-// pass on the inner 62 columns of those 20 rows (the Y pass's inputs its
-// adjoint reads), then takes the Y pass's adjoint at those 62 columns and
-// the X pass's at the inner 60. Every value is computed by the same
-// per-pixel functions from the same operands as in the forward and in
-// k_ad_prop_pair_bwd, so the results are bitwise theirs; the saved X-pass
+// The same adjoint without the X pass's result. A workgroup of 64 x 16
+// threads (a wave per row) covers 64 columns: lane tx is column x0-2+tx. The
+// pair's input is loaded once per row of this wave (the own row, y) and of
+// two rows above and below the workgroup's 16 (the first four waves), with
+// reads clamped into the image as in the forward; the X pass is rerun on
+// those 20 rows with the row neighbours taken from the adjacent lanes (valid
+// on lanes 1..62) into LDS, with the output gradients of rows y0-1 ..
+// y0+16. Then the Y pass's adjoint at lanes 1..62, and the X pass's at lanes
+// 2..61 (columns x0 .. x0+59) with its inputs and the Y adjoint of the
+// neighbouring columns again taken from adjacent lanes. Every value comes
+// from the same per-pixel functions and operands as in the forward and in
+// k_ad_prop_pair_bwd, so the results are bitwise theirs; the X pass's
 // result (20 bytes per pixel written forward and read back) is not needed.
 constexpr int kPairRcCols = 60, kPairRcRows = kPairBwdRows + 4;
 template <bool PAY>
@@ -460,26 +464,30 @@ __global__ void __launch_bounds__(64 * kPairBwdRows)
     k_ad_prop_pair_bwd_rc(const float2* fin, const float* sin, const float2* vin, const float2* gfo,
                           const float* gso, const float2* gvo, float2* gfi, float* gsi, float2* gvi, Dims d, float dt,
                           int border, int acc) {
-    __shared__ Px5 LF[kPairRcRows][64];  // pair input, rows y0-2 .. y0+17
-    __shared__ Px5 LM[kPairRcRows][64];  // its X pass; later the Y pass's adjoint (rows y0 .. y0+15)
+    __shared__ Px5 LM[kPairRcRows][64];        // X pass of rows y0-2 .. y0+17
+    __shared__ Px5 LG[kPairBwdRows + 2][64];   // output gradients of rows y0-1 .. y0+16
     const int tx = threadIdx.x, ty = threadIdx.y, b = blockIdx.z;
     const int x = (int)blockIdx.x * kPairRcCols - 2 + tx;  // column of this lane
     const int y0 = (int)blockIdx.y * kPairBwdRows, y = y0 + ty;
     const int xc = clampi(x, 0, d.W - 1);
     const bool xin = x >= border && x < d.W - border;
-    for (int i = ty; i < kPairRcRows; i += kPairBwdRows)
-        LF[i][tx] = load_px<PAY>(fin, sin, vin, at(b, clampi(y0 - 2 + i, 0, d.H - 1), xc, d));
+    // X pass of staged row i (image row clamp(y0-2+i)); returns its input
+    auto x_pass = [&](int i) {
+        const int yc = clampi(y0 - 2 + i, 0, d.H - 1);
+        const Px5 z = load_px<PAY>(fin, sin, vin, at(b, yc, xc, d));
+        const Px5 m = lane_shift_px<true, PAY>(z), p = lane_shift_px<false, PAY>(z);
+        const bool in = xin && yc >= border && yc < d.H - border;
+        LM[i][tx] = upwind_px<0, PAY>(m, z, p, dt, in);
+        return z;
+    };
+    const Px5 zown = x_pass(ty + 2);  // the pair's input at (y, x)
+    if (ty < 4) x_pass(ty < 2 ? ty : kPairBwdRows + ty);  // rows 0, 1, 18, 19
+    LG[ty + 1][tx] = load_grad_px<PAY>(gfo, gso, gvo, at(b, clampi(y, 0, d.H - 1), xc, d));
+    if (ty == 0) LG[0][tx] = load_grad_px<PAY>(gfo, gso, gvo, at(b, clampi(y0 - 1, 0, d.H - 1), xc, d));
+    if (ty == kPairBwdRows - 1)
+        LG[kPairBwdRows + 1][tx] = load_grad_px<PAY>(gfo, gso, gvo, at(b, clampi(y0 + kPairBwdRows, 0, d.H - 1), xc, d));
     __syncthreads();
-    // X pass: lane tx-1 / tx+1 hold the clamped reads of columns x-1 / x+1
-    if (tx >= 1 && tx <= 62) {
-        for (int i = ty; i < kPairRcRows; i += kPairBwdRows) {
-            const int yc = clampi(y0 - 2 + i, 0, d.H - 1);
-            const bool in = xin && yc >= border && yc < d.H - border;
-            LM[i][tx] = upwind_px<0, PAY>(LF[i][tx - 1], LF[i][tx], LF[i][tx + 1], dt, in);
-        }
-    }
-    __syncthreads();
-    // adjoint of the Y pass at columns x0-1 .. x0+60 (zero outside the image)
+    // adjoint of the Y pass (zero outside the image)
     Px5 gm;
     gm.f = make_float2(0.f, 0.f);
     gm.s = 0.f;
@@ -489,22 +497,20 @@ __global__ void __launch_bounds__(64 * kPairBwdRows)
 #pragma unroll
         for (int k = 0; k < 5; ++k) X[k] = LM[ty + k][tx];  // rows clamp(y-2+k)
 #pragma unroll
-        for (int k = 0; k < 3; ++k) G[k] = load_grad_px<PAY>(gfo, gso, gvo, at(b, clampi(y - 1 + k, 0, d.H - 1), x, d));
+        for (int k = 0; k < 3; ++k) G[k] = LG[ty + k][tx];  // rows clamp(y-1+k)
         auto interior = [&](int o) { return xin && o >= border && o < d.H - border; };
         gm = upwind_adjoint_px<1, PAY>(X, G, y, d.H, interior, dt);
     }
-    __syncthreads();  // every read of the X pass is done
-    LM[ty][tx] = gm;
-    __syncthreads();
-    // adjoint of the X pass at columns x0 .. x0+59
-    if (tx >= 2 && tx <= 61 && x < d.W && y < d.H) {
-        Px5 X[5];
-        const Px5 G[3] = {LM[ty][tx - 1], LM[ty][tx], LM[ty][tx + 1]};
-#pragma unroll
-        for (int k = 0; k < 5; ++k) X[k] = LF[ty + 2][tx - 2 + k];  // columns clamp(x-2+k) of row y
+    // adjoint of the X pass: the Y adjoint at columns x-1 .. x+1 and the
+    // pair's input at columns clamp(x-2) .. clamp(x+2) of row y, from lanes
+    // tx-1 .. tx+1 and tx-2 .. tx+2 (every lane active here)
+    const Px5 G3[3] = {lane_shift_px<true, PAY>(gm), gm, lane_shift_px<false, PAY>(gm)};
+    const Px5 m1 = lane_shift_px<true, PAY>(zown), p1 = lane_shift_px<false, PAY>(zown);
+    const Px5 X5[5] = {lane_shift_px<true, PAY>(m1), m1, zown, p1, lane_shift_px<false, PAY>(p1)};
+    if (tx >= 2 && tx < kPairRcCols + 2 && x < d.W && y < d.H) {
         const bool yin = y >= border && y < d.H - border;
         auto interior = [&](int o) { return yin && o >= border && o < d.W - border; };
-        put_px<PAY>(gfi, gsi, gvi, at(b, y, x, d), upwind_adjoint_px<0, PAY>(X, G, x, d.W, interior, dt), acc);
+        put_px<PAY>(gfi, gsi, gvi, at(b, y, x, d), upwind_adjoint_px<0, PAY>(X5, G3, x, d.W, interior, dt), acc);
     }
 }
 
