@@ -735,7 +735,9 @@ __global__ void __launch_bounds__(BX* BY)
 // workgroup of 64 x 16 threads computes the X pass on 16 rows (clamped into
 // the image) into LDS and the Y pass on the inner 12, reading and writing
 // the flow once instead of twice. Same arithmetic as two k_ad_smooth
-// launches.
+// launches. (Taking the x taps from the adjacent lanes instead of loading
+// them was measured slower here: the five float2 loads are served by the
+// caches.)
 constexpr int kSmoothRows = 12;
 __global__ void __launch_bounds__(64 * (kSmoothRows + 4)) k_ad_smooth_pair(const float2* in, float2* out, Dims d) {
     __shared__ float2 L[kSmoothRows + 4][64];
@@ -768,30 +770,41 @@ __global__ void __launch_bounds__(64 * (kSmoothRows + 4)) k_ad_smooth_pair(const
 }
 
 // Adjoint of k_ad_smooth_pair for the output gradient gout + gout2 (gout2
-// may be null): a workgroup of 64 x 16 threads takes the Y pass's adjoint
-// at 64 columns (its 60 output columns and two on each side) of 16 rows
-// into LDS, then the X pass's adjoint at the inner 60. Same arithmetic as
-// k_ad_smooth_bwd<1> followed by k_ad_smooth_bwd<0>.
+// may be null): a workgroup of 64 x 16 threads stages the summed output
+// gradient of its rows and two on each side in LDS (each element loaded
+// once), takes the Y pass's adjoint at 64 columns (its 60 output columns and
+// two on each side) of 16 rows into LDS, then the X pass's adjoint at the
+// inner 60. Same arithmetic as k_ad_smooth_bwd<1> followed by
+// k_ad_smooth_bwd<0>.
 constexpr int kSmoothBwdCols = 60, kSmoothBwdRows = 16;
 __global__ void __launch_bounds__(64 * kSmoothBwdRows)
     k_ad_smooth_pair_bwd(const float2* gout, const float2* gout2, float2* gin, Dims d, int acc) {
+    __shared__ float2 LGo[kSmoothBwdRows + 4][64];  // gout + gout2 of rows y0-2 .. y0+17
     __shared__ float2 L[kSmoothBwdRows][64];
     const int tx = threadIdx.x, ty = threadIdx.y, b = blockIdx.z;
     const int x = (int)blockIdx.x * kSmoothBwdCols - 2 + tx;
-    const int y = (int)blockIdx.y * kSmoothBwdRows + ty;
+    const int y0 = (int)blockIdx.y * kSmoothBwdRows, y = y0 + ty;
+    const int xc = clampi(x, 0, d.W - 1);
+    // each row of output gradients loaded once (clamped; rows and columns
+    // outside the image are never read below)
+    auto stage = [&](int r) {
+        const size_t i = at(b, clampi(y0 - 2 + r, 0, d.H - 1), xc, d);
+        float2 v = gout[i];
+        if (gout2) {
+            const float2 u = gout2[i];
+            v.x += u.x;
+            v.y += u.y;
+        }
+        LGo[r][tx] = v;
+    };
+    stage(ty + 2);
+    if (ty < 4) stage(ty < 2 ? ty : kSmoothBwdRows + ty);  // rows 0, 1, 18, 19
+    __syncthreads();
     float2 s = make_float2(0.f, 0.f);
     if (x >= 0 && x < d.W && y < d.H) {
-        auto g = [&](size_t i) {
-            float2 v = gout[i];
-            if (gout2) {
-                const float2 u = gout2[i];
-                v.x += u.x;
-                v.y += u.y;
-            }
-            return v;
-        };
-        s.x = corr5_adjoint(y, d.H, WB{}, [&](int i) { return g(at(b, i, x, d)).x; });
-        s.y = corr5_adjoint(y, d.H, WB{}, [&](int i) { return g(at(b, i, x, d)).y; });
+        // image row i is staged row i - y0 + 2
+        s.x = corr5_adjoint(y, d.H, WB{}, [&](int i) { return LGo[i - y0 + 2][tx].x; });
+        s.y = corr5_adjoint(y, d.H, WB{}, [&](int i) { return LGo[i - y0 + 2][tx].y; });
     }
     L[ty][tx] = s;
     __syncthreads();
