@@ -387,6 +387,12 @@ FF_API ff_status ff_ad_propagate_pass(int axis, const float* flow, const float* 
 FF_API ff_status ff_ad_propagate_pair(const float* flow, const float* scalar, const float* vector, float* flow_mid,
                                       float* scalar_mid, float* vector_mid, float* flow_out, float* scalar_out,
                                       float* vector_out, int B, int H, int W, float dt, int border, void* stream);
+/* Two iterations -- X, Y, X and Y passes -- in one launch (the same results
+ * as two ff_ad_propagate_pair calls), for forwards that keep no
+ * intermediates; no output may be its input. */
+FF_API ff_status ff_ad_propagate_quad(const float* flow, const float* scalar, const float* vector, float* flow_out,
+                                      float* scalar_out, float* vector_out, int B, int H, int W, float dt, int border,
+                                      void* stream);
 /* Adjoint of ff_ad_propagate_pair, given the pair's input and the X pass's
  * result (flow_mid ...); the same results as the two passes' adjoints.
  * With flow_mid, scalar_mid and vector_mid all NULL the X pass's result is

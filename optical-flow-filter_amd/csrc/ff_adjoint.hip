@@ -288,6 +288,64 @@ __global__ void __launch_bounds__(64 * (kPairRows + 2))
     }
 }
 
+// Two propagation iterations (X, Y, X, Y passes) in one launch, for the
+// forward of frames whose intermediates are not kept: a workgroup of
+// 64 x 16 threads (a wave per row) holds 16 rows (clamped into the image),
+// every pass's result in LDS, and writes the 12 inner rows of its 62 inner
+// columns (lanes 1..62); it reads and writes each field once per two
+// iterations instead of twice. The first X pass takes its row neighbours
+// from the adjacent lanes (the edge lanes load theirs); the second from the
+// adjacent lanes only, so lanes 0 and 63 serve as its halo. A row or column
+// outside the image holds the values of the edge row or column it is
+// clamped to (the clamped reads of the pass that follows), and each Y pass
+// reads its rows' neighbours at the clamped row indices. Every pixel's value
+// is computed by the same per-pixel function from the same operands as in
+// two k_ad_prop_pair launches, so the results are bitwise theirs.
+constexpr int kQuadRows = 12, kQuadCols = 62;
+template <bool PAY>
+__global__ void __launch_bounds__(64 * (kQuadRows + 4))
+    k_ad_prop_quad(const float2* fin, const float* sin, const float2* vin, float2* fout, float* sout, float2* vout,
+                   Dims d, float dt, int border) {
+    __shared__ Px5 LA[kQuadRows + 4][64];  // X passes
+    __shared__ Px5 LB[kQuadRows + 4][64];  // first Y pass
+    const int tx = threadIdx.x, ty = threadIdx.y, b = blockIdx.z;
+    const int x = (int)blockIdx.x * kQuadCols - 1 + tx;  // column of this lane
+    const int y0 = (int)blockIdx.y * kQuadRows;
+    const int yr = y0 - 2 + ty;  // staged row ty
+    const int yc = clampi(yr, 0, d.H - 1);
+    const int xc = clampi(x, 0, d.W - 1);
+    const bool xin = x >= border && x < d.W - border;
+    const bool in = xin && yc >= border && yc < d.H - border;
+    // staged row holding image row r (r within the staged range)
+    auto srow = [&](int r) { return clampi(r, 0, d.H - 1) - (y0 - 2); };
+    // first X pass on all 16 rows
+    {
+        const Px5 z = load_px<PAY>(fin, sin, vin, at(b, yc, xc, d));
+        Px5 m = lane_shift_px<true, PAY>(z), p = lane_shift_px<false, PAY>(z);
+        if (tx == 0) m = load_px<PAY>(fin, sin, vin, at(b, yc, clampi(x - 1, 0, d.W - 1), d));
+        if (tx == 63) p = load_px<PAY>(fin, sin, vin, at(b, yc, clampi(x + 1, 0, d.W - 1), d));
+        LA[ty][tx] = upwind_px<0, PAY>(m, z, p, dt, in);
+    }
+    __syncthreads();
+    // first Y pass on staged rows 1..14 (image rows clamp(y0-1 .. y0+12))
+    if (ty >= 1 && ty <= kQuadRows + 2)
+        LB[ty][tx] = upwind_px<1, PAY>(LA[srow(yc - 1)][tx], LA[srow(yc)][tx], LA[srow(yc + 1)][tx], dt, in);
+    __syncthreads();
+    // second X pass on the same rows, neighbours from the adjacent lanes
+    // (valid on lanes 1..62)
+    if (ty >= 1 && ty <= kQuadRows + 2) {
+        const Px5 z = LB[ty][tx];
+        const Px5 m = lane_shift_px<true, PAY>(z), p = lane_shift_px<false, PAY>(z);
+        LA[ty][tx] = upwind_px<0, PAY>(m, z, p, dt, in);
+    }
+    __syncthreads();
+    // second Y pass on the 12 output rows
+    if (ty >= 2 && ty < kQuadRows + 2 && yr < d.H && tx >= 1 && tx <= kQuadCols && x < d.W) {
+        const Px5 o = upwind_px<1, PAY>(LA[srow(yc - 1)][tx], LA[srow(yc)][tx], LA[srow(yc + 1)][tx], dt, in);
+        store_px<PAY>(fout, sout, vout, at(b, yr, x, d), o);
+    }
+}
+
 // Adjoint of one upwind pass. Output o = z + nd * D with nd = -dt*Ud and
 // D = back ? z - m : p - z, for every field, where Ud = (|p_u| > |m_u|) ?
 // p_u : m_u is read from the flow component along the pass (u). Holding
@@ -875,6 +933,17 @@ void prop_pair(const float* fin, const float* sin, const float* vin, float* fmid
         hipLaunchKernelGGL((k_ad_prop_pair<false, true>), g, blk, 0, s, F(fin), sin, F(vin), O(fmid), smid, O(vmid), O(fout), sout, O(vout), d, dt, border);
     else
         hipLaunchKernelGGL((k_ad_prop_pair<false, false>), g, blk, 0, s, F(fin), sin, F(vin), O(fmid), smid, O(vmid), O(fout), sout, O(vout), d, dt, border);
+}
+void prop_quad(const float* fin, const float* sin, const float* vin, float* fout, float* sout, float* vout, int B, int H,
+               int W, float dt, int border, hipStream_t s) {
+    const dim3 g((W + kQuadCols - 1) / kQuadCols, (H + kQuadRows - 1) / kQuadRows, B), blk(64, kQuadRows + 4);
+    const Dims d{H, W};
+    if (sin)
+        hipLaunchKernelGGL(k_ad_prop_quad<true>, g, blk, 0, s, (const float2*)fin, sin, (const float2*)vin, (float2*)fout,
+                           sout, (float2*)vout, d, dt, border);
+    else
+        hipLaunchKernelGGL(k_ad_prop_quad<false>, g, blk, 0, s, (const float2*)fin, sin, (const float2*)vin,
+                           (float2*)fout, sout, (float2*)vout, d, dt, border);
 }
 void prop_pair_bwd(const float* fin, const float* sin, const float* vin, const float* fmid, const float* smid,
                    const float* vmid, const float* gfo, const float* gso, const float* gvo, float* gfi, float* gsi,

@@ -691,6 +691,22 @@ ff_status ff_ad_propagate_pair(const float* flow, const float* scalar, const flo
         check_launch("ff_ad_propagate_pair");
     });
 }
+ff_status ff_ad_propagate_quad(const float* flow, const float* scalar, const float* vector, float* flow_out,
+                               float* scalar_out, float* vector_out, int B, int H, int W, float dt, int border,
+                               void* stream) {
+    return guard([&] {
+        dims(B, H, W);
+        if (border < 1) throw invalid_argument("border must be at least 1");
+        need(flow, "flow"), need(flow_out, "flow_out");
+        const bool pay = scalar != nullptr;
+        if (pay != (vector != nullptr) || pay != (scalar_out != nullptr) || pay != (vector_out != nullptr))
+            throw invalid_argument("payloads: pass all of scalar, vector and their outputs, or none");
+        if (flow_out == flow || (pay && (scalar_out == scalar || vector_out == vector)))
+            throw invalid_argument("outputs must not be the inputs (the passes read neighbours)");
+        ad::prop_quad(flow, scalar, vector, flow_out, scalar_out, vector_out, B, H, W, dt, border, as_stream(stream));
+        check_launch("ff_ad_propagate_quad");
+    });
+}
 ff_status ff_ad_propagate_pair_backward(const float* flow, const float* scalar, const float* vector,
                                         const float* flow_mid, const float* scalar_mid, const float* vector_mid,
                                         const float* g_flow_out, const float* g_scalar_out, const float* g_vector_out,

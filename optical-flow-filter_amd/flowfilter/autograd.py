@@ -178,16 +178,24 @@ def _frame_forward(spec, state, image, gammas, save):
         vc = state[l]["d"] if l < top else None
         passes = []
         pay = sc is not None
-        for _ in range(lv["N"]):
-            # an X and a Y pass in one launch; the adjoint keeps the pair's
-            # input and recomputes the X pass's result itself
+        it = 0
+        while it < lv["N"]:
             fo = _empty(B, h, w, 2, like=image)
             so = _empty(B, h, w, like=image) if pay else None
             vo = _empty(B, h, w, 2, like=image) if pay else None
-            N.call("ff_ad_propagate_pair", _p(f), _p(sc), _p(vc), None, None, None, _p(fo), _p(so), _p(vo),
-                   B, h, w, ctypes.c_float(lv["dt"]), spec.border, s)
-            if save:
-                passes.append((f, sc, vc))
+            if not save and it + 2 <= lv["N"]:
+                # two iterations in one launch when nothing is kept
+                N.call("ff_ad_propagate_quad", _p(f), _p(sc), _p(vc), _p(fo), _p(so), _p(vo), B, h, w,
+                       ctypes.c_float(lv["dt"]), spec.border, s)
+                it += 2
+            else:
+                # an X and a Y pass in one launch; the adjoint keeps the
+                # pair's input and recomputes the X pass's result itself
+                N.call("ff_ad_propagate_pair", _p(f), _p(sc), _p(vc), None, None, None, _p(fo), _p(so), _p(vo),
+                       B, h, w, ctypes.c_float(lv["dt"]), spec.border, s)
+                if save:
+                    passes.append((f, sc, vc))
+                it += 1
             f, sc, vc = fo, so, vo
         prop[l] = (f, sc, vc)
         if save:

@@ -144,6 +144,31 @@ def test_pass_pair_equals_two_passes(ad, shape, pay):
             assert torch.equal(a, b_)
 
 
+@pytest.mark.parametrize("shape", [(2, 45, 70), (1, 9, 130), (3, 31, 64), (1, 1, 5), (1, 13, 62), (2, 25, 124),
+                                   (1, 3, 3), (1, 40, 2), (1, 12, 63)])
+@pytest.mark.parametrize("pay", [False, True])
+def test_pass_quad_equals_two_pairs(ad, shape, pay):
+    """ff_ad_propagate_quad (two iterations in one launch) is bitwise two
+    ff_ad_propagate_pair launches."""
+    B, H, W = shape
+    r = rng(51 + H + W)
+    f = cuda(r.uniform(-3, 3, size=(B, H, W, 2)).astype(np.float32))
+    sc = cuda(r.uniform(0, 1, size=(B, H, W)).astype(np.float32)) if pay else None
+    vc = cuda(r.uniform(-1, 1, size=(B, H, W, 2)).astype(np.float32)) if pay else None
+    dt = ctypes_f(float(np.float32(1) / np.float32(3)))
+    e2, e1 = (lambda: torch.empty(B, H, W, 2, device="cuda")), (lambda: torch.empty(B, H, W, device="cuda"))
+    m = [e2(), e1() if pay else None, e2() if pay else None]
+    o = [e2(), e1() if pay else None, e2() if pay else None]
+    _call(ad, "ff_ad_propagate_pair", f, sc, vc, None, None, None, *m, B, H, W, dt, 2)
+    _call(ad, "ff_ad_propagate_pair", *m, None, None, None, *o, B, H, W, dt, 2)
+    q = [e2(), e1() if pay else None, e2() if pay else None]
+    _call(ad, "ff_ad_propagate_quad", f, sc, vc, *q, B, H, W, dt, 2)
+    torch.cuda.synchronize()
+    for a, b_ in zip(o, q):
+        if a is not None:
+            assert torch.equal(a, b_)
+
+
 @pytest.mark.parametrize("shape", [(2, 45, 70), (1, 9, 130), (3, 33, 62), (1, 1, 5), (2, 17, 121), (1, 40, 2)])
 @pytest.mark.parametrize("pay", [False, True])
 def test_pass_pair_adjoint_equals_two_adjoints(ad, shape, pay):
@@ -542,6 +567,8 @@ def test_errors(ad):
         _call(ad, "ff_ad_propagate_pair_backward", f, sc, f, None, sc, None, gf, gs, gf, gf, gs, gf, 1, 8, 8, dt, 1, 0)
     with pytest.raises(ValueError):
         _call(ad, "ff_ad_propagate_pair_backward", f, sc, f, f, None, None, gf, gs, gf, gf, gs, gf, 1, 8, 8, dt, 1, 0)
+    with pytest.raises(ValueError):
+        _call(ad, "ff_ad_propagate_quad", f, None, None, f, None, None, 1, 8, 8, dt, 1)
     # smoothing pairs read neighbours: no output may be an input
     with pytest.raises(ValueError):
         _call(ad, "ff_ad_smooth_pair", f, f, 1, 8, 8)
