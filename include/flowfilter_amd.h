@@ -393,6 +393,15 @@ FF_API ff_status ff_ad_propagate_pair(const float* flow, const float* scalar, co
 FF_API ff_status ff_ad_propagate_quad(const float* flow, const float* scalar, const float* vector, float* flow_out,
                                       float* scalar_out, float* vector_out, int B, int H, int W, float dt, int border,
                                       void* stream);
+/* Adjoint of ff_ad_propagate_quad given the quad's input only (its three
+ * inner passes are recomputed inside the launch); bitwise the adjoints of
+ * the two pairs. g_*_out may be NULL (zero); accumulate as for the pass
+ * backward; no input gradient may be an output gradient. */
+FF_API ff_status ff_ad_propagate_quad_backward(const float* flow, const float* scalar, const float* vector,
+                                               const float* g_flow_out, const float* g_scalar_out,
+                                               const float* g_vector_out, float* g_flow, float* g_scalar,
+                                               float* g_vector, int B, int H, int W, float dt, int border,
+                                               int accumulate, void* stream);
 /* Adjoint of ff_ad_propagate_pair, given the pair's input and the X pass's
  * result (flow_mid ...); the same results as the two passes' adjoints.
  * With flow_mid, scalar_mid and vector_mid all NULL the X pass's result is

@@ -572,6 +572,115 @@ __global__ void __launch_bounds__(64 * kPairBwdRows)
     }
 }
 
+// Adjoint of k_ad_prop_quad (two iterations), recomputing the forward
+// passes inside the launch. A workgroup of 64 x 16 threads (a wave per row)
+// covers 64 columns, lane tx being column x0-3+tx, and 16 rows, wave w being
+// row y0-4+w (reads clamped into the image). Forward: the first X pass on
+// the 16 rows (row neighbours from the adjacent lanes, the edge lanes load
+// theirs) into LDS, the first Y pass on rows 1..14, the second X pass on
+// those rows (adjacent lanes; valid on lanes 1..62) into LDS, beside the
+// output gradients of rows 2..13. Adjoints, each with the same per-pixel
+// function as the single passes: the second Y pass's at rows 3..12 (lanes
+// 1..62), the second X pass's at those rows (lanes 2..61; its inputs and
+// the Y adjoint of the neighbouring columns from the adjacent lanes), the
+// first Y pass's at rows 4..11, the first X pass's at those rows, written
+// for lanes 3..60 (columns x0 .. x0+57, rows y0 .. y0+7). Bitwise the
+// adjoints of two pass pairs.
+constexpr int kQuadBwdRows = 8, kQuadBwdStage = kQuadBwdRows + 8, kQuadBwdCols = 58;
+template <bool PAY>
+__global__ void __launch_bounds__(64 * kQuadBwdStage)
+    k_ad_prop_quad_bwd(const float2* fin, const float* sin, const float2* vin, const float2* gfo, const float* gso,
+                       const float2* gvo, float2* gfi, float* gsi, float2* gvi, Dims d, float dt, int border,
+                       int acc) {
+    __shared__ Px5 LA[kQuadBwdStage][64];  // first X pass
+    __shared__ Px5 LB[kQuadBwdStage][64];  // second X pass; later the gradient of the first Y pass's output
+    __shared__ Px5 LG[kQuadBwdStage][64];  // output gradients
+    const int tx = threadIdx.x, w = threadIdx.y, b = blockIdx.z;
+    const int x = (int)blockIdx.x * kQuadBwdCols - 3 + tx;  // column of this lane
+    const int y0 = (int)blockIdx.y * kQuadBwdRows;
+    const int yr = y0 - 4 + w;  // row of this wave
+    const int yc = clampi(yr, 0, d.H - 1), xc = clampi(x, 0, d.W - 1);
+    const bool xin = x >= border && x < d.W - border;
+    const bool in = xin && yc >= border && yc < d.H - border;
+    const bool pix = x >= 0 && x < d.W && yr >= 0 && yr < d.H;  // an image pixel
+    auto srow = [&](int r) { return clampi(r, 0, d.H - 1) - (y0 - 4); };  // staged row of image row clamp(r)
+    Px5 zero;
+    zero.f = make_float2(0.f, 0.f);
+    zero.s = 0.f;
+    zero.v = make_float2(0.f, 0.f);
+
+    // ---- forward
+    const Px5 F = load_px<PAY>(fin, sin, vin, at(b, yc, xc, d));
+    {
+        Px5 m = lane_shift_px<true, PAY>(F), p = lane_shift_px<false, PAY>(F);
+        if (tx == 0) m = load_px<PAY>(fin, sin, vin, at(b, yc, clampi(x - 1, 0, d.W - 1), d));
+        if (tx == 63) p = load_px<PAY>(fin, sin, vin, at(b, yc, clampi(x + 1, 0, d.W - 1), d));
+        LA[w][tx] = upwind_px<0, PAY>(m, F, p, dt, in);
+    }
+    if (w >= 2 && w <= kQuadBwdStage - 3) LG[w][tx] = load_grad_px<PAY>(gfo, gso, gvo, at(b, yc, xc, d));
+    __syncthreads();
+    const bool mid = w >= 1 && w <= kQuadBwdStage - 2;  // rows of the first Y and second X pass
+    Px5 Y1 = zero;
+    if (mid) Y1 = upwind_px<1, PAY>(LA[srow(yc - 1)][tx], LA[srow(yc)][tx], LA[srow(yc + 1)][tx], dt, in);
+    {
+        const Px5 m = lane_shift_px<true, PAY>(Y1), p = lane_shift_px<false, PAY>(Y1);
+        if (mid) LB[w][tx] = upwind_px<0, PAY>(m, Y1, p, dt, in);
+    }
+    __syncthreads();
+
+    // ---- adjoints
+    const bool rows2 = w >= 3 && w <= kQuadBwdStage - 4;  // second pair's adjoints
+    const bool rows1 = w >= 4 && w <= kQuadBwdStage - 5;  // first pair's adjoints
+    // second Y pass: gradient of the second X pass's output
+    Px5 g_x2 = zero;
+    if (rows2 && pix && tx >= 1 && tx <= 62) {
+        Px5 X[5], G[3];
+#pragma unroll
+        for (int k = 0; k < 5; ++k) X[k] = LB[srow(yr - 2 + k)][tx];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) G[k] = LG[w - 1 + k][tx];
+        auto interior = [&](int o) { return xin && o >= border && o < d.H - border; };
+        g_x2 = upwind_adjoint_px<1, PAY>(X, G, yr, d.H, interior, dt);
+    }
+    // second X pass: gradient of the first Y pass's output (lanes 2..61)
+    Px5 g_y1 = zero;
+    {
+        const Px5 G3[3] = {lane_shift_px<true, PAY>(g_x2), g_x2, lane_shift_px<false, PAY>(g_x2)};
+        const Px5 m1 = lane_shift_px<true, PAY>(Y1), p1 = lane_shift_px<false, PAY>(Y1);
+        const Px5 X5[5] = {lane_shift_px<true, PAY>(m1), m1, Y1, p1, lane_shift_px<false, PAY>(p1)};
+        if (rows2 && pix && tx >= 2 && tx <= 61) {
+            const bool yin = yr >= border && yr < d.H - border;
+            auto interior = [&](int o) { return yin && o >= border && o < d.W - border; };
+            g_y1 = upwind_adjoint_px<0, PAY>(X5, G3, x, d.W, interior, dt);
+        }
+    }
+    __syncthreads();  // every read of LB (second X pass) is done
+    LB[w][tx] = g_y1;
+    __syncthreads();
+    // first Y pass: gradient of the first X pass's output (lanes 2..61)
+    Px5 g_x1 = zero;
+    if (rows1 && pix && tx >= 2 && tx <= 61) {
+        Px5 X[5], G[3];
+#pragma unroll
+        for (int k = 0; k < 5; ++k) X[k] = LA[srow(yr - 2 + k)][tx];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) G[k] = LB[w - 1 + k][tx];
+        auto interior = [&](int o) { return xin && o >= border && o < d.H - border; };
+        g_x1 = upwind_adjoint_px<1, PAY>(X, G, yr, d.H, interior, dt);
+    }
+    // first X pass: gradient of the quad's input (lanes 3..60)
+    {
+        const Px5 G3[3] = {lane_shift_px<true, PAY>(g_x1), g_x1, lane_shift_px<false, PAY>(g_x1)};
+        const Px5 m1 = lane_shift_px<true, PAY>(F), p1 = lane_shift_px<false, PAY>(F);
+        const Px5 X5[5] = {lane_shift_px<true, PAY>(m1), m1, F, p1, lane_shift_px<false, PAY>(p1)};
+        if (rows1 && pix && tx >= 3 && tx < kQuadBwdCols + 3) {
+            const bool yin = yr >= border && yr < d.H - border;
+            auto interior = [&](int o) { return yin && o >= border && o < d.W - border; };
+            put_px<PAY>(gfi, gsi, gvi, at(b, yr, x, d), upwind_adjoint_px<0, PAY>(X5, G3, x, d.W, interior, dt), acc);
+        }
+    }
+}
+
 // ----------------------------------------------------------------- update
 
 // Adjoint of the closed-form solve (update_k.cu:43-63). The solve is
@@ -944,6 +1053,21 @@ void prop_quad(const float* fin, const float* sin, const float* vin, float* fout
     else
         hipLaunchKernelGGL(k_ad_prop_quad<false>, g, blk, 0, s, (const float2*)fin, sin, (const float2*)vin,
                            (float2*)fout, sout, (float2*)vout, d, dt, border);
+}
+void prop_quad_bwd(const float* fin, const float* sin, const float* vin, const float* gfo, const float* gso,
+                   const float* gvo, float* gfi, float* gsi, float* gvi, int B, int H, int W, float dt, int border,
+                   int acc, hipStream_t s) {
+    const dim3 g((W + kQuadBwdCols - 1) / kQuadBwdCols, (H + kQuadBwdRows - 1) / kQuadBwdRows, B),
+        blk(64, kQuadBwdStage);
+    const Dims d{H, W};
+    auto F = [](const float* p) { return (const float2*)p; };
+    auto O = [](float* p) { return (float2*)p; };
+    if (sin)
+        hipLaunchKernelGGL(k_ad_prop_quad_bwd<true>, g, blk, 0, s, F(fin), sin, F(vin), F(gfo), gso, F(gvo), O(gfi), gsi,
+                           O(gvi), d, dt, border, acc);
+    else
+        hipLaunchKernelGGL(k_ad_prop_quad_bwd<false>, g, blk, 0, s, F(fin), sin, F(vin), F(gfo), gso, F(gvo), O(gfi),
+                           gsi, O(gvi), d, dt, border, acc);
 }
 void prop_pair_bwd(const float* fin, const float* sin, const float* vin, const float* fmid, const float* smid,
                    const float* vmid, const float* gfo, const float* gso, const float* gvo, float* gfi, float* gsi,

@@ -707,6 +707,26 @@ ff_status ff_ad_propagate_quad(const float* flow, const float* scalar, const flo
         check_launch("ff_ad_propagate_quad");
     });
 }
+ff_status ff_ad_propagate_quad_backward(const float* flow, const float* scalar, const float* vector,
+                                        const float* g_flow_out, const float* g_scalar_out, const float* g_vector_out,
+                                        float* g_flow, float* g_scalar, float* g_vector, int B, int H, int W, float dt,
+                                        int border, int accumulate, void* stream) {
+    return guard([&] {
+        dims(B, H, W);
+        if (border < 1) throw invalid_argument("border must be at least 1");
+        need(flow, "flow"), need(g_flow, "g_flow");
+        const bool pay = scalar != nullptr;
+        if (pay && (!vector || !g_scalar || !g_vector))
+            throw invalid_argument("payloads: pass scalar, vector and both of their gradients");
+        if (g_flow == g_flow_out || (pay && ((g_scalar_out && g_scalar == g_scalar_out) ||
+                                             (g_vector_out && g_vector == g_vector_out))))
+            throw invalid_argument("input gradients must not be the output gradients (the adjoint reads neighbours)");
+        ad::prop_quad_bwd(flow, scalar, vector, g_flow_out, pay ? g_scalar_out : nullptr,
+                          pay ? g_vector_out : nullptr, g_flow, g_scalar, g_vector, B, H, W, dt, border, accumulate,
+                          as_stream(stream));
+        check_launch("ff_ad_propagate_quad_backward");
+    });
+}
 ff_status ff_ad_propagate_pair_backward(const float* flow, const float* scalar, const float* vector,
                                         const float* flow_mid, const float* scalar_mid, const float* vector_mid,
                                         const float* g_flow_out, const float* g_scalar_out, const float* g_vector_out,

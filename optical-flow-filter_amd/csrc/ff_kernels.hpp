@@ -188,6 +188,9 @@ void prop_pair(const float* fin, const float* sin, const float* vin, float* fmid
                float* fout, float* sout, float* vout, int B, int H, int W, float dt, int border, hipStream_t s);
 void prop_quad(const float* fin, const float* sin, const float* vin, float* fout, float* sout, float* vout, int B, int H,
                int W, float dt, int border, hipStream_t s);
+void prop_quad_bwd(const float* fin, const float* sin, const float* vin, const float* gfo, const float* gso,
+                   const float* gvo, float* gfi, float* gsi, float* gvi, int B, int H, int W, float dt, int border,
+                   int acc, hipStream_t s);
 void prop_pair_bwd(const float* fin, const float* sin, const float* vin, const float* fmid, const float* smid,
                    const float* vmid, const float* gfo, const float* gso, const float* gvo, float* gfi, float* gsi,
                    float* gvi, int B, int H, int W, float dt, int border, int acc, hipStream_t s);

@@ -207,6 +207,7 @@ SIGNATURES = {
     "ff_ad_propagate_pass": [_i, _v, _v, _v, _v, _v, _v, _i, _i, _i, _f, _i, _v],
     "ff_ad_propagate_pair": [_v, _v, _v, _v, _v, _v, _v, _v, _v, _i, _i, _i, _f, _i, _v],
     "ff_ad_propagate_quad": [_v, _v, _v, _v, _v, _v, _i, _i, _i, _f, _i, _v],
+    "ff_ad_propagate_quad_backward": [_v, _v, _v, _v, _v, _v, _v, _v, _v, _i, _i, _i, _f, _i, _i, _v],
     "ff_ad_propagate_pair_backward": [_v, _v, _v, _v, _v, _v, _v, _v, _v, _v, _v, _v, _i, _i, _i, _f, _i, _i, _v],
     "ff_ad_propagate_pass_backward": [_i, _v, _v, _v, _v, _v, _v, _v, _v, _v, _i, _i, _i, _f, _i, _i, _v],
     "ff_ad_update": [_v, _v, _v, _v, _v, _i, _i, _i, _f, _f, _v],

@@ -183,19 +183,18 @@ def _frame_forward(spec, state, image, gammas, save):
             fo = _empty(B, h, w, 2, like=image)
             so = _empty(B, h, w, like=image) if pay else None
             vo = _empty(B, h, w, 2, like=image) if pay else None
-            if not save and it + 2 <= lv["N"]:
-                # two iterations in one launch when nothing is kept
+            # two iterations per launch, then one if N is odd; only their
+            # input is kept: the adjoints recompute the passes inside
+            n = 2 if it + 2 <= lv["N"] else 1
+            if n == 2:
                 N.call("ff_ad_propagate_quad", _p(f), _p(sc), _p(vc), _p(fo), _p(so), _p(vo), B, h, w,
                        ctypes.c_float(lv["dt"]), spec.border, s)
-                it += 2
             else:
-                # an X and a Y pass in one launch; the adjoint keeps the
-                # pair's input and recomputes the X pass's result itself
                 N.call("ff_ad_propagate_pair", _p(f), _p(sc), _p(vc), None, None, None, _p(fo), _p(so), _p(vo),
                        B, h, w, ctypes.c_float(lv["dt"]), spec.border, s)
-                if save:
-                    passes.append((f, sc, vc))
-                it += 1
+            if save:
+                passes.append((n, f, sc, vc))
+            it += n
             f, sc, vc = fo, so, vo
         prop[l] = (f, sc, vc)
         if save:
@@ -315,14 +314,16 @@ def _frame_backward(spec, saved, g_new, g_out, gammas, g_image_out):
         # not read by the delta update, so its gradient enters as zero (None)
         gf, gs, gv = None, gsp, gvp
         passes = saved["passes"][l]
-        npairs = len(passes)
-        for i in range(npairs):
-            fi, si, vi = passes[npairs - 1 - i]
-            acc_f = i == npairs - 1 and "f" in g_old[l]
+        for i, (n, fi, si, vi) in enumerate(reversed(passes)):
+            acc_f = i == len(passes) - 1 and "f" in g_old[l]
             nf = g_old[l]["f"] if acc_f else _empty(B, h, w, 2, like=gfl)
             ns, nv = _empty(B, h, w, like=gfl), _empty(B, h, w, 2, like=gfl)
-            N.call("ff_ad_propagate_pair_backward", _p(fi), _p(si), _p(vi), None, None, None, _p(gf), _p(gs),
-                   _p(gv), _p(nf), _p(ns), _p(nv), B, h, w, f32(lv["dt"]), spec.border, 1 if acc_f else 0, s)
+            if n == 2:
+                N.call("ff_ad_propagate_quad_backward", _p(fi), _p(si), _p(vi), _p(gf), _p(gs), _p(gv), _p(nf),
+                       _p(ns), _p(nv), B, h, w, f32(lv["dt"]), spec.border, 1 if acc_f else 0, s)
+            else:
+                N.call("ff_ad_propagate_pair_backward", _p(fi), _p(si), _p(vi), None, None, None, _p(gf), _p(gs),
+                       _p(gv), _p(nf), _p(ns), _p(nv), B, h, w, f32(lv["dt"]), spec.border, 1 if acc_f else 0, s)
             gf, gs, gv = nf, ns, nv
         g_old[l]["f"] = gf
         g_old[l]["a0"] = gs
@@ -347,11 +348,14 @@ def _frame_backward(spec, saved, g_new, g_out, gammas, g_image_out):
     g_g[top] = gg
     g_old[top]["a0"] = gao
     gf = gfp
-    passes = saved["passes"][top]
-    for i in reversed(range(len(passes))):
+    for n, fi, _, _ in reversed(saved["passes"][top]):
         nf = _empty(B, h, w, 2, like=gf)
-        N.call("ff_ad_propagate_pair_backward", _p(passes[i][0]), None, None, None, None, None,
-               _p(gf), None, None, _p(nf), None, None, B, h, w, f32(lv["dt"]), spec.border, 0, s)
+        if n == 2:
+            N.call("ff_ad_propagate_quad_backward", _p(fi), None, None, _p(gf), None, None, _p(nf), None, None,
+                   B, h, w, f32(lv["dt"]), spec.border, 0, s)
+        else:
+            N.call("ff_ad_propagate_pair_backward", _p(fi), None, None, None, None, None, _p(gf), None, None,
+                   _p(nf), None, None, B, h, w, f32(lv["dt"]), spec.border, 0, s)
         gf = nf
     g_old[top]["f"] = gf
     if saved["first"]:

@@ -169,6 +169,37 @@ def test_pass_quad_equals_two_pairs(ad, shape, pay):
             assert torch.equal(a, b_)
 
 
+@pytest.mark.parametrize("shape", [(2, 45, 70), (1, 9, 130), (3, 33, 62), (1, 1, 5), (2, 17, 117), (1, 40, 2),
+                                   (1, 8, 58), (1, 3, 3)])
+@pytest.mark.parametrize("pay", [False, True])
+def test_pass_quad_adjoint_equals_two_pair_adjoints(ad, shape, pay):
+    """ff_ad_propagate_quad_backward (recomputing its inner passes) is
+    bitwise the adjoints of the two pairs, zero and accumulated output
+    gradients included."""
+    B, H, W = shape
+    r = rng(61 + H + W)
+    rnd = lambda *sh, lo=-1.0, hi=1.0: cuda(r.uniform(lo, hi, size=sh).astype(np.float32))
+    f = rnd(B, H, W, 2, lo=-3, hi=3)
+    sc, vc = (rnd(B, H, W, lo=0.0), rnd(B, H, W, 2)) if pay else (None, None)
+    dt = ctypes_f(float(np.float32(1) / np.float32(4)))
+    e2, e1 = (lambda: torch.empty(B, H, W, 2, device="cuda")), (lambda: torch.empty(B, H, W, device="cuda"))
+    m = [e2(), e1() if pay else None, e2() if pay else None]  # the first pair's output
+    _call(ad, "ff_ad_propagate_pair", f, sc, vc, None, None, None, *m, B, H, W, dt, 2)
+    for zero_flow in (False, True):
+        g = [None if zero_flow else rnd(B, H, W, 2), rnd(B, H, W) if pay else None, rnd(B, H, W, 2) if pay else None]
+        gm = [e2(), e1() if pay else None, e2() if pay else None]
+        ref = [rnd(B, H, W, 2), e1() if pay else None, e2() if pay else None]
+        pre = ref[0].clone()
+        _call(ad, "ff_ad_propagate_pair_backward", *m, None, None, None, *g, *gm, B, H, W, dt, 2, 0)
+        _call(ad, "ff_ad_propagate_pair_backward", f, sc, vc, None, None, None, *gm, *ref, B, H, W, dt, 2, 1)
+        got = [pre.clone(), e1() if pay else None, e2() if pay else None]
+        _call(ad, "ff_ad_propagate_quad_backward", f, sc, vc, *g, *got, B, H, W, dt, 2, 1)
+        torch.cuda.synchronize()
+        for a, b_ in zip(ref, got):
+            if a is not None:
+                assert torch.equal(a, b_), zero_flow
+
+
 @pytest.mark.parametrize("shape", [(2, 45, 70), (1, 9, 130), (3, 33, 62), (1, 1, 5), (2, 17, 121), (1, 40, 2)])
 @pytest.mark.parametrize("pay", [False, True])
 def test_pass_pair_adjoint_equals_two_adjoints(ad, shape, pay):
