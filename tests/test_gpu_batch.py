@@ -23,6 +23,7 @@ STREAMS = [
     (272, 512, 2, [10.0, 50.0], 4.0, [2, 2]),
     (96, 160, 3, [15.0, 30.0, 45.0], 4.0, [1, 1, 1]),
     (300, 200, 2, [10.0, 50.0], 1.5, [3, 2]),
+    (480, 640, 4, [8.0, 20.0, 35.0, 50.0], 4.0, [2, 2, 1, 1]),
 ]
 
 
