@@ -573,110 +573,139 @@ __global__ void __launch_bounds__(64 * kPairBwdRows)
 }
 
 // Adjoint of k_ad_prop_quad (two iterations), recomputing the forward
-// passes inside the launch. A workgroup of 64 x 16 threads (a wave per row)
-// covers 64 columns, lane tx being column x0-3+tx, and 16 rows, wave w being
-// row y0-4+w (reads clamped into the image). Forward: the first X pass on
-// the 16 rows (row neighbours from the adjacent lanes, the edge lanes load
-// theirs) into LDS, the first Y pass on rows 1..14, the second X pass on
-// those rows (adjacent lanes; valid on lanes 1..62) into LDS, beside the
-// output gradients of rows 2..13. Adjoints, each with the same per-pixel
-// function as the single passes: the second Y pass's at rows 3..12 (lanes
-// 1..62), the second X pass's at those rows (lanes 2..61; its inputs and
-// the Y adjoint of the neighbouring columns from the adjacent lanes), the
-// first Y pass's at rows 4..11, the first X pass's at those rows, written
-// for lanes 3..60 (columns x0 .. x0+57, rows y0 .. y0+7). Bitwise the
+// passes inside the launch. A workgroup of 64 x 16 threads covers 64
+// columns, lane tx being column x0-3+tx, and S = 16 x RPW staged rows, wave
+// w holding staged rows i = w + 16k, image rows y0-4+i (reads clamped into
+// the image); R = S - 8 output rows. Forward: the first X pass on the S rows
+// (row neighbours from the adjacent lanes, the edge lanes load theirs) into
+// LDS, the first Y pass on rows 1..S-2, the second X pass on those rows
+// (adjacent lanes; valid on lanes 1..62) into LDS, beside the output
+// gradients of rows 2..S-3. Adjoints, each with the same per-pixel function
+// as the single passes: the second Y pass's at rows 3..S-4 (lanes 1..62),
+// the second X pass's at those rows (lanes 2..61; its inputs and the Y
+// adjoint of the neighbouring columns from the adjacent lanes), the first Y
+// pass's at rows 4..S-5, the first X pass's at those rows, written for
+// lanes 3..60 (columns x0 .. x0+57, rows y0 .. y0+R-1). Bitwise the
 // adjoints of two pass pairs.
-constexpr int kQuadBwdRows = 8, kQuadBwdStage = kQuadBwdRows + 8, kQuadBwdCols = 58;
-template <bool PAY>
-__global__ void __launch_bounds__(64 * kQuadBwdStage)
+constexpr int kQuadBwdWaves = 16, kQuadBwdCols = 58;
+// Rows per wave of the launches: with 2 (32 staged rows for 24 output rows
+// instead of 16 for 8) the LDS grows to 144 KB, one workgroup fits per CU,
+// and the 4K window took 153 ms instead of 146.
+constexpr int kQuadBwdRowsPerWave = 1;
+template <int RPW>
+struct QuadBwdShape {
+    static constexpr int stage = kQuadBwdWaves * RPW, rows = stage - 8;
+};
+template <bool PAY, int RPW>
+__global__ void __launch_bounds__(64 * kQuadBwdWaves)
     k_ad_prop_quad_bwd(const float2* fin, const float* sin, const float2* vin, const float2* gfo, const float* gso,
                        const float2* gvo, float2* gfi, float* gsi, float2* gvi, Dims d, float dt, int border,
                        int acc) {
-    __shared__ Px5 LA[kQuadBwdStage][64];  // first X pass
-    __shared__ Px5 LB[kQuadBwdStage][64];  // second X pass; later the gradient of the first Y pass's output
-    __shared__ Px5 LG[kQuadBwdStage][64];  // output gradients
+    constexpr int S = QuadBwdShape<RPW>::stage, R = QuadBwdShape<RPW>::rows;
+    __shared__ Px5 LA[S][64];  // first X pass
+    __shared__ Px5 LB[S][64];  // second X pass; later the gradient of the first Y pass's output
+    __shared__ Px5 LG[S][64];  // output gradients
     const int tx = threadIdx.x, w = threadIdx.y, b = blockIdx.z;
     const int x = (int)blockIdx.x * kQuadBwdCols - 3 + tx;  // column of this lane
-    const int y0 = (int)blockIdx.y * kQuadBwdRows;
-    const int yr = y0 - 4 + w;  // row of this wave
-    const int yc = clampi(yr, 0, d.H - 1), xc = clampi(x, 0, d.W - 1);
+    const int y0 = (int)blockIdx.y * R;
+    const int xc = clampi(x, 0, d.W - 1);
     const bool xin = x >= border && x < d.W - border;
-    const bool in = xin && yc >= border && yc < d.H - border;
-    const bool pix = x >= 0 && x < d.W && yr >= 0 && yr < d.H;  // an image pixel
     auto srow = [&](int r) { return clampi(r, 0, d.H - 1) - (y0 - 4); };  // staged row of image row clamp(r)
     Px5 zero;
     zero.f = make_float2(0.f, 0.f);
     zero.s = 0.f;
     zero.v = make_float2(0.f, 0.f);
+    // this wave's staged rows i = w + 16 k, image rows y0-4+i
+    Px5 F[RPW], Y1[RPW], gy1[RPW];
+    int yr[RPW], yc[RPW];
+    bool in[RPW], pix[RPW];
+#pragma unroll
+    for (int k = 0; k < RPW; ++k) {
+        const int i = w + kQuadBwdWaves * k;
+        yr[k] = y0 - 4 + i;
+        yc[k] = clampi(yr[k], 0, d.H - 1);
+        in[k] = xin && yc[k] >= border && yc[k] < d.H - border;
+        pix[k] = x >= 0 && x < d.W && yr[k] >= 0 && yr[k] < d.H;
+    }
 
     // ---- forward
-    const Px5 F = load_px<PAY>(fin, sin, vin, at(b, yc, xc, d));
-    {
-        Px5 m = lane_shift_px<true, PAY>(F), p = lane_shift_px<false, PAY>(F);
-        if (tx == 0) m = load_px<PAY>(fin, sin, vin, at(b, yc, clampi(x - 1, 0, d.W - 1), d));
-        if (tx == 63) p = load_px<PAY>(fin, sin, vin, at(b, yc, clampi(x + 1, 0, d.W - 1), d));
-        LA[w][tx] = upwind_px<0, PAY>(m, F, p, dt, in);
+#pragma unroll
+    for (int k = 0; k < RPW; ++k) {
+        const int i = w + kQuadBwdWaves * k;
+        F[k] = load_px<PAY>(fin, sin, vin, at(b, yc[k], xc, d));
+        Px5 m = lane_shift_px<true, PAY>(F[k]), p = lane_shift_px<false, PAY>(F[k]);
+        if (tx == 0) m = load_px<PAY>(fin, sin, vin, at(b, yc[k], clampi(x - 1, 0, d.W - 1), d));
+        if (tx == 63) p = load_px<PAY>(fin, sin, vin, at(b, yc[k], clampi(x + 1, 0, d.W - 1), d));
+        LA[i][tx] = upwind_px<0, PAY>(m, F[k], p, dt, in[k]);
+        if (i >= 2 && i <= S - 3) LG[i][tx] = load_grad_px<PAY>(gfo, gso, gvo, at(b, yc[k], xc, d));
     }
-    if (w >= 2 && w <= kQuadBwdStage - 3) LG[w][tx] = load_grad_px<PAY>(gfo, gso, gvo, at(b, yc, xc, d));
     __syncthreads();
-    const bool mid = w >= 1 && w <= kQuadBwdStage - 2;  // rows of the first Y and second X pass
-    Px5 Y1 = zero;
-    if (mid) Y1 = upwind_px<1, PAY>(LA[srow(yc - 1)][tx], LA[srow(yc)][tx], LA[srow(yc + 1)][tx], dt, in);
-    {
-        const Px5 m = lane_shift_px<true, PAY>(Y1), p = lane_shift_px<false, PAY>(Y1);
-        if (mid) LB[w][tx] = upwind_px<0, PAY>(m, Y1, p, dt, in);
+#pragma unroll
+    for (int k = 0; k < RPW; ++k) {
+        const int i = w + kQuadBwdWaves * k;
+        const bool mid = i >= 1 && i <= S - 2;  // rows of the first Y and second X pass
+        Y1[k] = zero;
+        if (mid)
+            Y1[k] = upwind_px<1, PAY>(LA[srow(yc[k] - 1)][tx], LA[srow(yc[k])][tx], LA[srow(yc[k] + 1)][tx], dt,
+                                      in[k]);
+        const Px5 m = lane_shift_px<true, PAY>(Y1[k]), p = lane_shift_px<false, PAY>(Y1[k]);
+        if (mid) LB[i][tx] = upwind_px<0, PAY>(m, Y1[k], p, dt, in[k]);
     }
     __syncthreads();
 
-    // ---- adjoints
-    const bool rows2 = w >= 3 && w <= kQuadBwdStage - 4;  // second pair's adjoints
-    const bool rows1 = w >= 4 && w <= kQuadBwdStage - 5;  // first pair's adjoints
-    // second Y pass: gradient of the second X pass's output
-    Px5 g_x2 = zero;
-    if (rows2 && pix && tx >= 1 && tx <= 62) {
-        Px5 X[5], G[3];
+    // ---- adjoints of the second pair (rows 3 .. S-4)
 #pragma unroll
-        for (int k = 0; k < 5; ++k) X[k] = LB[srow(yr - 2 + k)][tx];
+    for (int k = 0; k < RPW; ++k) {
+        const int i = w + kQuadBwdWaves * k;
+        const bool rows2 = i >= 3 && i <= S - 4;
+        Px5 g_x2 = zero;
+        if (rows2 && pix[k] && tx >= 1 && tx <= 62) {
+            Px5 X[5], G[3];
 #pragma unroll
-        for (int k = 0; k < 3; ++k) G[k] = LG[w - 1 + k][tx];
-        auto interior = [&](int o) { return xin && o >= border && o < d.H - border; };
-        g_x2 = upwind_adjoint_px<1, PAY>(X, G, yr, d.H, interior, dt);
-    }
-    // second X pass: gradient of the first Y pass's output (lanes 2..61)
-    Px5 g_y1 = zero;
-    {
+            for (int q = 0; q < 5; ++q) X[q] = LB[srow(yr[k] - 2 + q)][tx];
+#pragma unroll
+            for (int q = 0; q < 3; ++q) G[q] = LG[i - 1 + q][tx];
+            auto interior = [&](int o) { return xin && o >= border && o < d.H - border; };
+            g_x2 = upwind_adjoint_px<1, PAY>(X, G, yr[k], d.H, interior, dt);
+        }
         const Px5 G3[3] = {lane_shift_px<true, PAY>(g_x2), g_x2, lane_shift_px<false, PAY>(g_x2)};
-        const Px5 m1 = lane_shift_px<true, PAY>(Y1), p1 = lane_shift_px<false, PAY>(Y1);
-        const Px5 X5[5] = {lane_shift_px<true, PAY>(m1), m1, Y1, p1, lane_shift_px<false, PAY>(p1)};
-        if (rows2 && pix && tx >= 2 && tx <= 61) {
-            const bool yin = yr >= border && yr < d.H - border;
+        const Px5 m1 = lane_shift_px<true, PAY>(Y1[k]), p1 = lane_shift_px<false, PAY>(Y1[k]);
+        const Px5 X5[5] = {lane_shift_px<true, PAY>(m1), m1, Y1[k], p1, lane_shift_px<false, PAY>(p1)};
+        gy1[k] = zero;
+        if (rows2 && pix[k] && tx >= 2 && tx <= 61) {
+            const bool yin = yr[k] >= border && yr[k] < d.H - border;
             auto interior = [&](int o) { return yin && o >= border && o < d.W - border; };
-            g_y1 = upwind_adjoint_px<0, PAY>(X5, G3, x, d.W, interior, dt);
+            gy1[k] = upwind_adjoint_px<0, PAY>(X5, G3, x, d.W, interior, dt);
         }
     }
     __syncthreads();  // every read of LB (second X pass) is done
-    LB[w][tx] = g_y1;
+#pragma unroll
+    for (int k = 0; k < RPW; ++k) LB[w + kQuadBwdWaves * k][tx] = gy1[k];
     __syncthreads();
-    // first Y pass: gradient of the first X pass's output (lanes 2..61)
-    Px5 g_x1 = zero;
-    if (rows1 && pix && tx >= 2 && tx <= 61) {
-        Px5 X[5], G[3];
+
+    // ---- adjoints of the first pair (rows 4 .. S-5), written for lanes 3..60
 #pragma unroll
-        for (int k = 0; k < 5; ++k) X[k] = LA[srow(yr - 2 + k)][tx];
+    for (int k = 0; k < RPW; ++k) {
+        const int i = w + kQuadBwdWaves * k;
+        const bool rows1 = i >= 4 && i <= S - 5;
+        Px5 g_x1 = zero;
+        if (rows1 && pix[k] && tx >= 2 && tx <= 61) {
+            Px5 X[5], G[3];
 #pragma unroll
-        for (int k = 0; k < 3; ++k) G[k] = LB[w - 1 + k][tx];
-        auto interior = [&](int o) { return xin && o >= border && o < d.H - border; };
-        g_x1 = upwind_adjoint_px<1, PAY>(X, G, yr, d.H, interior, dt);
-    }
-    // first X pass: gradient of the quad's input (lanes 3..60)
-    {
+            for (int q = 0; q < 5; ++q) X[q] = LA[srow(yr[k] - 2 + q)][tx];
+#pragma unroll
+            for (int q = 0; q < 3; ++q) G[q] = LB[i - 1 + q][tx];
+            auto interior = [&](int o) { return xin && o >= border && o < d.H - border; };
+            g_x1 = upwind_adjoint_px<1, PAY>(X, G, yr[k], d.H, interior, dt);
+        }
         const Px5 G3[3] = {lane_shift_px<true, PAY>(g_x1), g_x1, lane_shift_px<false, PAY>(g_x1)};
-        const Px5 m1 = lane_shift_px<true, PAY>(F), p1 = lane_shift_px<false, PAY>(F);
-        const Px5 X5[5] = {lane_shift_px<true, PAY>(m1), m1, F, p1, lane_shift_px<false, PAY>(p1)};
-        if (rows1 && pix && tx >= 3 && tx < kQuadBwdCols + 3) {
-            const bool yin = yr >= border && yr < d.H - border;
+        const Px5 m1 = lane_shift_px<true, PAY>(F[k]), p1 = lane_shift_px<false, PAY>(F[k]);
+        const Px5 X5[5] = {lane_shift_px<true, PAY>(m1), m1, F[k], p1, lane_shift_px<false, PAY>(p1)};
+        if (rows1 && pix[k] && tx >= 3 && tx < kQuadBwdCols + 3) {
+            const bool yin = yr[k] >= border && yr[k] < d.H - border;
             auto interior = [&](int o) { return yin && o >= border && o < d.W - border; };
-            put_px<PAY>(gfi, gsi, gvi, at(b, yr, x, d), upwind_adjoint_px<0, PAY>(X5, G3, x, d.W, interior, dt), acc);
+            put_px<PAY>(gfi, gsi, gvi, at(b, yr[k], x, d), upwind_adjoint_px<0, PAY>(X5, G3, x, d.W, interior, dt),
+                        acc);
         }
     }
 }
@@ -1057,17 +1086,18 @@ void prop_quad(const float* fin, const float* sin, const float* vin, float* fout
 void prop_quad_bwd(const float* fin, const float* sin, const float* vin, const float* gfo, const float* gso,
                    const float* gvo, float* gfi, float* gsi, float* gvi, int B, int H, int W, float dt, int border,
                    int acc, hipStream_t s) {
-    const dim3 g((W + kQuadBwdCols - 1) / kQuadBwdCols, (H + kQuadBwdRows - 1) / kQuadBwdRows, B),
-        blk(64, kQuadBwdStage);
+    constexpr int RPW = kQuadBwdRowsPerWave;
+    constexpr int rows = QuadBwdShape<RPW>::rows;
+    const dim3 g((W + kQuadBwdCols - 1) / kQuadBwdCols, (H + rows - 1) / rows, B), blk(64, kQuadBwdWaves);
     const Dims d{H, W};
     auto F = [](const float* p) { return (const float2*)p; };
     auto O = [](float* p) { return (float2*)p; };
     if (sin)
-        hipLaunchKernelGGL(k_ad_prop_quad_bwd<true>, g, blk, 0, s, F(fin), sin, F(vin), F(gfo), gso, F(gvo), O(gfi), gsi,
-                           O(gvi), d, dt, border, acc);
+        hipLaunchKernelGGL((k_ad_prop_quad_bwd<true, RPW>), g, blk, 0, s, F(fin), sin, F(vin), F(gfo), gso, F(gvo),
+                           O(gfi), gsi, O(gvi), d, dt, border, acc);
     else
-        hipLaunchKernelGGL(k_ad_prop_quad_bwd<false>, g, blk, 0, s, F(fin), sin, F(vin), F(gfo), gso, F(gvo), O(gfi),
-                           gsi, O(gvi), d, dt, border, acc);
+        hipLaunchKernelGGL((k_ad_prop_quad_bwd<false, RPW>), g, blk, 0, s, F(fin), sin, F(vin), F(gfo), gso, F(gvo),
+                           O(gfi), gsi, O(gvi), d, dt, border, acc);
 }
 void prop_pair_bwd(const float* fin, const float* sin, const float* vin, const float* fmid, const float* smid,
                    const float* vmid, const float* gfo, const float* gso, const float* gvo, float* gfi, float* gsi,
