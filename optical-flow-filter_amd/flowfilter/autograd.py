@@ -18,8 +18,9 @@ is zero.
 
 Memory: the forward keeps only the filter state every ``checkpoint`` frames;
 the backward recomputes each segment of frames from its checkpoint,
-keeping that segment's intermediates, and runs the adjoints frame by frame
-in reverse.
+keeping that segment's intermediates (image models, upper flows, and the
+input of every two propagation iterations: their adjoint kernel reruns the
+passes in between), and runs the adjoints frame by frame in reverse.
 
 PyTorch must be imported before the library is loaded into the process:
 torch's HIP runtime and the system one share a soname, so the library then
