@@ -351,6 +351,8 @@ def test_cpp_api_demo(tmp_path):
     ((240, 320), 3, 4.0, [2, 2, 2], 3),
     ((37, 200), 2, 3.0, [2, 1], 2),
     ((5, 7), 1, 1.0, [1], 1),
+    ((300, 420), 2, 12.0, [2, 2], 3),
+    ((256, 300), 1, 26.0, [1], 2),
 ])
 def test_fused_matches_oracle(ffo, shape, levels, maxflow, smooth, border):
     from flowfilter import _native as N
@@ -381,6 +383,25 @@ def test_fused_matches_oracle(ffo, shape, levels, maxflow, smooth, border):
         check_frame(t, s.getFlow(), want, "staged")
         check_frame(t, g.getImage(), o.downloadImage(), "fused image")
     assert g.waitTimeouts() == 0
+
+
+@pytest.mark.parametrize("levels", [1, 2])
+def test_halo_beyond_a_tile_runs_stage_kernels(ffo, levels):
+    """maxflow 40 (40 propagation iterations at level 0): the halo of a
+    fused tile would exceed its 64 columns, so every frame runs the stage
+    kernels, with the oracle's results."""
+    from flowfilter import _native as N
+    h, w = 200, 260
+    say("halo beyond a tile", levels)
+    g, o = _pyr_pair(ffo, h, w, levels, [10.0, 40.0][:levels], 40.0, [2, 1][:levels], fused=True)
+    for t, img in enumerate(seq_blobs(h, w, 4)):
+        g.loadImage(img)
+        N.profile_begin("fused_frame")
+        g.compute()
+        _, n = N.profile_end()
+        assert n == 0, n
+        o.compute(img)
+        check_frame(t, g.getFlow(), o.downloadFlow(), "maxflow 40")
 
 
 @pytest.mark.parametrize("levels", [2, 3])
