@@ -103,6 +103,37 @@ def test_flowfilter_float_input(ffo):
         check_frame(t, g.getFlow(), o.downloadFlow(), "float input")
 
 
+@pytest.mark.parametrize("fused", [True, False])
+def test_flowfilter_float_input_with_nan_and_inf(ffo, fused):
+    """Float frames holding NaN and +-Inf pixels: the update's clamp and
+    sanitize steps (update_k.cu:66-71) turn what reaches the flow into
+    +-maxflow or 0 exactly as the oracle does; the flow stays bitwise the
+    oracle's, the brightness state too (NaN where the oracle has NaN)."""
+    import flowfilter.gpu.flowfilters as gf
+    from flowfilter.gpu.image import GPUImage
+    h, w = 96, 160
+    inp = GPUImage((h, w), 4)
+    g = gf.FlowFilter.fromGPUImage(inp)
+    g.fused = fused
+    g.gamma = 0.01
+    g.maxflow = 2.0
+    g.smoothIterations = 2
+    o = ffo.FlowFilter(h, w, 2, 2.0, 0.01, u8=False)
+    rng = np.random.default_rng(17)
+    for t, img in enumerate(seq_noise(h, w, 8)):
+        f = (img / 255.0).astype(np.float32)
+        if t in (2, 5):
+            ys, xs = rng.integers(0, h, 6), rng.integers(0, w, 6)
+            f[ys[:2], xs[:2]] = np.nan
+            f[ys[2:4], xs[2:4]] = np.inf
+            f[ys[4:], xs[4:]] = -np.inf
+        inp.upload(f)
+        g.compute()
+        o.compute(f)
+        check_frame(t, g.getFlow(), o.downloadFlow(), "float input with NaN/Inf")
+        assert np.array_equal(g.getImage(), o.downloadImage(), equal_nan=True), t
+
+
 def device_frames(frames):
     """All frames in one tall device image; returns (image, base pointer, pitch)."""
     from flowfilter.gpu.image import GPUImage
