@@ -176,3 +176,23 @@ def test_frozen_finite_differences_match_autograd():
         fd = (vals[0] - vals[1]) / (2 * step)
         g = float(x.grad[0, t, yy, xx])
         assert abs(fd - g) <= 1e-5 * max(abs(g), 1e-3 * float(x.grad.abs().max())), (t, yy, xx, fd, g)
+
+
+def test_upper_index_map_and_gather_ranges(ffo):
+    """The differentiable form's upper-level index map equals the oracle's
+    FP32 point-sampling rule (the filter library's) for even and odd sizes,
+    and the exact rule too for even sizes (for odd ones float rounding can
+    move a column to the next texel, e.g. 481 -> 240); the gather ranges of
+    its adjoint (k_ad_upper_gather) hold exactly the columns (rows) that map
+    to each upper texel."""
+    from flowfilter.autograd import _index_map, _ranges
+    for n in list(range(1, 40)) + [63, 64, 65, 127, 128, 255, 480, 481, 1088, 2047, 2048, 3840]:
+        nu = max(n // 2, 1)
+        m = _index_map(n, nu)
+        assert all(int(m[x]) == ffo.upper_index(x, n, nu) for x in range(n)), n
+        if n % 2 == 0:
+            assert all(int(m[x]) == ffo.upper_index(x, n, nu, prec="f64") for x in range(n)), n
+        lo, hi = _ranges(m, nu)
+        for v in range(nu):
+            want = [x for x in range(n) if m[x] == v]
+            assert list(range(lo[v], hi[v])) == want, (n, v)
