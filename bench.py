@@ -23,8 +23,10 @@ HIP events on its stream, over a second pass of the same K steps; `traffic`
 is the HBM bytes per launch measured with rocprofv3 FETCH_SIZE/WRITE_SIZE
 passes, calibrated on a copy kernel, read from
 profiles/fused_frame_2048x1088_2lvl_traffic.json), the 64-stream batched
-measurement, and a CPU baseline (the FP32 CPU oracle on the host cores,
-bounded sample).
+measurement, the differentiable-window measurements (forward and forward +
+backward of a 3-level 3840x2160 32-frame window and of 16 sequences of
+512x512 x 8 frames, with peak HBM), and a CPU baseline (the FP32 CPU oracle
+on the host cores, bounded sample).
 """
 import argparse
 import json
