@@ -287,7 +287,8 @@ FF_API ff_status ff_pyramidal_set_image_device(ff_stage* s, const void* dev, siz
 /* ----------------------------------------- execution controls (new) */
 /* Select how compute() runs a filter: 1 = fused per-level kernels (the
  * default), 0 = one launch per reference stage kernel. Both produce
- * identical results. Applies to FlowFilter and PyramidalFlowFilter. */
+ * identical results. Applies to FlowFilter, DeltaFlowFilter and
+ * PyramidalFlowFilter. */
 FF_API ff_status ff_filter_set_fused(ff_stage* s, int fused);
 FF_API ff_status ff_filter_get_fused(ff_stage* s, int* out);
 /* PyramidalFlowFilter and batches: replay each fused frame as a captured

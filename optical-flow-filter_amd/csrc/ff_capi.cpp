@@ -529,6 +529,7 @@ ff_status ff_filter_get_fused(ff_stage* s, int* out) {
         if (!s || !s->obj) throw null_handle("ff_stage");
         if (auto* f = dynamic_cast<FlowFilter*>(s->obj.get())) *out = f->fused();
         else if (auto* p = dynamic_cast<PyramidalFlowFilter*>(s->obj.get())) *out = p->fused();
+        else if (auto* d = dynamic_cast<DeltaFlowFilter*>(s->obj.get())) *out = d->fused();
         else throw Error(FF_ERR_WRONG_TYPE, std::string("stage is a ") + s->obj->name() + ", expected a filter");
     });
 }

@@ -103,6 +103,7 @@ public:
     int height() const { return input_.height(); }
     int width() const { return input_.width(); }
     void setFused(bool f) { fused_ = f; }
+    bool fused() const { return fused_; }
     void setTileWaves(int nw) { frame_.setWaves(nw); }
     int tileWaves() const { return frame_.waves(); }
     void setTileTopRows(int rows) { frame_.setTopRows(rows); }

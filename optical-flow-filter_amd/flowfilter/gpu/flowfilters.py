@@ -198,6 +198,14 @@ class DeltaFlowFilter(Stage):
     def propagationBorder(self, v):
         N.call("ff_deltaflowfilter_set_propagation_border", self.handle, int(v))
 
+    @property
+    def fused(self):
+        return bool(self._geti("ff_filter_get_fused"))
+
+    @fused.setter
+    def fused(self, v):
+        N.call("ff_filter_set_fused", self.handle, int(bool(v)))
+
 
 class PyramidalFlowFilter(Stage):
     """Pyramidal optical flow filter (flowfilters.pyx:213-444)."""

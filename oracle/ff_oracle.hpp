@@ -68,6 +68,30 @@ struct Field {
     void zero() { std::fill(v.begin(), v.end(), T(0)); }
 };
 
+// Planted mistakes, for the tests that show each pinning check fails on a
+// wrong oracle (0, the default, is the faithful restatement). Each one is a
+// plausible transcription error of the reference:
+//   MUT_SMOOTHED_TOP_FEED  level L-2 reads the top level's smoothed flow
+//                          instead of its unsmoothed one (flowfilter.cu:663-672)
+//   MUT_DESCENDING_UPDATE  lower levels update from L-2 down to 0 instead of
+//                          0 up to L-2 (flowfilter.cu:710-712)
+//   MUT_TOP_UPDATES_LAST   the top level updates after the lower levels
+//                          (flowfilter.cu:708)
+//   MUT_PYR_TAP_SHIFT      pyramid taps at 2i, 2i+1, 2i+2 instead of 2i-1,
+//                          2i, 2i+1 (pyramid_k.cu:33-35, 67-69, 100-102,
+//                          134-136)
+//   MUT_PYR_SUM_DOUBLE     the float pyramid adds its side taps in double
+//                          instead of float (pyramid_k.cu:105, 139)
+enum Mutation {
+    MUT_NONE = 0,
+    MUT_SMOOTHED_TOP_FEED = 1,
+    MUT_DESCENDING_UPDATE = 2,
+    MUT_TOP_UPDATES_LAST = 3,
+    MUT_PYR_TAP_SHIFT = 4,
+    MUT_PYR_SUM_DOUBLE = 5,
+};
+inline int& mutation() { static int m = MUT_NONE; return m; }
+
 inline float mad(float a, float b, float c) { return std::fmaf(a, b, c); }
 inline double mad(double a, double b, double c) { return std::fma(a, b, c); }
 inline float fmin_(float a, float b) { return std::fminf(a, b); }
@@ -381,8 +405,11 @@ inline uint8_t down_u8(uint8_t km, uint8_t k0, uint8_t kp, double) {
     return (uint8_t)((2 * (int)k0 + (int)km + (int)kp) / 4);
 }
 
-// imageDownX_float_k / imageDownY_float_k (pyramid_k.cu:81-145).
+// imageDownX_float_k / imageDownY_float_k (pyramid_k.cu:81-145): the side
+// taps are added in float, the rest in double (0.5 and 0.25 are double
+// literals), then rounded to float.
 inline float down_f32(float tm, float t0, float tp, float) {
+    if (mutation() == MUT_PYR_SUM_DOUBLE) return (float)(0.5 * (double)t0 + 0.25 * ((double)tm + (double)tp));
     const float s = tm + tp;
     return (float)(0.5 * (double)t0 + 0.25 * (double)s);
 }
@@ -396,25 +423,29 @@ template <class T, class Tin>
 std::vector<Field<Tin>> pyramid(const Field<Tin>& img, int levels) {
     std::vector<Field<Tin>> out;
     out.push_back(img);
+    // first of the three taps: 2i - 1 (MUT_PYR_TAP_SHIFT: 2i)
+    const int t0 = mutation() == MUT_PYR_TAP_SHIFT ? 0 : -1;
     for (int l = 1; l < levels; ++l) {
         const Field<Tin>& src = out.back();
         const int h = src.h, w2 = src.w / 2;
         Field<Tin> fx(h, w2, 1);
         for (int y = 0; y < h; ++y)
             for (int x = 0; x < w2; ++x) {
+                const int c = 2 * x + t0;
                 if constexpr (sizeof(Tin) == 1)
-                    fx.at(y, x) = down_u8(src.tex(y, 2 * x - 1), src.tex(y, 2 * x), src.tex(y, 2 * x + 1), T(0));
+                    fx.at(y, x) = down_u8(src.tex(y, c), src.tex(y, c + 1), src.tex(y, c + 2), T(0));
                 else
-                    fx.at(y, x) = down_f32(src.tex(y, 2 * x - 1), src.tex(y, 2 * x), src.tex(y, 2 * x + 1), T(0));
+                    fx.at(y, x) = down_f32(src.tex(y, c), src.tex(y, c + 1), src.tex(y, c + 2), T(0));
             }
         const int h2 = h / 2;
         Field<Tin> fy(h2, w2, 1);
         for (int y = 0; y < h2; ++y)
             for (int x = 0; x < w2; ++x) {
+                const int r = 2 * y + t0;
                 if constexpr (sizeof(Tin) == 1)
-                    fy.at(y, x) = down_u8(fx.tex(2 * y - 1, x), fx.tex(2 * y, x), fx.tex(2 * y + 1, x), T(0));
+                    fy.at(y, x) = down_u8(fx.tex(r, x), fx.tex(r + 1, x), fx.tex(r + 2, x), T(0));
                 else
-                    fy.at(y, x) = down_f32(fx.tex(2 * y - 1, x), fx.tex(2 * y, x), fx.tex(2 * y + 1, x), T(0));
+                    fy.at(y, x) = down_f32(fx.tex(r, x), fx.tex(r + 1, x), fx.tex(r + 2, x), T(0));
             }
         out.push_back(std::move(fy));
     }
@@ -507,9 +538,22 @@ struct DeltaLevel {
         propagate_payload<T>(smoothed, image_upd, dflow_upd, prop_iters, border, prop_flow, prop_scalar,
                              prop_vec);
     }
-    // DeltaFlowFilter::computeUpdate (flowfilter.cu:480-503).
+    // DeltaFlowFilter::computeUpdate (flowfilter.cu:480-503): on the first
+    // frame both the old image and the propagated scalar payload take a0.
     void compute_update(const Field<T>& upper) {
         if (first) { image_upd = a0; prop_scalar = a0; first = false; }
+        run_update(upper);
+    }
+    // DeltaFlowFilter::compute (flowfilter.cu:429-457), the standalone
+    // filter: on the first frame only the old image takes a0, before the
+    // propagation, which then carries it as the scalar payload.
+    template <class Tin> void compute(const Field<Tin>& img, const Field<T>& upper) {
+        compute_image_model(img);
+        if (first) { image_upd = a0; first = false; }
+        compute_propagation();
+        run_update(upper);
+    }
+    void run_update(const Field<T>& upper) {
         delta_update<T>(a0, grad, prop_scalar, prop_vec, upper, gamma, maxflow, dflow_upd, flow_upd, image_upd);
         smoothed = smooth<T>(flow_upd, smooth_iters);
     }
@@ -548,16 +592,27 @@ struct Pyramidal {
     // Upper-level flow read by level l: the top level's *unsmoothed*
     // updated flow for l = L-2, else level l+1's smoothed flow
     // (flowfilter.cu:663-672).
-    const Field<T>& upper_of(int l) const { return l == L - 2 ? top.flow_upd : low[l + 1].smoothed; }
-    // PyramidalFlowFilter::compute (flowfilter.cu:685-716).
+    const Field<T>& upper_of(int l) const {
+        if (l == L - 2) return mutation() == MUT_SMOOTHED_TOP_FEED ? top.smoothed : top.flow_upd;
+        return low[l + 1].smoothed;
+    }
+    // PyramidalFlowFilter::compute (flowfilter.cu:685-716): image models and
+    // propagation of every level, then the updates: the top level first
+    // (:708), then levels 0, 1, ..., L-2 (:710-712), so that level l < L-2
+    // reads level l+1's smoothed flow of the previous frame.
     void compute(const Field<uint8_t>& img) {
         std::vector<Field<uint8_t>> pyr = pyramid<T, uint8_t>(img, L);
         if (L == 1) { top.compute(pyr[0]); return; }
         top.compute_image_model(pyr[L - 1]);
         top.compute_propagation();
         for (int l = 0; l < L - 1; ++l) { low[l].compute_image_model(pyr[l]); low[l].compute_propagation(); }
-        top.compute_update();
-        for (int l = 0; l < L - 1; ++l) low[l].compute_update(upper_of(l));
+        const int m = mutation();
+        if (m != MUT_TOP_UPDATES_LAST) top.compute_update();
+        if (m == MUT_DESCENDING_UPDATE)
+            for (int l = L - 2; l >= 0; --l) low[l].compute_update(upper_of(l));
+        else
+            for (int l = 0; l < L - 1; ++l) low[l].compute_update(upper_of(l));
+        if (m == MUT_TOP_UPDATES_LAST) top.compute_update();
     }
     // downloadFlow (flowfilter.cu:734-741): smoothed flow of level 0.
     const Field<T>& flow() const { return L == 1 ? top.smoothed : low[0].smoothed; }

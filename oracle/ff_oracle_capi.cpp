@@ -37,6 +37,10 @@ extern "C" {
 void ffo_set_threads(int n) { num_threads() = n < 1 ? 1 : n; }
 int ffo_get_threads() { return num_threads(); }
 
+// Planted mistake (ffo::Mutation; 0 = the faithful oracle).
+void ffo_set_mutation(int m) { mutation() = m; }
+int ffo_get_mutation() { return mutation(); }
+
 // ---- image model: u8 or f32 input --------------------------------------
 #define IMAGE_MODEL(SUF, T)                                                                         \
     void ffo_image_model_u8_##SUF(int h, int w, const uint8_t* img, T* a0, T* grad) {              \
@@ -178,6 +182,43 @@ PYRAMID(f64, double)
     }
 FLOWFILTER(f32, float)
 FLOWFILTER(f64, double)
+
+// ---- standalone delta filter (DeltaFlowFilter::compute) ------------------
+// The upper-level flow (hu x wu x 2) is passed with every compute, as the
+// flow the reference filter's inputFlow image holds at that point.
+#define DELTAFILTER(SUF, T)                                                                         \
+    void* ffo_deltafilter_create_##SUF(int h, int w, int u8) {                                     \
+        auto* f = new DeltaLevel<T>();                                                              \
+        f->init(h, w, u8 != 0);                                                                     \
+        return f;                                                                                   \
+    }                                                                                               \
+    void ffo_deltafilter_destroy_##SUF(void* p) { delete (DeltaLevel<T>*)p; }                      \
+    void ffo_deltafilter_set_gamma_##SUF(void* p, float g) { ((DeltaLevel<T>*)p)->set_gamma(g); }  \
+    void ffo_deltafilter_set_maxflow_##SUF(void* p, float m) { ((DeltaLevel<T>*)p)->set_maxflow(m); } \
+    void ffo_deltafilter_set_smooth_iterations_##SUF(void* p, int n) {                              \
+        ((DeltaLevel<T>*)p)->smooth_iters = n;                                                      \
+    }                                                                                               \
+    void ffo_deltafilter_set_border_##SUF(void* p, int b) { ((DeltaLevel<T>*)p)->border = b; }     \
+    void ffo_deltafilter_compute_u8_##SUF(void* p, const uint8_t* img, int hu, int wu,             \
+                                          const float* upper) {                                     \
+        auto* f = (DeltaLevel<T>*)p;                                                                \
+        f->compute(load_u8(img, f->h, f->w), load<T>(upper, hu, wu, 2));                            \
+    }                                                                                               \
+    void ffo_deltafilter_compute_f32_##SUF(void* p, const float* img, int hu, int wu,              \
+                                           const float* upper) {                                    \
+        auto* f = (DeltaLevel<T>*)p;                                                                \
+        f->compute(load<float>(img, f->h, f->w, 1), load<T>(upper, hu, wu, 2));                     \
+    }                                                                                               \
+    /* which: 0 smoothed flow (getFlow), 1 updated flow, 2 updated image (getImage), */             \
+    /* 3 delta flow, 4 propagated scalar payload                                     */             \
+    void ffo_deltafilter_get_##SUF(void* p, int which, T* out) {                                   \
+        auto* f = (DeltaLevel<T>*)p;                                                                \
+        const Field<T>* src[] = {&f->smoothed, &f->flow_upd, &f->image_upd, &f->dflow_upd,         \
+                                 &f->prop_scalar};                                                  \
+        store(*src[which], out);                                                                    \
+    }
+DELTAFILTER(f32, float)
+DELTAFILTER(f64, double)
 
 // ---- pyramidal filter ----------------------------------------------------------
 #define PYRAMIDAL(SUF, T)                                                                           \

@@ -34,6 +34,8 @@ def lib():
         _lib.ffo_pyramidal_create_f64.restype = _vp
         _lib.ffo_flowfilter_create_f32.restype = _vp
         _lib.ffo_flowfilter_create_f64.restype = _vp
+        _lib.ffo_deltafilter_create_f32.restype = _vp
+        _lib.ffo_deltafilter_create_f64.restype = _vp
         _lib.ffo_down_u8_f32.restype = ctypes.c_uint8
         _lib.ffo_down_u8_f64.restype = ctypes.c_uint8
     return _lib
@@ -41,6 +43,32 @@ def lib():
 
 def set_threads(n):
     lib().ffo_set_threads(int(n))
+
+
+# Planted mistakes (ffo::Mutation in ff_oracle.hpp), for the tests that show
+# a pinning check fails on a wrong oracle.
+MUT_NONE = 0
+MUT_SMOOTHED_TOP_FEED = 1
+MUT_DESCENDING_UPDATE = 2
+MUT_TOP_UPDATES_LAST = 3
+MUT_PYR_TAP_SHIFT = 4
+MUT_PYR_SUM_DOUBLE = 5
+
+
+class mutated:
+    """Context manager: run the oracle with planted mistake ``m``."""
+
+    def __init__(self, m):
+        self.m = int(m)
+
+    def __enter__(self):
+        self.prev = lib().ffo_get_mutation()
+        lib().ffo_set_mutation(self.m)
+        return self
+
+    def __exit__(self, *exc):
+        lib().ffo_set_mutation(self.prev)
+        return False
 
 
 def _dt(prec):
@@ -211,6 +239,64 @@ class FlowFilter:
 
     def downloadImage(self):
         return self._get(2, 1)
+
+
+class DeltaFlowFilter:
+    """Standalone DeltaFlowFilter (its own compute(), flowfilter.cu:429-457) on
+    the CPU. The upper-level flow is passed to every compute(): the flow the
+    reference filter's inputFlow image holds at that point."""
+
+    def __init__(self, height, width, u8=True, prec="f32"):
+        self.h, self.w, self.prec, self.u8 = height, width, prec, u8
+        self._h = _fn("ffo_deltafilter_create", prec)(height, width, int(u8))
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _fn("ffo_deltafilter_destroy", self.prec)(ctypes.c_void_p(self._h))
+            self._h = None
+
+    def setGamma(self, g):
+        _fn("ffo_deltafilter_set_gamma", self.prec)(ctypes.c_void_p(self._h), ctypes.c_float(g))
+
+    def setMaxFlow(self, m):
+        _fn("ffo_deltafilter_set_maxflow", self.prec)(ctypes.c_void_p(self._h), ctypes.c_float(m))
+
+    def setSmoothIterations(self, n):
+        _fn("ffo_deltafilter_set_smooth_iterations", self.prec)(ctypes.c_void_p(self._h), int(n))
+
+    def setPropagationBorder(self, b):
+        _fn("ffo_deltafilter_set_border", self.prec)(ctypes.c_void_p(self._h), int(b))
+
+    def compute(self, img, upper):
+        up = _f32(upper)
+        hu, wu = up.shape[:2]
+        if self.u8:
+            _fn("ffo_deltafilter_compute_u8", self.prec)(ctypes.c_void_p(self._h),
+                                                         _p(np.ascontiguousarray(img, np.uint8)), hu, wu, _p(up))
+        else:
+            _fn("ffo_deltafilter_compute_f32", self.prec)(ctypes.c_void_p(self._h), _p(_f32(img)), hu, wu, _p(up))
+
+    def _get(self, which, ch):
+        shape = (self.h, self.w, ch) if ch > 1 else (self.h, self.w)
+        out = np.empty(shape, _dt(self.prec))
+        _fn("ffo_deltafilter_get", self.prec)(ctypes.c_void_p(self._h), int(which), _p(out))
+        return out
+
+    def getFlow(self):
+        """Smoothed flow (DeltaFlowFilter::getFlow, flowfilter.cu:545-547)."""
+        return self._get(0, 2)
+
+    def updatedFlow(self):
+        return self._get(1, 2)
+
+    def getImage(self):
+        return self._get(2, 1)
+
+    def deltaFlow(self):
+        return self._get(3, 2)
+
+    def propagatedScalar(self):
+        return self._get(4, 1)
 
 
 class PyramidalFlowFilter:

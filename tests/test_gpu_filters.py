@@ -330,25 +330,39 @@ def test_pyramidal_device_input_and_timing(ffo, graphs):
     assert g.elapsedTime() > 0.0
 
 
-def test_delta_flow_filter_standalone(ffo):
-    """DeltaFlowFilter.compute() on its own (image model, propagation, update, smoothing)."""
+@pytest.mark.parametrize("shape", [(48, 64), (272, 512)])
+@pytest.mark.parametrize("fused", [True, False])
+def test_delta_flow_filter_standalone(ffo, shape, fused):
+    """DeltaFlowFilter.compute() on its own (image model, first-frame copy,
+    propagation, update, smoothing; flowfilter.cu:429-457) against the
+    oracle's standalone delta filter, bitwise, with a new upper-level flow
+    uploaded before every frame."""
     import flowfilter.gpu.flowfilters as gf
     from flowfilter.gpu.image import GPUImage
-    h, w = 48, 64
+    h, w = shape
+    say("standalone delta filter", shape, "fused" if fused else "staged")
     img_g = GPUImage((h, w), 1)
     up = GPUImage((h // 2, w // 2, 2), 4)
-    upv = np.random.default_rng(3).uniform(-1, 1, (h // 2, w // 2, 2)).astype(np.float32)
-    up.upload(upv)
+    rng = np.random.default_rng(3)
     d = gf.DeltaFlowFilter(img_g, up)
+    d.fused = fused
     d.gamma = 50.0
     d.maxflow = 2.0
+    d.smoothIterations = 2
     assert d.propagationIterations == 2 and (d.height, d.width) == (h, w)
-    frames = seq_noise(h, w, 5)
-    for img in frames:
+    o = ffo.DeltaFlowFilter(h, w)
+    o.setGamma(50.0)
+    o.setMaxFlow(2.0)
+    o.setSmoothIterations(2)
+    for t, img in enumerate(seq_noise(h, w, 6, (0.9, -0.6))):
+        upv = rng.uniform(-0.5, 0.5, (h // 2, w // 2, 2)).astype(np.float32)
+        up.upload(upv)
         img_g.upload(img)
         d.compute()
-    out = d.getFlow().download(np.float32)
-    assert np.all(np.isfinite(out)) and np.abs(out).max() <= 2.0
+        o.compute(img, upv)
+        check_frame(t, d.getFlow().download(np.float32), o.getFlow(), "standalone delta flow")
+        check_frame(t, d.getImage().download(np.float32), o.getImage(), "standalone delta image")
+    assert np.abs(o.getFlow()).max() > 0.1
 
 
 def test_cpp_api_demo(tmp_path):
