@@ -11,7 +11,7 @@ noise), uploaded to HBM before timing and cycled.
 One step = compute() of one frame read in place from HBM (the frame's
 device address handed over with setImageDevice; the fused frame kernel
 loads it directly). W warm-up steps, then K timed steps between two HIP
-events on the library stream, with the stream synchronized before and
+events on the filter's stream, with the device synchronized before and
 after. The same K steps with a device loadImage (a copy into the filter's
 input image) before each compute() are reported as `value_with_copy`, and
 with a host loadImage (the reference's boundary, PCIe included; wall clock)
@@ -43,25 +43,37 @@ HBM_PEAK_GBPS = 8000.0
 
 
 def algorithmic_bytes():
-    """Compulsory HBM bytes per launch of each kernel at this workload.
+    """Compulsory HBM bytes of one frame of the filter at this workload
+    (the reference algorithm's state in + state out + input, halos excluded).
 
     Per pixel of its level: delta level 43 B (u8 image 1, flow 8, delta flow 8,
     old brightness 4, upper-level flow 8 B at 1/4 density = 2, new flow 8,
-    delta flow 8, brightness 4); top level 32 B when it reduces the level-0
-    image while loading it (flow 8, brightness 4, new smoothed flow 8,
-    brightness 4, unsmoothed flow 8; its image bytes are the level-0 image
-    already counted at level 0). The fused frame kernel runs both levels:
-    43 B per level-0 pixel + 32 B per level-1 pixel. One payload propagation
+    delta flow 8, brightness 4); top level 33 B (u8 image 1, flow 8,
+    brightness 4, new smoothed flow 8, brightness 4, unsmoothed flow 8, which
+    the level below reads); pyramid 1 B per level-1 pixel (the u8 write of
+    the reduced image). 43 * 2,228,224 + 33 * 557,056 + 557,056 =
+    114,753,536 B per frame, which the fused frame kernel moves in one launch
+    (its top level reduces the level-0 image while loading it, so its
+    actual compulsory traffic is 1.1 MB lower). One payload propagation
     pass of the staged path: 40 B (read and write flow 8, scalar 4, vector 8)."""
     n0 = H * W
     n1 = (H // 2) * (W // 2)
     return {
-        "fused_frame": 43 * n0 + 32 * n1,
+        "fused_frame": 43 * n0 + 33 * n1 + n1,
         "propagate_payload": 40 * n0,
     }
 
 
 def _cpu_model():
+    """The host CPU's model name (lscpu), else /proc/cpuinfo's."""
+    import subprocess
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.strip().startswith("Model name:"):
+                return line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
     try:
         with open("/proc/cpuinfo") as fh:
             for line in fh:
@@ -72,10 +84,18 @@ def _cpu_model():
     return "unknown"
 
 
+def host_cores():
+    """Cores this process may run on (its CPU affinity)."""
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
 def cpu_baseline(frames, budget_s=12.0, threads=None):
-    """The FP32 CPU oracle on the host: OpenMP over rows on up to 16 threads
-    (the value), then on one thread (reported beside it), each for about
-    budget_s / 2 seconds of frames."""
+    """The FP32 CPU oracle on the host: OpenMP over rows on every core this
+    process may use (the value), then on one thread (reported beside it),
+    each for about budget_s / 2 seconds of frames."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import ffo
 
@@ -91,7 +111,7 @@ def cpu_baseline(frames, budget_s=12.0, threads=None):
             if (el >= budget and n >= 2) or n >= 2000:
                 return n, el
 
-    threads = threads or min(16, os.cpu_count() or 1)
+    threads = threads or host_cores()
     n, el = run(threads, budget_s / 2)
     n1, el1 = run(1, budget_s / 2)
     return {"value": round(H * W * n / el / 1e6, 3), "unit": "Mpix/s", "cores": threads, "kind": "port",
@@ -103,12 +123,16 @@ def cpu_baseline(frames, budget_s=12.0, threads=None):
 
 def batched_bench(args, N, gf, GPUImage):
     """64 camera streams (synthetic.camera_streams, seed 3: 320x240 to
-    1920x1080, 1-3 levels each) resident in HBM. One step loads every
-    stream's next frame from device memory and advances all of them:
-    batched = BatchedFlowFilter.loadImagesDevice (one copy launch) and
-    compute() (one fused launch), plain = each filter's loadImageDevice and
-    compute() in a loop. Each mode is timed three times (best run kept).
-    Returns aggregate level-0 Mpix/s of both and their ratio."""
+    1920x1080, 1-3 levels each) resident in HBM. One step hands every
+    stream its next frame in place (device address, no copy) and advances
+    all of them: batched = BatchedFlowFilter.setImagesDevice and compute()
+    (one fused launch), plain = each filter's setImageDevice and compute()
+    in a loop, the filters sharing one stream (the reference runs every
+    filter on stream 0, one after the other). `plain_own_streams` repeats
+    the loop with each filter on its own stream (the library's default,
+    which lets the GPU overlap the filters' launches). Each mode is timed
+    three times (best run kept). Returns aggregate level-0 Mpix/s and the
+    ratios."""
     import numpy as np
     from flowfilter import synthetic
     t0 = time.perf_counter()
@@ -138,32 +162,38 @@ def batched_bench(args, N, gf, GPUImage):
 
     px = sum(sp["h"] * sp["w"] for sp in streams)
     res = {}
-    for mode in ("batched", "plain"):
+    for mode in ("batched", "plain", "plain_own_streams"):
         fs = build()
         batch = gf.BatchedFlowFilter(fs) if mode == "batched" else None
+        if mode == "plain":
+            # every filter on the first one's stream
+            for f in fs[1:]:
+                f.stream = fs[0].stream
 
         def step(i):
             ptrs = frame_ptrs(i)
             if batch is not None:
-                batch.loadImagesDevice(ptrs, pitches)
+                batch.setImagesDevice(ptrs, pitches)
                 batch.compute()
             else:
                 for f, p, q in zip(fs, ptrs, pitches):
-                    f.loadImageDevice(p, q)
+                    f.setImageDevice(p, q)
                     f.compute()
 
         for i in range(5):
             step(i)
         N.synchronize()
         # best of three timed runs per mode (the host-driven plain loop is
-        # noisy run to run)
+        # noisy run to run); wall clock between device-wide waits, which
+        # covers every stream the mode uses
         ms = None
         for rep in range(3):
-            m0 = N.Marker()
+            N.synchronize()
+            t0 = time.perf_counter()
             for i in range(args.batched_steps):
                 step(5 + rep * args.batched_steps + i)
-            m1 = N.Marker()
-            t = m0.elapsed_ms(m1)
+            N.synchronize()
+            t = (time.perf_counter() - t0) * 1e3
             ms = t if ms is None else min(ms, t)
         res[mode] = px * args.batched_steps / (ms / 1e3) / 1e6
         if batch is not None:
@@ -172,13 +202,69 @@ def batched_bench(args, N, gf, GPUImage):
             res["timeouts"] = batch.waitTimeouts()
         print("%s: %.3f ms/step, %.1f Mpix/s" % (mode, ms / args.batched_steps, res[mode]), file=sys.stderr,
               flush=True)
+        if mode == "plain":
+            for f in fs[1:]:
+                f.stream = None  # before the first filter (and its stream) goes
     return {"streams": len(streams), "mpix_per_step": round(px / 1e6, 3), "steps": args.batched_steps,
             "value": round(res["batched"], 3), "plain_value": round(res["plain"], 3), "unit": "Mpix/s",
-            "speedup": round(res["batched"] / res["plain"], 3), "launch_blocks": res["blocks"],
+            "speedup": round(res["batched"] / res["plain"], 3),
+            "plain_own_streams_value": round(res["plain_own_streams"], 3),
+            "speedup_vs_own_streams": round(res["batched"] / res["plain_own_streams"], 3),
+            "launch_blocks": res["blocks"],
             "tile": res["tile"],
             "wait_timeouts": res["timeouts"],
             "workload": "64 camera streams (seed 3): 320x240..1920x1080, 1-3 levels, gamma U[5,50], "
-                        "maxflow 4, smooth 1-2, affine motion; step = load 64 frames from HBM + advance 64 filters"}
+                        "maxflow 4, smooth 1-2, affine motion; step = hand each stream its next frame in HBM "
+                        "(in place, no copy) + advance 64 filters; plain = the same loop of filters on one stream"}
+
+
+def display_rotation_bench(args, N, GPUImage):
+    """The display and image-prediction stages at 2048x1088 (library stream):
+    FlowToColor of a flow field (12 B/px: flow 8 in, RGBA 4 out; the wheel
+    is L2-resident), and RotationalFlowImagePredictor with 4 iterations on a
+    float image (rotational flow 8 B/px written, then 8 Lax-Wendroff passes
+    of 16 B/px: flow 8 + image 4 in, image 4 out). Kernel times from HIP
+    events around each launch; GB/s = algorithmic bytes / kernel time."""
+    import numpy as np
+    from flowfilter.gpu.camera import createPerspectiveCamera
+    from flowfilter.gpu.display import FlowToColor
+    from flowfilter.gpu.rotation import RotationalFlowImagePredictor
+    rng = np.random.default_rng(5)
+    n = H * W
+    flow = GPUImage((H, W, 2), 4)
+    flow.upload(rng.uniform(-5, 5, (H, W, 2)).astype(np.float32))
+    color = FlowToColor(flow, 4.0)
+    img = GPUImage((H, W), 4)
+    img.upload(rng.uniform(0, 1, (H, W)).astype(np.float32))
+    pred = RotationalFlowImagePredictor(createPerspectiveCamera(4.0, H, W, 3.6, 4.8), img)
+    pred.iterations = 4
+    pred.setAngularVelocity(0.01, -0.02, 0.005)
+    out = {}
+    for name, stage, kernels in (("flow_to_color", color, {"flow_to_color": 12}),
+                                 ("rotational_predictor", pred, {"rotational_flow": 8, "lax_wendroff": 16})):
+        for _ in range(args.warmup):
+            stage.compute()
+        N.synchronize()
+        m0 = N.Marker()
+        for _ in range(args.steps):
+            stage.compute()
+        m1 = N.Marker()
+        ms = m0.elapsed_ms(m1) / args.steps
+        res = {"ms_per_compute": round(ms, 5), "mpix_s": round(n / (ms / 1e3) / 1e6, 1)}
+        for k, bpp in kernels.items():
+            N.profile_begin(k)
+            for _ in range(args.steps):
+                stage.compute()
+            tot, launches = N.profile_end()
+            avg = tot / max(launches, 1)
+            gbs = bpp * n / (avg / 1e3) / 1e9
+            res[k] = {"launches_per_compute": launches // args.steps, "avg_launch_ms": round(avg, 5),
+                      "bytes_per_px": bpp, "achieved_GBs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBPS, 4)}
+        out[name] = res
+        print("%s: %s" % (name, res), file=sys.stderr, flush=True)
+    out["workload"] = ("2048x1088: FlowToColor (maxflow 4) of a random flow; RotationalFlowImagePredictor "
+                       "(f 4 mm, sensor 3.6x4.8 mm, w (0.01, -0.02, 0.005), 4 iterations) of a float image")
+    return out
 
 
 def copy_bench(args, N, GPUImage):
@@ -280,6 +366,8 @@ def main():
     ap.add_argument("--no-copy-variant", action="store_true",
                     help="skip the pass that loads each frame with a copy (value_with_copy)")
     ap.add_argument("--batched-steps", type=int, default=60)
+    ap.add_argument("--no-display", action="store_true",
+                    help="skip the FlowToColor / rotational predictor measurements")
     ap.add_argument("--no-differentiable", action="store_true",
                     help="skip the differentiable-window measurements (and the torch import they need)")
     ap.add_argument("--ad-reps", type=int, default=2)
@@ -342,10 +430,10 @@ def main():
     def timed(first, fn=step):
         N.synchronize()
         w0 = time.perf_counter()
-        m0 = N.Marker()
+        m0 = N.Marker(f)
         for i in range(args.steps):
             fn(first + i)
-        m1 = N.Marker()
+        m1 = N.Marker(f)
         enq = time.perf_counter() - w0
         ms = m0.elapsed_ms(m1)
         return ms, time.perf_counter() - w0, enq
@@ -423,6 +511,8 @@ def main():
     }
     if not args.no_batched:
         out["batched"] = batched_bench(args, N, gf, GPUImage)
+    if not args.no_display:
+        out["display_rotation"] = display_rotation_bench(args, N, GPUImage)
     if not args.no_differentiable:
         out["differentiable"] = differentiable_bench(args)
     if not args.no_cpu:

@@ -23,10 +23,19 @@
  *    handle sharing the buffer; release it with ff_gpuimage_destroy. Stages
  *    hold their own references, so a buffer lives until its last holder is
  *    gone.
- *  - Work is enqueued on the library stream (ff_get_stream). compute()
- *    returns without waiting; elapsed_time, download and upload calls wait
- *    for the work they depend on. Objects are not thread-safe: use one
- *    object per thread.
+ *  - Streams. Every filter (FlowFilter, DeltaFlowFilter,
+ *    PyramidalFlowFilter) and every batch enqueues its work on a HIP stream
+ *    of its own (ff_stage_get_stream; ff_stage_set_stream substitutes a
+ *    caller's); a filter added to a batch follows the batch's stream. The
+ *    other stages, and images created outside a filter, use the library
+ *    stream (ff_get_stream / ff_set_stream). A device image's own
+ *    transfers run on the stream of the object that created it, so a
+ *    download through a filter's image handle waits for that filter's
+ *    work; where a stage reads an image another object owns, the two
+ *    streams are ordered with events. compute() returns without waiting;
+ *    elapsed_time, download and upload calls wait for the work they depend
+ *    on. Objects are not thread-safe: use each object from one thread at a
+ *    time; filters on different threads do not wait for each other.
  */
 #ifndef FLOWFILTER_AMD_H_
 #define FLOWFILTER_AMD_H_
@@ -68,9 +77,12 @@ typedef struct ff_stage ff_stage;       /* flowfilter::gpu::Stage and subclasses
 /* ------------------------------------------------------------- library */
 FF_API const char* ff_last_error(void);
 FF_API ff_status ff_version(int* major, int* minor);
-/* Wait for all work enqueued by the library. */
+/* Wait for all work enqueued by the library (device-wide). */
 FF_API ff_status ff_synchronize(void);
-/* Enqueue on the given hipStream_t (NULL restores the library stream). */
+/* The library stream: standalone stages and images created outside a
+ * filter enqueue on it. set substitutes the given hipStream_t (NULL
+ * restores the library's own); call it while no other thread is using the
+ * library. */
 FF_API ff_status ff_set_stream(void* hip_stream);
 FF_API ff_status ff_get_stream(void** hip_stream);
 
@@ -110,11 +122,34 @@ FF_API ff_status ff_gpuimage_clear(ff_gpuimage* img);
 FF_API ff_status ff_gpuimage_upload_device(ff_gpuimage* img, const void* dev, size_t pitch);
 FF_API ff_status ff_gpuimage_download_device(const ff_gpuimage* img, void* dev, size_t pitch);
 
+/* DLPack export of a device image (no reference counterpart; the zero-copy
+ * form of getFlowDevice, python/flowfilter/gpu/flowfilters.pyx:244-251).
+ * Writes a DLManagedTensor* (DLPack ABI v0.8: ROCm device, uint8 or
+ * float32, shape (height, width) or (height, width, depth), strides in
+ * elements from the row pitch) viewing the image's buffer as it is now; the
+ * tensor holds a share of the buffer until its deleter runs. With sync, the
+ * consumer's hipStream_t first waits for the work enqueued on the image's
+ * stream. A filter's state images alternate between two buffers from one
+ * compute() to the next: export again after each compute(). */
+FF_API ff_status ff_gpuimage_to_dlpack(const ff_gpuimage* img, void* consumer_stream, int sync, void** managed);
+/* Runs the deleter of a managed tensor that no consumer took. */
+FF_API ff_status ff_dlpack_delete(void* managed);
+
 /* ------------------------------------------ Stage (gpu/pipeline.h:29-78) */
 FF_API ff_status ff_stage_configure(ff_stage* s);              /* configure() */
 FF_API ff_status ff_stage_compute(ff_stage* s);                /* compute() */
 FF_API ff_status ff_stage_elapsed_time(ff_stage* s, float* ms); /* elapsedTime() */
 FF_API ff_status ff_stage_destroy(ff_stage* s);
+/* The hipStream_t the stage's work is enqueued on (no reference
+ * counterpart; the reference uses stream 0 for everything,
+ * src/gpu/pipeline.cu:21-89): a filter's or batch's own, else the library
+ * stream. */
+FF_API ff_status ff_stage_get_stream(ff_stage* s, void** hip_stream);
+/* Filters and batches (no reference counterpart): enqueue on the caller's
+ * hip_stream from now on (NULL returns to the filter's own stream). Waits
+ * for the work already enqueued on the previous stream. Other stages:
+ * FF_ERR_WRONG_TYPE. */
+FF_API ff_status ff_stage_set_stream(ff_stage* s, void* hip_stream);
 
 /* ------------------------------ ImageModel (gpu/imagemodel.h:22-63) */
 /* ImageModel() / ImageModel(GPUImage) (src/gpu/imagemodel.cu:24-43); input may be NULL */
@@ -202,6 +237,61 @@ FF_API ff_status ff_imagepyramid_set_levels(ff_stage* s, int levels);           
 FF_API ff_status ff_imagepyramid_get_levels(ff_stage* s, int* out);                    /* :222-224 */
 FF_API ff_status ff_imagepyramid_get_image(ff_stage* s, int level, ff_gpuimage** out); /* :207-219 */
 
+/* ------------------------------- FlowToColor (gpu/display.h:23-93) */
+/* getColorWheelRGBA (include/flowfilter/colorwheel.h:26, src/colorwheel.cpp:
+ * 21862-21873): the 256 x 256 RGBA colour wheel (u8, pitch 1024) as a host
+ * image; its data is the library's and read-only. */
+FF_API ff_status ff_get_color_wheel_rgba(ff_image_t* out);
+/* FlowToColor() / FlowToColor(inputFlow, maxflow) (src/gpu/display.cu:20-42);
+ * inputFlow may be NULL. Colours flow vectors (float2) as RGBA (4 x u8): the
+ * wheel read at ((u + m) / 2m, (v + m) / 2m) with clamped,
+ * normalised-coordinate point sampling (display_k.cu:14-43). */
+FF_API ff_status ff_flowtocolor_create(const ff_gpuimage* inputFlow, float maxflow, ff_stage** out);
+FF_API ff_status ff_flowtocolor_set_input_flow(ff_stage* s, const ff_gpuimage* flow);   /* :100-117 */
+FF_API ff_status ff_flowtocolor_get_color_flow(ff_stage* s, ff_gpuimage** out);         /* :119-121 */
+FF_API ff_status ff_flowtocolor_get_max_flow(ff_stage* s, float* out);                  /* :124-126 */
+FF_API ff_status ff_flowtocolor_set_max_flow(ff_stage* s, float maxflow);               /* :129-137 */
+FF_API ff_status ff_flowtocolor_download_color_flow(ff_stage* s, ff_image_t* color);    /* :140-142 */
+
+/* ----------- camera, rotational prediction, Lax-Wendroff propagation */
+/* perspectiveCamera (include/flowfilter/gpu/camera.h:20-25), layout-identical. */
+typedef struct ff_camera {
+    float alphaX;
+    float alphaY;
+    float centerX;
+    float centerY;
+} ff_camera;
+/* createPerspectiveCamera (src/gpu/camera.cu:14-29) */
+FF_API ff_status ff_create_perspective_camera(float focalLength, int height, int width, float sensorHeight,
+                                              float sensorWidth, ff_camera* out);
+/* LaxWendroffPropagator() / (inputFlow, inputImage) (src/gpu/propagation.cu:
+ * 509-525; gpu/propagation.h:218-291); both inputs or neither. Propagates a
+ * float image (depth 1 or 4) along a flow field: iterations of a Y then an
+ * X Lax-Wendroff pass (propagation_k.h:49-132), dt = 1/iterations. */
+FF_API ff_status ff_laxwendroff_create(const ff_gpuimage* inputFlow, const ff_gpuimage* inputImage, ff_stage** out);
+FF_API ff_status ff_laxwendroff_set_iterations(ff_stage* s, int n);                     /* :647-658 */
+FF_API ff_status ff_laxwendroff_get_iterations(ff_stage* s, int* out);                  /* :660-662 */
+FF_API ff_status ff_laxwendroff_get_dt(ff_stage* s, float* out);                        /* :665-667 */
+FF_API ff_status ff_laxwendroff_set_input_flow(ff_stage* s, const ff_gpuimage* flow);   /* :671-687 */
+FF_API ff_status ff_laxwendroff_set_input_image(ff_stage* s, const ff_gpuimage* img);   /* :690-714 */
+FF_API ff_status ff_laxwendroff_get_flow(ff_stage* s, ff_gpuimage** out);               /* :717-719 */
+FF_API ff_status ff_laxwendroff_get_propagated_image(ff_stage* s, ff_gpuimage** out);   /* :722-724 */
+/* RotationalFlowImagePredictor() / (cam) / (cam, inputImage) (src/gpu/
+ * rotation.cu:22-42; gpu/rotation.h:24-88): cam NULL -> the first, image NULL
+ * -> the second. compute() writes the rotational flow of the camera's
+ * angular velocity (rotation_k.cu:17-43) and propagates the input image
+ * along it. */
+FF_API ff_status ff_rotation_create(const ff_camera* cam, const ff_gpuimage* inputImage, ff_stage** out);
+FF_API ff_status ff_rotation_set_input_image(ff_stage* s, const ff_gpuimage* img);      /* :91-97 */
+FF_API ff_status ff_rotation_get_predicted_image(ff_stage* s, ff_gpuimage** out);       /* :100-103 */
+FF_API ff_status ff_rotation_get_optical_flow(ff_stage* s, ff_gpuimage** out);          /* :105-108 */
+FF_API ff_status ff_rotation_set_camera(ff_stage* s, const ff_camera* cam);             /* :111-113 */
+/* the camera set last (no reference counterpart: the Python wrapper keeps it) */
+FF_API ff_status ff_rotation_get_camera(ff_stage* s, ff_camera* out);
+FF_API ff_status ff_rotation_set_angular_velocity(ff_stage* s, float wx, float wy, float wz); /* :116-121 */
+FF_API ff_status ff_rotation_set_iterations(ff_stage* s, int n);                        /* :124-127 */
+FF_API ff_status ff_rotation_get_iterations(ff_stage* s, int* out);                     /* :129-132 */
+
 /* ------------------------------ FlowFilter (gpu/flowfilter.h:31-145) */
 /* FlowFilter(h, w, smoothIterations, maxflow, gamma) (src/gpu/flowfilter.cu:45-76) */
 FF_API ff_status ff_flowfilter_create(int height, int width, int smoothIterations, float maxflow, float gamma,
@@ -278,10 +368,11 @@ FF_API ff_status ff_pyramidal_load_image_device(ff_stage* s, const void* dev, si
 /* Frame in place (no reference counterpart): the next compute() reads its
  * frame from `dev` (u8, the filter's height x width, row pitch `pitch`
  * bytes) where it lies, without the copy into the input image; the memory
- * must hold the frame until that compute() has run on the library stream.
+ * must hold the frame until that compute() has run on the filter's stream.
  * The filter copies it in instead on its first frame, for pyramids deeper
- * than three levels, on the staged path, inside a batch, or when `dev` or
- * `pitch` is not a multiple of 4. A later load_image* replaces it. */
+ * than three levels, on the staged path, when `dev` or `pitch` is not a
+ * multiple of 4 or pitch * height reaches 2^31, and in a batch beyond its
+ * first 128 frames in place. A later load_image* replaces it. */
 FF_API ff_status ff_pyramidal_set_image_device(ff_stage* s, const void* dev, size_t pitch);
 
 /* ----------------------------------------- execution controls (new) */
@@ -299,11 +390,21 @@ FF_API ff_status ff_filter_get_fused(ff_stage* s, int* out);
  * 1 % of a 64-stream step). */
 FF_API ff_status ff_filter_set_graphs(ff_stage* s, int graphs);
 FF_API ff_status ff_filter_get_graphs(ff_stage* s, int* out);
-/* PyramidalFlowFilter only (diagnostic, no reference counterpart): number of
- * times a level of a fused frame stopped waiting for the level above it
- * after the spin limit instead of seeing it complete; any nonzero count
- * means the flow of those frames is not valid. Waits for the device. */
+/* PyramidalFlowFilter and batches (diagnostic, no reference counterpart):
+ * number of times a level of a fused frame stopped waiting for the level
+ * above it after the spin limit instead of seeing it complete; any nonzero
+ * count means the flow of those frames is not valid. Waits for the filter's
+ * stream. Such a wait is also reported as FF_ERR_RUNTIME by the next
+ * ff_stage_elapsed_time, ff_pyramidal_download_flow or
+ * ff_pyramidal_download_image of the filter (of every filter of the batch,
+ * and ff_stage_elapsed_time of the batch). */
 FF_API ff_status ff_filter_get_wait_timeouts(ff_stage* s, unsigned long long* out);
+/* Test hook (no reference counterpart), PyramidalFlowFilter and batches:
+ * spins > 0 makes the waits of later fused frames wait for a completion
+ * count that no launch reaches, so each gives up after `spins` polls and
+ * counts a timeout (every kernel still ends; the flow of those frames is
+ * not valid); 0 restores the normal waits. For tests of the error path. */
+FF_API ff_status ff_filter_set_wait_test(ff_stage* s, int spins);
 
 /* Tile height of the fused frame kernel, in waves per workgroup (no
  * reference counterpart): 0 (the default) times the candidates 16, 15, 8, 5
@@ -342,6 +443,12 @@ FF_API ff_status ff_batch_size(ff_stage* batch, int* out);
  * launch for all of them; n must equal the batch size. Enqueued on the
  * library stream, like ff_pyramidal_load_image_device. */
 FF_API ff_status ff_batch_load_images_device(ff_stage* batch, const void* const* frames, const size_t* pitches, int n);
+/* The next ff_stage_compute(batch) reads every filter's frame in place:
+ * ff_pyramidal_set_image_device of filter i with frames[i], pitches[i]. The
+ * batched launch reads up to 128 such frames where they lie (no copy
+ * launch); the memory must hold the frames until that compute has run on
+ * the batch's stream. n must equal the batch size. */
+FF_API ff_status ff_batch_set_images_device(ff_stage* batch, const void* const* frames, const size_t* pitches, int n);
 /* Workgroups of the last fused launch (diagnostic). */
 FF_API ff_status ff_batch_get_blocks(ff_stage* batch, int* out);
 
@@ -480,6 +587,8 @@ FF_API ff_status ff_profile_end(double* total_ms, long long* count);
  * between the two markers. For timing loops of compute() calls. */
 typedef struct ff_marker ff_marker;
 FF_API ff_status ff_marker_record(ff_marker** out);
+/* The same on the stream of stage s (ff_stage_get_stream). */
+FF_API ff_status ff_marker_record_on(ff_stage* s, ff_marker** out);
 FF_API ff_status ff_marker_elapsed(ff_marker* start, ff_marker* stop, float* ms);
 FF_API ff_status ff_marker_destroy(ff_marker* m);
 

@@ -49,6 +49,13 @@ inline image_t createImage(int height, int width, std::size_t pixelSize) {
     return createImage(height, width, 1, pixelSize);
 }
 inline void destroyImage(image_t& img) { check(ff_destroy_image(&img)); }
+// getColorWheelRGBA (include/flowfilter/colorwheel.h:26): the 256 x 256
+// RGBA colour wheel of FlowToColor (read-only library data).
+inline image_t getColorWheelRGBA() {
+    image_t w{};
+    check(ff_get_color_wheel_rgba(&w));
+    return w;
+}
 
 namespace gpu {
 
@@ -298,6 +305,25 @@ private:
     void make(const ff_gpuimage* i, int levels) {
         ff_stage* s = nullptr;
         check(ff_imagepyramid_create(i, levels, &s));
+        reset(s);
+    }
+};
+
+// FlowToColor (include/flowfilter/gpu/display.h:23-93).
+class FlowToColor : public Stage {
+public:
+    FlowToColor() { make(nullptr, 1.0f); }
+    FlowToColor(GPUImage inputFlow, const float maxflow) { make(H(inputFlow), maxflow); }
+    void downloadColorFlow(image_t& colorFlow) { check(ff_flowtocolor_download_color_flow(s_.get(), &colorFlow)); }
+    void setInputFlow(GPUImage f) { check(ff_flowtocolor_set_input_flow(s_.get(), H(f))); }
+    GPUImage getColorFlow() { return img(ff_flowtocolor_get_color_flow); }
+    float getMaxFlow() const { return getf(ff_flowtocolor_get_max_flow); }
+    void setMaxFlow(const float maxflow) { check(ff_flowtocolor_set_max_flow(s_.get(), maxflow)); }
+
+private:
+    void make(const ff_gpuimage* f, float m) {
+        ff_stage* s = nullptr;
+        check(ff_flowtocolor_create(f, m, &s));
         reset(s);
     }
 };

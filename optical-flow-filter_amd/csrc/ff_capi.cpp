@@ -47,6 +47,16 @@ ff_status guard(F&& f) {
     }
 }
 
+// guard() with the stream of stage `s` (a filter or batch) bound to the
+// calling thread for the call.
+template <class F>
+ff_status guard_on(const ff_stage* s, F&& f) {
+    return guard([&] {
+        StreamScope sc(s && s->obj ? s->obj->box() : nullptr);
+        f();
+    });
+}
+
 Error null_handle(const char* what) { return Error(FF_ERR_NULL_HANDLE, std::string("null ") + what); }
 
 const Image& img_of(const ff_gpuimage* h) {
@@ -209,17 +219,126 @@ ff_status ff_gpuimage_download_device(const ff_gpuimage* img, void* dev, size_t 
     return guard([&] { img_of(img).download_device(dev, pitch); });
 }
 
+// ------------------------------------------------------------------ DLPack
+// The DLPack exchange structs (dmlc/dlpack, the ABI of v0.8 that consumers
+// read from a "dltensor" capsule).
+namespace {
+struct DLDevice {
+    int32_t device_type;
+    int32_t device_id;
+};
+struct DLDataType {
+    uint8_t code;
+    uint8_t bits;
+    uint16_t lanes;
+};
+struct DLTensor {
+    void* data;
+    DLDevice device;
+    int32_t ndim;
+    DLDataType dtype;
+    int64_t* shape;
+    int64_t* strides;
+    uint64_t byte_offset;
+};
+struct DLManagedTensor {
+    DLTensor dl_tensor;
+    void* manager_ctx;
+    void (*deleter)(DLManagedTensor*);
+};
+constexpr int32_t kDLROCM = 10;
+constexpr uint8_t kDLUInt = 1, kDLFloat = 2;
+
+// An exported view: the managed tensor, its shape and strides, and a share
+// of the image (keeps the buffer alive until the consumer lets go).
+struct DLExport {
+    DLManagedTensor m;
+    int64_t shape[3];
+    int64_t strides[3];
+    Image img;
+};
+void dl_delete(DLManagedTensor* m) {
+    if (m) delete static_cast<DLExport*>(m->manager_ctx);
+}
+}  // namespace
+
+ff_status ff_gpuimage_to_dlpack(const ff_gpuimage* img, void* consumer_stream, int sync, void** out) {
+    return guard([&] {
+        out_ptr(out, "managed tensor");
+        const Image& im = img_of(img);
+        if (!im.allocated()) throw logic_error("GPUImage to DLPack: unallocated image");
+        if (im.itemSize() != 1 && im.itemSize() != 4)
+            throw invalid_argument("GPUImage to DLPack: item size must be 1 (uint8) or 4 (float32)");
+        if (im.pitch() % (size_t)im.itemSize())
+            throw invalid_argument("GPUImage to DLPack: pitch is not a multiple of the item size");
+        if (sync) {
+            // the consumer's stream waits for the work enqueued on the image
+            hipStream_t home = im.homeStream();
+            if ((hipStream_t)consumer_stream != home) {
+                hipEvent_t e = im.home()->event();
+                FF_HIP(hipEventRecord(e, home));
+                FF_HIP(hipStreamWaitEvent((hipStream_t)consumer_stream, e, 0));
+            }
+        }
+        int dev = 0;
+        FF_HIP(hipGetDevice(&dev));
+        auto* x = new DLExport();
+        x->img = im;
+        const int nd = im.depth() > 1 ? 3 : 2;
+        x->shape[0] = im.height();
+        x->shape[1] = im.width();
+        x->shape[2] = im.depth();
+        x->strides[0] = (int64_t)(im.pitch() / (size_t)im.itemSize());
+        x->strides[1] = im.depth();
+        x->strides[2] = 1;
+        DLTensor& t = x->m.dl_tensor;
+        t.data = im.data();
+        t.device = DLDevice{kDLROCM, dev};
+        t.ndim = nd;
+        t.dtype = im.itemSize() == 1 ? DLDataType{kDLUInt, 8, 1} : DLDataType{kDLFloat, 32, 1};
+        t.shape = x->shape;
+        t.strides = x->strides;
+        t.byte_offset = 0;
+        x->m.manager_ctx = x;
+        x->m.deleter = dl_delete;
+        *out = &x->m;
+    });
+}
+
+ff_status ff_dlpack_delete(void* managed) {
+    return guard([&] {
+        auto* m = static_cast<DLManagedTensor*>(managed);
+        if (m && m->deleter) m->deleter(m);
+    });
+}
+
 // ------------------------------------------------------------------- Stage
 ff_status ff_stage_configure(ff_stage* s) {
-    return guard([&] { as<Stage>(s, "Stage").configure(); });
+    return guard_on(s, [&] { as<Stage>(s, "Stage").configure(); });
 }
 ff_status ff_stage_compute(ff_stage* s) {
-    return guard([&] { as<Stage>(s, "Stage").compute(); });
+    return guard_on(s, [&] { as<Stage>(s, "Stage").compute(); });
 }
 ff_status ff_stage_elapsed_time(ff_stage* s, float* ms) {
-    return guard([&] {
+    return guard_on(s, [&] {
         out_ptr(ms, "ms");
         *ms = as<Stage>(s, "Stage").elapsedTime();
+    });
+}
+ff_status ff_stage_get_stream(ff_stage* s, void** out) {
+    return guard_on(s, [&] {
+        out_ptr(out, "stream");
+        if (!s || !s->obj) throw null_handle("ff_stage");
+        *out = (void*)stream();
+    });
+}
+ff_status ff_stage_set_stream(ff_stage* s, void* hip_stream) {
+    return guard([&] {
+        if (!s || !s->obj) throw null_handle("ff_stage");
+        if (!s->obj->box())
+            throw Error(FF_ERR_WRONG_TYPE, std::string("stage is a ") + s->obj->name() +
+                                               ", expected a filter or a batch (only they have streams of their own)");
+        s->obj->box()->use((hipStream_t)hip_stream);
     });
 }
 ff_status ff_stage_destroy(ff_stage* s) {
@@ -229,26 +348,26 @@ ff_status ff_stage_destroy(ff_stage* s) {
 // -------------------------------------------------------- helper macros
 #define SETTER(FN, CLS, TYPE, CALL)                                  \
     ff_status FN(ff_stage* s, TYPE v) {                              \
-        return guard([&] { as<CLS>(s, #CLS).CALL(v); });             \
+        return guard_on(s, [&] { as<CLS>(s, #CLS).CALL(v); });             \
     }
 #define GETTER(FN, CLS, TYPE, CALL)                                  \
     ff_status FN(ff_stage* s, TYPE* out) {                           \
-        return guard([&] {                                           \
+        return guard_on(s, [&] {                                           \
             out_ptr(out, #FN);                                       \
             *out = (TYPE)as<CLS>(s, #CLS).CALL();                    \
         });                                                          \
     }
 #define IMG_SETTER(FN, CLS, CALL)                                    \
     ff_status FN(ff_stage* s, const ff_gpuimage* img) {              \
-        return guard([&] { as<CLS>(s, #CLS).CALL(img_of(img)); });   \
+        return guard_on(s, [&] { as<CLS>(s, #CLS).CALL(img_of(img)); });   \
     }
 #define IMG_GETTER(FN, CLS, CALL)                                    \
     ff_status FN(ff_stage* s, ff_gpuimage** out) {                   \
-        return guard([&] { new_image(as<CLS>(s, #CLS).CALL(), out); }); \
+        return guard_on(s, [&] { new_image(as<CLS>(s, #CLS).CALL(), out); }); \
     }
 #define ACTION(FN, CLS, CALL)                                        \
     ff_status FN(ff_stage* s) {                                      \
-        return guard([&] { as<CLS>(s, #CLS).CALL(); });              \
+        return guard_on(s, [&] { as<CLS>(s, #CLS).CALL(); });              \
     }
 
 // -------------------------------------------------------------- ImageModel
@@ -275,7 +394,7 @@ GETTER(ff_flowpropagator_get_dt, FlowPropagator, float, getDt)
 SETTER(ff_flowpropagator_set_border, FlowPropagator, int, setBorder)
 GETTER(ff_flowpropagator_get_border, FlowPropagator, int, getBorder)
 ff_status ff_flowpropagator_set_invert_input_flow(ff_stage* s, int invert) {
-    return guard([&] { as<FlowPropagator>(s, "FlowPropagator").setInvertInputFlow(invert != 0); });
+    return guard_on(s, [&] { as<FlowPropagator>(s, "FlowPropagator").setInvertInputFlow(invert != 0); });
 }
 GETTER(ff_flowpropagator_get_invert_input_flow, FlowPropagator, int, getInvertInputFlow)
 IMG_SETTER(ff_flowpropagator_set_input_flow, FlowPropagator, setInputFlow)
@@ -388,7 +507,7 @@ IMG_SETTER(ff_imagepyramid_set_input_image, ImagePyramid, setInputImage)
 SETTER(ff_imagepyramid_set_levels, ImagePyramid, int, setLevels)
 GETTER(ff_imagepyramid_get_levels, ImagePyramid, int, getLevels)
 ff_status ff_imagepyramid_get_image(ff_stage* s, int level, ff_gpuimage** out) {
-    return guard([&] { new_image(as<ImagePyramid>(s, "ImagePyramid").getImage(level), out); });
+    return guard_on(s, [&] { new_image(as<ImagePyramid>(s, "ImagePyramid").getImage(level), out); });
 }
 
 // -------------------------------------------------------------- FlowFilter
@@ -403,13 +522,13 @@ ACTION(ff_flowfilter_compute_propagation, FlowFilter, computePropagation)
 ACTION(ff_flowfilter_compute_update, FlowFilter, computeUpdate)
 IMG_SETTER(ff_flowfilter_set_input_image, FlowFilter, setInputImage)
 ff_status ff_flowfilter_load_image(ff_stage* s, const ff_image_t* image) {
-    return guard([&] { as<FlowFilter>(s, "FlowFilter").loadImage(host_img(image)); });
+    return guard_on(s, [&] { as<FlowFilter>(s, "FlowFilter").loadImage(host_img(image)); });
 }
 ff_status ff_flowfilter_download_flow(ff_stage* s, ff_image_t* flow) {
-    return guard([&] { as<FlowFilter>(s, "FlowFilter").downloadFlow(host_img(flow)); });
+    return guard_on(s, [&] { as<FlowFilter>(s, "FlowFilter").downloadFlow(host_img(flow)); });
 }
 ff_status ff_flowfilter_download_image(ff_stage* s, ff_image_t* image) {
-    return guard([&] { as<FlowFilter>(s, "FlowFilter").downloadImage(host_img(image)); });
+    return guard_on(s, [&] { as<FlowFilter>(s, "FlowFilter").downloadImage(host_img(image)); });
 }
 IMG_GETTER(ff_flowfilter_get_flow, FlowFilter, getFlow)
 GETTER(ff_flowfilter_get_gamma, FlowFilter, float, getGamma)
@@ -424,14 +543,18 @@ GETTER(ff_flowfilter_get_propagation_iterations, FlowFilter, int, getPropagation
 GETTER(ff_flowfilter_height, FlowFilter, int, height)
 GETTER(ff_flowfilter_width, FlowFilter, int, width)
 ff_status ff_flowfilter_load_image_device(ff_stage* s, const void* dev, size_t pitch) {
-    return guard([&] { as<FlowFilter>(s, "FlowFilter").loadImageDevice(dev, pitch); });
+    return guard_on(s, [&] { as<FlowFilter>(s, "FlowFilter").loadImageDevice(dev, pitch); });
 }
 
 // --------------------------------------------------------- DeltaFlowFilter
 ff_status ff_deltaflowfilter_create(const ff_gpuimage* image, const ff_gpuimage* flow, ff_stage** out) {
     return guard([&] {
-        if (image && flow) new_stage<DeltaFlowFilter>(out, image->img, flow->img);
-        else new_stage<DeltaFlowFilter>(out);
+        if (image && flow) {
+            new_stage<DeltaFlowFilter>(out, image->img, flow->img);
+        } else {
+            new_stage<DeltaFlowFilter>(out);
+            (*out)->obj->ensureOwnStream();
+        }
     });
 }
 ACTION(ff_deltaflowfilter_compute_image_model, DeltaFlowFilter, computeImageModel)
@@ -459,41 +582,41 @@ ff_status ff_pyramidal_create(int h, int w, int levels, ff_stage** out) {
 }
 IMG_GETTER(ff_pyramidal_get_flow, PyramidalFlowFilter, getFlow)
 ff_status ff_pyramidal_load_image(ff_stage* s, const ff_image_t* image) {
-    return guard([&] { as<PyramidalFlowFilter>(s, "PyramidalFlowFilter").loadImage(host_img(image)); });
+    return guard_on(s, [&] { as<PyramidalFlowFilter>(s, "PyramidalFlowFilter").loadImage(host_img(image)); });
 }
 ff_status ff_pyramidal_download_flow(ff_stage* s, ff_image_t* flow) {
-    return guard([&] { as<PyramidalFlowFilter>(s, "PyramidalFlowFilter").downloadFlow(host_img(flow)); });
+    return guard_on(s, [&] { as<PyramidalFlowFilter>(s, "PyramidalFlowFilter").downloadFlow(host_img(flow)); });
 }
 ff_status ff_pyramidal_download_image(ff_stage* s, ff_image_t* image) {
-    return guard([&] { as<PyramidalFlowFilter>(s, "PyramidalFlowFilter").downloadImage(host_img(image)); });
+    return guard_on(s, [&] { as<PyramidalFlowFilter>(s, "PyramidalFlowFilter").downloadImage(host_img(image)); });
 }
 ff_status ff_pyramidal_get_gamma(ff_stage* s, int level, float* out) {
-    return guard([&] {
+    return guard_on(s, [&] {
         out_ptr(out, "gamma");
         *out = as<PyramidalFlowFilter>(s, "PyramidalFlowFilter").getGamma(level);
     });
 }
 ff_status ff_pyramidal_set_gamma(ff_stage* s, int level, float gamma) {
-    return guard([&] { as<PyramidalFlowFilter>(s, "PyramidalFlowFilter").setGamma(level, gamma); });
+    return guard_on(s, [&] { as<PyramidalFlowFilter>(s, "PyramidalFlowFilter").setGamma(level, gamma); });
 }
 ff_status ff_pyramidal_set_gamma_vector(ff_stage* s, const float* gamma, int n) {
-    return guard([&] {
+    return guard_on(s, [&] {
         if (n > 0 && !gamma) throw Error(FF_ERR_INVALID_ARGUMENT, "null gamma vector");
         std::vector<float> v(gamma, gamma + (n > 0 ? n : 0));
         as<PyramidalFlowFilter>(s, "PyramidalFlowFilter").setGamma(v);
     });
 }
 ff_status ff_pyramidal_get_smooth_iterations(ff_stage* s, int level, int* out) {
-    return guard([&] {
+    return guard_on(s, [&] {
         out_ptr(out, "iterations");
         *out = as<PyramidalFlowFilter>(s, "PyramidalFlowFilter").getSmoothIterations(level);
     });
 }
 ff_status ff_pyramidal_set_smooth_iterations(ff_stage* s, int level, int n) {
-    return guard([&] { as<PyramidalFlowFilter>(s, "PyramidalFlowFilter").setSmoothIterations(level, n); });
+    return guard_on(s, [&] { as<PyramidalFlowFilter>(s, "PyramidalFlowFilter").setSmoothIterations(level, n); });
 }
 ff_status ff_pyramidal_set_smooth_iterations_vector(ff_stage* s, const int* iters, int n) {
-    return guard([&] {
+    return guard_on(s, [&] {
         if (n > 0 && !iters) throw Error(FF_ERR_INVALID_ARGUMENT, "null iterations vector");
         std::vector<int> v(iters, iters + (n > 0 ? n : 0));
         as<PyramidalFlowFilter>(s, "PyramidalFlowFilter").setSmoothIterations(v);
@@ -507,15 +630,15 @@ GETTER(ff_pyramidal_height, PyramidalFlowFilter, int, height)
 GETTER(ff_pyramidal_width, PyramidalFlowFilter, int, width)
 GETTER(ff_pyramidal_levels, PyramidalFlowFilter, int, levels)
 ff_status ff_pyramidal_load_image_device(ff_stage* s, const void* dev, size_t pitch) {
-    return guard([&] { as<PyramidalFlowFilter>(s, "PyramidalFlowFilter").loadImageDevice(dev, pitch); });
+    return guard_on(s, [&] { as<PyramidalFlowFilter>(s, "PyramidalFlowFilter").loadImageDevice(dev, pitch); });
 }
 ff_status ff_pyramidal_set_image_device(ff_stage* s, const void* dev, size_t pitch) {
-    return guard([&] { as<PyramidalFlowFilter>(s, "PyramidalFlowFilter").setImageDevice(dev, pitch); });
+    return guard_on(s, [&] { as<PyramidalFlowFilter>(s, "PyramidalFlowFilter").setImageDevice(dev, pitch); });
 }
 
 // ------------------------------------------------------- execution controls
 ff_status ff_filter_set_fused(ff_stage* s, int fused) {
-    return guard([&] {
+    return guard_on(s, [&] {
         if (!s || !s->obj) throw null_handle("ff_stage");
         if (auto* f = dynamic_cast<FlowFilter*>(s->obj.get())) f->setFused(fused != 0);
         else if (auto* p = dynamic_cast<PyramidalFlowFilter*>(s->obj.get())) p->setFused(fused != 0);
@@ -524,7 +647,7 @@ ff_status ff_filter_set_fused(ff_stage* s, int fused) {
     });
 }
 ff_status ff_filter_get_fused(ff_stage* s, int* out) {
-    return guard([&] {
+    return guard_on(s, [&] {
         out_ptr(out, "fused");
         if (!s || !s->obj) throw null_handle("ff_stage");
         if (auto* f = dynamic_cast<FlowFilter*>(s->obj.get())) *out = f->fused();
@@ -535,14 +658,14 @@ ff_status ff_filter_get_fused(ff_stage* s, int* out) {
 }
 
 ff_status ff_filter_set_graphs(ff_stage* s, int graphs) {
-    return guard([&] {
+    return guard_on(s, [&] {
         if (!s || !s->obj) throw null_handle("ff_stage");
         if (auto* b = dynamic_cast<BatchedFlowFilter*>(s->obj.get())) b->setGraphs(graphs != 0);
         else as<PyramidalFlowFilter>(s, "PyramidalFlowFilter").setGraphs(graphs != 0);
     });
 }
 ff_status ff_filter_get_graphs(ff_stage* s, int* out) {
-    return guard([&] {
+    return guard_on(s, [&] {
         out_ptr(out, "graphs");
         if (!s || !s->obj) throw null_handle("ff_stage");
         if (auto* b = dynamic_cast<BatchedFlowFilter*>(s->obj.get())) *out = b->graphs();
@@ -551,7 +674,7 @@ ff_status ff_filter_get_graphs(ff_stage* s, int* out) {
 }
 
 ff_status ff_filter_get_wait_timeouts(ff_stage* s, unsigned long long* out) {
-    return guard([&] {
+    return guard_on(s, [&] {
         out_ptr(out, "timeouts");
         if (!s || !s->obj) throw null_handle("ff_stage");
         if (auto* b = dynamic_cast<BatchedFlowFilter*>(s->obj.get())) *out = b->waitTimeouts();
@@ -559,20 +682,28 @@ ff_status ff_filter_get_wait_timeouts(ff_stage* s, unsigned long long* out) {
     });
 }
 
+ff_status ff_filter_set_wait_test(ff_stage* s, int spins) {
+    return guard_on(s, [&] {
+        if (!s || !s->obj) throw null_handle("ff_stage");
+        if (auto* b = dynamic_cast<BatchedFlowFilter*>(s->obj.get())) b->setWaitTest(spins);
+        else as<PyramidalFlowFilter>(s, "PyramidalFlowFilter").setWaitTest(spins);
+    });
+}
+
 ff_status ff_filter_set_tile_waves(ff_stage* s, int waves) {
-    return guard([&] { on_fused_filter(s, [&](auto& f) { f.setTileWaves(waves); }); });
+    return guard_on(s, [&] { on_fused_filter(s, [&](auto& f) { f.setTileWaves(waves); }); });
 }
 ff_status ff_filter_get_tile_waves(ff_stage* s, int* out) {
-    return guard([&] {
+    return guard_on(s, [&] {
         out_ptr(out, "waves");
         on_fused_filter(s, [&](auto& f) { *out = f.tileWaves(); });
     });
 }
 ff_status ff_filter_set_tile_top_rows(ff_stage* s, int rows) {
-    return guard([&] { on_fused_filter(s, [&](auto& f) { f.setTileTopRows(rows); }); });
+    return guard_on(s, [&] { on_fused_filter(s, [&](auto& f) { f.setTileTopRows(rows); }); });
 }
 ff_status ff_filter_get_tile_top_rows(ff_stage* s, int* out) {
-    return guard([&] {
+    return guard_on(s, [&] {
         out_ptr(out, "rows");
         on_fused_filter(s, [&](auto& f) { *out = f.tileTopRows(); });
     });
@@ -583,27 +714,109 @@ ff_status ff_batch_create(ff_stage** out) {
     return guard([&] { new_stage<BatchedFlowFilter>(out); });
 }
 ff_status ff_batch_add(ff_stage* batch, ff_stage* filter) {
-    return guard([&] {
+    return guard_on(batch, [&] {
         BatchedFlowFilter& b = as<BatchedFlowFilter>(batch, "BatchedFlowFilter");
         as<PyramidalFlowFilter>(filter, "PyramidalFlowFilter");
         b.add(std::static_pointer_cast<PyramidalFlowFilter>(filter->obj));
     });
 }
 ff_status ff_batch_size(ff_stage* batch, int* out) {
-    return guard([&] {
+    return guard_on(batch, [&] {
         out_ptr(out, "size");
         *out = as<BatchedFlowFilter>(batch, "BatchedFlowFilter").size();
     });
 }
 ff_status ff_batch_load_images_device(ff_stage* batch, const void* const* frames, const size_t* pitches, int n) {
-    return guard([&] { as<BatchedFlowFilter>(batch, "BatchedFlowFilter").loadImagesDevice(frames, pitches, n); });
+    return guard_on(batch, [&] { as<BatchedFlowFilter>(batch, "BatchedFlowFilter").loadImagesDevice(frames, pitches, n); });
+}
+ff_status ff_batch_set_images_device(ff_stage* batch, const void* const* frames, const size_t* pitches, int n) {
+    return guard_on(batch, [&] { as<BatchedFlowFilter>(batch, "BatchedFlowFilter").setImagesDevice(frames, pitches, n); });
 }
 ff_status ff_batch_get_blocks(ff_stage* batch, int* out) {
-    return guard([&] {
+    return guard_on(batch, [&] {
         out_ptr(out, "blocks");
         *out = as<BatchedFlowFilter>(batch, "BatchedFlowFilter").lastBlocks();
     });
 }
+
+// ------------------------------------------------------------- FlowToColor
+ff_status ff_get_color_wheel_rgba(ff_image_t* out) {
+    return guard([&] {
+        out_ptr(out, "ff_image_t*");
+        *out = colorWheelRGBA();
+    });
+}
+ff_status ff_flowtocolor_create(const ff_gpuimage* inputFlow, float maxflow, ff_stage** out) {
+    return guard([&] {
+        if (inputFlow) new_stage<FlowToColor>(out, inputFlow->img, maxflow);
+        else new_stage<FlowToColor>(out);
+    });
+}
+IMG_SETTER(ff_flowtocolor_set_input_flow, FlowToColor, setInputFlow)
+IMG_GETTER(ff_flowtocolor_get_color_flow, FlowToColor, getColorFlow)
+GETTER(ff_flowtocolor_get_max_flow, FlowToColor, float, getMaxFlow)
+SETTER(ff_flowtocolor_set_max_flow, FlowToColor, float, setMaxFlow)
+ff_status ff_flowtocolor_download_color_flow(ff_stage* s, ff_image_t* color) {
+    return guard_on(s, [&] { as<FlowToColor>(s, "FlowToColor").downloadColorFlow(host_img(color)); });
+}
+
+// ------------------------------------------- camera, rotation, Lax-Wendroff
+static_assert(sizeof(ff_camera) == sizeof(Camera), "ff_camera and Camera differ");
+namespace {
+Camera cam_of(const ff_camera* c) {
+    if (!c) throw Error(FF_ERR_INVALID_ARGUMENT, "null ff_camera");
+    return Camera{c->alphaX, c->alphaY, c->centerX, c->centerY};
+}
+}  // namespace
+ff_status ff_create_perspective_camera(float focalLength, int height, int width, float sensorHeight,
+                                       float sensorWidth, ff_camera* out) {
+    return guard([&] {
+        out_ptr(out, "ff_camera*");
+        const Camera c = createPerspectiveCamera(focalLength, height, width, sensorHeight, sensorWidth);
+        *out = ff_camera{c.alphaX, c.alphaY, c.centerX, c.centerY};
+    });
+}
+ff_status ff_laxwendroff_create(const ff_gpuimage* inputFlow, const ff_gpuimage* inputImage, ff_stage** out) {
+    return guard([&] {
+        if (inputFlow && inputImage) new_stage<LaxWendroffPropagator>(out, inputFlow->img, inputImage->img);
+        else new_stage<LaxWendroffPropagator>(out);
+    });
+}
+SETTER(ff_laxwendroff_set_iterations, LaxWendroffPropagator, int, setIterations)
+GETTER(ff_laxwendroff_get_iterations, LaxWendroffPropagator, int, getIterations)
+GETTER(ff_laxwendroff_get_dt, LaxWendroffPropagator, float, getDt)
+IMG_SETTER(ff_laxwendroff_set_input_flow, LaxWendroffPropagator, setInputFlow)
+IMG_SETTER(ff_laxwendroff_set_input_image, LaxWendroffPropagator, setInputImage)
+IMG_GETTER(ff_laxwendroff_get_flow, LaxWendroffPropagator, getFlow)
+IMG_GETTER(ff_laxwendroff_get_propagated_image, LaxWendroffPropagator, getPropagatedImage)
+
+ff_status ff_rotation_create(const ff_camera* cam, const ff_gpuimage* inputImage, ff_stage** out) {
+    return guard([&] {
+        if (!cam) new_stage<RotationalFlowImagePredictor>(out);
+        else if (!inputImage) new_stage<RotationalFlowImagePredictor>(out, cam_of(cam));
+        else new_stage<RotationalFlowImagePredictor>(out, cam_of(cam), inputImage->img);
+    });
+}
+IMG_SETTER(ff_rotation_set_input_image, RotationalFlowImagePredictor, setInputImage)
+IMG_GETTER(ff_rotation_get_predicted_image, RotationalFlowImagePredictor, getPredictedImage)
+IMG_GETTER(ff_rotation_get_optical_flow, RotationalFlowImagePredictor, getOpticalFlow)
+ff_status ff_rotation_set_camera(ff_stage* s, const ff_camera* cam) {
+    return guard_on(s, [&] { as<RotationalFlowImagePredictor>(s, "RotationalFlowImagePredictor").setCamera(cam_of(cam)); });
+}
+ff_status ff_rotation_get_camera(ff_stage* s, ff_camera* out) {
+    return guard_on(s, [&] {
+        out_ptr(out, "ff_camera*");
+        const Camera& c = as<RotationalFlowImagePredictor>(s, "RotationalFlowImagePredictor").getCamera();
+        *out = ff_camera{c.alphaX, c.alphaY, c.centerX, c.centerY};
+    });
+}
+ff_status ff_rotation_set_angular_velocity(ff_stage* s, float wx, float wy, float wz) {
+    return guard_on(s, [&] {
+        as<RotationalFlowImagePredictor>(s, "RotationalFlowImagePredictor").setAngularVelocity(wx, wy, wz);
+    });
+}
+SETTER(ff_rotation_set_iterations, RotationalFlowImagePredictor, int, setIterations)
+GETTER(ff_rotation_get_iterations, RotationalFlowImagePredictor, int, getIterations)
 
 // ------------------------------------------------------ differentiable form
 namespace {
@@ -893,6 +1106,20 @@ ff_status ff_profile_end(double* total_ms, long long* count) {
 // ------------------------------------------------------------ stream markers
 ff_status ff_marker_record(ff_marker** out) {
     return guard([&] {
+        out_ptr(out, "ff_marker**");
+        hipEvent_t e = nullptr;
+        FF_HIP(hipEventCreate(&e));
+        const hipError_t r = hipEventRecord(e, stream());
+        if (r != hipSuccess) {
+            (void)hipEventDestroy(e);
+            throw_hip(r, "hipEventRecord", __FILE__, __LINE__);
+        }
+        *out = reinterpret_cast<ff_marker*>(e);
+    });
+}
+ff_status ff_marker_record_on(ff_stage* s, ff_marker** out) {
+    return guard_on(s, [&] {
+        if (!s || !s->obj) throw null_handle("ff_stage");
         out_ptr(out, "ff_marker**");
         hipEvent_t e = nullptr;
         FF_HIP(hipEventCreate(&e));

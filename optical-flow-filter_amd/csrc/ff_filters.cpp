@@ -19,6 +19,8 @@ void ensureLike(Image& spare, const Image& like) {
 
 // FlowFilter(GPUImage) (flowfilter.cu:32-38).
 FlowFilter::FlowFilter(const Image& inputImage) {
+    ownStream();
+    StreamScope sc(box_);
     setInputImage(inputImage);
     configure();
 }
@@ -29,7 +31,10 @@ FlowFilter::FlowFilter(int height, int width, int smoothIterations, float maxflo
         throw invalid_argument("FlowFilter::FlowFilter(): height should be greater than zero, got: " + num(height));
     if (width <= 0)
         throw invalid_argument("FlowFilter::FlowFilter(): width should be greater than zero, got: " + num(width));
+    ownStream();
+    StreamScope sc(box_);
     setInputImage(Image(height, width, 1, 1));
+    ownInput_ = true;
     configure();
     setGamma(gamma);
     setMaxFlow(maxflow);
@@ -123,6 +128,7 @@ void FlowFilter::compute() {
         throw logic_error("FlowFilter::compute(): input image shape changed since configure()");
     startTiming();
     if (fused_ && fusedSupported()) {
+        CrossStream x{&input_};
         const int t = frame_.prepare({fusedArgs()}, stream());
         frame_.launch(t, stream());
         check_launch("FlowFilter::compute (fused)");
@@ -137,12 +143,14 @@ void FlowFilter::compute() {
         update_.run();
         smoother_.run();
     }
+    if (ownInput_) frames_.readBy(stream());
     stopTiming();
 }
 
 void FlowFilter::computeImageModel() {
     startTiming();
     runImageModel();
+    if (ownInput_) frames_.readBy(stream());
     stopTiming();
 }
 
@@ -169,9 +177,17 @@ void FlowFilter::setInputImage(const Image& img) {
     h_ = img.height();
     w_ = img.width();
     inputSet_ = true;
+    ownInput_ = false;
+    frames_ = InputFrames();
 }
 
-void FlowFilter::loadImage(const ff_image_t& img) { input_.upload(img); }
+// FlowFilter::loadImage (flowfilter.cu:218-236). A filter that owns its
+// input image loads host frames double buffered, overlapping the previous
+// frame's compute().
+void FlowFilter::loadImage(const ff_image_t& img) {
+    if (ownInput_) frames_.load(input_, img);
+    else input_.upload(img);
+}
 void FlowFilter::loadImageDevice(const void* dev, size_t pitch) { input_.upload_device(dev, pitch); }
 void FlowFilter::downloadFlow(ff_image_t& flow) { smoother_.getSmoothedFlow().download(flow); }
 void FlowFilter::downloadImage(ff_image_t& img) { update_.getUpdatedImage().download(img); }
@@ -192,6 +208,8 @@ void FlowFilter::setMaxFlow(float m) {
 
 // DeltaFlowFilter(inputImage, inputFlow) (flowfilter.cu:349-362).
 DeltaFlowFilter::DeltaFlowFilter(const Image& inputImage, const Image& inputFlow) {
+    ownStream();
+    StreamScope sc(box_);
     setInputImage(inputImage);
     setInputFlow(inputFlow);
     configure();
@@ -303,6 +321,7 @@ void DeltaFlowFilter::compute() {
     requireConfigured("compute()");
     startTiming();
     if (fused_ && fusedSupported()) {
+        CrossStream x{&input_, &inputFlow_};
         const int t = frame_.prepare({fusedArgs()}, stream());
         frame_.launch(t, stream());
         check_launch("DeltaFlowFilter::compute (fused)");
@@ -375,6 +394,8 @@ void DeltaFlowFilter::setMaxFlow(float m) {
 // PyramidalFlowFilter(h, w, levels) (flowfilter.cu:631-641).
 PyramidalFlowFilter::PyramidalFlowFilter(int height, int width, int levels)
     : h_(height), w_(width), levels_(levels) {
+    ownStream();
+    StreamScope sc(box_);
     configure();
 }
 
@@ -444,7 +465,7 @@ bool same_args(const LevelArgs& a, const LevelArgs& b) {
            same_img(a.flow_upd_out, b.flow_upd_out) && same_img(a.upper, b.upper) && a.xmap == b.xmap &&
            a.ymap == b.ymap && a.gamma == b.gamma && a.maxflow == b.maxflow && a.dt == b.dt &&
            a.prop_iters == b.prop_iters && a.smooth_iters == b.smooth_iters && a.border == b.border &&
-           a.delta == b.delta && a.input_frame == b.input_frame;
+           a.delta == b.delta && a.frame_slot == b.frame_slot;
 }
 }  // namespace
 
@@ -523,7 +544,7 @@ bool PyramidalFlowFilter::fusedPlan(std::vector<LevelArgs>& args) {
     for (LevelArgs& a : args) {
         if (!fused_level_supported(a)) return false;
         // levels that read the input frame itself may take it in place
-        a.input_frame = reduce && a.img.data == input_.view().data ? 1 : 0;
+        a.frame_slot = reduce && a.img.data == input_.view().data ? 0 : -1;
     }
     return true;
 }
@@ -537,9 +558,9 @@ void PyramidalFlowFilter::commitFused() {
     for (int l = 0; l < levels_ - 1; ++l) low_[l].fusedCommit();
 }
 
-void PyramidalFlowFilter::enqueueFused(int table, hipStream_t s, const void* frame, size_t frame_pitch) {
+void PyramidalFlowFilter::enqueueFused(int table, hipStream_t s, const FrameRef* frame) {
     enqueuePreFused();
-    frame_.launch(table, s, frame, frame_pitch);
+    frame_.launch(table, s, frame, frame ? 1 : 0);
     check_launch("PyramidalFlowFilter::compute (fused frame)");
 }
 
@@ -583,10 +604,17 @@ void PyramidalFlowFilter::computeFused(const std::vector<LevelArgs>& args, const
     // a graph pays off only when the frame is more than the one launch
     // (pyramid kernels for more than three levels); replaying a one-node
     // graph costs more than launching the kernel
-    if (run_maybe_graphed(graphs_on_ && levels_ > 3, graphs_, args, frame_.token(table), s,
-                          [&] { enqueueFused(table, s, frame, frame_pitch); }))
+    const FrameRef fr{frame, frame_pitch};
+    // a frame in place is a launch argument, which a replayed graph would
+    // not update: such frames are launched directly
+    if (run_maybe_graphed(graphs_on_ && levels_ > 3 && !frame, graphs_, args, frame_.token(table), s,
+                          [&] { enqueueFused(table, s, frame ? &fr : nullptr); }))
         frame_.replayed();
     commitFused();
+}
+
+bool PyramidalFlowFilter::frameFits(const void* dev, size_t pitch) const {
+    return dev && (uintptr_t)dev % 4 == 0 && pitch % 4 == 0 && (unsigned long long)pitch * (unsigned long long)h_ < 0x7fffffffULL;
 }
 
 void PyramidalFlowFilter::compute() {
@@ -600,17 +628,32 @@ void PyramidalFlowFilter::compute() {
     // a frame passed in place is read by the fused launch (up to three
     // levels: every level reduces the input frame while loading it), else
     // copied into the input image first
-    const bool in_place = frame && fused && levels_ <= 3 && ((uintptr_t)frame % 4 == 0) && (pitch % 4 == 0);
+    const bool in_place = frame && fused && levels_ <= 3 && frameFits(frame, pitch);
     if (frame && !in_place) input_.upload_device(frame, pitch);
     if (fused) computeFused(args, in_place ? frame : nullptr, in_place ? pitch : 0);
     else computeStaged();
+    frames_.readBy(stream());
     stopTiming();
+}
+
+void PyramidalFlowFilter::checkWaits() {
+    waitOwn_.follow(frame_.watch());
+    waitOwn_.check("PyramidalFlowFilter");
+    waitBatch_.check("PyramidalFlowFilter (in a batch)");
+}
+
+float PyramidalFlowFilter::elapsedTime() {
+    const float t = Stage::elapsedTime();
+    checkWaits();
+    return t;
 }
 
 Image PyramidalFlowFilter::getFlow() const { return levels_ == 1 ? top_.getFlow() : low_[0].getFlow(); }
 
+// PyramidalFlowFilter::loadImage (flowfilter.cu:728-731): double buffered
+// (InputFrames), so the copy overlaps the previous frame's compute().
 void PyramidalFlowFilter::loadImage(const ff_image_t& img) {
-    input_.upload(img);
+    frames_.load(input_, img);
     frame_dev_ = nullptr;
 }
 void PyramidalFlowFilter::loadImageDevice(const void* dev, size_t pitch) {
@@ -630,11 +673,13 @@ void PyramidalFlowFilter::setImageDevice(const void* dev, size_t pitch) {
 void PyramidalFlowFilter::downloadFlow(ff_image_t& flow) {
     if (levels_ == 1) top_.downloadFlow(flow);
     else low_[0].getFlow().download(flow);
+    checkWaits();
 }
 
 void PyramidalFlowFilter::downloadImage(ff_image_t& img) {
     if (levels_ == 1) top_.downloadImage(img);
     else low_[0].getImage().download(img);
+    checkWaits();
 }
 
 void PyramidalFlowFilter::checkLevel(int level, const char* who) const {
@@ -705,13 +750,32 @@ void PyramidalFlowFilter::setPropagationBorder(int b) {
 
 // ------------------------------------------------------- BatchedFlowFilter
 
+// A filter in a batch enqueues all its work on the batch's stream (its own
+// compute() and transfers too), so the batched launch, which reads and
+// writes its state, is ordered with them.
 int BatchedFlowFilter::add(const std::shared_ptr<PyramidalFlowFilter>& f) {
     if (!f) throw invalid_argument("BatchedFlowFilter::add(): null filter");
     for (const auto& g : streams_)
         if (g == f) throw invalid_argument("BatchedFlowFilter::add(): filter already in the batch");
+    f->box()->follow(box_);
     streams_.push_back(f);
     graphs_.clear();
     return (int)streams_.size() - 1;
+}
+
+void BatchedFlowFilter::setImagesDevice(const void* const* ptrs, const size_t* pitches, int n) {
+    if (n != (int)streams_.size())
+        throw invalid_argument("BatchedFlowFilter::setImagesDevice(): " + num(n) + " frames for " +
+                               num((long)streams_.size()) + " streams");
+    if (n > 0 && (!ptrs || !pitches)) throw invalid_argument("BatchedFlowFilter::setImagesDevice(): null array");
+    for (int i = 0; i < n; ++i) streams_[i]->setImageDevice(ptrs[i], pitches[i]);
+}
+
+float BatchedFlowFilter::elapsedTime() {
+    const float t = Stage::elapsedTime();
+    waitOwn_.follow(frame_.watch());
+    waitOwn_.check("BatchedFlowFilter");
+    return t;
 }
 
 void BatchedFlowFilter::loadImagesDevice(const void* const* ptrs, const size_t* pitches, int n) {
@@ -747,17 +811,26 @@ void BatchedFlowFilter::compute() {
     startTiming();
     std::vector<LevelArgs> all, one;
     std::vector<PyramidalFlowFilter*> fused;
+    std::vector<FrameRef> frames;
     for (const auto& f : streams_) {
-        // a frame a stream was given in place is copied in: the batched
-        // launch reads every stream's input image
-        if (f->frame_dev_) {
-            f->input_.upload_device(f->frame_dev_, f->frame_pitch_);
-            f->frame_dev_ = nullptr;
-        }
-        if (f->fused() && f->fusedPlan(one)) {
+        // a filter moved to another stream since it joined comes back
+        if (f->box()->get() != box_->get()) f->box()->follow(box_);
+        const void* fp = f->frame_dev_;
+        const size_t pp = f->frame_pitch_;
+        f->frame_dev_ = nullptr;
+        const bool fusedOk = f->fused() && f->fusedPlan(one);
+        // a frame given in place is read by the launch where it lies (one
+        // launch argument per stream), else copied into the input image
+        const bool in_place = fp && fusedOk && f->levels_ <= 3 && f->frameFits(fp, pp) &&
+                              (int)frames.size() < kMaxLaunchFrames;
+        if (fp && !in_place) f->input_.upload_device(fp, pp);
+        if (fusedOk) {
+            const int slot = in_place ? (int)frames.size() : -1;
+            if (in_place) frames.push_back(FrameRef{fp, pp});
             const int off = (int)all.size();
             for (LevelArgs a : one) {
                 if (a.wait_on >= 0) a.wait_on += off;
+                a.frame_slot = a.frame_slot >= 0 ? slot : -1;
                 all.push_back(a);
             }
             fused.push_back(f.get());
@@ -770,16 +843,20 @@ void BatchedFlowFilter::compute() {
         const int table = frame_.prepare(all, s);
         last_blocks_ = frame_.blocks(table);
         // the step is replayed as a HIP graph when graphs are on (the
-        // default): about 1 % of a 64-stream step, against the single
-        // filter's one-launch frames, where a replay costs more than it saves
-        if (run_maybe_graphed(graphs_on_, graphs_, all, frame_.token(table), s, [&] {
+        // default) and no frame is read in place (a launch argument, which
+        // a replay would not update)
+        if (run_maybe_graphed(graphs_on_ && frames.empty(), graphs_, all, frame_.token(table), s, [&] {
                 for (PyramidalFlowFilter* f : fused) f->enqueuePreFused();
-                frame_.launch(table, s);
+                frame_.launch(table, s, frames.data(), (int)frames.size());
                 check_launch("BatchedFlowFilter::compute (fused frame)");
             }))
             frame_.replayed();
-        for (PyramidalFlowFilter* f : fused) f->commitFused();
+        for (PyramidalFlowFilter* f : fused) {
+            f->commitFused();
+            f->waitBatch_.follow(frame_.watch());
+        }
     }
+    for (const auto& f : streams_) f->frames_.readBy(ffamd::stream());
     stopTiming();
 }
 

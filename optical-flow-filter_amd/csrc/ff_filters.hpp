@@ -68,7 +68,11 @@ private:
     void requireConfigured(const char* who) const;
     int h_ = 0, w_ = 0;
     bool configured_ = false, firstLoad_ = false, inputSet_ = false, fused_ = true;
+    // the input image is the filter's own (FlowFilter(h, w, ...)): host
+    // frames load double buffered (InputFrames)
+    bool ownInput_ = false;
     Image input_;
+    InputFrames frames_;
     ImageModel model_;
     FlowUpdate update_;
     FlowSmoother smoother_;
@@ -178,6 +182,9 @@ public:
     void setImageDevice(const void* dev, size_t pitch);
     void downloadFlow(ff_image_t& flow);
     void downloadImage(ff_image_t& img);
+    // Waits for the last compute(); FF_ERR_RUNTIME when a wait of the fused
+    // kernel gave up since the last synchronizing call (downloads too).
+    float elapsedTime() override;
     float getGamma(int level) const;
     void setGamma(int level, float g);
     void setGamma(const std::vector<float>& g);
@@ -200,6 +207,11 @@ public:
     }
     bool graphs() const { return graphs_on_; }
     unsigned long long waitTimeouts() const { return frame_.timeouts(); }
+    // Test hook (FusedFrame::setWaitTest).
+    void setWaitTest(int spins) {
+        frame_.setWaitTest(spins);
+        graphs_.clear();
+    }
     // Waves per workgroup of the fused frame kernel: 0 = autotune.
     void setTileWaves(int nw) { frame_.setWaves(nw); }
     int tileWaves() const { return frame_.waves(); }
@@ -209,6 +221,11 @@ public:
 private:
     friend class BatchedFlowFilter;
     void checkLevel(int level, const char* who) const;
+    // FF_ERR_RUNTIME when a wait of this filter's fused launches (or of a
+    // batch's that advanced it) gave up since the last check.
+    void checkWaits();
+    // Whether a frame at (dev, pitch) can be read in place by the fused kernel.
+    bool frameFits(const void* dev, size_t pitch) const;
     void computeStaged();
     // Pyramid kernels a fused frame needs before its launch (deeper pyramids)
     // and the state exchange after it.
@@ -218,15 +235,17 @@ private:
     // (first frame) or not covered by the fused kernel.
     bool fusedPlan(std::vector<LevelArgs>& args);
     void computeFused(const std::vector<LevelArgs>& args, const void* frame = nullptr, size_t frame_pitch = 0);
-    void enqueueFused(int table, hipStream_t s, const void* frame = nullptr, size_t frame_pitch = 0);
+    void enqueueFused(int table, hipStream_t s, const FrameRef* frame = nullptr);
     bool configured_ = false, fused_ = true, graphs_on_ = true;
     // frame passed in place for the next compute() (setImageDevice)
     const void* frame_dev_ = nullptr;
     size_t frame_pitch_ = 0;
     GraphCache graphs_;
     FusedFrame frame_;
+    WaitCheck waitOwn_, waitBatch_;
     int h_ = 0, w_ = 0, levels_ = 0;
     Image input_;
+    InputFrames frames_;
     ImagePyramid pyramid_;
     FlowFilter top_;
     std::vector<DeltaFlowFilter> low_;
@@ -239,7 +258,7 @@ private:
 // stage kernels first. No reference counterpart.
 class BatchedFlowFilter : public Stage {
 public:
-    BatchedFlowFilter() = default;
+    BatchedFlowFilter() { ownStream(); }
     void configure() override {}
     void compute() override;
     const char* name() const override { return "BatchedFlowFilter"; }
@@ -248,6 +267,11 @@ public:
     // Load the next frame of every stream from device memory (u8, pitched)
     // in one copy launch; n must equal size().
     void loadImagesDevice(const void* const* ptrs, const size_t* pitches, int n);
+    // The next compute() reads every stream's frame where the caller keeps
+    // it (setImageDevice of each stream; the fused launch takes them in
+    // place, up to kMaxLaunchFrames of them).
+    void setImagesDevice(const void* const* ptrs, const size_t* pitches, int n);
+    float elapsedTime() override;
     const std::shared_ptr<PyramidalFlowFilter>& stream(int i) const;
     void setGraphs(bool g) {
         graphs_on_ = g;
@@ -255,6 +279,10 @@ public:
     }
     bool graphs() const { return graphs_on_; }
     unsigned long long waitTimeouts() const { return frame_.timeouts(); }
+    void setWaitTest(int spins) {
+        frame_.setWaitTest(spins);
+        graphs_.clear();
+    }
     void setTileWaves(int nw) { frame_.setWaves(nw); }
     int tileWaves() const { return frame_.waves(); }
     void setTileTopRows(int rows) { frame_.setTopRows(rows); }
@@ -268,6 +296,7 @@ private:
     int last_blocks_ = 0;
     GraphCache graphs_;
     FusedFrame frame_;
+    WaitCheck waitOwn_;
 };
 
 }  // namespace ffamd

@@ -73,7 +73,7 @@ namespace {
 
 constexpr int LANES = 64;
 constexpr int OOB = (int)0x80000000;  // buffer offset beyond any image: access dropped
-constexpr int kSpinLimit = 1 << 23;   // about 2 s of polling at s_sleep 8
+constexpr int kSpinLimit = 1 << 23;   // polls before a wait gives up: about 2 s at s_sleep 8
 
 // One level of one frame, as the kernel reads it. Blocks first_block ..
 // first_block + tiles_x * tiles_y - 1 of the launch are its tiles.
@@ -97,9 +97,10 @@ struct LevelJob {
     int delta, img_mode, first_block;
     // completion counters: wait until counter wait_idx has counted
     // wait_target more tiles in this launch than before it; signal counter
-    // signal_idx. img_arg: the level's image is the filter's input frame,
-    // which a launch may pass in place (read where the caller keeps it).
-    int wait_idx, wait_target, img_arg, signal_idx;
+    // signal_idx. frame_slot >= 0: the level's image is a filter's input
+    // frame, which a launch may pass in place as its frame frame_slot (read
+    // where the caller keeps it).
+    int wait_idx, wait_target, frame_slot, signal_idx;
     float dt, gamma, maxflow;
     int pad_;
 };
@@ -220,14 +221,18 @@ __device__ __forceinline__ int xcd_tile(int b, int nt) {
 // published (the unsmoothed flow) is read with device-coherent loads after
 // the wait, so no cache invalidation is needed; the workgroup barrier
 // orders every wave's loads after the polling thread saw the count.
-__device__ __forceinline__ void wait_upper(const LevelJob& p, unsigned* counters, int timeout_idx, unsigned epoch) {
+// The wait gives up after `spin_limit` polls and counts a timeout (the host
+// reports it as an error); `bias` is added to the target (0, or a count no
+// launch reaches, which a test uses to make the wait give up).
+__device__ __forceinline__ void wait_upper(const LevelJob& p, unsigned* counters, int timeout_idx, unsigned epoch,
+                                           int spin_limit, unsigned bias) {
     if (threadIdx.x == 0) {
         const unsigned* done = counters + 2 * p.wait_idx;
-        const unsigned target = epoch * (unsigned)p.wait_target;
+        const unsigned target = epoch * (unsigned)p.wait_target + bias;
         int spins = 0;
         while ((int)(__hip_atomic_load(done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - target) < 0) {
             __builtin_amdgcn_s_sleep(8);
-            if (++spins > kSpinLimit) {
+            if (++spins > spin_limit) {
                 __hip_atomic_fetch_add(counters + timeout_idx, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 break;
             }
@@ -638,9 +643,19 @@ __device__ __forceinline__ float* xrow(float* xch, int nw, int slot, int wave) {
     return xch + ((slot * nw + wave) * 10) * LANES;
 }
 
+// Per-launch arguments of the frame kernel besides the job table.
+struct WaitArgs {
+    int spin_limit;
+    unsigned bias;
+};
+struct FrameArgs {
+    const unsigned char* ptr[kMaxLaunchFrames];
+    int pitch[kMaxLaunchFrames];
+};
+
 template <bool DELTA, int MODE, int R>
 __device__ __forceinline__ void level_tile(const LevelJob& p, int tile, float* xch, int nw, unsigned* counters, unsigned epoch,
-                                           int timeout_idx, ImgSrc frame) {
+                                           int timeout_idx, const WaitArgs& wa, ImgSrc frame) {
     Geo g;
     g.lane = threadIdx.x & (LANES - 1);
     g.wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -696,7 +711,7 @@ __device__ __forceinline__ void level_tile(const LevelJob& p, int tile, float* x
         }
     }
     ImageRegs<MODE, R> ir;
-    load_image<MODE, R>(ir, p, g, (p.img_arg && frame.data) ? frame : ImgSrc{p.img, p.img_pitch});
+    load_image<MODE, R>(ir, p, g, frame.data ? frame : ImgSrc{p.img, p.img_pitch});
 
     int slot = 0;
 
@@ -762,7 +777,7 @@ __device__ __forceinline__ void level_tile(const LevelJob& p, int tile, float* x
     iteration(std::true_type{});
 
     // ------------------------------------------------ image model + update
-    if (DELTA && p.wait_idx >= 0) wait_upper(p, counters, timeout_idx, epoch);
+    if (DELTA && p.wait_idx >= 0) wait_upper(p, counters, timeout_idx, epoch, wa.spin_limit, wa.bias);
     float aold[R];  // unused (and removed) on delta levels
     float2 up[R];
     int offa[R], offf[R];
@@ -863,21 +878,23 @@ constexpr int kTopRowChoices[] = {4, 8};
 template <int RTOP>
 __global__ void __launch_bounds__(1024)
     k_fused_frame(const LevelJob* __restrict__ jobs, const int* __restrict__ block_job, unsigned* counters,
-                  unsigned epoch, int timeout_idx, GroupEnds ends, int nw, const unsigned char* frame,
-                  int frame_pitch) {
+                  unsigned epoch, int timeout_idx, GroupEnds ends, int nw, WaitArgs wa, FrameArgs frames) {
     // row exchange between the waves of the workgroup (2 * nw * 10 * LANES floats)
     extern __shared__ float xch[];
     const LevelJob& p = jobs[block_job[blockIdx.x]];
     const int tile = (int)blockIdx.x - p.first_block;
     const unsigned b = blockIdx.x;
-    const ImgSrc fr{frame, frame_pitch};
-    if (b < (unsigned)ends.e[0]) level_tile<false, IMG_F32, kRows>(p, tile, xch, nw, counters, epoch, timeout_idx, fr);
-    else if (b < (unsigned)ends.e[1]) level_tile<false, IMG_U8, kRows>(p, tile, xch, nw, counters, epoch, timeout_idx, fr);
-    else if (b < (unsigned)ends.e[2]) level_tile<false, IMG_U8_DOWN, RTOP>(p, tile, xch, nw, counters, epoch, timeout_idx, fr);
-    else if (b < (unsigned)ends.e[3]) level_tile<false, IMG_U8_DOWN2, RTOP>(p, tile, xch, nw, counters, epoch, timeout_idx, fr);
-    else if (b < (unsigned)ends.e[4]) level_tile<true, IMG_F32, kRows>(p, tile, xch, nw, counters, epoch, timeout_idx, fr);
-    else if (b < (unsigned)ends.e[5]) level_tile<true, IMG_U8, kRows>(p, tile, xch, nw, counters, epoch, timeout_idx, fr);
-    else level_tile<true, IMG_U8_DOWN, kRows>(p, tile, xch, nw, counters, epoch, timeout_idx, fr);
+    // the job's input frame passed in place, if any (frame_slot is
+    // uniform, so is the selection)
+    ImgSrc fr{nullptr, 0};
+    if (p.frame_slot >= 0) fr = ImgSrc{frames.ptr[p.frame_slot], frames.pitch[p.frame_slot]};
+    if (b < (unsigned)ends.e[0]) level_tile<false, IMG_F32, kRows>(p, tile, xch, nw, counters, epoch, timeout_idx, wa, fr);
+    else if (b < (unsigned)ends.e[1]) level_tile<false, IMG_U8, kRows>(p, tile, xch, nw, counters, epoch, timeout_idx, wa, fr);
+    else if (b < (unsigned)ends.e[2]) level_tile<false, IMG_U8_DOWN, RTOP>(p, tile, xch, nw, counters, epoch, timeout_idx, wa, fr);
+    else if (b < (unsigned)ends.e[3]) level_tile<false, IMG_U8_DOWN2, RTOP>(p, tile, xch, nw, counters, epoch, timeout_idx, wa, fr);
+    else if (b < (unsigned)ends.e[4]) level_tile<true, IMG_F32, kRows>(p, tile, xch, nw, counters, epoch, timeout_idx, wa, fr);
+    else if (b < (unsigned)ends.e[5]) level_tile<true, IMG_U8, kRows>(p, tile, xch, nw, counters, epoch, timeout_idx, wa, fr);
+    else level_tile<true, IMG_U8_DOWN, kRows>(p, tile, xch, nw, counters, epoch, timeout_idx, wa, fr);
 }
 
 __global__ void k_clear_counters(unsigned* c, int n) {
@@ -995,6 +1012,7 @@ struct FusedFrame::Impl {
         int rtop = 0;
         size_t map_offset = 0;
         unsigned long long token = 0;
+        bool waits = false;  // some job waits for another
     };
     Table slot[2];
     int next = 0;
@@ -1011,12 +1029,14 @@ struct FusedFrame::Impl {
     int fixed = 0, fixed_rtop = 0;  // waves per workgroup, top-level rows per thread; 0: autotune
     std::vector<int> tuned_for;
     int tuned = 0, tuned_rtop = 0;
+    std::shared_ptr<WaitWatch> watch = std::make_shared<WaitWatch>();
+    int test_spins = 0;  // > 0: the waits give up after this many polls (test hook)
 
-    int table(const std::vector<LevelArgs>& jobs, int nw, int rtop);
+    int table(const std::vector<LevelArgs>& jobs, int nw, int rtop, hipStream_t s);
     unsigned* region(int si) const { return (unsigned*)counters.data() + (size_t)si * 2 * npairs; }
     int timeout_word() const { return 4 * npairs; }
     void clear_counters(int si, hipStream_t s) const;
-    void run(int t, hipStream_t s, const void* frame = nullptr, int frame_pitch = 0);
+    void run(int t, hipStream_t s, const FrameRef* frames = nullptr, int nframes = 0);
     void tune(const std::vector<LevelArgs>& jobs, hipStream_t s);
 };
 
@@ -1030,9 +1050,9 @@ FusedFrame& FusedFrame::operator=(FusedFrame&& o) noexcept {
 
 // Job table of `jobs` tiled with nw waves per workgroup and rtop rows per
 // thread on reduced top levels: found among the two resident tables or
-// built and uploaded (after a device synchronization, since the slot it
-// replaces may be in use).
-int FusedFrame::Impl::table(const std::vector<LevelArgs>& jobs, int nw, int rtop) {
+// built and uploaded (after waiting for stream s, the stream of every launch
+// of this FusedFrame, since the slot it replaces may be in use).
+int FusedFrame::Impl::table(const std::vector<LevelArgs>& jobs, int nw, int rtop, hipStream_t s) {
     // launch order: grouped by tile body, stable within a group; every top
     // level precedes every delta level, so producers precede consumers
     std::vector<int> order(jobs.size()), pos(jobs.size());
@@ -1087,7 +1107,7 @@ int FusedFrame::Impl::table(const std::vector<LevelArgs>& jobs, int nw, int rtop
         p.first_block = blocks;
         p.wait_idx = -1;
         p.signal_idx = -1;
-        p.img_arg = a.input_frame ? 1 : 0;
+        p.frame_slot = a.frame_slot < kMaxLaunchFrames ? a.frame_slot : -1;
         p.dt = a.dt;
         p.gamma = a.gamma;
         p.maxflow = a.maxflow;
@@ -1126,13 +1146,18 @@ int FusedFrame::Impl::table(const std::vector<LevelArgs>& jobs, int nw, int rtop
         if (slot[i].key == key) return i;
 
     // a table (or the counters) may be in use by launches still in flight
-    FF_HIP(hipDeviceSynchronize());
+    // on the stream
+    FF_HIP(hipStreamSynchronize(s));
     const int need = (int)jobs.size();
     if (npairs < need) {
         const int n = need < 64 ? 64 : need;
+        // the timed-out waits counted so far are kept
+        watch->stream = s;
+        watch->read();
         std::vector<unsigned> zero(4 * (size_t)n + 2, 0u);
         counters.assign(zero.data(), zero.size() * sizeof(unsigned));
         npairs = n;
+        watch->retire((const unsigned*)counters.data() + timeout_word());
         // launches recorded with the old counters must not be replayed
         for (Table& o : slot) o.token = ++uploads;
     }
@@ -1145,6 +1170,8 @@ int FusedFrame::Impl::table(const std::vector<LevelArgs>& jobs, int nw, int rtop
     t.nw = nw;
     t.rtop = rtop;
     t.map_offset = map_offset;
+    t.waits = false;
+    for (const LevelJob& r : rec) t.waits = t.waits || r.wait_idx >= 0;
     for (int g = 0; g < kGroups - 1; ++g) t.ends.e[g] = ends[g];
     t.token = ++uploads;
     next ^= 1;
@@ -1156,7 +1183,7 @@ void FusedFrame::Impl::clear_counters(int si, hipStream_t s) const {
     hipLaunchKernelGGL(k_clear_counters, dim3((n + 255) / 256), dim3(256), 0, s, region(si), n);
 }
 
-void FusedFrame::Impl::run(int ti, hipStream_t s, const void* frame, int frame_pitch) {
+void FusedFrame::Impl::run(int ti, hipStream_t s, const FrameRef* frames, int nframes) {
     const int si = ti & 1;
     const Table& t = slot[si];
     if (!t.blocks) throw logic_error("fused frame: launch of an unprepared table");
@@ -1186,6 +1213,15 @@ void FusedFrame::Impl::run(int ti, hipStream_t s, const void* frame, int frame_p
         }
         e = ++epoch[si];
     }
+    const WaitArgs wa{test_spins > 0 ? test_spins : kSpinLimit, test_spins > 0 ? 0x40000000u : 0u};
+    FrameArgs fa;
+    std::memset(&fa, 0, sizeof(fa));
+    for (int k = 0; k < nframes; ++k) {
+        fa.ptr[k] = (const unsigned char*)frames[k].ptr;
+        fa.pitch[k] = (int)frames[k].pitch;
+    }
+    watch->stream = s;
+    if (t.waits) watch->pending = true;
     ProfileScope ps("fused_frame", s);
     const char* base = (const char*)t.dev.data();
     // the timeout word, relative to this slot's region
@@ -1193,11 +1229,11 @@ void FusedFrame::Impl::run(int ti, hipStream_t s, const void* frame, int frame_p
     if (t.rtop == 8)
         hipLaunchKernelGGL(k_fused_frame<8>, dim3(t.blocks), dim3(t.nw * LANES), lds_bytes(t.nw), s,
                            (const LevelJob*)base, (const int*)(base + t.map_offset), region(si), e, timeout_idx, t.ends,
-                           t.nw, (const unsigned char*)frame, frame_pitch);
+                           t.nw, wa, fa);
     else
         hipLaunchKernelGGL(k_fused_frame<4>, dim3(t.blocks), dim3(t.nw * LANES), lds_bytes(t.nw), s,
                            (const LevelJob*)base, (const int*)(base + t.map_offset), region(si), e, timeout_idx, t.ends,
-                           t.nw, (const unsigned char*)frame, frame_pitch);
+                           t.nw, wa, fa);
 }
 
 // Time the launches of every candidate tiling (waves per workgroup and, when
@@ -1222,7 +1258,7 @@ void FusedFrame::Impl::tune(const std::vector<LevelArgs>& jobs, hipStream_t s) {
                 bool ok = true;
                 for (const LevelArgs& a : jobs) ok = ok && tiles_with(a, nw * rows_per_thread(a, rtop));
                 if (!ok) continue;
-                const int t = table(jobs, nw, rtop);
+                const int t = table(jobs, nw, rtop, s);
                 run(t, s);  // warm-up
                 constexpr int kReps = 3;
                 FF_HIP(hipEventRecord(e0, s));
@@ -1265,12 +1301,16 @@ int FusedFrame::prepare(const std::vector<LevelArgs>& jobs, hipStream_t s) {
         nw = impl_->tuned;
         rtop = impl_->tuned_rtop;
     }
-    return impl_->table(jobs, nw, rtop);
+    return impl_->table(jobs, nw, rtop, s);
 }
 
-void FusedFrame::launch(int table, hipStream_t s, const void* frame, size_t frame_pitch) {
-    if (frame && frame_pitch > 0x7fffffffu) throw invalid_argument("fused frame: frame pitch too large");
-    impl_->run(table, s, frame, (int)frame_pitch);
+void FusedFrame::launch(int table, hipStream_t s, const FrameRef* frames, int nframes) {
+    if (nframes < 0 || nframes > kMaxLaunchFrames) throw invalid_argument("fused frame: too many frames in place");
+    for (int k = 0; k < nframes; ++k)
+        if (frames[k].ptr && (frames[k].pitch > 0x7fffffffu || frames[k].pitch % 4 || (uintptr_t)frames[k].ptr % 4))
+            throw invalid_argument("fused frame: a frame in place needs a pitch and address that are multiples of 4 "
+                                   "and a pitch below 2^31");
+    impl_->run(table, s, frames, nframes);
 }
 
 void FusedFrame::replayed() { impl_->counted[0] = impl_->counted[1] = 0; }
@@ -1300,13 +1340,54 @@ void FusedFrame::setTopRows(int rows) {
 
 int FusedFrame::topRows() const { return impl_->fixed_rtop ? impl_->fixed_rtop : impl_->tuned_rtop; }
 
-unsigned long long FusedFrame::timeouts() const {
-    if (!impl_->npairs) return 0;
-    FF_HIP(hipDeviceSynchronize());
-    unsigned v = 0;
-    const unsigned* c = (const unsigned*)impl_->counters.data();
-    FF_HIP(hipMemcpy(&v, c + impl_->timeout_word(), sizeof(v), hipMemcpyDeviceToHost));
-    return v;
+unsigned long long FusedFrame::timeouts() const { return impl_->watch->read(); }
+
+const std::shared_ptr<WaitWatch>& FusedFrame::watch() const { return impl_->watch; }
+
+void FusedFrame::setWaitTest(int spins) {
+    if (spins < 0) throw invalid_argument("fused frame: wait test spins must be >= 0");
+    impl_->test_spins = spins;
+}
+
+int FusedFrame::waitTest() const { return impl_->test_spins; }
+
+// ------------------------------------------------------------- WaitWatch
+
+WaitWatch::~WaitWatch() {
+    if (host) (void)hipHostFree(host);
+}
+
+unsigned long long WaitWatch::read() {
+    if (!pending || !word) return total;
+    if (!host) FF_HIP(hipHostMalloc((void**)&host, sizeof(unsigned), hipHostMallocDefault));
+    FF_HIP(hipMemcpyAsync(host, word, sizeof(unsigned), hipMemcpyDeviceToHost, stream));
+    FF_HIP(hipStreamSynchronize(stream));
+    total = base + *host;
+    pending = false;
+    return total;
+}
+
+void WaitWatch::retire(const unsigned* next) {
+    base = total;
+    word = next;
+}
+
+void WaitCheck::follow(const std::shared_ptr<WaitWatch>& w) {
+    if (watch == w) return;
+    watch = w;
+    seen = w ? w->total : 0;
+}
+
+void WaitCheck::check(const char* who) {
+    if (!watch) return;
+    const unsigned long long n = watch->read();
+    if (n > seen) {
+        const unsigned long long d = n - seen;
+        seen = n;
+        throw runtime_error(std::string(who) + ": " + std::to_string(d) +
+                            " wait(s) of the fused frame kernel for the level above gave up after the spin limit "
+                            "since the last check; the flow of those frames is not valid");
+    }
 }
 
 }  // namespace ffamd

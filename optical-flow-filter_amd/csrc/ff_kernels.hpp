@@ -10,6 +10,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <memory>
 #include <vector>
 
 namespace ffamd {
@@ -73,6 +74,27 @@ struct CopyRegion {
 };
 void launch_copy_regions(const CopyRegion* regions, int n, hipStream_t s);
 
+// flowToColor_k: RGBA colour (4 bytes per pixel) of every flow vector, from
+// the colour wheel image (u8, 4 channels) read at the flow normalised by
+// maxflow (ff_display.hip).
+void launch_flow_to_color(const DevImg& flow, const DevImg& wheel, float maxflow, const DevImg& color, hipStream_t s);
+// The colour wheel table (256 x 256 RGBA, rows top to bottom) and its size.
+constexpr int kWheelHeight = 256, kWheelWidth = 256, kWheelDepth = 4;
+const unsigned char* colorwheel_rgba(size_t* bytes);
+
+// Perspective camera intrinsics (perspectiveCamera,
+// include/flowfilter/gpu/camera.h:20-25), layout-identical to ff_camera.
+struct Camera {
+    float alphaX, alphaY, centerX, centerY;
+};
+// rotationalOpticalFlow_k: the flow of a camera rotating with angular
+// velocity (wx, wy, wz), every pixel of `flow` (float2) (ff_rotation.hip).
+void launch_rotational_flow(const Camera& cam, float wx, float wy, float wz, const DevImg& flow, hipStream_t s);
+// LaxWendroffX_k / LaxWendroffY_k: one pass (axis 0: x, 1: y) over an image
+// of `channels` (1 or 4) floats per pixel, advected by `flow` (float2).
+void launch_lax_wendroff_pass(int axis, int channels, const DevImg& flow, const DevImg& in, const DevImg& out, float dt,
+                              hipStream_t s);
+
 // ------------------------------------------------------------ fused frame
 // One kernel launch runs every level of a frame (and, for batched filters,
 // of many frames): each level is a job whose output tiles are workgroups of
@@ -114,9 +136,48 @@ struct LevelArgs {
     // reads as `upper`, or -1 when `upper` is not written in this launch.
     // The job must come earlier in the list.
     int wait_on = -1;
-    // 1: `img` is the filter's input frame, which a launch may instead pass
-    // in place (FusedFrame::launch's frame argument)
-    int input_frame = 0;
+    // k >= 0: `img` is a filter's input frame, which a launch may instead
+    // pass in place as its frame k (FusedFrame::launch); -1: always `img`.
+    int frame_slot = -1;
+};
+
+// A frame a fused launch reads in place: device u8 image, row pitch in
+// bytes (multiples of 4; pitch * rows below 2^31).
+struct FrameRef {
+    const void* ptr = nullptr;
+    size_t pitch = 0;
+};
+// Frames one launch can take in place (a batch of more streams copies the
+// rest into their input images).
+constexpr int kMaxLaunchFrames = 128;
+
+// Count of the waits of a FusedFrame's launches that gave up after their
+// spin limit (the results of those frames are not valid). The kernel counts
+// in a device word; read() copies it on the stream of the launches and
+// waits for that stream only.
+struct WaitWatch {
+    WaitWatch() = default;
+    WaitWatch(const WaitWatch&) = delete;
+    WaitWatch& operator=(const WaitWatch&) = delete;
+    ~WaitWatch();
+    const unsigned* word = nullptr;  // the kernel's count (device)
+    hipStream_t stream = nullptr;    // stream of the launches that count
+    unsigned long long base = 0;     // counts of words since replaced
+    unsigned long long total = 0;    // base + word, as last read
+    bool pending = false;            // launches with waits since the last read
+    unsigned* host = nullptr;        // pinned copy of the word
+    unsigned long long read();
+    // The word is replaced by a new zero word: keep its count in base.
+    void retire(const unsigned* next);
+};
+
+// One observer's view of a WaitWatch: check() raises FF_ERR_RUNTIME when the
+// count grew since this observer's last check.
+struct WaitCheck {
+    std::shared_ptr<WaitWatch> watch;
+    unsigned long long seen = 0;
+    void follow(const std::shared_ptr<WaitWatch>& w);
+    void check(const char* who);
 };
 
 // Returns false when the level's parameters are outside what the fused
@@ -124,8 +185,8 @@ struct LevelArgs {
 bool fused_level_supported(const LevelArgs& a);
 
 // Device tables and completion counters of fused frame launches. prepare()
-// builds the job table of a list of levels (uploading it, after a device
-// synchronization, when it matches none of the two most recent tables),
+// builds the job table of a list of levels (uploading it, after waiting for
+// the work on stream s, when it matches none of the two most recent tables),
 // choosing the tile height by timing the candidates the first time it sees
 // a set of level shapes; it must not run under stream capture. launch()
 // only enqueues the kernel.
@@ -141,10 +202,10 @@ public:
     // The first call for a set of level shapes times every candidate tiling
     // on stream s (except what setWaves / setTopRows fixed).
     int prepare(const std::vector<LevelArgs>& jobs, hipStream_t s);
-    // frame (optional): device u8 image of the input frame's size, with row
-    // pitch frame_pitch (a multiple of 4), read in place by the jobs marked
-    // input_frame instead of their img.
-    void launch(int table, hipStream_t s, const void* frame = nullptr, size_t frame_pitch = 0);
+    // frames (optional, nframes <= kMaxLaunchFrames): the jobs whose
+    // frame_slot is k read frames[k] in place instead of their img (when
+    // its ptr is not null).
+    void launch(int table, hipStream_t s, const FrameRef* frames = nullptr, int nframes = 0);
     // A graph holding launches of this frame was replayed on the stream:
     // the next direct launch clears the completion counters first.
     void replayed();
@@ -159,8 +220,14 @@ public:
     void setTopRows(int rows);
     int topRows() const;
     // Waits that gave up after the spin limit (a scheduling fault; results
-    // of such a frame are not valid). Synchronizes the device.
+    // of such a frame are not valid). Waits for the stream of the launches.
     unsigned long long timeouts() const;
+    const std::shared_ptr<WaitWatch>& watch() const;
+    // Test hook: spins > 0 makes the waits of later launches wait for a
+    // count no launch reaches and give up after `spins` polls (every
+    // kernel still ends); 0 restores the normal waits.
+    void setWaitTest(int spins);
+    int waitTest() const;
     // Workgroups of a prepared table.
     int blocks(int table) const;
     // Identity of the device memory a launch of this table uses; it changes

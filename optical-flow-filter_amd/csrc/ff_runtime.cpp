@@ -25,27 +25,88 @@ void check_launch(const char* what) {
 
 // ------------------------------------------------------------------ stream
 
+StreamBox::~StreamBox() {
+    if (ev_) (void)hipEventDestroy(ev_);
+    if (own_) (void)hipStreamDestroy(own_);
+}
+
+std::shared_ptr<StreamBox> StreamBox::create() {
+    auto b = std::make_shared<StreamBox>();
+    FF_HIP(hipStreamCreateWithFlags(&b->own_, hipStreamNonBlocking));
+    return b;
+}
+
+const std::shared_ptr<StreamBox>& StreamBox::library() {
+    // kept for the process (never destroyed: static destructors may run
+    // after the HIP runtime has shut down)
+    static const std::shared_ptr<StreamBox>* lib = new std::shared_ptr<StreamBox>(create());
+    return *lib;
+}
+
+hipStream_t StreamBox::get() const {
+    if (alias_) return alias_->get();
+    const hipStream_t u = user_.load(std::memory_order_acquire);
+    return u ? u : own_;
+}
+
+void StreamBox::drain() { FF_HIP(hipStreamSynchronize(get())); }
+
+void StreamBox::use(hipStream_t s) {
+    drain();
+    alias_.reset();
+    user_.store(s, std::memory_order_release);
+}
+
+void StreamBox::follow(const std::shared_ptr<StreamBox>& to) {
+    if (to.get() == this) return;
+    drain();
+    alias_ = to;
+}
+
+hipEvent_t StreamBox::event() {
+    if (!ev_) FF_HIP(hipEventCreateWithFlags(&ev_, hipEventDisableTiming));
+    return ev_;
+}
+
 namespace {
-std::mutex g_mu;
-hipStream_t g_own = nullptr;
-hipStream_t g_user = nullptr;
-bool g_has_user = false;
+thread_local std::shared_ptr<StreamBox> t_bound;
 }  // namespace
 
-hipStream_t stream() {
-    std::lock_guard<std::mutex> lk(g_mu);
-    if (g_has_user) return g_user;
-    if (!g_own) FF_HIP(hipStreamCreateWithFlags(&g_own, hipStreamNonBlocking));
-    return g_own;
+StreamScope::StreamScope(const std::shared_ptr<StreamBox>& b) : prev_(t_bound) {
+    if (b) t_bound = b;
+}
+StreamScope::~StreamScope() { t_bound = prev_; }
+
+const std::shared_ptr<StreamBox>& current_box() { return t_bound ? t_bound : StreamBox::library(); }
+
+bool stream_bound() { return t_bound != nullptr; }
+
+hipStream_t stream() { return current_box()->get(); }
+
+void set_stream(hipStream_t s) { StreamBox::library()->use(s); }
+
+void synchronize() { FF_HIP(hipDeviceSynchronize()); }
+
+CrossStream::CrossStream(std::initializer_list<const Image*> imgs) : cur_(stream()), cur_box_(current_box()) {
+    for (const Image* im : imgs) {
+        if (!im || !im->allocated()) continue;
+        const std::shared_ptr<StreamBox>& h = im->home();
+        if (!h || h->get() == cur_) continue;
+        bool seen = false;
+        for (const auto& o : others_) seen = seen || o->get() == h->get();
+        if (seen) continue;
+        FF_HIP(hipEventRecord(h->event(), h->get()));
+        FF_HIP(hipStreamWaitEvent(cur_, h->event(), 0));
+        others_.push_back(h);
+    }
 }
 
-void set_stream(hipStream_t s) {
-    std::lock_guard<std::mutex> lk(g_mu);
-    g_user = s;
-    g_has_user = s != nullptr;
+CrossStream::~CrossStream() {
+    if (others_.empty()) return;
+    hipEvent_t e = cur_box_->event();
+    if (hipEventRecord(e, cur_) != hipSuccess) return;
+    for (const auto& o : others_) (void)hipStreamWaitEvent(o->get(), e, 0);
 }
-
-void synchronize() { FF_HIP(hipStreamSynchronize(stream())); }
 
 // ----------------------------------------------------------------- staging
 
@@ -87,12 +148,19 @@ void Image::allocate() {
     if (h_ < 0 || w_ < 0 || d_ < 0 || item_ < 0) throw invalid_argument("GPUImage: negative dimension " + shapeString());
     pitch_ = pitch_for(w_, d_, item_);
     st_ = std::make_shared<Storage>();
+    st_->home = current_box();
     const size_t bytes = pitch_ * (size_t)h_;
     if (bytes > 0) {
         FF_HIP(hipMalloc(&st_->ptr, bytes));
         st_->bytes = bytes;
     }
 }
+
+const std::shared_ptr<StreamBox>& Image::home() const {
+    return st_ && st_->home ? st_->home : StreamBox::library();
+}
+
+hipStream_t Image::homeStream() const { return home()->get(); }
 
 std::string Image::shapeString() const {
     std::ostringstream os;
@@ -116,13 +184,23 @@ void Image::upload(const ff_image_t& host) {
         item_ = (int)host.itemSize;
         allocate();
     }
+    upload(host, homeStream());
+}
+
+void Image::upload(const ff_image_t& host, hipStream_t s) {
+    if (!st_) {
+        h_ = host.height;
+        w_ = host.width;
+        d_ = host.depth;
+        item_ = (int)host.itemSize;
+        allocate();
+    }
     if (!matches(host))
         throw invalid_argument("GPUImage::upload(): shapes do not match. Required: " + shapeString() +
                                ", passed: " + shapeString(host));
     if (rowBytes() == 0 || h_ == 0) return;
     if (!host.data) throw invalid_argument("GPUImage::upload(): null host data");
     if (host.pitch < rowBytes()) throw invalid_argument("GPUImage::upload(): host pitch smaller than a row");
-    hipStream_t s = stream();
     const size_t span = (size_t)(h_ - 1) * pitch_ + rowBytes();
     if (host.pitch == pitch_) {
         // same row layout: one linear copy
@@ -150,7 +228,7 @@ void Image::download(ff_image_t& host) const {
     if (rowBytes() == 0 || h_ == 0) return;
     if (!host.data) throw invalid_argument("GPUImage::download(): null host data");
     if (host.pitch < rowBytes()) throw invalid_argument("GPUImage::download(): host pitch smaller than a row");
-    hipStream_t s = stream();
+    hipStream_t s = homeStream();
     const size_t span = (size_t)(h_ - 1) * pitch_ + rowBytes();
     if (host.pitch == pitch_) {
         FF_HIP(hipMemcpyAsync(host.data, data(), span, hipMemcpyDeviceToHost, s));
@@ -171,7 +249,7 @@ void Image::upload_device(const void* dev, size_t pitch) {
     if (!dev) throw invalid_argument("GPUImage::upload_device(): null device pointer");
     if (pitch < rowBytes()) throw invalid_argument("GPUImage::upload_device(): pitch smaller than a row");
     const CopyRegion r{dev, data(), pitch, pitch_, rowBytes(), h_};
-    launch_copy_regions(&r, 1, stream());
+    launch_copy_regions(&r, 1, homeStream());
     check_launch("GPUImage::upload_device");
 }
 
@@ -180,7 +258,7 @@ void Image::download_device(void* dev, size_t pitch) const {
     if (rowBytes() == 0 || h_ == 0) return;
     if (!dev) throw invalid_argument("GPUImage::download_device(): null device pointer");
     if (pitch < rowBytes()) throw invalid_argument("GPUImage::download_device(): pitch smaller than a row");
-    FF_HIP(hipMemcpy2DAsync(dev, pitch, data(), pitch_, rowBytes(), h_, hipMemcpyDeviceToDevice, stream()));
+    FF_HIP(hipMemcpy2DAsync(dev, pitch, data(), pitch_, rowBytes(), h_, hipMemcpyDeviceToDevice, homeStream()));
 }
 
 // GPUImage::copyFrom (src/gpu/image.cu:152-170): same shape and pitch
@@ -192,6 +270,8 @@ void Image::copyFrom(const Image& src) {
                                ", passed: " + src.shapeString());
     if (!st_ || !src.st_) throw logic_error("GPUImage::copyFrom(): unallocated image");
     if (rowBytes() == 0 || h_ == 0) return;
+    StreamScope sc(home());
+    CrossStream x{&src};
     launch_copy2d(src.view(), view(), rowBytes(), stream());
     check_launch("copyFrom");
 }
@@ -199,7 +279,7 @@ void Image::copyFrom(const Image& src) {
 // GPUImage::clear (src/gpu/image.cu:172-176).
 void Image::clear() {
     if (!st_ || !st_->ptr) return;
-    FF_HIP(hipMemset2DAsync(data(), pitch_, 0, rowBytes(), h_, stream()));
+    FF_HIP(hipMemset2DAsync(data(), pitch_, 0, rowBytes(), h_, homeStream()));
 }
 
 void Image::swapBuffers(Image& o) {
@@ -207,6 +287,52 @@ void Image::swapBuffers(Image& o) {
     if (!st_ || !o.st_) throw logic_error("swapBuffers: unallocated image");
     std::swap(st_->ptr, o.st_->ptr);
     std::swap(st_->bytes, o.st_->bytes);
+}
+
+// ------------------------------------------------------------- InputFrames
+
+struct InputFrames::State {
+    Image spare;
+    std::shared_ptr<StreamBox> copy;
+    hipEvent_t read[2] = {nullptr, nullptr};  // per buffer: its last readers
+    bool recorded[2] = {false, false};
+    int cur = 0;  // buffer the input image holds
+    ~State() {
+        for (hipEvent_t e : read)
+            if (e) (void)hipEventDestroy(e);
+    }
+};
+
+InputFrames::InputFrames() = default;
+InputFrames::InputFrames(InputFrames&&) noexcept = default;
+InputFrames& InputFrames::operator=(InputFrames&&) noexcept = default;
+InputFrames::~InputFrames() = default;
+
+void InputFrames::load(Image& input, const ff_image_t& host) {
+    if (!input.matches(host))
+        throw invalid_argument("GPUImage::upload(): shapes do not match. Required: " + input.shapeString() +
+                               ", passed: " + Image::shapeString(host));
+    if (!st_) st_.reset(new State);
+    State& t = *st_;
+    if (!t.spare.allocated() || !t.spare.sameShape(input) || t.spare.pitch() != input.pitch()) {
+        t.spare = Image(input.height(), input.width(), input.depth(), input.itemSize());
+        t.recorded[0] = t.recorded[1] = false;
+    }
+    if (!t.copy) t.copy = StreamBox::create();
+    for (hipEvent_t& e : t.read)
+        if (!e) FF_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    const int sp = t.cur ^ 1;
+    hipStream_t cs = t.copy->get();
+    if (t.recorded[sp]) FF_HIP(hipStreamWaitEvent(cs, t.read[sp], 0));
+    t.spare.upload(host, cs);  // waits for the copy (and the readers before it)
+    input.swapBuffers(t.spare);
+    t.cur = sp;
+}
+
+void InputFrames::readBy(hipStream_t s) {
+    if (!st_ || !st_->read[st_->cur]) return;
+    FF_HIP(hipEventRecord(st_->read[st_->cur], s));
+    st_->recorded[st_->cur] = true;
 }
 
 void DeviceArray::assign(const void* host, size_t bytes) {

@@ -10,12 +10,15 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstddef>
 #include <cstdint>
+#include <initializer_list>
 #include <memory>
 #include <stdexcept>
 #include <string>
 #include <utility>
+#include <vector>
 
 #include "../../include/flowfilter_amd.h"
 #include "ff_kernels.hpp"
@@ -48,19 +51,79 @@ inline Error invalid_argument(const std::string& m) { return Error(FF_ERR_INVALI
 inline Error logic_error(const std::string& m) { return Error(FF_ERR_LOGIC, m); }
 inline Error runtime_error(const std::string& m) { return Error(FF_ERR_RUNTIME, m); }
 
-// Library-wide stream (the reference runs everything on stream 0 of
-// device 0). Created on first use on the current device; a caller may
-// substitute its own stream.
+// ------------------------------------------------------------- streams
+//
+// The reference runs everything on stream 0 with a host wait after every
+// stage (src/gpu/pipeline.cu:63-82). Here every filter and batch owns a
+// stream, and everything else (standalone stages, images created outside a
+// filter) uses the library stream, which a caller may replace
+// (ff_set_stream). A call on a filter binds the filter's stream to the
+// calling thread for its duration (StreamScope); stream() is the bound
+// stream, or the library stream when none is bound. Objects built inside a
+// bound scope (a pyramid's levels) share the bound stream.
+
+// A stream the library enqueues work on: owned by the box, a caller's
+// stream, or (for a filter added to a batch) the stream of another box.
+class StreamBox {
+public:
+    StreamBox() = default;
+    StreamBox(const StreamBox&) = delete;
+    StreamBox& operator=(const StreamBox&) = delete;
+    ~StreamBox();
+    // A box with a new stream of its own (non-blocking, default priority).
+    static std::shared_ptr<StreamBox> create();
+    // The library stream's box.
+    static const std::shared_ptr<StreamBox>& library();
+    hipStream_t get() const;
+    // Enqueue on `s` (a caller's stream, not destroyed by the box); null
+    // returns to the box's own stream. Waits for the work already enqueued
+    // on the previous stream. For the library box, call it while no other
+    // thread enqueues work through the library.
+    void use(hipStream_t s);
+    // Follow `to` (a batch's box); null ends it. Waits for the work already
+    // enqueued on the previous stream.
+    void follow(const std::shared_ptr<StreamBox>& to);
+    // An event recorded only on this box's stream (reused).
+    hipEvent_t event();
+
+private:
+    void drain();
+    hipStream_t own_ = nullptr;
+    std::atomic<hipStream_t> user_{nullptr};
+    std::shared_ptr<StreamBox> alias_;
+    hipEvent_t ev_ = nullptr;
+};
+
+// Binds a box to the calling thread while in scope (nests).
+class StreamScope {
+public:
+    explicit StreamScope(const std::shared_ptr<StreamBox>& b);
+    ~StreamScope();
+    StreamScope(const StreamScope&) = delete;
+    StreamScope& operator=(const StreamScope&) = delete;
+
+private:
+    std::shared_ptr<StreamBox> prev_;
+};
+
+// The bound box (the library box when none is bound) and its stream.
+const std::shared_ptr<StreamBox>& current_box();
+// Whether a box is bound to the calling thread.
+bool stream_bound();
 hipStream_t stream();
 void set_stream(hipStream_t s);
+// Waits for all work the library enqueued on the device.
 void synchronize();
 
 // Device allocation shared by every handle of one image. The pointer is
 // mutable so that a filter can exchange the buffers of two same-shape
-// images (ping-pong state) while every outstanding handle follows.
+// images (ping-pong state) while every outstanding handle follows. `home`
+// is the box bound when it was allocated: the image's own copies and
+// clears run on its stream.
 struct Storage {
     void* ptr = nullptr;
     size_t bytes = 0;
+    std::shared_ptr<StreamBox> home;
     Storage() = default;
     Storage(const Storage&) = delete;
     Storage& operator=(const Storage&) = delete;
@@ -88,14 +151,20 @@ public:
     DevImg view() const { return DevImg{data(), h_, w_, pitch_}; }
 
     // Host transfers wait for completion (the reference issues synchronous
-    // cudaMemcpy2D, src/gpu/image.cu:105,133).
+    // cudaMemcpy2D, src/gpu/image.cu:105,133). They run on the image's home
+    // stream, after the work already enqueued there.
     void upload(const ff_image_t& host);
+    // The same on stream s (the caller orders s after the image's readers).
+    void upload(const ff_image_t& host, hipStream_t s);
     void download(ff_image_t& host) const;
-    // Device-memory transfers are enqueued on the library stream.
+    // Device-memory transfers and clears are enqueued on the home stream.
     void upload_device(const void* dev, size_t pitch);
     void download_device(void* dev, size_t pitch) const;
     void copyFrom(const Image& src);
     void clear();
+    // Stream of the image's home box (the library stream when unallocated).
+    hipStream_t homeStream() const;
+    const std::shared_ptr<StreamBox>& home() const;
 
     bool sameShape(const Image& o) const {
         return h_ == o.h_ && w_ == o.w_ && d_ == o.d_ && item_ == o.item_;
@@ -114,6 +183,45 @@ private:
     int h_ = 0, w_ = 0, d_ = 0, item_ = 0;
     size_t pitch_ = 0;
     std::shared_ptr<Storage> st_;
+};
+
+// Orders work across streams for images another object owns: on
+// construction the current stream waits for the work enqueued so far on
+// each image's home stream (when that is another stream); on destruction
+// each such home stream waits for the work enqueued meanwhile on the
+// current stream (so later writes there cannot overtake this work's reads).
+class CrossStream {
+public:
+    CrossStream(std::initializer_list<const Image*> imgs);
+    ~CrossStream();
+    CrossStream(const CrossStream&) = delete;
+    CrossStream& operator=(const CrossStream&) = delete;
+
+private:
+    hipStream_t cur_;
+    std::shared_ptr<StreamBox> cur_box_;
+    std::vector<std::shared_ptr<StreamBox>> others_;
+};
+
+// Host frames for a filter's own input image, double buffered: load()
+// copies a host frame into the spare buffer on a copy stream of its own,
+// after the last work that read that buffer, while the filter's stream may
+// still be reading the current one; then it exchanges the two buffers
+// behind the input image's handles. It returns once the host memory has
+// been read. readBy(s) records that the work enqueued so far on s reads the
+// current buffer (call it after each compute()).
+class InputFrames {
+public:
+    InputFrames();
+    InputFrames(InputFrames&&) noexcept;
+    InputFrames& operator=(InputFrames&&) noexcept;
+    ~InputFrames();
+    void load(Image& input, const ff_image_t& host);
+    void readBy(hipStream_t s);
+
+private:
+    struct State;
+    std::unique_ptr<State> st_;
 };
 
 // Small device array (index maps and similar).

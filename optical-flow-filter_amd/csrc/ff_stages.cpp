@@ -70,6 +70,7 @@ void ImageModel::run() {
     if (!configured_) throw logic_error("ImageModel::compute() stage not configured.");
     require_shape(bound_, a0_, "ImageModel::compute()");
     require_alloc(bound_, "ImageModel::compute()");
+    CrossStream x{&bound_, &a0_, &grad_};
     launch_image_model(bound_.view(), bound_.itemSize() == 1, a0_.view(), grad_.view(), stream());
     check_launch("ImageModel::compute");
 }
@@ -124,6 +125,7 @@ void FlowPropagator::run() {
     if (!configured_) throw logic_error("FlowPropagator::compute() stage not configured.");
     require_shape(bound_, bufX_, "FlowPropagator::compute()");
     require_alloc(bound_, "FlowPropagator::compute()");
+    CrossStream x{&bound_, &bufX_, &bufY_};
     hipStream_t s = stream();
     const DevImg bx = bufX_.view(), by = bufY_.view();
     DevImg src = bound_.view();
@@ -231,6 +233,7 @@ void FlowPropagatorPayload::run() {
     require_alloc(bFlow_, "FlowPropagatorPayload::compute()");
     require_alloc(bScalar_, "FlowPropagatorPayload::compute()");
     require_alloc(bVector_, "FlowPropagatorPayload::compute()");
+    CrossStream x{&bFlow_, &bScalar_, &bVector_, &flowX_, &flowY_, &scalarX_, &scalarY_, &vectorX_, &vectorY_};
     hipStream_t s = stream();
     DevImg fx = flowX_.view(), fy = flowY_.view(), sx = scalarX_.view(), sy = scalarY_.view(),
            vx = vectorX_.view(), vy = vectorY_.view();
@@ -319,6 +322,7 @@ void FlowUpdate::run() {
     require_alloc(flow_, "FlowUpdate::compute()");
     require_alloc(image_, "FlowUpdate::compute()");
     require_alloc(grad_, "FlowUpdate::compute()");
+    CrossStream x{&flow_, &image_, &grad_, &imageUpd_, &flowUpd_};
     launch_update(image_.view(), grad_.view(), imageUpd_.view(), flow_.view(), imageUpd_.view(), flowUpd_.view(),
                   gamma_, maxflow_, stream());
     check_launch("FlowUpdate::compute");
@@ -438,6 +442,7 @@ void DeltaFlowUpdate::run() {
     for (const Image* im : {&dflow_, &imageOld_, &image_, &grad_, &bFlow_}) require_alloc(*im, "DeltaFlowUpdate::compute()");
     if (flowUpd_.height() > 0 && flowUpd_.width() > 0 && (bFlow_.height() <= 0 || bFlow_.width() <= 0))
         throw logic_error("DeltaFlowUpdate::compute(): empty upper-level flow");
+    CrossStream x{&dflow_, &imageOld_, &image_, &grad_, &bFlow_, &imageUpd_, &dflowUpd_, &flowUpd_};
     launch_delta_update(image_.view(), grad_.view(), imageOld_.view(), dflow_.view(), bFlow_.view(), xmap(), ymap(),
                         imageUpd_.view(), dflowUpd_.view(), flowUpd_.view(), gamma_, maxflow_, stream());
     check_launch("DeltaFlowUpdate::compute");
@@ -486,6 +491,7 @@ void FlowSmoother::run() {
     if (!configured_) throw logic_error("FlowSmoother::compute() stage not configured.");
     require_shape(bound_, bufX_, "FlowSmoother::compute()");
     require_alloc(bound_, "FlowSmoother::compute()");
+    CrossStream x{&bound_, &bufX_, &bufY_};
     hipStream_t s = stream();
     const DevImg bx = bufX_.view(), by = bufY_.view();
     launch_smooth_pass(0, bound_.view(), bx, s);
@@ -553,6 +559,7 @@ void ImagePyramid::run() {
     if (!configured_) throw runtime_error("ImagePyramid::compute(): stage not configured");
     require_alloc(bound_, "ImagePyramid::compute()");
     const bool u8 = bound_.itemSize() == 1;
+    CrossStream x{&bound_};
     hipStream_t s = stream();
     for (size_t l = 0; l < py_.size(); ++l) {
         const Image& src = l == 0 ? bound_ : py_[l - 1];
@@ -566,6 +573,212 @@ void ImagePyramid::compute() {
     startTiming();
     run();
     stopTiming();
+}
+
+// =============================================================== FlowToColor
+// src/gpu/display.cu:20-140
+
+ff_image_t colorWheelRGBA() {
+    size_t bytes = 0;
+    const unsigned char* p = colorwheel_rgba(&bytes);
+    if (bytes != (size_t)kWheelHeight * kWheelWidth * kWheelDepth)
+        throw runtime_error("colour wheel table has " + num((long)bytes) + " bytes");
+    ff_image_t w{};
+    w.height = kWheelHeight;
+    w.width = kWheelWidth;
+    w.depth = kWheelDepth;
+    w.itemSize = 1;
+    w.pitch = (size_t)kWheelWidth * kWheelDepth;
+    w.data = const_cast<unsigned char*>(p);
+    return w;
+}
+
+// FlowToColor(inputFlow, maxflow) (display.cu:31-42).
+FlowToColor::FlowToColor(const Image& inputFlow, float maxflow) {
+    setInputFlow(inputFlow);
+    setMaxFlow(maxflow);
+    configure();
+}
+
+// FlowToColor::configure (display.cu:50-78): the wheel uploaded as a
+// 256 x 256 RGBA image and the output colour image of the input's size.
+void FlowToColor::configure() {
+    if (!inputSet_) throw runtime_error("FlowToColor::configure(): input flow not set");
+    const ff_image_t w = colorWheelRGBA();
+    wheel_ = Image(w.height, w.width, w.depth, 1);
+    wheel_.upload(w);
+    color_ = Image(input_.height(), input_.width(), 4, 1);
+    configured_ = true;
+}
+
+void FlowToColor::run() {
+    if (!configured_) throw logic_error("FlowToColor::compute(): stage not configured");
+    require_shape(input_, color_, "FlowToColor::compute()");
+    require_alloc(input_, "FlowToColor::compute()");
+    CrossStream x{&input_, &wheel_, &color_};
+    launch_flow_to_color(input_.view(), wheel_.view(), maxflow_, color_.view(), stream());
+    check_launch("FlowToColor::compute");
+}
+
+// FlowToColor::compute (display.cu:81-97).
+void FlowToColor::compute() {
+    startTiming();
+    run();
+    stopTiming();
+}
+
+// FlowToColor::setInputFlow (display.cu:100-117).
+void FlowToColor::setInputFlow(const Image& flow) {
+    if (flow.depth() != 2)
+        throw runtime_error("FlowToColor::setInputFlow(): input flow should have depth 2: " + num(flow.depth()));
+    if (flow.itemSize() != 4)
+        throw runtime_error("FlowToColor::setInputFlow(): input flow should have item size 4: " + num(flow.itemSize()));
+    input_ = flow;
+    inputSet_ = true;
+}
+
+// FlowToColor::setMaxFlow (display.cu:130-137).
+void FlowToColor::setMaxFlow(float m) {
+    if (!(m > 0.0f)) throw runtime_error("FlowToColor::setMaxFlow(): maxflow should be greater than 0.0: " + std::to_string(m));
+    maxflow_ = m;
+}
+
+// ===================================================== LaxWendroffPropagator
+// src/gpu/propagation.cu:505-724
+
+LaxWendroffPropagator::LaxWendroffPropagator(const Image& inputFlow, const Image& inputImage) {
+    setInputFlow(inputFlow);
+    setInputImage(inputImage);
+    configure();
+}
+
+// LaxWendroffPropagator::configure (propagation.cu:533-569).
+void LaxWendroffPropagator::configure() {
+    if (!flowSet_) throw runtime_error("LaxWendroffPropagator::configure(): input flow has not been set");
+    if (!imageSet_) throw runtime_error("LaxWendroffPropagator::configure(): input image has not been set");
+    bufX_ = Image(flow_.height(), flow_.width(), image_.depth(), 4);
+    bufY_ = Image(flow_.height(), flow_.width(), image_.depth(), 4);
+    configured_ = true;
+}
+
+// LaxWendroffPropagator::compute (propagation.cu:572-644): a Y pass then an
+// X pass, `iterations` times (at least once).
+void LaxWendroffPropagator::run() {
+    if (!configured_) throw runtime_error("LaxWendroffPropagator::compute(): stage not configured");
+    require_shape(flow_, bufX_, "LaxWendroffPropagator::compute()");
+    require_shape(image_, bufX_, "LaxWendroffPropagator::compute()");
+    require_alloc(flow_, "LaxWendroffPropagator::compute()");
+    require_alloc(image_, "LaxWendroffPropagator::compute()");
+    if (image_.depth() != bufX_.depth())
+        throw logic_error("LaxWendroffPropagator::compute(): input image depth changed since configure()");
+    CrossStream x{&flow_, &image_, &bufX_, &bufY_};
+    hipStream_t s = stream();
+    const int c = image_.depth();
+    launch_lax_wendroff_pass(1, c, flow_.view(), image_.view(), bufY_.view(), dt_, s);
+    launch_lax_wendroff_pass(0, c, flow_.view(), bufY_.view(), bufX_.view(), dt_, s);
+    for (int k = 0; k < iterations_ - 1; ++k) {
+        launch_lax_wendroff_pass(1, c, flow_.view(), bufX_.view(), bufY_.view(), dt_, s);
+        launch_lax_wendroff_pass(0, c, flow_.view(), bufY_.view(), bufX_.view(), dt_, s);
+    }
+    check_launch("LaxWendroffPropagator::compute");
+}
+
+void LaxWendroffPropagator::compute() {
+    startTiming();
+    run();
+    stopTiming();
+}
+
+// LaxWendroffPropagator::setIterations (propagation.cu:647-658).
+void LaxWendroffPropagator::setIterations(int n) {
+    if (n <= 0) throw runtime_error("LaxWendroffPropagator::setIterations(): iterations less than zero: " + num(n));
+    iterations_ = n;
+    dt_ = 1.0f / float(iterations_);
+}
+
+// LaxWendroffPropagator::setInputFlow (propagation.cu:671-687).
+void LaxWendroffPropagator::setInputFlow(const Image& flow) {
+    if (flow.depth() != 2)
+        throw runtime_error("LaxWendroffPropagator::setInputFlow(): input flow should have depth 2: " +
+                            num(flow.depth()));
+    if (flow.itemSize() != 4)
+        throw runtime_error("LaxWendroffPropagator::setInputFlow(): input flow should have item size 4: " +
+                            num(flow.itemSize()));
+    flow_ = flow;
+    flowSet_ = true;
+}
+
+// LaxWendroffPropagator::setInputImage (propagation.cu:690-714): depth 1 or
+// 4, float, the input flow's size.
+void LaxWendroffPropagator::setInputImage(const Image& img) {
+    if (img.depth() != 1 && img.depth() != 4)
+        throw runtime_error("LaxWendroffPropagator::setInputImage(): input image should have depth 1 or 4, got: " +
+                            num(img.depth()));
+    if (img.itemSize() != 4)
+        throw runtime_error("LaxWendroffPropagator::setInputImage(): input image should have item size 4: " +
+                            num(img.itemSize()));
+    if (img.height() != flow_.height() || img.width() != flow_.width())
+        throw runtime_error("LaxWendroffPropagator::setInputImage(): image shape " + img.shapeString() +
+                            " does not match the input flow " + flow_.shapeString());
+    image_ = img;
+    imageSet_ = true;
+}
+
+// ============================================== RotationalFlowImagePredictor
+// src/gpu/rotation.cu:20-134
+
+RotationalFlowImagePredictor::RotationalFlowImagePredictor(const Camera& cam, const Image& inputImage) {
+    setCamera(cam);
+    setInputImage(inputImage);
+    configure();
+}
+
+// RotationalFlowImagePredictor::configure (rotation.cu:50-68). As in the
+// reference, the propagator is made anew: its iteration count returns to
+// 0 (one pass pair with dt = 0) until setIterations.
+void RotationalFlowImagePredictor::configure() {
+    if (!imageSet_) throw runtime_error("RotationalFlowImagePredictor::configure(): input image not set");
+    flow_ = Image(image_.height(), image_.width(), 2, 4);
+    propagator_ = LaxWendroffPropagator(flow_, image_);
+    configured_ = true;
+}
+
+// RotationalFlowImagePredictor::compute (rotation.cu:71-88).
+void RotationalFlowImagePredictor::compute() {
+    startTiming();
+    if (!configured_) throw runtime_error("RotationalFlowImagePredictor::compute(): stage not configured");
+    CrossStream x{&flow_};
+    launch_rotational_flow(camera_, w_[0], w_[1], w_[2], flow_.view(), stream());
+    check_launch("RotationalFlowImagePredictor::compute");
+    propagator_.run();
+    stopTiming();
+}
+
+// RotationalFlowImagePredictor::setInputImage (rotation.cu:91-97): the
+// reference does not check the image here; configure() does (through the
+// propagator).
+void RotationalFlowImagePredictor::setInputImage(const Image& img) {
+    image_ = img;
+    imageSet_ = true;
+}
+
+// RotationalFlowImagePredictor::setAngularVelocity (rotation.cu:116-121).
+void RotationalFlowImagePredictor::setAngularVelocity(float wx, float wy, float wz) {
+    w_[0] = wx;
+    w_[1] = wy;
+    w_[2] = wz;
+}
+
+// createPerspectiveCamera (camera.cu:14-29): pixel ranges width-1 and
+// height-1, focal lengths in pixels, the principal point at the centre.
+Camera createPerspectiveCamera(float focalLength, int height, int width, float sensorHeight, float sensorWidth) {
+    const int imgX = width - 1, imgY = height - 1;
+    Camera c;
+    c.alphaX = focalLength * (float)imgX / sensorWidth;
+    c.alphaY = focalLength * (float)imgY / sensorHeight;
+    c.centerX = 0.5f * (float)imgX;
+    c.centerY = 0.5f * (float)imgY;
+    return c;
 }
 
 }  // namespace ffamd

@@ -34,12 +34,24 @@ public:
     virtual void configure() = 0;
     virtual void compute() = 0;
     virtual const char* name() const = 0;
-    float elapsedTime() { return timer_.elapsed(); }
+    virtual float elapsedTime() { return timer_.elapsed(); }
+    // The stream box of a filter or batch; null for the other stages, which
+    // run on the stream bound by their caller (the library stream, or the
+    // stream of the filter they belong to).
+    const std::shared_ptr<StreamBox>& box() const { return box_; }
+    // A filter made without arguments: give it a stream of its own.
+    void ensureOwnStream() {
+        if (!box_) box_ = StreamBox::create();
+    }
 
 protected:
     void startTiming() { timer_.start(); }
     void stopTiming() { timer_.stop(); }
+    // Filters and batches: share the bound box (a level built inside a
+    // pyramid) or create a stream of their own.
+    void ownStream() { box_ = stream_bound() ? current_box() : StreamBox::create(); }
     Timer timer_;
+    std::shared_ptr<StreamBox> box_;
 };
 
 // ---------------------------------------------------------------- ImageModel
@@ -228,6 +240,93 @@ private:
     Image input_, bound_;
     std::vector<Image> px_, py_;
 };
+
+// --------------------------------------------------------------- FlowToColor
+// include/flowfilter/gpu/display.h:23-93: RGBA colour coding of a flow field
+// (the demos' display of the filter's output).
+class FlowToColor : public Stage {
+public:
+    FlowToColor() = default;
+    FlowToColor(const Image& inputFlow, float maxflow);
+    void configure() override;
+    void compute() override;
+    const char* name() const override { return "FlowToColor"; }
+    void setInputFlow(const Image& flow);
+    Image getColorFlow() const { return color_; }
+    float getMaxFlow() const { return maxflow_; }
+    void setMaxFlow(float m);
+    void downloadColorFlow(ff_image_t& img) const { color_.download(img); }
+    void run();
+
+private:
+    bool configured_ = false, inputSet_ = false;
+    float maxflow_ = 1.0f;
+    Image input_, wheel_, color_;
+};
+
+// ----------------------------------------------------- LaxWendroffPropagator
+// include/flowfilter/gpu/propagation.h:218-291: N iterations of a Y and an X
+// Lax-Wendroff pass (dt = 1/N) of a float image (1 or 4 channels) along a
+// flow field; 0 iterations (never set) runs one pair with dt = 0.
+class LaxWendroffPropagator : public Stage {
+public:
+    LaxWendroffPropagator() = default;
+    LaxWendroffPropagator(const Image& inputFlow, const Image& inputImage);
+    void configure() override;
+    void compute() override;
+    const char* name() const override { return "LaxWendroffPropagator"; }
+    void setIterations(int n);
+    int getIterations() const { return iterations_; }
+    float getDt() const { return dt_; }
+    void setInputFlow(const Image& flow);
+    void setInputImage(const Image& img);
+    Image getFlow() const { return flow_; }
+    Image getPropagatedImage() const { return bufX_; }
+    void run();
+
+private:
+    bool configured_ = false, flowSet_ = false, imageSet_ = false;
+    int iterations_ = 0;
+    float dt_ = 0.0f;
+    Image flow_, image_, bufX_, bufY_;
+};
+
+// ---------------------------------------------- RotationalFlowImagePredictor
+// include/flowfilter/gpu/rotation.h:24-88: the optical flow of a rotating
+// camera and the image it predicts (the input image propagated along that
+// flow with LaxWendroffPropagator).
+class RotationalFlowImagePredictor : public Stage {
+public:
+    RotationalFlowImagePredictor() = default;
+    explicit RotationalFlowImagePredictor(const Camera& cam) { setCamera(cam); }
+    RotationalFlowImagePredictor(const Camera& cam, const Image& inputImage);
+    void configure() override;
+    void compute() override;
+    const char* name() const override { return "RotationalFlowImagePredictor"; }
+    void setInputImage(const Image& img);
+    Image getPredictedImage() const { return propagator_.getPropagatedImage(); }
+    Image getOpticalFlow() const { return flow_; }
+    void setCamera(const Camera& cam) { camera_ = cam; }
+    const Camera& getCamera() const { return camera_; }
+    void setAngularVelocity(float wx, float wy, float wz);
+    void setIterations(int n) { propagator_.setIterations(n); }
+    int getIterations() const { return propagator_.getIterations(); }
+
+private:
+    bool configured_ = false, imageSet_ = false;
+    Camera camera_{0.0f, 0.0f, 0.0f, 0.0f};
+    float w_[3] = {0.0f, 0.0f, 0.0f};
+    Image image_, flow_;
+    LaxWendroffPropagator propagator_;
+};
+
+// createPerspectiveCamera (src/gpu/camera.cu:14-29).
+Camera createPerspectiveCamera(float focalLength, int height, int width, float sensorHeight, float sensorWidth);
+
+// The colour wheel as a host image descriptor (getColorWheelRGBA,
+// src/colorwheel.cpp:21862-21873): 256 x 256 x 4 u8, read-only data of the
+// library.
+ff_image_t colorWheelRGBA();
 
 // Upper-level texel index of every output column (or row) for the
 // normalized-coordinate point sampling of deltaFlowUpdate_k

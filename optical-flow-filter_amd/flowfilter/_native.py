@@ -19,6 +19,12 @@ class image_t(ctypes.Structure):
                 ("pitch", ctypes.c_size_t), ("itemSize", ctypes.c_size_t), ("data", ctypes.c_void_p)]
 
 
+class ff_camera(ctypes.Structure):
+    """ff_camera / flowfilter::gpu::perspectiveCamera (camera.h:20-25)."""
+    _fields_ = [("alphaX", ctypes.c_float), ("alphaY", ctypes.c_float), ("centerX", ctypes.c_float),
+                ("centerY", ctypes.c_float)]
+
+
 _S = ctypes.c_void_p             # ff_stage*
 _I = ctypes.c_void_p             # ff_gpuimage*
 _pS = ctypes.POINTER(ctypes.c_void_p)
@@ -54,10 +60,41 @@ SIGNATURES = {
     "ff_gpuimage_upload_device": [_I, _v, _z],
     "ff_gpuimage_download_device": [_I, _v, _z],
     # Stage
+    "ff_gpuimage_to_dlpack": [_I, _v, _i, _pv],
+    "ff_dlpack_delete": [_v],
+    # FlowToColor
+    "ff_get_color_wheel_rgba": [_h],
+    "ff_flowtocolor_create": [_I, _f, _pS],
+    "ff_flowtocolor_set_input_flow": [_S, _I],
+    "ff_flowtocolor_get_color_flow": [_S, _pI],
+    "ff_flowtocolor_get_max_flow": [_S, _pf],
+    "ff_flowtocolor_set_max_flow": [_S, _f],
+    "ff_flowtocolor_download_color_flow": [_S, _h],
+    # camera, rotation, Lax-Wendroff
+    "ff_create_perspective_camera": [_f, _i, _i, _f, _f, ctypes.POINTER(ff_camera)],
+    "ff_laxwendroff_create": [_I, _I, _pS],
+    "ff_laxwendroff_set_iterations": [_S, _i],
+    "ff_laxwendroff_get_iterations": [_S, _pi],
+    "ff_laxwendroff_get_dt": [_S, _pf],
+    "ff_laxwendroff_set_input_flow": [_S, _I],
+    "ff_laxwendroff_set_input_image": [_S, _I],
+    "ff_laxwendroff_get_flow": [_S, _pI],
+    "ff_laxwendroff_get_propagated_image": [_S, _pI],
+    "ff_rotation_create": [ctypes.POINTER(ff_camera), _I, _pS],
+    "ff_rotation_set_input_image": [_S, _I],
+    "ff_rotation_get_predicted_image": [_S, _pI],
+    "ff_rotation_get_optical_flow": [_S, _pI],
+    "ff_rotation_set_camera": [_S, ctypes.POINTER(ff_camera)],
+    "ff_rotation_get_camera": [_S, ctypes.POINTER(ff_camera)],
+    "ff_rotation_set_angular_velocity": [_S, _f, _f, _f],
+    "ff_rotation_set_iterations": [_S, _i],
+    "ff_rotation_get_iterations": [_S, _pi],
     "ff_stage_configure": [_S],
     "ff_stage_compute": [_S],
     "ff_stage_elapsed_time": [_S, _pf],
     "ff_stage_destroy": [_S],
+    "ff_stage_get_stream": [_S, _pv],
+    "ff_stage_set_stream": [_S, _v],
     # ImageModel
     "ff_imagemodel_create": [_I, _pS],
     "ff_imagemodel_set_input_image": [_S, _I],
@@ -194,6 +231,7 @@ SIGNATURES = {
     "ff_filter_set_graphs": [_S, _i],
     "ff_filter_get_graphs": [_S, _pi],
     "ff_filter_get_wait_timeouts": [_S, ctypes.POINTER(ctypes.c_ulonglong)],
+    "ff_filter_set_wait_test": [_S, _i],
     "ff_filter_set_tile_waves": [_S, _i],
     "ff_filter_get_tile_waves": [_S, _pi],
     "ff_filter_set_tile_top_rows": [_S, _i],
@@ -223,12 +261,14 @@ SIGNATURES = {
     "ff_batch_add": [_S, _S],
     "ff_batch_size": [_S, _pi],
     "ff_batch_load_images_device": [_S, ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_size_t), _i],
+    "ff_batch_set_images_device": [_S, ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_size_t), _i],
     "ff_batch_get_blocks": [_S, _pi],
     # profiling
     "ff_profile_begin": [ctypes.c_char_p],
     "ff_profile_end": [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_longlong)],
     # stream markers
     "ff_marker_record": [_pv],
+    "ff_marker_record_on": [_S, _pv],
     "ff_marker_elapsed": [_v, _v, _pf],
     "ff_marker_destroy": [_v],
 }
@@ -290,10 +330,14 @@ def get_stream():
 
 
 class Marker(object):
-    """A timing event recorded on the library stream at construction."""
+    """A timing event recorded at construction on the stream of ``stage`` (a
+    filter or batch object, or its handle), or on the library stream."""
 
-    def __init__(self):
-        self._h = get_handle("ff_marker_record")
+    def __init__(self, stage=None):
+        if stage is None:
+            self._h = get_handle("ff_marker_record")
+        else:
+            self._h = get_handle("ff_marker_record_on", getattr(stage, "handle", stage))
 
     def elapsed_ms(self, stop):
         """Milliseconds from this marker to ``stop`` (waits for ``stop``)."""

@@ -38,6 +38,23 @@ class Stage(object):
         """Milliseconds of the last compute() (waits for it to finish)."""
         return N.get_float("ff_stage_elapsed_time", self.handle)
 
+    @property
+    def stream(self):
+        """The hipStream_t (as an int) this stage's work is enqueued on: a
+        filter's or batch's own, else the library stream."""
+        return N.get_handle("ff_stage_get_stream", self.handle)
+
+    @stream.setter
+    def stream(self, hip_stream):
+        """Filters and batches: enqueue on a caller's hipStream_t (an int
+        pointer; None returns to the filter's own stream)."""
+        N.call("ff_stage_set_stream", self.handle, ctypes.c_void_p(hip_stream) if hip_stream else None)
+
+    def setWaitTest(self, spins):
+        """Test hook (PyramidalFlowFilter and batches): the fused kernel's
+        waits give up after ``spins`` polls (0 restores them)."""
+        N.call("ff_filter_set_wait_test", self.handle, int(spins))
+
     # helpers for subclasses
     def _img(self, fn):
         return GPUImage._wrap(N.get_handle(fn, self.handle))

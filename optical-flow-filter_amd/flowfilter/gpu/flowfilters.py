@@ -319,7 +319,7 @@ class PyramidalFlowFilter(Stage):
 
     def waitTimeouts(self):
         """Waits of a fused frame that ended at the spin limit (0 when every
-        level saw the level above it complete). Synchronizes the device."""
+        level saw the level above it complete). Waits for the filter's stream."""
         v = ctypes.c_ulonglong(0)
         N.call("ff_filter_get_wait_timeouts", self.handle, ctypes.byref(v))
         return v.value
@@ -362,6 +362,17 @@ class BatchedFlowFilter(Stage):
         q = (ctypes.c_size_t * max(n, 1))(*[int(v) for v in pitches])
         N.call("ff_batch_load_images_device", self.handle, p, q, n)
 
+    def setImagesDevice(self, ptrs, pitches):
+        """Next frame of every filter read in place by the next compute():
+        ptrs[i] (device u8, row pitch pitches[i]) must hold filter i's frame
+        until that compute() has run on the batch's stream."""
+        n = len(ptrs)
+        if len(pitches) != n:
+            raise ValueError("setImagesDevice: %d pointers, %d pitches" % (n, len(pitches)))
+        p = (ctypes.c_void_p * max(n, 1))(*[int(v) for v in ptrs])
+        q = (ctypes.c_size_t * max(n, 1))(*[int(v) for v in pitches])
+        N.call("ff_batch_set_images_device", self.handle, p, q, n)
+
     def __getitem__(self, i):
         return self._filters[i]
 
@@ -380,7 +391,7 @@ class BatchedFlowFilter(Stage):
 
     def waitTimeouts(self):
         """Waits of a fused step that ended at the spin limit (0 when every
-        level saw the level above it complete). Synchronizes the device."""
+        level saw the level above it complete). Waits for the batch's stream."""
         v = ctypes.c_ulonglong(0)
         N.call("ff_filter_get_wait_timeouts", self.handle, ctypes.byref(v))
         return v.value

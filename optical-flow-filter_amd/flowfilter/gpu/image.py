@@ -3,7 +3,9 @@
 Mirrors python/flowfilter/gpu/image.pyx:16-65 of the reference
 (GPUImage(shape, itemSize), upload, download(dtype, output), shape) over
 ff_gpuimage_* of the C ABI. Handles share the device buffer like the C++
-GPUImage (shared pointer semantics).
+GPUImage (shared pointer semantics). Adds the DLPack protocol
+(``__dlpack__`` / ``__dlpack_device__``), so ``torch.from_dlpack`` takes a
+device image (e.g. a filter's getFlowDevice()) without a copy.
 """
 import ctypes
 
@@ -11,6 +13,27 @@ import numpy as np
 
 from .. import _native as N
 from ..image import Image
+
+_KDLROCM = 10
+
+# PyCapsule plumbing for the DLPack exchange (CPython C API through ctypes).
+_PyCapsule_New = ctypes.pythonapi.PyCapsule_New
+_PyCapsule_New.restype = ctypes.py_object
+_PyCapsule_New.argtypes = [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_void_p]
+_PyCapsule_IsValid = ctypes.pythonapi.PyCapsule_IsValid
+_PyCapsule_IsValid.restype = ctypes.c_int
+_PyCapsule_IsValid.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
+_PyCapsule_GetPointer = ctypes.pythonapi.PyCapsule_GetPointer
+_PyCapsule_GetPointer.restype = ctypes.c_void_p
+_PyCapsule_GetPointer.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
+
+
+@ctypes.CFUNCTYPE(None, ctypes.c_void_p)
+def _capsule_destructor(capsule):
+    # a capsule no consumer took (a consumer renames it "used_dltensor" and
+    # calls the tensor's deleter itself)
+    if _PyCapsule_IsValid(capsule, b"dltensor"):
+        N.lib.ff_dlpack_delete(ctypes.c_void_p(_PyCapsule_GetPointer(capsule, b"dltensor")))
 
 
 class GPUImage(object):
@@ -71,6 +94,25 @@ class GPUImage(object):
 
     def downloadDevice(self, ptr, pitch):
         N.call("ff_gpuimage_download_device", self.handle, ctypes.c_void_p(ptr), ctypes.c_size_t(pitch))
+
+    def __dlpack_device__(self):
+        """(kDLROCM, device id) -- the DLPack device of the buffer."""
+        return (_KDLROCM, 0)
+
+    def __dlpack__(self, stream=None, max_version=None, dl_device=None):
+        """DLPack capsule viewing this image's buffer (uint8 or float32; shape
+        (H, W) or (H, W, depth), row stride from the pitch), zero-copy. The
+        consumer's stream (an int hipStream_t; None: the null stream; -1: no
+        synchronization) first waits for the work enqueued on the image. The
+        view holds the buffer alive. A filter's state images alternate
+        between two buffers from one compute() to the next: take a new view
+        after each compute()."""
+        if dl_device is not None and tuple(dl_device) != self.__dlpack_device__():
+            raise BufferError("GPUImage lives on device %s, not %s" % (self.__dlpack_device__(), dl_device))
+        sync = 0 if stream == -1 else 1
+        s = None if stream in (None, -1) else ctypes.c_void_p(int(stream))
+        m = N.get_handle("ff_gpuimage_to_dlpack", self.handle, s, sync)
+        return _PyCapsule_New(m, b"dltensor", ctypes.cast(_capsule_destructor, ctypes.c_void_p))
 
     @property
     def height(self):

@@ -1,7 +1,6 @@
-"""flowfilter.gpu.propagation (python/flowfilter/gpu/propagation.pyx:78-315).
-
-LaxWendroffPropagator (:317-404) belongs to the rotational image-prediction
-path, which this package does not provide.
+"""flowfilter.gpu.propagation (python/flowfilter/gpu/propagation.pyx:78-330):
+FlowPropagator, FlowPropagatorPayload and LaxWendroffPropagator (the image
+propagation of the rotational image-prediction path, :254-330).
 """
 from .. import _native as N
 from ._stage import Stage, _hnd
@@ -100,3 +99,37 @@ class FlowPropagatorPayload(Stage):
     @border.setter
     def border(self, b):
         N.call("ff_flowpropagatorpayload_set_border", self.handle, int(b))
+
+
+class LaxWendroffPropagator(Stage):
+    """Lax-Wendroff propagation of a float image (1 or 4 channels) along a
+    flow field (propagation.pyx:254-330)."""
+
+    def __init__(self, inputFlow=None, inputImage=None):
+        if inputFlow is None or inputImage is None:
+            inputFlow = inputImage = None
+        self._set_handle(N.get_handle("ff_laxwendroff_create", _hnd(inputFlow), _hnd(inputImage)))
+
+    def setInputFlow(self, inputFlow):
+        N.call("ff_laxwendroff_set_input_flow", self.handle, inputFlow.handle)
+
+    def setInputImage(self, img):
+        N.call("ff_laxwendroff_set_input_image", self.handle, img.handle)
+
+    def getFlow(self):
+        return self._img("ff_laxwendroff_get_flow")
+
+    def getPropagatedImage(self):
+        return self._img("ff_laxwendroff_get_propagated_image")
+
+    @property
+    def iterations(self):
+        return self._geti("ff_laxwendroff_get_iterations")
+
+    @iterations.setter
+    def iterations(self, n):
+        N.call("ff_laxwendroff_set_iterations", self.handle, int(n))
+
+    @property
+    def dt(self):
+        return self._getf("ff_laxwendroff_get_dt")

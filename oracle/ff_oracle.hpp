@@ -452,6 +452,114 @@ std::vector<Field<Tin>> pyramid(const Field<Tin>& img, int levels) {
     return out;
 }
 
+// --------------------------------------------------------------- display
+
+// Texel read by a normalised-coordinate, point-filtered, clamped texture
+// (the colour wheel of FlowToColor::configure, display.cu:59-67) at
+// coordinate c on an axis of n texels: floor(c * n) clamped to [0, n-1]; a
+// NaN coordinate reads texel 0 (the texture unit's own behaviour for it is
+// not pinned by the reference).
+inline int wheel_texel_index(float c, int n) {
+    const float s = c * (float)n;
+    if (!(s > 0.0f)) return 0;  // negative, zero or NaN
+    if (s >= (float)(n - 1)) return n - 1;
+    return (int)std::floor(s);
+}
+
+// flowToColor_k (display_k.cu:14-43): out (h x w x 4) = wheel texel at
+// ((u + m) / (2m), (v + m) / (2m)), the normalisation in float with IEEE
+// division; wheel is wh x ww x 4 u8, row-major.
+inline void flow_to_color(const Field<float>& flow, float maxflow, const Field<uint8_t>& wheel,
+                          Field<uint8_t>& out) {
+    out = Field<uint8_t>(flow.h, flow.w, 4);
+    const float den = 2.0f * maxflow;
+    for (int y = 0; y < flow.h; ++y)
+        for (int x = 0; x < flow.w; ++x) {
+            const float cu = (flow.at(y, x, 0) + maxflow) / den;
+            const float cv = (flow.at(y, x, 1) + maxflow) / den;
+            const int ix = wheel_texel_index(cu, wheel.w), iy = wheel_texel_index(cv, wheel.h);
+            for (int k = 0; k < 4; ++k) out.at(y, x, k) = wheel.at(iy, ix, k);
+        }
+}
+
+// ------------------------------------- rotational flow and Lax-Wendroff
+
+// perspectiveCamera and createPerspectiveCamera (camera.h:20-25,
+// camera.cu:14-29).
+struct Camera {
+    float alphaX, alphaY, centerX, centerY;
+};
+inline Camera create_perspective_camera(float focal, int h, int w, float sensor_h, float sensor_w) {
+    const int ix = w - 1, iy = h - 1;
+    return Camera{focal * (float)ix / sensor_w, focal * (float)iy / sensor_h, 0.5f * (float)ix, 0.5f * (float)iy};
+}
+
+// rotationalOpticalFlow_k (rotation_k.cu:17-43): p = ((x - cx)/ax,
+// (y - cy)/ay, 1) (camera_k.h:22-27), c = w x p (math_k.h:21-26), flow =
+// (ax c.x + (cx - x) c.z, ay c.y + (cy - y) c.z). Contraction policy (the
+// GPU kernel's too): in a*b - c*d and a*b + c*d the first product is fused.
+inline void rotational_flow(const Camera& cam, float wx, float wy, float wz, int h, int w, Field<float>& flow) {
+    flow = Field<float>(h, w, 2);
+    for (int y = 0; y < h; ++y)
+        for (int x = 0; x < w; ++x) {
+            const float fx = (float)x, fy = (float)y;
+            const float px = (fx - cam.centerX) / cam.alphaX, py = (fy - cam.centerY) / cam.alphaY, pz = 1.0f;
+            const float c0 = std::fmaf(wy, pz, -(wz * py));
+            const float c1 = std::fmaf(wz, px, -(wx * pz));
+            const float c2 = std::fmaf(wx, py, -(wy * px));
+            flow.at(y, x, 0) = std::fmaf(cam.alphaX, c0, (cam.centerX - fx) * c2);
+            flow.at(y, x, 1) = std::fmaf(cam.alphaY, c1, (cam.centerY - fy) * c2);
+        }
+}
+
+// One Lax-Wendroff pass (LaxWendroffX_k axis 0 / LaxWendroffY_k axis 1,
+// propagation_k.h:49-132) of an image of c float channels (1 or 4):
+// out = z - R/2 (p - m) + R^2/2 (m - 2z + p), R = dt * flow[axis], neighbours
+// clamped. One channel: the update in double (the reference's 0.5 is a
+// double literal), the second difference in float along x (2 * z) and in
+// double along y (2.0 * z); four channels: everything in float (the
+// reference's float4 operators), with R/2 and R^2/2 rounded to float. The
+// products that feed the two updates are fused.
+inline void lax_wendroff_pass(int axis, const Field<float>& flow, const Field<float>& in, Field<float>& out, float dt) {
+    const int h = in.h, w = in.w, c = in.c;
+    out = Field<float>(h, w, c);
+    const int dx = axis == 0 ? 1 : 0, dy = axis == 1 ? 1 : 0;
+    for (int y = 0; y < h; ++y)
+        for (int x = 0; x < w; ++x) {
+            const float R = dt * flow.at(y, x, axis);
+            for (int k = 0; k < c; ++k) {
+                const float m = in.tex(y - dy, x - dx, k), z = in.at(y, x, k), p = in.tex(y + dy, x + dx, k);
+                const float d0 = p - m;
+                if (c == 1) {
+                    const double d2 = axis == 0 ? (double)((m - 2.0f * z) + p)
+                                                : ((double)m - 2.0 * (double)z) + (double)p;
+                    const double hR = 0.5 * (double)R;
+                    const double t = std::fma(-hR, (double)d0, (double)z);
+                    out.at(y, x, k) = (float)std::fma(hR * (double)R, d2, t);
+                } else {
+                    const float d2 = (m - 2.0f * z) + p;
+                    const float hR = (float)(0.5 * (double)R), kR = (float)(0.5 * (double)R * (double)R);
+                    out.at(y, x, k) = std::fmaf(kR, d2, std::fmaf(-hR, d0, z));
+                }
+            }
+        }
+}
+
+// LaxWendroffPropagator::compute (propagation.cu:572-644): a Y pass then an X
+// pass, max(iterations, 1) times; dt = 1/iterations, 0 when iterations is 0
+// (never set).
+inline Field<float> lax_wendroff(const Field<float>& flow, const Field<float>& img, int iterations) {
+    const float dt = iterations > 0 ? 1.0f / float(iterations) : 0.0f;
+    Field<float> by, bx;
+    lax_wendroff_pass(1, flow, img, by, dt);
+    lax_wendroff_pass(0, flow, by, bx, dt);
+    for (int n = 0; n < iterations - 1; ++n) {
+        lax_wendroff_pass(1, flow, bx, by, dt);
+        lax_wendroff_pass(0, flow, by, bx, dt);
+    }
+    return bx;
+}
+
 // --------------------------------------------------------------- filters
 
 // Single-level filter state and recurrence, FlowFilter (flowfilter.cu:23-330).

@@ -146,6 +146,38 @@ SMOOTH(f64, double)
 PYRAMID(f32, float)
 PYRAMID(f64, double)
 
+// ---- display (FlowToColor) ---------------------------------------------------
+void ffo_flow_to_color(int h, int w, const float* flow, float maxflow, int wh, int ww, const uint8_t* wheel,
+                       uint8_t* out) {
+    Field<uint8_t> wh4(wh, ww, 4), o;
+    std::memcpy(wh4.v.data(), wheel, wh4.v.size());
+    flow_to_color(load<float>(flow, h, w, 2), maxflow, wh4, o);
+    std::memcpy(out, o.v.data(), o.v.size());
+}
+int ffo_wheel_texel_index(float c, int n) { return wheel_texel_index(c, n); }
+
+// ---- camera, rotational flow, Lax-Wendroff ------------------------------------
+void ffo_create_perspective_camera(float focal, int h, int w, float sensor_h, float sensor_w, float* out4) {
+    const Camera c = create_perspective_camera(focal, h, w, sensor_h, sensor_w);
+    out4[0] = c.alphaX;
+    out4[1] = c.alphaY;
+    out4[2] = c.centerX;
+    out4[3] = c.centerY;
+}
+void ffo_rotational_flow(const float* cam4, float wx, float wy, float wz, int h, int w, float* out) {
+    Field<float> f;
+    rotational_flow(Camera{cam4[0], cam4[1], cam4[2], cam4[3]}, wx, wy, wz, h, w, f);
+    store(f, out);
+}
+void ffo_lax_wendroff(int h, int w, int c, const float* flow, const float* img, int iterations, float* out) {
+    store(lax_wendroff(load<float>(flow, h, w, 2), load<float>(img, h, w, c), iterations), out);
+}
+void ffo_lax_wendroff_pass(int axis, int h, int w, int c, const float* flow, const float* img, float dt, float* out) {
+    Field<float> o;
+    lax_wendroff_pass(axis, load<float>(flow, h, w, 2), load<float>(img, h, w, c), o, dt);
+    store(o, out);
+}
+
 // ---- single-level filter (FlowFilter) ------------------------------------
 #define FLOWFILTER(SUF, T)                                                                          \
     void* ffo_flowfilter_create_##SUF(int h, int w, int u8, int smooth_iters, float maxflow,       \

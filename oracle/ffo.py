@@ -188,6 +188,57 @@ def down_u8(km, k0, kp, prec="f32"):
     return _fn("ffo_down_u8", prec)(int(km), int(k0), int(kp))
 
 
+def flow_to_color(flow, maxflow, wheel):
+    """RGBA colours (h, w, 4) uint8 of a flow field (h, w, 2) read from the
+    colour wheel (wh, ww, 4) uint8 (flowToColor_k)."""
+    h, w, _ = flow.shape
+    wh, ww, _ = wheel.shape
+    out = np.empty((h, w, 4), np.uint8)
+    lib().ffo_flow_to_color(h, w, _p(_f32(flow)), ctypes.c_float(maxflow), wh, ww,
+                            _p(np.ascontiguousarray(wheel, np.uint8)), _p(out))
+    return out
+
+
+def wheel_texel_index(c, n):
+    f = lib().ffo_wheel_texel_index
+    f.argtypes = [ctypes.c_float, ctypes.c_int]
+    return f(float(c), int(n))
+
+
+def create_perspective_camera(focal, h, w, sensor_h, sensor_w):
+    """(alphaX, alphaY, centerX, centerY) as float32."""
+    out = np.empty(4, np.float32)
+    lib().ffo_create_perspective_camera(ctypes.c_float(focal), int(h), int(w), ctypes.c_float(sensor_h),
+                                        ctypes.c_float(sensor_w), _p(out))
+    return out
+
+
+def rotational_flow(cam, wx, wy, wz, h, w):
+    """Flow (h, w, 2) of a camera (alphaX, alphaY, centerX, centerY) rotating
+    with angular velocity (wx, wy, wz)."""
+    out = np.empty((h, w, 2), np.float32)
+    lib().ffo_rotational_flow(_p(_f32(np.asarray(cam))), ctypes.c_float(wx), ctypes.c_float(wy), ctypes.c_float(wz),
+                              int(h), int(w), _p(out))
+    return out
+
+
+def lax_wendroff(flow, img, iterations):
+    """LaxWendroffPropagator on a float image (h, w) or (h, w, 4)."""
+    h, w = img.shape[:2]
+    c = 1 if img.ndim == 2 else img.shape[2]
+    out = np.empty(img.shape, np.float32)
+    lib().ffo_lax_wendroff(h, w, c, _p(_f32(flow)), _p(_f32(img)), int(iterations), _p(out))
+    return out
+
+
+def lax_wendroff_pass(axis, flow, img, dt):
+    h, w = img.shape[:2]
+    c = 1 if img.ndim == 2 else img.shape[2]
+    out = np.empty(img.shape, np.float32)
+    lib().ffo_lax_wendroff_pass(int(axis), h, w, c, _p(_f32(flow)), _p(_f32(img)), ctypes.c_float(dt), _p(out))
+    return out
+
+
 class FlowFilter:
     """Single-level filter (FlowFilter) on the CPU.
 

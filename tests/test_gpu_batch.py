@@ -156,12 +156,52 @@ def test_batch_device_frames_match_host_loads(ffo):
         a.loadImagesDevice([dev[0].data], [dev[0].pitch])
 
 
-def test_batch_sixty_four_streams_hundred_twenty_steps():
+def test_batch_frames_in_place_match_copies(ffo):
+    """setImagesDevice (every stream's frame read where it lies by the
+    batched launch; the 4-level stream and the first frames copy instead)
+    gives the same results as loadImagesDevice (one copy launch)."""
+    import flowfilter.gpu.flowfilters as gf
+    from flowfilter.gpu.image import GPUImage
+    n = 5
+    specs = STREAMS
+    a = gf.BatchedFlowFilter([make(*sp) for sp in specs])
+    b = gf.BatchedFlowFilter([make(*sp) for sp in specs])
+    seqs = [frames_for(30 + i, sp[0], sp[1], n) for i, sp in enumerate(specs)]
+    dev = []
+    for i, sp in enumerate(specs):
+        g = GPUImage((n * sp[0], sp[1]), 1)
+        g.upload(np.concatenate(seqs[i], axis=0))
+        dev.append(g)
+    for t in range(n):
+        say("batch frames in place, step", t)
+        ptrs = [g.data + t * sp[0] * g.pitch for g, sp in zip(dev, specs)]
+        a.setImagesDevice(ptrs, [g.pitch for g in dev])
+        b.loadImagesDevice(ptrs, [g.pitch for g in dev])
+        a.compute()
+        b.compute()
+        for i in range(len(specs)):
+            assert np.array_equal(a[i].getFlow(), b[i].getFlow()), (i, t)
+            assert np.array_equal(a[i].getImage(), b[i].getImage()), (i, t)
+    h0, w0, l0, g0, m0, s0 = specs[2]
+    o = ffo.PyramidalFlowFilter(h0, w0, l0)
+    o.configure_like(g0, m0, s0)
+    for img in seqs[2]:
+        o.compute(img)
+    assert np.array_equal(a[2].getFlow(), o.downloadFlow())
+    assert a.waitTimeouts() == 0
+    from flowfilter import errors
+    with pytest.raises(errors.InvalidArgument):
+        a.setImagesDevice([dev[0].data], [dev[0].pitch])
+
+
+@pytest.mark.parametrize("in_place", [False, True])
+def test_batch_sixty_four_streams_hundred_twenty_steps(in_place):
     """The 64-stream workload (seed 3: 320x240 to 1920x1080, 1-3 levels,
     gamma U[5,50], smooth 1-2, affine motion), 120 steps over each stream's
-    frames (cycled), frames loaded from device memory in one launch: every
-    stream's flow and brightness bitwise equal to its filter computed alone
-    (loop of loadImageDevice + compute) at steps 1, 60 and 119."""
+    frames (cycled), frames loaded from device memory in one launch (or read
+    in place by the batched launch): every stream's flow and brightness
+    bitwise equal to its filter computed alone (loop of loadImageDevice +
+    compute, or setImageDevice + compute) at steps 1, 60 and 119."""
     import flowfilter.gpu.flowfilters as gf
     from flowfilter import synthetic
     from flowfilter.gpu.image import GPUImage
@@ -180,13 +220,19 @@ def test_batch_sixty_four_streams_hundred_twenty_steps():
     batch = gf.BatchedFlowFilter(batched)
     for t in range(120):
         ptrs = [g.data + (t % 3) * sp["h"] * g.pitch for sp, g in zip(streams, dev)]
-        batch.loadImagesDevice(ptrs, pitches)
+        if in_place:
+            batch.setImagesDevice(ptrs, pitches)
+        else:
+            batch.loadImagesDevice(ptrs, pitches)
         batch.compute()
         for f, p, q in zip(alone, ptrs, pitches):
-            f.loadImageDevice(p, q)
+            if in_place:
+                f.setImageDevice(p, q)
+            else:
+                f.loadImageDevice(p, q)
             f.compute()
         if t in (1, 60, 119):
-            say("64 streams, step", t)
+            say("64 streams, step", t, "in place" if in_place else "copied")
             for i in range(len(streams)):
                 a, b = batched[i].getFlow(), alone[i].getFlow()
                 assert np.array_equal(a, b), "stream %d step %d: max |diff| %g" % (i, t, np.abs(a - b).max())
