@@ -536,6 +536,22 @@ FF_API ff_status ff_ad_update_backward(const float* a0, const float* grad, const
                                        const float* g_flow_out, float* g_a0, float* g_grad, float* g_a0_old,
                                        float* g_flow, float* g_gamma_partial, int B, int H, int W, float gamma,
                                        float maxflow, int accumulate, void* stream);
+/* Allocation-free form of ff_ad_update_backward. It has no counterpart in
+ * the reference; it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). Same arguments and results, except that the gamma gradient's
+ * per-workgroup partial sums stay in a library-owned device array and are
+ * summed in the library, in double and in a fixed order, into *g_gamma (one
+ * double in device memory; bit 4 of accumulate adds to it instead of
+ * overwriting it). The array holds 1,048,576 partial sums (4 MiB on each GPU
+ * the library runs on, kept until the process ends); B *
+ * ff_ad_partial_count(H, W) above that is FF_ERR_INVALID_ARGUMENT. The call
+ * allocates nothing. Every call on one GPU uses that GPU's array: calls that
+ * overlap on one GPU (different streams, no ordering between them) race. */
+FF_API ff_status ff_ad_update_backward_noalloc(const float* a0, const float* grad, const float* a0_old,
+                                               const float* flow, const float* g_flow_out, float* g_a0, float* g_grad,
+                                               float* g_a0_old, float* g_flow, double* g_gamma, int B, int H, int W,
+                                               float gamma, float maxflow, int accumulate, void* stream);
 /* Delta-level update (update_k.cu:82-173); upper is the upper level's flow
  * (Hu x Wu), xmap/ymap the upper texel index of every column/row.
  * Backward leaves the upper-flow gradient per pixel of this level in

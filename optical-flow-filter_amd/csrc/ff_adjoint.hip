@@ -18,6 +18,10 @@
 // output-gradient input reads as zero.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+#include <string>
+#include <vector>
+
 #include "ff_device.hpp"
 #include "ff_kernels.hpp"
 #include "ff_runtime.hpp"
@@ -1014,7 +1018,51 @@ __global__ void __launch_bounds__(64 * kSmoothBwdRows)
     }
 }
 
+// Sum of n partials in double, in a fixed order (deterministic): thread t
+// adds partials t, t + 256, t + 512, ... in sequence, then the 256 sums are
+// combined pairwise (t += t + 128, then t += t + 64, ...). Thread 0 writes
+// the total, or adds it to *out when acc is set.
+constexpr int kSumThreads = 256;
+__global__ void __launch_bounds__(kSumThreads) k_ad_sum_partials(const float* p, int n, double* out, int acc) {
+    __shared__ double red[kSumThreads];
+    const int t = threadIdx.x;
+    double s = 0.0;
+    for (int i = t; i < n; i += kSumThreads) s += (double)p[i];
+    red[t] = s;
+    __syncthreads();
+    for (int k = kSumThreads / 2; k > 0; k >>= 1) {
+        if (t < k) red[t] += red[t + k];
+        __syncthreads();
+    }
+    if (t == 0) *out = acc ? *out + red[0] : red[0];
+}
+
 }  // namespace
+
+// Scratch of the allocation-free update adjoint (update_bwd_noalloc): the
+// gamma partial sums of one call, one value per workgroup, 4 MiB. Each GPU
+// holds its own copy of the array from the library's load until the
+// process ends. In host code `scratch` names a host-side handle, not device
+// memory: only the address scratch_ptr() looks up may be passed to kernels.
+static __device__ __align__(256) float scratch[ad::kScratchPartials];
+
+// Device address of `scratch` on the current device, looked up once per
+// device (hipGetSymbolAddress with that device current) and kept.
+static float* scratch_ptr() {
+    static std::mutex mu;
+    static std::vector<float*> ptrs;
+    int dev = 0;
+    FF_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(mu);
+    if ((int)ptrs.size() <= dev) ptrs.resize(dev + 1, nullptr);
+    if (!ptrs[dev]) {
+        void* p = nullptr;
+        FF_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) throw runtime_error("update adjoint: no device address for the scratch array");
+        ptrs[dev] = (float*)p;
+    }
+    return ptrs[dev];
+}
 
 // ------------------------------------------------------------- launchers
 
@@ -1149,6 +1197,21 @@ void update_bwd(const float* a0, const float* g, const float* aold, const float*
     hipLaunchKernelGGL(k_ad_update_bwd, grid3(B, H, W), dim3(BX, BY), 0, s, a0, (const float2*)g, aold,
                        (const float2*)f, (const float2*)gout, ga0, (float2*)gg, gaold, (float2*)gf, gamma_partial,
                        Dims{H, W}, gamma, maxflow, acc);
+}
+// The update adjoint with its gamma partial sums in `scratch` (allocates
+// nothing), reduced into *g_gamma. The partials are all rewritten by the
+// first launch, so nothing needs clearing between calls.
+void update_bwd_noalloc(const float* a0, const float* g, const float* aold, const float* f, const float* gout,
+                        float* ga0, float* gg, float* gaold, float* gf, double* g_gamma, int B, int H, int W,
+                        float gamma, float maxflow, int acc, hipStream_t s) {
+    const long long n = (long long)B * blocks(H, W);
+    if (n > kScratchPartials)
+        throw invalid_argument("ff_ad_update_backward_noalloc: B * ff_ad_partial_count(H, W) = " + std::to_string(n) +
+                               " exceeds the scratch array's " + std::to_string(kScratchPartials) + " partial sums");
+    float* part = scratch_ptr();
+    update_bwd(a0, g, aold, f, gout, ga0, gg, gaold, gf, part, B, H, W, gamma, maxflow, acc & 15, s);
+    hipLaunchKernelGGL(k_ad_sum_partials, dim3(1), dim3(kSumThreads), 0, s, (const float*)part, (int)n, g_gamma,
+                       (acc >> 4) & 1);
 }
 void delta_update(const float* a0, const float* g, const float* sprop, const float* dprop, const float* upper,
                   const int* xmap, const int* ymap, float* dout, float* fout, int B, int H, int W, int Hu, int Wu,

@@ -1005,6 +1005,21 @@ ff_status ff_ad_update_backward(const float* a0, const float* grad, const float*
         check_launch("ff_ad_update_backward");
     });
 }
+ff_status ff_ad_update_backward_noalloc(const float* a0, const float* grad, const float* a0_old, const float* flow,
+                                        const float* g_flow_out, float* g_a0, float* g_grad, float* g_a0_old,
+                                        float* g_flow, double* g_gamma, int B, int H, int W, float gamma,
+                                        float maxflow, int accumulate, void* stream) {
+    return guard([&] {
+        dims(B, H, W);
+        for (const void* p : {(const void*)a0, (const void*)grad, (const void*)a0_old, (const void*)flow,
+                              (const void*)g_flow_out, (const void*)g_a0, (const void*)g_grad, (const void*)g_a0_old,
+                              (const void*)g_flow, (const void*)g_gamma})
+            need(p, "argument");
+        ad::update_bwd_noalloc(a0, grad, a0_old, flow, g_flow_out, g_a0, g_grad, g_a0_old, g_flow, g_gamma, B, H, W,
+                               gamma, maxflow, accumulate, as_stream(stream));
+        check_launch("ff_ad_update_backward_noalloc");
+    });
+}
 ff_status ff_ad_delta_update(const float* a0, const float* grad, const float* a0_prop, const float* dflow_prop,
                              const float* upper, const int* xmap, const int* ymap, float* dflow_out, float* flow_out,
                              int B, int H, int W, int Hu, int Wu, float gamma, float maxflow, void* stream) {

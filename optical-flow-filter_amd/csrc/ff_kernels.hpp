@@ -270,6 +270,11 @@ void update(const float* a0, const float* g, const float* aold, const float* f, 
 void update_bwd(const float* a0, const float* g, const float* aold, const float* f, const float* gout, float* ga0,
                 float* gg, float* gaold, float* gf, float* gamma_partial, int B, int H, int W, float gamma,
                 float maxflow, int acc, hipStream_t s);
+// Largest B * blocks(H, W) update_bwd_noalloc accepts (its scratch array).
+constexpr int kScratchPartials = 1 << 20;
+void update_bwd_noalloc(const float* a0, const float* g, const float* aold, const float* f, const float* gout,
+                        float* ga0, float* gg, float* gaold, float* gf, double* g_gamma, int B, int H, int W,
+                        float gamma, float maxflow, int acc, hipStream_t s);
 void delta_update(const float* a0, const float* g, const float* sprop, const float* dprop, const float* upper,
                   const int* xmap, const int* ymap, float* dout, float* fout, int B, int H, int W, int Hu, int Wu,
                   float gamma, float maxflow, hipStream_t s);

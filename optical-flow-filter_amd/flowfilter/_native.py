@@ -250,6 +250,7 @@ SIGNATURES = {
     "ff_ad_propagate_pass_backward": [_i, _v, _v, _v, _v, _v, _v, _v, _v, _v, _i, _i, _i, _f, _i, _i, _v],
     "ff_ad_update": [_v, _v, _v, _v, _v, _i, _i, _i, _f, _f, _v],
     "ff_ad_update_backward": [_v, _v, _v, _v, _v, _v, _v, _v, _v, _v, _i, _i, _i, _f, _f, _i, _v],
+    "ff_ad_update_backward_noalloc": [_v, _v, _v, _v, _v, _v, _v, _v, _v, _v, _i, _i, _i, _f, _f, _i, _v],
     "ff_ad_delta_update": [_v, _v, _v, _v, _v, _v, _v, _v, _v, _i, _i, _i, _i, _i, _f, _f, _v],
     "ff_ad_delta_update_backward": [_v, _v, _v, _v, _v, _v, _v, _v, _v, _v, _v, _v, _v, _v, _v,
                                     _i, _i, _i, _i, _i, _f, _f, _i, _v],
