@@ -204,11 +204,13 @@ private:
 };
 
 // Host frames for a filter's own input image, double buffered: load()
-// copies a host frame into the spare buffer on a copy stream of its own,
-// after the last work that read that buffer, while the filter's stream may
-// still be reading the current one; then it exchanges the two buffers
-// behind the input image's handles. It returns once the host memory has
-// been read. readBy(s) records that the work enqueued so far on s reads the
+// copies a host frame into a pinned host buffer (on a few threads), then
+// enqueues its transfer into the spare device buffer on a copy stream of
+// its own, after the last work that read that buffer, and exchanges the two
+// buffers behind the input image's handles; the input's home stream waits
+// for the transfer. It returns once the host memory has been read, without
+// waiting for the device, so the transfer overlaps the previous frame's
+// compute(). readBy(s) records that the work enqueued so far on s reads the
 // current buffer (call it after each compute()).
 class InputFrames {
 public:

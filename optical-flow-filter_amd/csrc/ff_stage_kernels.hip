@@ -66,13 +66,22 @@ __global__ void __launch_bounds__(BX* BY) k_scale_f2(DevImg in, float s, DevImg 
 }
 
 __global__ void __launch_bounds__(256) k_copy2d(DevImg src, DevImg dst, size_t row_bytes) {
-    // 4-byte granules when every row start and the row length allow it,
-    // bytes otherwise (decided by the host, see launch_copy2d).
+    // 8- or 4-byte granules when every row start and the row length allow
+    // it (k_copy2d_u64, k_copy2d_u32), bytes otherwise (launch_copy2d).
     const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
     const int y = blockIdx.y;
     if (i >= row_bytes) return;
     const char* s = (const char*)src.data + (size_t)y * src.pitch;
     char* d = (char*)dst.data + (size_t)y * dst.pitch;
+    d[i] = s[i];
+}
+
+__global__ void __launch_bounds__(256) k_copy2d_u64(DevImg src, DevImg dst, size_t row_words) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    const int y = blockIdx.y;
+    if (i >= row_words) return;
+    const uint64_t* s = (const uint64_t*)((const char*)src.data + (size_t)y * src.pitch);
+    uint64_t* d = (uint64_t*)((char*)dst.data + (size_t)y * dst.pitch);
     d[i] = s[i];
 }
 
@@ -381,9 +390,14 @@ void launch_copy_regions(const CopyRegion* regions, int n, hipStream_t s) {
 void launch_copy2d(const DevImg& src, const DevImg& dst, size_t row_bytes, hipStream_t s) {
     ProfileScope ps("copy2d", s);
     if (src.h <= 0 || row_bytes == 0) return;
-    const bool words = (row_bytes % 4 == 0) && (src.pitch % 4 == 0) && (dst.pitch % 4 == 0) &&
-                       ((uintptr_t)src.data % 4 == 0) && ((uintptr_t)dst.data % 4 == 0);
-    if (words) {
+    auto aligned = [&](size_t a) {
+        return row_bytes % a == 0 && src.pitch % a == 0 && dst.pitch % a == 0 && (uintptr_t)src.data % a == 0 &&
+               (uintptr_t)dst.data % a == 0;
+    };
+    if (aligned(8)) {
+        const size_t n = row_bytes / 8;
+        hipLaunchKernelGGL(k_copy2d_u64, dim3((unsigned)((n + 255) / 256), src.h), dim3(256), 0, s, src, dst, n);
+    } else if (aligned(4)) {
         const size_t n = row_bytes / 4;
         hipLaunchKernelGGL(k_copy2d_u32, dim3((unsigned)((n + 255) / 256), src.h), dim3(256), 0, s, src, dst, n);
     } else {

@@ -248,9 +248,16 @@ def warp_previous(prev, flow):
 
 
 def flow_loss(frames, flows, truth, photometric_weight=0.1):
-    """The loss of flowfilter.autograd.flow_loss (truth [B,2]) with
-    warp_previous in place of grid_sample."""
+    """The loss of flowfilter.autograd.flow_loss (truth [B,2] per sequence,
+    or a field [B,H,W,2] / [B,T,H,W,2]) with warp_previous in place of
+    grid_sample."""
     T = frames.shape[1]
-    l2 = ((flows[:, 1:] - truth[:, None, None, None, :]) ** 2).mean()
+    if truth.dim() == 2:
+        truth = truth[:, None, None, None, :]
+    elif truth.dim() == 4:
+        truth = truth[:, None]
+    else:
+        truth = truth[:, 1:]
+    l2 = ((flows[:, 1:] - truth) ** 2).mean()
     ph = sum(((frames[:, t] - warp_previous(frames[:, t - 1], flows[:, t])) ** 2).mean() for t in range(1, T))
     return l2 + photometric_weight * ph / (T - 1)
