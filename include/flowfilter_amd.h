@@ -587,6 +587,23 @@ FF_API ff_status ff_ad_smooth_pass_backward(int axis, const float* g_flow_out, c
 FF_API ff_status ff_ad_smooth_pair(const float* flow, float* flow_out, int B, int H, int W, void* stream);
 FF_API ff_status ff_ad_smooth_pair_backward(const float* g_flow_out, const float* g_flow_out2, float* g_flow, int B,
                                             int H, int W, int accumulate, void* stream);
+/* Photometric term of the differentiable window's loss, fused (no
+ * reference counterpart; it replaces the grid_sample form of
+ * flowfilter.autograd.photometric_error). Frames [B][T][H][W] float, flows
+ * [B][T][H][W] float2, T >= 2: for every frame t >= 1, frame t-1 is sampled
+ * bilinearly at x - flow_t(x) with the coordinates clamped into the image,
+ * and partial (ff_ad_photometric_partial_count floats, one per workgroup)
+ * receives the sums of the squared differences to frame t. Backward: for the
+ * loss scale *scale (device float, d loss / d sum of squares) adds the
+ * gradients into g_frames and g_flows (flows of frame 0 untouched); the
+ * sampled pixels' gradients are atomic adds, so their summation order
+ * varies from run to run. */
+FF_API ff_status ff_ad_photometric_partial_count(int B, int T, int height, int width, int* out);
+FF_API ff_status ff_ad_photometric(const float* frames, const float* flows, float* partial, int B, int T, int H, int W,
+                                   void* stream);
+FF_API ff_status ff_ad_photometric_backward(const float* frames, const float* flows, const float* scale,
+                                            float* g_frames, float* g_flows, int B, int T, int H, int W,
+                                            void* stream);
 
 /* Kernel profiling (no reference counterpart; the reference times whole
  * stages only, src/gpu/pipeline.cu:63-89). Between begin and end, every

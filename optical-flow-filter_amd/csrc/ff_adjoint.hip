@@ -42,6 +42,12 @@ __device__ __forceinline__ size_t at(int b, int y, int x, const Dims& d) {
 
 inline dim3 grid3(int B, int H, int W) { return dim3((W + BX - 1) / BX, (H + BY - 1) / BY, B); }
 
+// threadIdx.y in a workgroup 64 threads wide (every kernel here): one value
+// per wave, so it is read into a scalar register, and the row indices,
+// clamps and row conditions derived from it stay on the scalar unit (the
+// compiler cannot tell that threadIdx.y is uniform across a wave).
+__device__ __forceinline__ int wave_y() { return __builtin_amdgcn_readfirstlane((int)threadIdx.y); }
+
 __device__ __forceinline__ void put(float* p, size_t i, float v, int acc) { p[i] = acc ? p[i] + v : v; }
 __device__ __forceinline__ void put2(float2* p, size_t i, float2 v, int acc) {
     if (acc) {
@@ -82,7 +88,7 @@ struct WB {
 
 // imagePrefilter_k + imageModel_k on a float image (imagemodel_k.cu:31-133).
 __global__ void __launch_bounds__(BX* BY) k_ad_image_model(const float* I, float* a0, float2* g, Dims d) {
-    const int x = blockIdx.x * BX + threadIdx.x, y = blockIdx.y * BY + threadIdx.y, b = blockIdx.z;
+    const int x = blockIdx.x * BX + threadIdx.x, y = blockIdx.y * BY + wave_y(), b = blockIdx.z;
     if (x >= d.W || y >= d.H) return;
     float dx = 0.0f, dy = 0.0f, sm = 0.0f;
 #pragma unroll
@@ -110,7 +116,7 @@ __global__ void __launch_bounds__(BX* BY) k_ad_image_model(const float* I, float
 // adjoint first forms g_sy = Sx^T g_a0 + Dx^T g_gx and g_sx = Dy^T g_gy.
 __global__ void __launch_bounds__(BX* BY)
     k_ad_image_model_bwd1(const float* ga0, const float2* gg, float* gsy, float* gsx, Dims d) {
-    const int x = blockIdx.x * BX + threadIdx.x, y = blockIdx.y * BY + threadIdx.y, b = blockIdx.z;
+    const int x = blockIdx.x * BX + threadIdx.x, y = blockIdx.y * BY + wave_y(), b = blockIdx.z;
     if (x >= d.W || y >= d.H) return;
     const float s1 = corr5_adjoint(x, d.W, WS{}, [&](int i) { return ga0[at(b, y, i, d)]; });
     const float s2 = corr5_adjoint(x, d.W, WD{}, [&](int i) { return gg[at(b, y, i, d)].x; });
@@ -121,7 +127,7 @@ __global__ void __launch_bounds__(BX* BY)
 // g_I = Sy^T g_sy + Sx^T g_sx.
 __global__ void __launch_bounds__(BX* BY)
     k_ad_image_model_bwd2(const float* gsy, const float* gsx, float* gI, Dims d, int acc) {
-    const int x = blockIdx.x * BX + threadIdx.x, y = blockIdx.y * BY + threadIdx.y, b = blockIdx.z;
+    const int x = blockIdx.x * BX + threadIdx.x, y = blockIdx.y * BY + wave_y(), b = blockIdx.z;
     if (x >= d.W || y >= d.H) return;
     const float a = corr5_adjoint(y, d.H, WS{}, [&](int i) { return gsy[at(b, i, x, d)]; });
     const float c = corr5_adjoint(x, d.W, WS{}, [&](int i) { return gsx[at(b, y, i, d)]; });
@@ -134,7 +140,7 @@ __global__ void __launch_bounds__(BX* BY)
 // (H, W/2) from (H, W) along x, or (H/2, W) from (H, W) along y.
 template <int AXIS>
 __global__ void __launch_bounds__(BX* BY) k_ad_down(const float* in, float* out, Dims di, Dims dout) {
-    const int x = blockIdx.x * BX + threadIdx.x, y = blockIdx.y * BY + threadIdx.y, b = blockIdx.z;
+    const int x = blockIdx.x * BX + threadIdx.x, y = blockIdx.y * BY + wave_y(), b = blockIdx.z;
     if (x >= dout.W || y >= dout.H) return;
     float tm, t0, tp;
     if (AXIS == 0) {
@@ -154,7 +160,7 @@ __global__ void __launch_bounds__(BX* BY) k_ad_down(const float* in, float* out,
 // tap it is and 0.25 from outputs whose side taps (clamped) read it.
 template <int AXIS>
 __global__ void __launch_bounds__(BX* BY) k_ad_down_bwd(const float* gout, float* gin, Dims di, Dims dout, int acc) {
-    const int x = blockIdx.x * BX + threadIdx.x, y = blockIdx.y * BY + threadIdx.y, b = blockIdx.z;
+    const int x = blockIdx.x * BX + threadIdx.x, y = blockIdx.y * BY + wave_y(), b = blockIdx.z;
     if (x >= di.W || y >= di.H) return;
     const int j = AXIS == 0 ? x : y, n = AXIS == 0 ? di.W : di.H, no = AXIS == 0 ? dout.W : dout.H;
     float s = 0.0f;
@@ -243,7 +249,7 @@ template <int AXIS, bool PAY>
 __global__ void __launch_bounds__(BX* BY) k_ad_prop(const float2* fin, const float* sin, const float2* vin,
                                                     float2* fout, float* sout, float2* vout, Dims d, float dt,
                                                     int border) {
-    const int x = blockIdx.x * BX + threadIdx.x, y = blockIdx.y * BY + threadIdx.y, b = blockIdx.z;
+    const int x = blockIdx.x * BX + threadIdx.x, y = blockIdx.y * BY + wave_y(), b = blockIdx.z;
     if (x >= d.W || y >= d.H) return;
     const int ym = AXIS == 1 ? clampi(y - 1, 0, d.H - 1) : y, yp = AXIS == 1 ? clampi(y + 1, 0, d.H - 1) : y;
     const int xm = AXIS == 0 ? clampi(x - 1, 0, d.W - 1) : x, xp = AXIS == 0 ? clampi(x + 1, 0, d.W - 1) : x;
@@ -268,7 +274,7 @@ __global__ void __launch_bounds__(64 * (kPairRows + 2))
     k_ad_prop_pair(const float2* fin, const float* sin, const float2* vin, float2* fmid, float* smid, float2* vmid,
                    float2* fout, float* sout, float2* vout, Dims d, float dt, int border) {
     __shared__ Px5 L[kPairRows + 2][64];
-    const int tx = threadIdx.x, ty = threadIdx.y, b = blockIdx.z;
+    const int tx = threadIdx.x, ty = wave_y(), b = blockIdx.z;
     const int x = blockIdx.x * 64 + tx;
     const int yr = (int)blockIdx.y * kPairRows - 1 + ty;  // row of this thread's X pass
     const int yc = clampi(yr, 0, d.H - 1);
@@ -312,7 +318,7 @@ __global__ void __launch_bounds__(64 * (kQuadRows + 4))
                    Dims d, float dt, int border) {
     __shared__ Px5 LA[kQuadRows + 4][64];  // X passes
     __shared__ Px5 LB[kQuadRows + 4][64];  // first Y pass
-    const int tx = threadIdx.x, ty = threadIdx.y, b = blockIdx.z;
+    const int tx = threadIdx.x, ty = wave_y(), b = blockIdx.z;
     const int x = (int)blockIdx.x * kQuadCols - 1 + tx;  // column of this lane
     const int y0 = (int)blockIdx.y * kQuadRows;
     const int yr = y0 - 2 + ty;  // staged row ty
@@ -424,6 +430,100 @@ __device__ __forceinline__ Px5 upwind_adjoint_px(const Px5 (&X)[5], const Px5 (&
     return r;
 }
 
+// upwind_adjoint_px in two halves, for kernels that hold every output of a
+// pass: out_terms computes, once per output o, what o contributes to its
+// three inputs (the coefficients of the inputs at o-1, o, o+1 and the
+// adjoint of its speed, routed to the neighbour the speed came from), and
+// gather_px sums the terms of outputs j-1, j, j+1 for input j. Both use
+// upwind_adjoint_px's expressions in its order, so together they give its
+// results bit for bit, while each output's terms are computed once instead
+// of once per input that reads them.
+struct OutTerms {
+    float cm, cz, cp;  // coefficients of the inputs at o-1, o, o+1
+    float gud;         // adjoint of the speed
+    unsigned flags;    // bit 0: o inside the border; bit 1: speed taken from o+1
+};
+
+template <int C, bool PAY>
+__device__ __forceinline__ OutTerms out_terms(const Px5& xm, const Px5& x0, const Px5& xp, const Px5& g, bool interior,
+                                              float dt) {
+    OutTerms t;
+    t.cm = t.cz = t.cp = t.gud = 0.f;
+    t.flags = 0u;
+    if (!interior) return t;
+    const float um = C == 0 ? xm.f.x : xm.f.y, up = C == 0 ? xp.f.x : xp.f.y;
+    const bool selp = (fabsf(up) - fabsf(um)) > 0.0f;
+    const float Ud = selp ? up : um;
+    const float nd = -(dt * Ud);
+    const bool back = Ud >= 0.0f;
+    t.cz = 1.0f + (back ? nd : -nd);
+    t.cm = back ? -nd : 0.f;
+    t.cp = back ? 0.f : nd;
+    auto D = [&](float m, float z, float p) { return back ? (z - m) : (p - z); };
+    float gUd = g.f.x * D(xm.f.x, x0.f.x, xp.f.x) + g.f.y * D(xm.f.y, x0.f.y, xp.f.y);
+    if constexpr (PAY)
+        gUd += g.s * D(xm.s, x0.s, xp.s) + g.v.x * D(xm.v.x, x0.v.x, xp.v.x) + g.v.y * D(xm.v.y, x0.v.y, xp.v.y);
+    gUd *= -dt;
+    t.gud = gUd;
+    t.flags = 1u | (selp ? 2u : 0u);
+    return t;
+}
+
+// Input j from outputs j-1, j, j+1 (T, G; valid: the output is inside the
+// line).
+template <int C, bool PAY>
+__device__ __forceinline__ Px5 gather_px(const OutTerms (&T)[3], const Px5 (&G)[3], const bool (&valid)[3]) {
+    float2 gf = make_float2(0.f, 0.f), gv = make_float2(0.f, 0.f);
+    float gs = 0.f;
+#pragma unroll
+    for (int oi = 0; oi < 3; ++oi) {
+        if (!valid[oi]) continue;
+        const Px5& g = G[oi];
+        if (!(T[oi].flags & 1u)) {
+            if (oi == 1) {  // out = z
+                gf.x += g.f.x;
+                gf.y += g.f.y;
+                if (PAY) {
+                    gs += g.s;
+                    gv.x += g.v.x;
+                    gv.y += g.v.y;
+                }
+            }
+            continue;
+        }
+        const float c = (oi == 1) ? T[oi].cz : ((oi == 2) ? T[oi].cm : T[oi].cp);
+        gf.x = fmaf(c, g.f.x, gf.x);
+        gf.y = fmaf(c, g.f.y, gf.y);
+        if constexpr (PAY) {
+            gs = fmaf(c, g.s, gs);
+            gv.x = fmaf(c, g.v.x, gv.x);
+            gv.y = fmaf(c, g.v.y, gv.y);
+        }
+        const bool selp = (T[oi].flags & 2u) != 0u;
+        const bool from = selp ? (oi == 0) : (oi == 2);
+        if (from) {
+            if (C == 0) gf.x += T[oi].gud;
+            else gf.y += T[oi].gud;
+        }
+    }
+    Px5 r;
+    r.f = gf;
+    r.s = gs;
+    r.v = gv;
+    return r;
+}
+
+template <bool PREV>
+__device__ __forceinline__ OutTerms lane_shift_terms(const OutTerms& t) {
+    OutTerms r;
+    r.cm = lane_shift<PREV>(t.cm);
+    r.cz = lane_shift<PREV>(t.cz);
+    r.cp = lane_shift<PREV>(t.cp);
+    r.gud = lane_shift<PREV>(t.gud);
+    r.flags = (unsigned)__builtin_amdgcn_update_dpp(0, (int)t.flags, PREV ? 0x138 : 0x130, 0xf, 0xf, true);
+    return r;
+}
+
 template <bool PAY>
 __device__ __forceinline__ Px5 load_grad_px(const float2* f, const float* s, const float2* v, size_t i) {
     Px5 r;
@@ -446,7 +546,7 @@ template <int AXIS, bool PAY>
 __global__ void __launch_bounds__(BX* BY)
     k_ad_prop_bwd(const float2* fin, const float* sin, const float2* vin, const float2* gfo, const float* gso,
                   const float2* gvo, float2* gfi, float* gsi, float2* gvi, Dims d, float dt, int border, int acc) {
-    const int x = blockIdx.x * BX + threadIdx.x, y = blockIdx.y * BY + threadIdx.y, b = blockIdx.z;
+    const int x = blockIdx.x * BX + threadIdx.x, y = blockIdx.y * BY + wave_y(), b = blockIdx.z;
     if (x >= d.W || y >= d.H) return;
     const int n = AXIS == 0 ? d.W : d.H, j = AXIS == 0 ? x : y;
     auto idx = [&](int k) { return AXIS == 0 ? at(b, y, k, d) : at(b, k, x, d); };
@@ -477,7 +577,7 @@ __global__ void __launch_bounds__(64 * kPairBwdRows)
                        const float2* vmid, const float2* gfo, const float* gso, const float2* gvo, float2* gfi,
                        float* gsi, float2* gvi, Dims d, float dt, int border, int acc) {
     __shared__ Px5 LG[kPairBwdRows][64];
-    const int tx = threadIdx.x, ty = threadIdx.y, b = blockIdx.z;
+    const int tx = threadIdx.x, ty = wave_y(), b = blockIdx.z;
     const int x = (int)blockIdx.x * kPairCols - 1 + tx;
     const int y = (int)blockIdx.y * kPairBwdRows + ty;
     const bool xin = x >= border && x < d.W - border;
@@ -528,7 +628,7 @@ __global__ void __launch_bounds__(64 * kPairBwdRows)
                           int border, int acc) {
     __shared__ Px5 LM[kPairRcRows][64];        // X pass of rows y0-2 .. y0+17
     __shared__ Px5 LG[kPairBwdRows + 2][64];   // output gradients of rows y0-1 .. y0+16
-    const int tx = threadIdx.x, ty = threadIdx.y, b = blockIdx.z;
+    const int tx = threadIdx.x, ty = wave_y(), b = blockIdx.z;
     const int x = (int)blockIdx.x * kPairRcCols - 2 + tx;  // column of this lane
     const int y0 = (int)blockIdx.y * kPairBwdRows, y = y0 + ty;
     const int xc = clampi(x, 0, d.W - 1);
@@ -576,40 +676,88 @@ __global__ void __launch_bounds__(64 * kPairBwdRows)
     }
 }
 
+// Rows of pixels (Px5) in LDS as planes (flow, scalar, vector), 20 bytes
+// per pixel with consecutive lanes at consecutive addresses (an array of
+// Px5 takes 24 bytes, its padded size, and strides the banks). A row can
+// hold a pixel or an output's OutTerms instead.
+template <int S>
+struct LdsRows {
+    float2 f[S][64];
+    float s[S][64];
+    float2 v[S][64];
+    template <bool PAY>
+    __device__ __forceinline__ void put(int i, int tx, const Px5& p) {
+        f[i][tx] = p.f;
+        if constexpr (PAY) {
+            s[i][tx] = p.s;
+            v[i][tx] = p.v;
+        }
+    }
+    template <bool PAY>
+    __device__ __forceinline__ Px5 get(int i, int tx) const {
+        Px5 r;
+        r.f = f[i][tx];
+        r.s = PAY ? s[i][tx] : 0.f;
+        r.v = PAY ? v[i][tx] : make_float2(0.f, 0.f);
+        return r;
+    }
+    __device__ __forceinline__ void put_terms(int i, int tx, const OutTerms& t) {
+        f[i][tx] = make_float2(t.cm, t.cz);
+        s[i][tx] = t.cp;
+        v[i][tx] = make_float2(t.gud, __uint_as_float(t.flags));
+    }
+    __device__ __forceinline__ OutTerms get_terms(int i, int tx) const {
+        const float2 a = f[i][tx], c = v[i][tx];
+        OutTerms t;
+        t.cm = a.x;
+        t.cz = a.y;
+        t.cp = s[i][tx];
+        t.gud = c.x;
+        t.flags = __float_as_uint(c.y);
+        return t;
+    }
+};
+
 // Adjoint of k_ad_prop_quad (two iterations), recomputing the forward
-// passes inside the launch. A workgroup of 64 x 16 threads covers 64
-// columns, lane tx being column x0-3+tx, and S = 16 x RPW staged rows, wave
-// w holding staged rows i = w + 16k, image rows y0-4+i (reads clamped into
+// passes inside the launch. A workgroup of 64 x NW threads covers 64
+// columns, lane tx being column x0-3+tx, and S = NW x RPW staged rows, wave
+// w holding staged rows i = w + NW k, image rows y0-4+i (reads clamped into
 // the image); R = S - 8 output rows. Forward: the first X pass on the S rows
 // (row neighbours from the adjacent lanes, the edge lanes load theirs) into
 // LDS, the first Y pass on rows 1..S-2, the second X pass on those rows
-// (adjacent lanes; valid on lanes 1..62) into LDS, beside the output
-// gradients of rows 2..S-3. Adjoints, each with the same per-pixel function
-// as the single passes: the second Y pass's at rows 3..S-4 (lanes 1..62),
-// the second X pass's at those rows (lanes 2..61; its inputs and the Y
-// adjoint of the neighbouring columns from the adjacent lanes), the first Y
-// pass's at rows 4..S-5, the first X pass's at those rows, written for
-// lanes 3..60 (columns x0 .. x0+57, rows y0 .. y0+R-1). Bitwise the
-// adjoints of two pass pairs.
-constexpr int kQuadBwdWaves = 16, kQuadBwdCols = 58;
-// Rows per wave of the launches: with 2 (32 staged rows for 24 output rows
-// instead of 16 for 8) the LDS grows to 144 KB, one workgroup fits per CU,
-// and the 4K window took 153 ms instead of 146.
-constexpr int kQuadBwdRowsPerWave = 1;
-template <int RPW>
+// (adjacent lanes; valid on lanes 1..62) into LDS. Adjoints, each as
+// per-output terms (out_terms) and a gather (gather_px), which give the
+// single passes' adjoints bit for bit:
+//   second Y pass: terms of rows 2..S-3 (lanes 1..62; the output gradients
+//     read from memory) into LDS in place of the second X pass, gathered at
+//     rows 3..S-4;
+//   second X pass: terms at every lane (its inputs and neighbours from the
+//     adjacent lanes), gathered from the adjacent lanes at lanes 2..61;
+//     the result, the first Y pass's output gradient, into LDS;
+//   first Y pass: terms of rows 3..S-4 into LDS in place of the first X
+//     pass, gathered at rows 4..S-5;
+//   first X pass: as the second, written for lanes 3..60 (columns
+//     x0 .. x0+57, rows y0 .. y0+R-1).
+constexpr int kQuadBwdCols = 58;
+// Workgroup shape of the launches: 8 waves of 4 rows, 32 staged rows for 24
+// output rows, two 40 KB sets of LDS planes (two workgroups per CU). With
+// 16 waves of one row (16 staged rows for 8 output rows, the gradients
+// staged in LDS too, each input recomputing the terms of the three outputs
+// that read it) the 3840x2160 launch took 249 us.
+constexpr int kQuadBwdWaves = 8, kQuadBwdRowsPerWave = 4;
+template <int NW, int RPW>
 struct QuadBwdShape {
-    static constexpr int stage = kQuadBwdWaves * RPW, rows = stage - 8;
+    static constexpr int stage = NW * RPW, rows = stage - 8;
 };
-template <bool PAY, int RPW>
-__global__ void __launch_bounds__(64 * kQuadBwdWaves)
+template <bool PAY, int NW, int RPW>
+__global__ void __launch_bounds__(64 * NW)
     k_ad_prop_quad_bwd(const float2* fin, const float* sin, const float2* vin, const float2* gfo, const float* gso,
                        const float2* gvo, float2* gfi, float* gsi, float2* gvi, Dims d, float dt, int border,
                        int acc) {
-    constexpr int S = QuadBwdShape<RPW>::stage, R = QuadBwdShape<RPW>::rows;
-    __shared__ Px5 LA[S][64];  // first X pass
-    __shared__ Px5 LB[S][64];  // second X pass; later the gradient of the first Y pass's output
-    __shared__ Px5 LG[S][64];  // output gradients
-    const int tx = threadIdx.x, w = threadIdx.y, b = blockIdx.z;
+    constexpr int S = QuadBwdShape<NW, RPW>::stage, R = QuadBwdShape<NW, RPW>::rows;
+    __shared__ LdsRows<S> LA;  // first X pass; later the first Y pass's terms
+    __shared__ LdsRows<S> LB;  // second X pass; later the second Y pass's terms, then the first Y pass's output gradient
+    const int tx = threadIdx.x, w = wave_y(), b = blockIdx.z;
     const int x = (int)blockIdx.x * kQuadBwdCols - 3 + tx;  // column of this lane
     const int y0 = (int)blockIdx.y * R;
     const int xc = clampi(x, 0, d.W - 1);
@@ -619,98 +767,136 @@ __global__ void __launch_bounds__(64 * kQuadBwdWaves)
     zero.f = make_float2(0.f, 0.f);
     zero.s = 0.f;
     zero.v = make_float2(0.f, 0.f);
-    // this wave's staged rows i = w + 16 k, image rows y0-4+i
+    const bool xvalid[3] = {x - 1 >= 0 && x - 1 < d.W, x >= 0 && x < d.W, x + 1 >= 0 && x + 1 < d.W};
+    // this wave's staged rows i = w + NW k, image rows y0-4+i
     Px5 F[RPW], Y1[RPW], gy1[RPW];
     int yr[RPW], yc[RPW];
     bool in[RPW], pix[RPW];
 #pragma unroll
     for (int k = 0; k < RPW; ++k) {
-        const int i = w + kQuadBwdWaves * k;
+        const int i = w + NW * k;
         yr[k] = y0 - 4 + i;
         yc[k] = clampi(yr[k], 0, d.H - 1);
         in[k] = xin && yc[k] >= border && yc[k] < d.H - border;
         pix[k] = x >= 0 && x < d.W && yr[k] >= 0 && yr[k] < d.H;
     }
+    auto yinterior = [&](int o) { return xin && o >= border && o < d.H - border; };
 
     // ---- forward
 #pragma unroll
     for (int k = 0; k < RPW; ++k) {
-        const int i = w + kQuadBwdWaves * k;
+        const int i = w + NW * k;
         F[k] = load_px<PAY>(fin, sin, vin, at(b, yc[k], xc, d));
         Px5 m = lane_shift_px<true, PAY>(F[k]), p = lane_shift_px<false, PAY>(F[k]);
         if (tx == 0) m = load_px<PAY>(fin, sin, vin, at(b, yc[k], clampi(x - 1, 0, d.W - 1), d));
         if (tx == 63) p = load_px<PAY>(fin, sin, vin, at(b, yc[k], clampi(x + 1, 0, d.W - 1), d));
-        LA[i][tx] = upwind_px<0, PAY>(m, F[k], p, dt, in[k]);
-        if (i >= 2 && i <= S - 3) LG[i][tx] = load_grad_px<PAY>(gfo, gso, gvo, at(b, yc[k], xc, d));
+        LA.template put<PAY>(i, tx, upwind_px<0, PAY>(m, F[k], p, dt, in[k]));
     }
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < RPW; ++k) {
-        const int i = w + kQuadBwdWaves * k;
+        const int i = w + NW * k;
         const bool mid = i >= 1 && i <= S - 2;  // rows of the first Y and second X pass
         Y1[k] = zero;
         if (mid)
-            Y1[k] = upwind_px<1, PAY>(LA[srow(yc[k] - 1)][tx], LA[srow(yc[k])][tx], LA[srow(yc[k] + 1)][tx], dt,
-                                      in[k]);
+            Y1[k] = upwind_px<1, PAY>(LA.template get<PAY>(srow(yc[k] - 1), tx), LA.template get<PAY>(srow(yc[k]), tx),
+                                      LA.template get<PAY>(srow(yc[k] + 1), tx), dt, in[k]);
         const Px5 m = lane_shift_px<true, PAY>(Y1[k]), p = lane_shift_px<false, PAY>(Y1[k]);
-        if (mid) LB[i][tx] = upwind_px<0, PAY>(m, Y1[k], p, dt, in[k]);
+        if (mid) LB.template put<PAY>(i, tx, upwind_px<0, PAY>(m, Y1[k], p, dt, in[k]));
     }
     __syncthreads();
 
-    // ---- adjoints of the second pair (rows 3 .. S-4)
+    // ---- second Y pass: terms of output rows 2 .. S-3 (inside the image)
+    OutTerms t[RPW];
 #pragma unroll
     for (int k = 0; k < RPW; ++k) {
-        const int i = w + kQuadBwdWaves * k;
-        const bool rows2 = i >= 3 && i <= S - 4;
-        Px5 g_x2 = zero;
-        if (rows2 && pix[k] && tx >= 1 && tx <= 62) {
-            Px5 X[5], G[3];
-#pragma unroll
-            for (int q = 0; q < 5; ++q) X[q] = LB[srow(yr[k] - 2 + q)][tx];
-#pragma unroll
-            for (int q = 0; q < 3; ++q) G[q] = LG[i - 1 + q][tx];
-            auto interior = [&](int o) { return xin && o >= border && o < d.H - border; };
-            g_x2 = upwind_adjoint_px<1, PAY>(X, G, yr[k], d.H, interior, dt);
-        }
-        const Px5 G3[3] = {lane_shift_px<true, PAY>(g_x2), g_x2, lane_shift_px<false, PAY>(g_x2)};
-        const Px5 m1 = lane_shift_px<true, PAY>(Y1[k]), p1 = lane_shift_px<false, PAY>(Y1[k]);
-        const Px5 X5[5] = {lane_shift_px<true, PAY>(m1), m1, Y1[k], p1, lane_shift_px<false, PAY>(p1)};
-        gy1[k] = zero;
-        if (rows2 && pix[k] && tx >= 2 && tx <= 61) {
-            const bool yin = yr[k] >= border && yr[k] < d.H - border;
-            auto interior = [&](int o) { return yin && o >= border && o < d.W - border; };
-            gy1[k] = upwind_adjoint_px<0, PAY>(X5, G3, x, d.W, interior, dt);
+        const int i = w + NW * k;
+        t[k] = out_terms<1, PAY>(zero, zero, zero, zero, false, dt);
+        if (i >= 2 && i <= S - 3 && pix[k] && tx >= 1 && tx <= 62) {
+            const Px5 g = load_grad_px<PAY>(gfo, gso, gvo, at(b, yr[k], xc, d));
+            t[k] = out_terms<1, PAY>(LB.template get<PAY>(srow(yr[k] - 1), tx), LB.template get<PAY>(srow(yr[k]), tx),
+                                     LB.template get<PAY>(srow(yr[k] + 1), tx), g, yinterior(yr[k]), dt);
         }
     }
     __syncthreads();  // every read of LB (second X pass) is done
 #pragma unroll
-    for (int k = 0; k < RPW; ++k) LB[w + kQuadBwdWaves * k][tx] = gy1[k];
+    for (int k = 0; k < RPW; ++k) LB.put_terms(w + NW * k, tx, t[k]);
     __syncthreads();
 
-    // ---- adjoints of the first pair (rows 4 .. S-5), written for lanes 3..60
+    // ---- gathers of the second pair (rows 3 .. S-4)
 #pragma unroll
     for (int k = 0; k < RPW; ++k) {
-        const int i = w + kQuadBwdWaves * k;
+        const int i = w + NW * k;
+        const bool rows2 = i >= 3 && i <= S - 4;
+        Px5 g_x2 = zero;
+        if (rows2 && pix[k] && tx >= 1 && tx <= 62) {
+            OutTerms T[3];
+            Px5 G[3];
+            bool valid[3];
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                const int o = yr[k] - 1 + q;  // output row, staged row i-1+q
+                valid[q] = o >= 0 && o < d.H;
+                T[q] = LB.get_terms(i - 1 + q, tx);
+                G[q] = load_grad_px<PAY>(gfo, gso, gvo, at(b, clampi(o, 0, d.H - 1), xc, d));
+            }
+            g_x2 = gather_px<1, PAY>(T, G, valid);
+        }
+        // second X pass: terms at this lane's column, gathered from the
+        // adjacent lanes
+        const bool yin = yr[k] >= border && yr[k] < d.H - border;
+        const Px5 m1 = lane_shift_px<true, PAY>(Y1[k]), p1 = lane_shift_px<false, PAY>(Y1[k]);
+        const OutTerms to = out_terms<0, PAY>(m1, Y1[k], p1, g_x2, yin && x >= border && x < d.W - border, dt);
+        const OutTerms T3[3] = {lane_shift_terms<true>(to), to, lane_shift_terms<false>(to)};
+        const Px5 G3[3] = {lane_shift_px<true, PAY>(g_x2), g_x2, lane_shift_px<false, PAY>(g_x2)};
+        gy1[k] = zero;
+        if (rows2 && pix[k] && tx >= 2 && tx <= 61) gy1[k] = gather_px<0, PAY>(T3, G3, xvalid);
+    }
+    __syncthreads();  // every read of LB (the second Y pass's terms) is done
+#pragma unroll
+    for (int k = 0; k < RPW; ++k) LB.template put<PAY>(w + NW * k, tx, gy1[k]);
+    __syncthreads();
+
+    // ---- first Y pass: terms of output rows 3 .. S-4
+#pragma unroll
+    for (int k = 0; k < RPW; ++k) {
+        const int i = w + NW * k;
+        t[k] = out_terms<1, PAY>(zero, zero, zero, zero, false, dt);
+        if (i >= 3 && i <= S - 4 && pix[k] && tx >= 2 && tx <= 61)
+            t[k] = out_terms<1, PAY>(LA.template get<PAY>(srow(yr[k] - 1), tx), LA.template get<PAY>(srow(yr[k]), tx),
+                                     LA.template get<PAY>(srow(yr[k] + 1), tx), LB.template get<PAY>(i, tx), yinterior(yr[k]), dt);
+    }
+    __syncthreads();  // every read of LA (first X pass) is done
+#pragma unroll
+    for (int k = 0; k < RPW; ++k) LA.put_terms(w + NW * k, tx, t[k]);
+    __syncthreads();
+
+    // ---- gathers of the first pair (rows 4 .. S-5), written for lanes 3..60
+#pragma unroll
+    for (int k = 0; k < RPW; ++k) {
+        const int i = w + NW * k;
         const bool rows1 = i >= 4 && i <= S - 5;
         Px5 g_x1 = zero;
         if (rows1 && pix[k] && tx >= 2 && tx <= 61) {
-            Px5 X[5], G[3];
+            OutTerms T[3];
+            Px5 G[3];
+            bool valid[3];
 #pragma unroll
-            for (int q = 0; q < 5; ++q) X[q] = LA[srow(yr[k] - 2 + q)][tx];
-#pragma unroll
-            for (int q = 0; q < 3; ++q) G[q] = LB[i - 1 + q][tx];
-            auto interior = [&](int o) { return xin && o >= border && o < d.H - border; };
-            g_x1 = upwind_adjoint_px<1, PAY>(X, G, yr[k], d.H, interior, dt);
+            for (int q = 0; q < 3; ++q) {
+                const int o = yr[k] - 1 + q;
+                valid[q] = o >= 0 && o < d.H;
+                T[q] = LA.get_terms(i - 1 + q, tx);
+                G[q] = LB.template get<PAY>(i - 1 + q, tx);
+            }
+            g_x1 = gather_px<1, PAY>(T, G, valid);
         }
-        const Px5 G3[3] = {lane_shift_px<true, PAY>(g_x1), g_x1, lane_shift_px<false, PAY>(g_x1)};
+        const bool yin = yr[k] >= border && yr[k] < d.H - border;
         const Px5 m1 = lane_shift_px<true, PAY>(F[k]), p1 = lane_shift_px<false, PAY>(F[k]);
-        const Px5 X5[5] = {lane_shift_px<true, PAY>(m1), m1, F[k], p1, lane_shift_px<false, PAY>(p1)};
-        if (rows1 && pix[k] && tx >= 3 && tx < kQuadBwdCols + 3) {
-            const bool yin = yr[k] >= border && yr[k] < d.H - border;
-            auto interior = [&](int o) { return yin && o >= border && o < d.W - border; };
-            put_px<PAY>(gfi, gsi, gvi, at(b, yr[k], x, d), upwind_adjoint_px<0, PAY>(X5, G3, x, d.W, interior, dt),
-                        acc);
-        }
+        const OutTerms to = out_terms<0, PAY>(m1, F[k], p1, g_x1, yin && x >= border && x < d.W - border, dt);
+        const OutTerms T3[3] = {lane_shift_terms<true>(to), to, lane_shift_terms<false>(to)};
+        const Px5 G3[3] = {lane_shift_px<true, PAY>(g_x1), g_x1, lane_shift_px<false, PAY>(g_x1)};
+        if (rows1 && pix[k] && tx >= 3 && tx < kQuadBwdCols + 3)
+            put_px<PAY>(gfi, gsi, gvi, at(b, yr[k], x, d), gather_px<0, PAY>(T3, G3, xvalid), acc);
     }
 }
 
@@ -750,7 +936,7 @@ __device__ __forceinline__ bool inside(float v, float lim) { return v > -lim && 
 __global__ void __launch_bounds__(BX* BY) k_ad_update(const float* a0, const float2* g, const float* aold,
                                                       const float2* f, float2* fout, Dims d, float gamma,
                                                       float maxflow) {
-    const int x = blockIdx.x * BX + threadIdx.x, y = blockIdx.y * BY + threadIdx.y, b = blockIdx.z;
+    const int x = blockIdx.x * BX + threadIdx.x, y = blockIdx.y * BY + wave_y(), b = blockIdx.z;
     if (x >= d.W || y >= d.H) return;
     const size_t i = at(b, y, x, d);
     const float2 gg = g[i], ff = f[i];
@@ -780,7 +966,7 @@ __global__ void __launch_bounds__(BX* BY)
     k_ad_update_bwd(const float* a0, const float2* g, const float* aold, const float2* f, const float2* gout,
                     float* ga0, float2* gg, float* gaold, float2* gf, float* gamma_partial, Dims d, float gamma,
                     float maxflow, int acc) {
-    const int x = blockIdx.x * BX + threadIdx.x, y = blockIdx.y * BY + threadIdx.y, b = blockIdx.z;
+    const int x = blockIdx.x * BX + threadIdx.x, y = blockIdx.y * BY + wave_y(), b = blockIdx.z;
     float ggam = 0.0f;
     if (x < d.W && y < d.H) {
         const size_t i = at(b, y, x, d);
@@ -809,7 +995,7 @@ __global__ void __launch_bounds__(BX* BY)
     k_ad_delta_update(const float* a0, const float2* g, const float* sprop, const float2* dprop, const float2* upper,
                       const int* xmap, const int* ymap, float2* dout, float2* fout, Dims d, Dims du, float gamma,
                       float maxflow) {
-    const int x = blockIdx.x * BX + threadIdx.x, y = blockIdx.y * BY + threadIdx.y, b = blockIdx.z;
+    const int x = blockIdx.x * BX + threadIdx.x, y = blockIdx.y * BY + wave_y(), b = blockIdx.z;
     if (x >= d.W || y >= d.H) return;
     const size_t i = at(b, y, x, d);
     const float2 gg = g[i], dd = dprop[i];
@@ -837,7 +1023,7 @@ __global__ void __launch_bounds__(BX* BY)
                           const float2* upper, const int* xmap, const int* ymap, const float2* gdout,
                           const float2* gfout, float* ga0, float2* gg, float* gsprop, float2* gdprop, float2* gup,
                           float* gamma_partial, Dims d, Dims du, float gamma, float maxflow, int acc) {
-    const int x = blockIdx.x * BX + threadIdx.x, y = blockIdx.y * BY + threadIdx.y, b = blockIdx.z;
+    const int x = blockIdx.x * BX + threadIdx.x, y = blockIdx.y * BY + wave_y(), b = blockIdx.z;
     float ggam = 0.0f;
     if (x < d.W && y < d.H) {
         const size_t i = at(b, y, x, d);
@@ -876,7 +1062,7 @@ __global__ void __launch_bounds__(BX* BY)
 __global__ void __launch_bounds__(BX* BY)
     k_ad_upper_gather(const float2* gup, const int* xlo, const int* xhi, const int* ylo, const int* yhi, float2* gupper,
                       Dims d, Dims du, int acc) {
-    const int X = blockIdx.x * BX + threadIdx.x, Y = blockIdx.y * BY + threadIdx.y, b = blockIdx.z;
+    const int X = blockIdx.x * BX + threadIdx.x, Y = blockIdx.y * BY + wave_y(), b = blockIdx.z;
     if (X >= du.W || Y >= du.H) return;
     float2 s = make_float2(0.f, 0.f);
     for (int y = ylo[Y]; y < yhi[Y]; ++y)
@@ -893,7 +1079,7 @@ __global__ void __launch_bounds__(BX* BY)
 // flowSmoothX_k / flowSmoothY_k (flowsmoothing_k.cu:18-87) on a batch.
 template <int AXIS>
 __global__ void __launch_bounds__(BX* BY) k_ad_smooth(const float2* in, float2* out, Dims d) {
-    const int x = blockIdx.x * BX + threadIdx.x, y = blockIdx.y * BY + threadIdx.y, b = blockIdx.z;
+    const int x = blockIdx.x * BX + threadIdx.x, y = blockIdx.y * BY + wave_y(), b = blockIdx.z;
     if (x >= d.W || y >= d.H) return;
     float sx = 0.0f, sy = 0.0f;
 #pragma unroll
@@ -909,7 +1095,7 @@ __global__ void __launch_bounds__(BX* BY) k_ad_smooth(const float2* in, float2* 
 template <int AXIS>
 __global__ void __launch_bounds__(BX* BY)
     k_ad_smooth_bwd(const float2* gout, const float2* gout2, float2* gin, Dims d, int acc) {
-    const int x = blockIdx.x * BX + threadIdx.x, y = blockIdx.y * BY + threadIdx.y, b = blockIdx.z;
+    const int x = blockIdx.x * BX + threadIdx.x, y = blockIdx.y * BY + wave_y(), b = blockIdx.z;
     if (x >= d.W || y >= d.H) return;
     auto g = [&](size_t i) {
         float2 v = gout[i];
@@ -941,7 +1127,7 @@ __global__ void __launch_bounds__(BX* BY)
 constexpr int kSmoothRows = 12;
 __global__ void __launch_bounds__(64 * (kSmoothRows + 4)) k_ad_smooth_pair(const float2* in, float2* out, Dims d) {
     __shared__ float2 L[kSmoothRows + 4][64];
-    const int tx = threadIdx.x, ty = threadIdx.y, b = blockIdx.z;
+    const int tx = threadIdx.x, ty = wave_y(), b = blockIdx.z;
     const int x = (int)blockIdx.x * 64 + tx;
     const int yr = (int)blockIdx.y * kSmoothRows - 2 + ty;  // row of this thread's X pass
     const int yc = clampi(yr, 0, d.H - 1);
@@ -981,7 +1167,7 @@ __global__ void __launch_bounds__(64 * kSmoothBwdRows)
     k_ad_smooth_pair_bwd(const float2* gout, const float2* gout2, float2* gin, Dims d, int acc) {
     __shared__ float2 LGo[kSmoothBwdRows + 4][64];  // gout + gout2 of rows y0-2 .. y0+17
     __shared__ float2 L[kSmoothBwdRows][64];
-    const int tx = threadIdx.x, ty = threadIdx.y, b = blockIdx.z;
+    const int tx = threadIdx.x, ty = wave_y(), b = blockIdx.z;
     const int x = (int)blockIdx.x * kSmoothBwdCols - 2 + tx;
     const int y0 = (int)blockIdx.y * kSmoothBwdRows, y = y0 + ty;
     const int xc = clampi(x, 0, d.W - 1);
@@ -1016,6 +1202,96 @@ __global__ void __launch_bounds__(64 * kSmoothBwdRows)
         r.y = corr5_adjoint(x, d.W, WB{}, [&](int i) { return L[ty][tx + (i - x)].y; });
         put2(gin, at(b, y, x, d), r, acc & 1);
     }
+}
+
+// ------------------------------------------------------------ photometric
+
+// The photometric term of the differentiable window's loss: frame t-1
+// sampled bilinearly at x - flow_t(x), the coordinates clamped into the image
+// (grid_sample with align_corners and border padding), against frame t.
+// Frames [B][T][H][W] float, flows [B][T][H][W] float2; z = b (T-1) + t - 1
+// indexes the frame pairs.
+struct Warp {
+    int x0, x1, y0, y1;  // sampled columns and rows (x1 = min(x0 + 1, W - 1))
+    float wx, wy;        // weights of x1, y1
+    bool fx, fy;         // the coordinate is not clamped (its derivative passes)
+};
+__device__ __forceinline__ Warp warp_at(int x, int y, float2 f, const Dims& d) {
+    Warp w;
+    const float ux = (float)x - f.x, uy = (float)y - f.y;
+    const float hx = (float)(d.W - 1), hy = (float)(d.H - 1);
+    w.fx = !(ux < 0.f) && !(ux > hx);
+    w.fy = !(uy < 0.f) && !(uy > hy);
+    const float sx = ux > hx ? hx : (ux < 0.f ? 0.f : ux);
+    const float sy = uy > hy ? hy : (uy < 0.f ? 0.f : uy);
+    const float x0 = floorf(sx), y0 = floorf(sy);
+    w.wx = sx - x0;
+    w.wy = sy - y0;
+    // (a NaN coordinate converts to some index: kept inside the image)
+    w.x0 = clampi((int)x0, 0, d.W - 1);
+    w.y0 = clampi((int)y0, 0, d.H - 1);
+    w.x1 = min(w.x0 + 1, d.W - 1);
+    w.y1 = min(w.y0 + 1, d.H - 1);
+    return w;
+}
+struct Corners {
+    float p00, p01, p10, p11, top, bot, v;
+};
+__device__ __forceinline__ Corners sample(const float* prev, const Warp& w, int W) {
+    Corners c;
+    c.p00 = prev[(size_t)w.y0 * W + w.x0];
+    c.p01 = prev[(size_t)w.y0 * W + w.x1];
+    c.p10 = prev[(size_t)w.y1 * W + w.x0];
+    c.p11 = prev[(size_t)w.y1 * W + w.x1];
+    c.top = (1.0f - w.wx) * c.p00 + w.wx * c.p01;
+    c.bot = (1.0f - w.wx) * c.p10 + w.wx * c.p11;
+    c.v = (1.0f - w.wy) * c.top + w.wy * c.bot;
+    return c;
+}
+
+// Sum of the squared differences, one partial sum per workgroup.
+__global__ void __launch_bounds__(BX* BY)
+    k_ad_photometric(const float* frames, const float2* flows, float* partial, int T, Dims d) {
+    const int x = blockIdx.x * BX + threadIdx.x, y = blockIdx.y * BY + wave_y(), z = blockIdx.z;
+    const int b = z / (T - 1), t = z - b * (T - 1) + 1;
+    float e = 0.0f;
+    if (x < d.W && y < d.H) {
+        const size_t cur = (size_t)(b * T + t) * d.H * d.W, i = (size_t)y * d.W + x;
+        const Warp w = warp_at(x, y, flows[cur + i], d);
+        const Corners c = sample(frames + cur - (size_t)d.H * d.W, w, d.W);
+        const float diff = frames[cur + i] - c.v;
+        e = diff * diff;
+    }
+    block_sum(e, partial);
+}
+
+// Its adjoint for the loss scale *scale (d loss / d sum of squares):
+// frame t gets s, the four sampled pixels of frame t-1 get -s times their
+// weights (atomic adds: several pixels sample the same ones), and the flow
+// gets s times the derivative of the sample along each unclamped
+// coordinate, with s = 2 *scale * difference. Adds into gframes and gflows.
+__global__ void __launch_bounds__(BX* BY)
+    k_ad_photometric_bwd(const float* frames, const float2* flows, const float* scale, float* gframes,
+                         float2* gflows, int T, Dims d) {
+    const int x = blockIdx.x * BX + threadIdx.x, y = blockIdx.y * BY + wave_y(), z = blockIdx.z;
+    if (x >= d.W || y >= d.H) return;
+    const int b = z / (T - 1), t = z - b * (T - 1) + 1;
+    const size_t cur = (size_t)(b * T + t) * d.H * d.W, prev = cur - (size_t)d.H * d.W, i = (size_t)y * d.W + x;
+    const Warp w = warp_at(x, y, flows[cur + i], d);
+    const Corners c = sample(frames + prev, w, d.W);
+    const float sd = 2.0f * *scale * (frames[cur + i] - c.v);
+    unsafeAtomicAdd(gframes + cur + i, sd);
+    float* gp = gframes + prev;
+    unsafeAtomicAdd(gp + (size_t)w.y0 * d.W + w.x0, -sd * ((1.0f - w.wy) * (1.0f - w.wx)));
+    unsafeAtomicAdd(gp + (size_t)w.y0 * d.W + w.x1, -sd * ((1.0f - w.wy) * w.wx));
+    unsafeAtomicAdd(gp + (size_t)w.y1 * d.W + w.x0, -sd * (w.wy * (1.0f - w.wx)));
+    unsafeAtomicAdd(gp + (size_t)w.y1 * d.W + w.x1, -sd * (w.wy * w.wx));
+    const float dvx = (1.0f - w.wy) * (c.p01 - c.p00) + w.wy * (c.p11 - c.p10);
+    const float dvy = c.bot - c.top;
+    float2 g = gflows[cur + i];
+    if (w.fx) g.x += sd * dvx;
+    if (w.fy) g.y += sd * dvy;
+    gflows[cur + i] = g;
 }
 
 // Sum of n partials in double, in a fixed order (deterministic): thread t
@@ -1134,18 +1410,18 @@ void prop_quad(const float* fin, const float* sin, const float* vin, float* fout
 void prop_quad_bwd(const float* fin, const float* sin, const float* vin, const float* gfo, const float* gso,
                    const float* gvo, float* gfi, float* gsi, float* gvi, int B, int H, int W, float dt, int border,
                    int acc, hipStream_t s) {
-    constexpr int RPW = kQuadBwdRowsPerWave;
-    constexpr int rows = QuadBwdShape<RPW>::rows;
-    const dim3 g((W + kQuadBwdCols - 1) / kQuadBwdCols, (H + rows - 1) / rows, B), blk(64, kQuadBwdWaves);
+    constexpr int NW = kQuadBwdWaves, RPW = kQuadBwdRowsPerWave;
+    constexpr int rows = QuadBwdShape<NW, RPW>::rows;
+    const dim3 g((W + kQuadBwdCols - 1) / kQuadBwdCols, (H + rows - 1) / rows, B), blk(64, NW);
     const Dims d{H, W};
     auto F = [](const float* p) { return (const float2*)p; };
     auto O = [](float* p) { return (float2*)p; };
     if (sin)
-        hipLaunchKernelGGL((k_ad_prop_quad_bwd<true, RPW>), g, blk, 0, s, F(fin), sin, F(vin), F(gfo), gso, F(gvo),
-                           O(gfi), gsi, O(gvi), d, dt, border, acc);
+        hipLaunchKernelGGL((k_ad_prop_quad_bwd<true, NW, RPW>), g, blk, 0, s, F(fin), sin, F(vin), F(gfo), gso,
+                           F(gvo), O(gfi), gsi, O(gvi), d, dt, border, acc);
     else
-        hipLaunchKernelGGL((k_ad_prop_quad_bwd<false, RPW>), g, blk, 0, s, F(fin), sin, F(vin), F(gfo), gso, F(gvo),
-                           O(gfi), gsi, O(gvi), d, dt, border, acc);
+        hipLaunchKernelGGL((k_ad_prop_quad_bwd<false, NW, RPW>), g, blk, 0, s, F(fin), sin, F(vin), F(gfo), gso,
+                           F(gvo), O(gfi), gsi, O(gvi), d, dt, border, acc);
 }
 void prop_pair_bwd(const float* fin, const float* sin, const float* vin, const float* fmid, const float* smid,
                    const float* vmid, const float* gfo, const float* gso, const float* gvo, float* gfi, float* gsi,
@@ -1253,6 +1529,17 @@ void smooth_bwd(int axis, const float* gout, const float* gout2, float* gin, int
     auto F = [](const float* p) { return (const float2*)p; };
     if (axis == 0) hipLaunchKernelGGL((k_ad_smooth_bwd<0>), grid3(B, H, W), dim3(BX, BY), 0, s, F(gout), F(gout2), (float2*)gin, Dims{H, W}, acc);
     else hipLaunchKernelGGL((k_ad_smooth_bwd<1>), grid3(B, H, W), dim3(BX, BY), 0, s, F(gout), F(gout2), (float2*)gin, Dims{H, W}, acc);
+}
+
+int photometric_partials(int B, int T, int H, int W) { return B * (T - 1) * blocks(H, W); }
+void photometric(const float* frames, const float* flows, float* partial, int B, int T, int H, int W, hipStream_t s) {
+    hipLaunchKernelGGL(k_ad_photometric, grid3(B * (T - 1), H, W), dim3(BX, BY), 0, s, frames, (const float2*)flows,
+                       partial, T, Dims{H, W});
+}
+void photometric_bwd(const float* frames, const float* flows, const float* scale, float* gframes, float* gflows, int B,
+                     int T, int H, int W, hipStream_t s) {
+    hipLaunchKernelGGL(k_ad_photometric_bwd, grid3(B * (T - 1), H, W), dim3(BX, BY), 0, s, frames,
+                       (const float2*)flows, scale, gframes, (float2*)gflows, T, Dims{H, W});
 }
 
 }  // namespace ad

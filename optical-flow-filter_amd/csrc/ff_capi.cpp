@@ -1110,6 +1110,38 @@ ff_status ff_ad_smooth_pair_backward(const float* g_flow_out, const float* g_flo
     });
 }
 
+ff_status ff_ad_photometric_partial_count(int B, int T, int height, int width, int* out) {
+    return guard([&] {
+        out_ptr(out, "count");
+        if (T < 2) throw invalid_argument("photometric term: needs at least two frames, got T = " + std::to_string(T));
+        dims(B, height, width);
+        *out = ad::photometric_partials(B, T, height, width);
+    });
+}
+ff_status ff_ad_photometric(const float* frames, const float* flows, float* partial, int B, int T, int H, int W,
+                            void* stream) {
+    return guard([&] {
+        if (T < 2) throw invalid_argument("photometric term: needs at least two frames, got T = " + std::to_string(T));
+        dims(B, H, W);
+        need(frames, "frames"), need(flows, "flows"), need(partial, "partial");
+        ad::photometric(frames, flows, partial, B, T, H, W, as_stream(stream));
+        check_launch("ff_ad_photometric");
+    });
+}
+ff_status ff_ad_photometric_backward(const float* frames, const float* flows, const float* scale, float* g_frames,
+                                     float* g_flows, int B, int T, int H, int W, void* stream) {
+    return guard([&] {
+        if (T < 2) throw invalid_argument("photometric term: needs at least two frames, got T = " + std::to_string(T));
+        dims(B, H, W);
+        need(frames, "frames"), need(flows, "flows"), need(scale, "scale"), need(g_frames, "g_frames"),
+            need(g_flows, "g_flows");
+        if ((const float*)g_frames == frames || (const float*)g_flows == flows)
+            throw invalid_argument("photometric adjoint: the gradients must not alias the inputs");
+        ad::photometric_bwd(frames, flows, scale, g_frames, g_flows, B, T, H, W, as_stream(stream));
+        check_launch("ff_ad_photometric_backward");
+    });
+}
+
 // --------------------------------------------------------------- profiling
 ff_status ff_profile_begin(const char* filter) {
     return guard([&] { profile_begin(filter ? filter : ""); });

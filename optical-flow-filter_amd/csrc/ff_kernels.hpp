@@ -290,6 +290,10 @@ void smooth_pair_bwd(const float* gout, const float* gout2, float* gin, int B, i
                      hipStream_t s);
 void smooth_bwd(int axis, const float* gout, const float* gout2, float* gin, int B, int H, int W, int acc,
                 hipStream_t s);
+int photometric_partials(int B, int T, int H, int W);
+void photometric(const float* frames, const float* flows, float* partial, int B, int T, int H, int W, hipStream_t s);
+void photometric_bwd(const float* frames, const float* flows, const float* scale, float* gframes, float* gflows, int B,
+                     int T, int H, int W, hipStream_t s);
 }  // namespace ad
 
 }  // namespace ffamd

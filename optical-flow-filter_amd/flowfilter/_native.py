@@ -259,6 +259,9 @@ SIGNATURES = {
     "ff_ad_smooth_pass_backward": [_i, _v, _v, _v, _i, _i, _i, _i, _v],
     "ff_ad_smooth_pair": [_v, _v, _i, _i, _i, _v],
     "ff_ad_smooth_pair_backward": [_v, _v, _v, _i, _i, _i, _i, _v],
+    "ff_ad_photometric_partial_count": [_i, _i, _i, _i, _pi],
+    "ff_ad_photometric": [_v, _v, _v, _i, _i, _i, _i, _v],
+    "ff_ad_photometric_backward": [_v, _v, _v, _v, _v, _i, _i, _i, _i, _v],
     "ff_batch_add": [_S, _S],
     "ff_batch_size": [_S, _pi],
     "ff_batch_load_images_device": [_S, ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_size_t), _i],

@@ -446,15 +446,49 @@ def flow_window(frames, gamma, spec, checkpoint=4):
 
 # ------------------------------------------------------------------ losses
 
+class _Photometric(torch.autograd.Function):
+    """The photometric term on the GPU in two kernels (ff_ad_photometric and
+    its adjoint): the sum of squared differences as per-workgroup partial
+    sums, added in double in a fixed order; the backward adds the frame and
+    flow gradients in one pass."""
+
+    @staticmethod
+    def forward(ctx, frames, flows):
+        B, T, H, W = frames.shape
+        frames = frames.contiguous()
+        flows = flows.contiguous()
+        n = N.get_int("ff_ad_photometric_partial_count", B, T, H, W)
+        part = torch.empty(n, dtype=torch.float32, device=frames.device)
+        N.call("ff_ad_photometric", _p(frames), _p(flows), _p(part), B, T, H, W, _stream())
+        count = B * (T - 1) * H * W
+        ctx.save_for_backward(frames, flows)
+        ctx.count = count
+        return (part.double().sum() / count).to(torch.float32)
+
+    @staticmethod
+    def backward(ctx, g):
+        frames, flows = ctx.saved_tensors
+        B, T, H, W = frames.shape
+        scale = (g.to(torch.float32) / ctx.count).reshape(1).contiguous()
+        gframes = torch.zeros_like(frames)
+        gflows = torch.zeros_like(flows)
+        N.call("ff_ad_photometric_backward", _p(frames), _p(flows), _p(scale), _p(gframes), _p(gflows), B, T, H, W,
+               _stream())
+        return gframes, gflows
+
+
 def photometric_error(frames, flows):
     """Mean over frames t >= 1 of the mean squared difference between frame
     t and frame t-1 sampled bilinearly at x - flow_t(x) (edge-clamped).
-    frames [B,T,H,W], flows [B,T,H,W,2] (pixels per frame). All frame
-    pairs go through one grid_sample call, so autograd handles two slices
-    of the frames instead of two per frame."""
+    frames [B,T,H,W], flows [B,T,H,W,2] (pixels per frame). Float32 tensors
+    on the GPU go through the fused kernels (_Photometric); other tensors
+    (the CPU and float64 checks of the composition) through one grid_sample
+    call over all frame pairs."""
     B, T, H, W = frames.shape
     if T < 2:
         return frames.sum() * 0.0
+    if frames.is_cuda and frames.dtype == torch.float32 and flows.dtype == torch.float32:
+        return _Photometric.apply(frames, flows)
     dev, dt = frames.device, frames.dtype
     xs = torch.arange(W, device=dev, dtype=dt)
     ys = torch.arange(H, device=dev, dtype=dt)[:, None]
