@@ -22,7 +22,10 @@ dominant kernel (algorithmic bytes per launch / its average duration from
 HIP events on its stream, over a second pass of the same K steps; `traffic`
 is the HBM bytes per launch measured with rocprofv3 FETCH_SIZE/WRITE_SIZE
 passes, calibrated on a copy kernel, read from
-profiles/fused_frame_2048x1088_2lvl_traffic.json), the 64-stream batched
+profiles/fused_frame_2048x1088_2lvl_traffic.json; `valu_issue` is the
+kernel's VALU-issue floor, its SQ_INSTS_VALU count from
+profiles/fused_frame_2048x1088_2lvl_valu.json over the CUs' issue rate,
+against the measured launch time: the bound that holds), the 64-stream batched
 measurement, the differentiable-window measurements (forward and forward +
 backward of a 3-level 3840x2160 32-frame window and of 16 sequences of
 512x512 x 8 frames, with peak HBM), and a CPU baseline (the FP32 CPU oracle
@@ -488,6 +491,18 @@ def main():
             t = json.load(fh)
         if t.get("algorithmic_bytes_per_launch") == algorithmic_bytes()[dominant]:
             traffic = t["traffic_bytes_per_launch"]
+    # what actually bounds the fused kernel: VALU issue. Its instruction
+    # count per launch (rocprofv3 SQ_INSTS_VALU, profiles/) over the CUs'
+    # issue rate (one wave-wide VALU instruction per CU per cycle, 256 CUs
+    # at 2.4 GHz) is the floor; frac = floor / this run's launch time.
+    valu = None
+    vf = os.path.join(ROOT, "profiles", "fused_frame_2048x1088_2lvl_valu.json")
+    if dominant == "fused_frame" and os.path.exists(vf):
+        with open(vf) as fh:
+            v = json.load(fh)
+        floor_ms = v["sq_insts_valu_per_launch"] / v["valu_issue_rate_per_s"] * 1e3
+        valu = {"instructions_per_launch": v["sq_insts_valu_per_launch"], "issue_floor_ms": round(floor_ms, 5),
+                "frac": round(floor_ms / avg_ms, 4), "source": "profiles/fused_frame_2048x1088_2lvl_valu_counters.md"}
 
     out = {
         "metric": "Mpix/s at 2048x1088 2-lvl; HBM GB/s % of peak; batched/plain speedup",
@@ -513,7 +528,7 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
                      "algorithmic_bytes_per_launch": alg_bytes, "avg_launch_ms": round(avg_ms, 5),
-                     "launches": launches},
+                     "launches": launches, "valu_issue": valu},
     }
     if not args.no_batched:
         out["batched"] = batched_bench(args, N, gf, GPUImage)
