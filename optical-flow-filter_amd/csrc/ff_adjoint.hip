@@ -1265,27 +1265,58 @@ __global__ void __launch_bounds__(BX* BY)
     block_sum(e, partial);
 }
 
-// Its adjoint for the loss scale *scale (d loss / d sum of squares):
-// frame t gets s, the four sampled pixels of frame t-1 get -s times their
-// weights (atomic adds: several pixels sample the same ones), and the flow
-// gets s times the derivative of the sample along each unclamped
-// coordinate, with s = 2 *scale * difference. Adds into gframes and gflows.
+// Lane i receives lane i-1's value (PREV) or lane i+1's; 0 where there is
+// none (every lane of the wave active).
+template <bool PREV>
+__device__ __forceinline__ int lane_shift_int(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, PREV ? 0x138 : 0x130, 0xf, 0xf, true);
+}
+
+// Its adjoint for the loss scale *scale (d loss / d sum of squares), for
+// the frame pairs t = 1 + parity, 3 + parity, ... (z = b * npairs + k, t =
+// 1 + parity + 2k): frame t gets s, the four sampled pixels of frame t-1 get
+// -s times their weights, and the flow gets s times the derivative of the
+// sample along each unclamped coordinate, with s = 2 *scale * difference;
+// all added into gframes and gflows. Within one launch a frame is either a
+// pair's target or the frame a pair samples, never both, so the target
+// term is a plain add; the sampled pixels take atomic adds (neighbouring
+// pixels sample the same ones), after each lane has taken over the
+// contribution of the lane on its left that lands on the same pixel (a
+// lane's (y0, x1) and (y1, x1) are its right neighbour's (y0, x0) and
+// (y1, x0) wherever the flow is smooth), which halves them. A wave is one
+// row of 64 columns: every lane stays active for the lane exchanges.
 __global__ void __launch_bounds__(BX* BY)
     k_ad_photometric_bwd(const float* frames, const float2* flows, const float* scale, float* gframes,
-                         float2* gflows, int T, Dims d) {
+                         float2* gflows, int T, int parity, int npairs, Dims d) {
     const int x = blockIdx.x * BX + threadIdx.x, y = blockIdx.y * BY + wave_y(), z = blockIdx.z;
-    if (x >= d.W || y >= d.H) return;
-    const int b = z / (T - 1), t = z - b * (T - 1) + 1;
-    const size_t cur = (size_t)(b * T + t) * d.H * d.W, prev = cur - (size_t)d.H * d.W, i = (size_t)y * d.W + x;
-    const Warp w = warp_at(x, y, flows[cur + i], d);
+    if (y >= d.H) return;  // the whole wave
+    const bool valid = x < d.W;
+    const int xc = min(x, d.W - 1);
+    const int b = z / npairs, t = 1 + parity + 2 * (z - b * npairs);
+    const size_t cur = (size_t)(b * T + t) * d.H * d.W, prev = cur - (size_t)d.H * d.W, i = (size_t)y * d.W + xc;
+    const Warp w = warp_at(xc, y, flows[cur + i], d);
     const Corners c = sample(frames + prev, w, d.W);
-    const float sd = 2.0f * *scale * (frames[cur + i] - c.v);
-    unsafeAtomicAdd(gframes + cur + i, sd);
+    const float sd = valid ? 2.0f * *scale * (frames[cur + i] - c.v) : 0.0f;
+    float c00 = -sd * ((1.0f - w.wy) * (1.0f - w.wx)), c01 = -sd * ((1.0f - w.wy) * w.wx);
+    float c10 = -sd * (w.wy * (1.0f - w.wx)), c11 = -sd * (w.wy * w.wx);
+    // the lane on the left: its x1 column and rows, its right-hand terms
+    const int v = valid ? 1 : 0;
+    const int lv = lane_shift_int<true>(v), lx1 = lane_shift_int<true>(w.x1);
+    const int ly0 = lane_shift_int<true>(w.y0), ly1 = lane_shift_int<true>(w.y1);
+    const float l01 = lane_shift<true>(c01), l11 = lane_shift<true>(c11);
+    // the lane on the right: whether it took this lane's right-hand terms
+    const int rv = lane_shift_int<false>(v), rx0 = lane_shift_int<false>(w.x0);
+    const int ry0 = lane_shift_int<false>(w.y0), ry1 = lane_shift_int<false>(w.y1);
+    if (!valid) return;
+    if (lv && lx1 == w.x0 && ly0 == w.y0) c00 += l01;
+    if (lv && lx1 == w.x0 && ly1 == w.y1) c10 += l11;
+    const bool gave0 = rv && rx0 == w.x1 && ry0 == w.y0, gave1 = rv && rx0 == w.x1 && ry1 == w.y1;
+    gframes[cur + i] += sd;
     float* gp = gframes + prev;
-    unsafeAtomicAdd(gp + (size_t)w.y0 * d.W + w.x0, -sd * ((1.0f - w.wy) * (1.0f - w.wx)));
-    unsafeAtomicAdd(gp + (size_t)w.y0 * d.W + w.x1, -sd * ((1.0f - w.wy) * w.wx));
-    unsafeAtomicAdd(gp + (size_t)w.y1 * d.W + w.x0, -sd * (w.wy * (1.0f - w.wx)));
-    unsafeAtomicAdd(gp + (size_t)w.y1 * d.W + w.x1, -sd * (w.wy * w.wx));
+    unsafeAtomicAdd(gp + (size_t)w.y0 * d.W + w.x0, c00);
+    if (!gave0) unsafeAtomicAdd(gp + (size_t)w.y0 * d.W + w.x1, c01);
+    unsafeAtomicAdd(gp + (size_t)w.y1 * d.W + w.x0, c10);
+    if (!gave1) unsafeAtomicAdd(gp + (size_t)w.y1 * d.W + w.x1, c11);
     const float dvx = (1.0f - w.wy) * (c.p01 - c.p00) + w.wy * (c.p11 - c.p10);
     const float dvy = c.bot - c.top;
     float2 g = gflows[cur + i];
@@ -1538,8 +1569,13 @@ void photometric(const float* frames, const float* flows, float* partial, int B,
 }
 void photometric_bwd(const float* frames, const float* flows, const float* scale, float* gframes, float* gflows, int B,
                      int T, int H, int W, hipStream_t s) {
-    hipLaunchKernelGGL(k_ad_photometric_bwd, grid3(B * (T - 1), H, W), dim3(BX, BY), 0, s, frames,
-                       (const float2*)flows, scale, gframes, (float2*)gflows, T, Dims{H, W});
+    // pairs t = 1, 3, ... then t = 2, 4, ... (see k_ad_photometric_bwd)
+    for (int parity = 0; parity < 2; ++parity) {
+        const int npairs = (T - parity) / 2;
+        if (npairs < 1) continue;
+        hipLaunchKernelGGL(k_ad_photometric_bwd, grid3(B * npairs, H, W), dim3(BX, BY), 0, s, frames,
+                           (const float2*)flows, scale, gframes, (float2*)gflows, T, parity, npairs, Dims{H, W});
+    }
 }
 
 }  // namespace ad
