@@ -298,61 +298,133 @@ __global__ void __launch_bounds__(64 * (kPairRows + 2))
     }
 }
 
+// What an output of a pass contributes to its three inputs (see out_terms).
+struct OutTerms {
+    float cm, cz, cp;  // coefficients of the inputs at o-1, o, o+1
+    float gud;         // adjoint of the speed
+    unsigned flags;    // bit 0: o inside the border; bit 1: speed taken from o+1
+};
+
+// Rows of pixels (Px5) in LDS as planes (flow, scalar, vector), 20 bytes
+// per pixel with consecutive lanes at consecutive addresses (an array of
+// Px5 takes 24 bytes, its padded size, and strides the banks). A row can
+// hold a pixel or an output's OutTerms instead.
+template <int S>
+struct LdsRows {
+    float2 f[S][64];
+    float s[S][64];
+    float2 v[S][64];
+    template <bool PAY>
+    __device__ __forceinline__ void put(int i, int tx, const Px5& p) {
+        f[i][tx] = p.f;
+        if constexpr (PAY) {
+            s[i][tx] = p.s;
+            v[i][tx] = p.v;
+        }
+    }
+    template <bool PAY>
+    __device__ __forceinline__ Px5 get(int i, int tx) const {
+        Px5 r;
+        r.f = f[i][tx];
+        r.s = PAY ? s[i][tx] : 0.f;
+        r.v = PAY ? v[i][tx] : make_float2(0.f, 0.f);
+        return r;
+    }
+    __device__ __forceinline__ void put_terms(int i, int tx, const OutTerms& t) {
+        f[i][tx] = make_float2(t.cm, t.cz);
+        s[i][tx] = t.cp;
+        v[i][tx] = make_float2(t.gud, __uint_as_float(t.flags));
+    }
+    __device__ __forceinline__ OutTerms get_terms(int i, int tx) const {
+        const float2 a = f[i][tx], c = v[i][tx];
+        OutTerms t;
+        t.cm = a.x;
+        t.cz = a.y;
+        t.cp = s[i][tx];
+        t.gud = c.x;
+        t.flags = __float_as_uint(c.y);
+        return t;
+    }
+};
+
 // Two propagation iterations (X, Y, X, Y passes) in one launch, for the
 // forward of frames whose intermediates are not kept: a workgroup of
-// 64 x 16 threads (a wave per row) holds 16 rows (clamped into the image),
-// every pass's result in LDS, and writes the 12 inner rows of its 62 inner
-// columns (lanes 1..62); it reads and writes each field once per two
-// iterations instead of twice. The first X pass takes its row neighbours
-// from the adjacent lanes (the edge lanes load theirs); the second from the
-// adjacent lanes only, so lanes 0 and 63 serve as its halo. A row or column
-// outside the image holds the values of the edge row or column it is
-// clamped to (the clamped reads of the pass that follows), and each Y pass
-// reads its rows' neighbours at the clamped row indices. Every pixel's value
-// is computed by the same per-pixel function from the same operands as in
-// two k_ad_prop_pair launches, so the results are bitwise theirs.
-constexpr int kQuadRows = 12, kQuadCols = 62;
-template <bool PAY>
-__global__ void __launch_bounds__(64 * (kQuadRows + 4))
+// 64 x NW threads covers 64 columns (lane tx: column x0-1+tx) and S = NW x RPW
+// staged rows (wave w: staged rows i = w + NW k, image rows y0-2+i, clamped
+// into the image), and writes the S-4 inner rows of its 62 inner columns
+// (lanes 1..62); it reads and writes each field once per two iterations
+// instead of twice. The first X pass (row neighbours from the adjacent
+// lanes, the edge lanes load theirs) goes to LDS; the first Y pass reads it
+// at rows 1..S-2, the second X pass follows in registers (neighbours from
+// the adjacent lanes only, so lanes 0 and 63 serve as its halo) into LDS;
+// the second Y pass reads that at the output rows. A row or column outside
+// the image holds the values of the edge row or column it is clamped to
+// (the clamped reads of the pass that follows), and each Y pass reads its
+// rows' neighbours at the clamped row indices. Every pixel's value is
+// computed by the same per-pixel function from the same operands as in two
+// k_ad_prop_pair launches, so the results are bitwise theirs.
+constexpr int kQuadCols = 62;
+// 16 waves of 2 rows: 32 staged rows for 28 output rows, two 40 KB sets of
+// LDS planes (two workgroups, 32 waves, per CU). With 16 waves of one row
+// (16 staged rows for 12, 24-byte Px5 in LDS, three barriers) the
+// 3840x2160 launch took 94 us.
+constexpr int kQuadWaves = 16, kQuadRowsPerWave = 2, kQuadRows = kQuadWaves * kQuadRowsPerWave - 4;
+template <bool PAY, int NW, int RPW>
+__global__ void __launch_bounds__(64 * NW)
     k_ad_prop_quad(const float2* fin, const float* sin, const float2* vin, float2* fout, float* sout, float2* vout,
                    Dims d, float dt, int border) {
-    __shared__ Px5 LA[kQuadRows + 4][64];  // X passes
-    __shared__ Px5 LB[kQuadRows + 4][64];  // first Y pass
-    const int tx = threadIdx.x, ty = wave_y(), b = blockIdx.z;
+    constexpr int S = NW * RPW;
+    __shared__ LdsRows<S> LA;  // first X pass
+    __shared__ LdsRows<S> LB;  // second X pass
+    const int tx = threadIdx.x, w = wave_y(), b = blockIdx.z;
     const int x = (int)blockIdx.x * kQuadCols - 1 + tx;  // column of this lane
-    const int y0 = (int)blockIdx.y * kQuadRows;
-    const int yr = y0 - 2 + ty;  // staged row ty
-    const int yc = clampi(yr, 0, d.H - 1);
+    const int y0 = (int)blockIdx.y * (S - 4);
     const int xc = clampi(x, 0, d.W - 1);
     const bool xin = x >= border && x < d.W - border;
-    const bool in = xin && yc >= border && yc < d.H - border;
     // staged row holding image row r (r within the staged range)
     auto srow = [&](int r) { return clampi(r, 0, d.H - 1) - (y0 - 2); };
-    // first X pass on all 16 rows
-    {
-        const Px5 z = load_px<PAY>(fin, sin, vin, at(b, yc, xc, d));
-        Px5 m = lane_shift_px<true, PAY>(z), p = lane_shift_px<false, PAY>(z);
-        if (tx == 0) m = load_px<PAY>(fin, sin, vin, at(b, yc, clampi(x - 1, 0, d.W - 1), d));
-        if (tx == 63) p = load_px<PAY>(fin, sin, vin, at(b, yc, clampi(x + 1, 0, d.W - 1), d));
-        LA[ty][tx] = upwind_px<0, PAY>(m, z, p, dt, in);
+    int yr[RPW], yc[RPW];
+    bool in[RPW];
+#pragma unroll
+    for (int k = 0; k < RPW; ++k) {
+        yr[k] = y0 - 2 + w + NW * k;
+        yc[k] = clampi(yr[k], 0, d.H - 1);
+        in[k] = xin && yc[k] >= border && yc[k] < d.H - border;
+    }
+    // first X pass on every staged row
+    Px5 z[RPW];
+#pragma unroll
+    for (int k = 0; k < RPW; ++k) z[k] = load_px<PAY>(fin, sin, vin, at(b, yc[k], xc, d));
+#pragma unroll
+    for (int k = 0; k < RPW; ++k) {
+        Px5 m = lane_shift_px<true, PAY>(z[k]), p = lane_shift_px<false, PAY>(z[k]);
+        if (tx == 0) m = load_px<PAY>(fin, sin, vin, at(b, yc[k], clampi(x - 1, 0, d.W - 1), d));
+        if (tx == 63) p = load_px<PAY>(fin, sin, vin, at(b, yc[k], clampi(x + 1, 0, d.W - 1), d));
+        LA.template put<PAY>(w + NW * k, tx, upwind_px<0, PAY>(m, z[k], p, dt, in[k]));
     }
     __syncthreads();
-    // first Y pass on staged rows 1..14 (image rows clamp(y0-1 .. y0+12))
-    if (ty >= 1 && ty <= kQuadRows + 2)
-        LB[ty][tx] = upwind_px<1, PAY>(LA[srow(yc - 1)][tx], LA[srow(yc)][tx], LA[srow(yc + 1)][tx], dt, in);
-    __syncthreads();
-    // second X pass on the same rows, neighbours from the adjacent lanes
+    // first Y pass on staged rows 1..S-2 and the second X pass after it
     // (valid on lanes 1..62)
-    if (ty >= 1 && ty <= kQuadRows + 2) {
-        const Px5 z = LB[ty][tx];
-        const Px5 m = lane_shift_px<true, PAY>(z), p = lane_shift_px<false, PAY>(z);
-        LA[ty][tx] = upwind_px<0, PAY>(m, z, p, dt, in);
+#pragma unroll
+    for (int k = 0; k < RPW; ++k) {
+        const int i = w + NW * k;
+        if (i < 1 || i > S - 2) continue;  // uniform across the wave
+        const Px5 y1 = upwind_px<1, PAY>(LA.template get<PAY>(srow(yc[k] - 1), tx), LA.template get<PAY>(srow(yc[k]), tx),
+                                         LA.template get<PAY>(srow(yc[k] + 1), tx), dt, in[k]);
+        const Px5 m = lane_shift_px<true, PAY>(y1), p = lane_shift_px<false, PAY>(y1);
+        LB.template put<PAY>(i, tx, upwind_px<0, PAY>(m, y1, p, dt, in[k]));
     }
     __syncthreads();
-    // second Y pass on the 12 output rows
-    if (ty >= 2 && ty < kQuadRows + 2 && yr < d.H && tx >= 1 && tx <= kQuadCols && x < d.W) {
-        const Px5 o = upwind_px<1, PAY>(LA[srow(yc - 1)][tx], LA[srow(yc)][tx], LA[srow(yc + 1)][tx], dt, in);
-        store_px<PAY>(fout, sout, vout, at(b, yr, x, d), o);
+    // second Y pass on the S-4 output rows
+#pragma unroll
+    for (int k = 0; k < RPW; ++k) {
+        const int i = w + NW * k;
+        if (i < 2 || i > S - 3 || yr[k] >= d.H) continue;
+        if (tx >= 1 && tx <= kQuadCols && x < d.W) {
+            const Px5 o = upwind_px<1, PAY>(LB.template get<PAY>(srow(yc[k] - 1), tx), LB.template get<PAY>(srow(yc[k]), tx),
+                                            LB.template get<PAY>(srow(yc[k] + 1), tx), dt, in[k]);
+            store_px<PAY>(fout, sout, vout, at(b, yr[k], x, d), o);
+        }
     }
 }
 
@@ -437,13 +509,7 @@ __device__ __forceinline__ Px5 upwind_adjoint_px(const Px5 (&X)[5], const Px5 (&
 // gather_px sums the terms of outputs j-1, j, j+1 for input j. Both use
 // upwind_adjoint_px's expressions in its order, so together they give its
 // results bit for bit, while each output's terms are computed once instead
-// of once per input that reads them.
-struct OutTerms {
-    float cm, cz, cp;  // coefficients of the inputs at o-1, o, o+1
-    float gud;         // adjoint of the speed
-    unsigned flags;    // bit 0: o inside the border; bit 1: speed taken from o+1
-};
-
+// of once per input that reads them (OutTerms above).
 template <int C, bool PAY>
 __device__ __forceinline__ OutTerms out_terms(const Px5& xm, const Px5& x0, const Px5& xp, const Px5& g, bool interior,
                                               float dt) {
@@ -676,47 +742,6 @@ __global__ void __launch_bounds__(64 * kPairBwdRows)
     }
 }
 
-// Rows of pixels (Px5) in LDS as planes (flow, scalar, vector), 20 bytes
-// per pixel with consecutive lanes at consecutive addresses (an array of
-// Px5 takes 24 bytes, its padded size, and strides the banks). A row can
-// hold a pixel or an output's OutTerms instead.
-template <int S>
-struct LdsRows {
-    float2 f[S][64];
-    float s[S][64];
-    float2 v[S][64];
-    template <bool PAY>
-    __device__ __forceinline__ void put(int i, int tx, const Px5& p) {
-        f[i][tx] = p.f;
-        if constexpr (PAY) {
-            s[i][tx] = p.s;
-            v[i][tx] = p.v;
-        }
-    }
-    template <bool PAY>
-    __device__ __forceinline__ Px5 get(int i, int tx) const {
-        Px5 r;
-        r.f = f[i][tx];
-        r.s = PAY ? s[i][tx] : 0.f;
-        r.v = PAY ? v[i][tx] : make_float2(0.f, 0.f);
-        return r;
-    }
-    __device__ __forceinline__ void put_terms(int i, int tx, const OutTerms& t) {
-        f[i][tx] = make_float2(t.cm, t.cz);
-        s[i][tx] = t.cp;
-        v[i][tx] = make_float2(t.gud, __uint_as_float(t.flags));
-    }
-    __device__ __forceinline__ OutTerms get_terms(int i, int tx) const {
-        const float2 a = f[i][tx], c = v[i][tx];
-        OutTerms t;
-        t.cm = a.x;
-        t.cz = a.y;
-        t.cp = s[i][tx];
-        t.gud = c.x;
-        t.flags = __float_as_uint(c.y);
-        return t;
-    }
-};
 
 // Adjoint of k_ad_prop_quad (two iterations), recomputing the forward
 // passes inside the launch. A workgroup of 64 x NW threads covers 64
@@ -1429,14 +1454,15 @@ void prop_pair(const float* fin, const float* sin, const float* vin, float* fmid
 }
 void prop_quad(const float* fin, const float* sin, const float* vin, float* fout, float* sout, float* vout, int B, int H,
                int W, float dt, int border, hipStream_t s) {
-    const dim3 g((W + kQuadCols - 1) / kQuadCols, (H + kQuadRows - 1) / kQuadRows, B), blk(64, kQuadRows + 4);
+    constexpr int NW = kQuadWaves, RPW = kQuadRowsPerWave;
+    const dim3 g((W + kQuadCols - 1) / kQuadCols, (H + kQuadRows - 1) / kQuadRows, B), blk(64, NW);
     const Dims d{H, W};
     if (sin)
-        hipLaunchKernelGGL(k_ad_prop_quad<true>, g, blk, 0, s, (const float2*)fin, sin, (const float2*)vin, (float2*)fout,
-                           sout, (float2*)vout, d, dt, border);
-    else
-        hipLaunchKernelGGL(k_ad_prop_quad<false>, g, blk, 0, s, (const float2*)fin, sin, (const float2*)vin,
+        hipLaunchKernelGGL((k_ad_prop_quad<true, NW, RPW>), g, blk, 0, s, (const float2*)fin, sin, (const float2*)vin,
                            (float2*)fout, sout, (float2*)vout, d, dt, border);
+    else
+        hipLaunchKernelGGL((k_ad_prop_quad<false, NW, RPW>), g, blk, 0, s, (const float2*)fin, sin,
+                           (const float2*)vin, (float2*)fout, sout, (float2*)vout, d, dt, border);
 }
 void prop_quad_bwd(const float* fin, const float* sin, const float* vin, const float* gfo, const float* gso,
                    const float* gvo, float* gfi, float* gsi, float* gvi, int B, int H, int W, float dt, int border,

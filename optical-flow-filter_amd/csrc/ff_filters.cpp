@@ -605,6 +605,7 @@ void PyramidalFlowFilter::computeFused(const std::vector<LevelArgs>& args, const
     // (pyramid kernels for more than three levels); replaying a one-node
     // graph costs more than launching the kernel
     const FrameRef fr{frame, frame_pitch};
+    const WaitingLaunch order(s, frame_.waits(table));
     // a frame in place is a launch argument, which a replayed graph would
     // not update: such frames are launched directly
     if (run_maybe_graphed(graphs_on_ && levels_ > 3 && !frame, graphs_, args, frame_.token(table), s,
@@ -842,6 +843,7 @@ void BatchedFlowFilter::compute() {
         hipStream_t s = ffamd::stream();
         const int table = frame_.prepare(all, s);
         last_blocks_ = frame_.blocks(table);
+        const WaitingLaunch order(s, frame_.waits(table));
         // the step is replayed as a HIP graph when graphs are on (the
         // default) and no frame is read in place (a launch argument, which
         // a replay would not update)

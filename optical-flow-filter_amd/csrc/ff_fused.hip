@@ -1222,6 +1222,7 @@ void FusedFrame::Impl::run(int ti, hipStream_t s, const FrameRef* frames, int nf
     }
     watch->stream = s;
     if (t.waits) watch->pending = true;
+    const WaitingLaunch order(s, t.waits);
     ProfileScope ps("fused_frame", s);
     const char* base = (const char*)t.dev.data();
     // the timeout word, relative to this slot's region
@@ -1317,6 +1318,7 @@ void FusedFrame::replayed() { impl_->counted[0] = impl_->counted[1] = 0; }
 
 int FusedFrame::blocks(int table) const { return impl_->slot[table & 1].blocks; }
 unsigned long long FusedFrame::token(int table) const { return impl_->slot[table & 1].token; }
+bool FusedFrame::waits(int table) const { return impl_->slot[table & 1].waits; }
 
 void FusedFrame::setWaves(int nw) {
     bool ok = nw == 0;

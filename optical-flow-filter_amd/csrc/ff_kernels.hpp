@@ -234,6 +234,8 @@ public:
     // whenever that memory is replaced, so recorded launches (graphs) keyed
     // by it never refer to freed memory.
     unsigned long long token(int table) const;
+    // Whether some job of the table waits for another (see WaitingLaunch).
+    bool waits(int table) const;
 
 private:
     struct Impl;

@@ -117,187 +117,6 @@ CrossStream::~CrossStream() {
 // ----------------------------------------------------------------- staging
 
 namespace {
-// Pinned host buffer for copies between host images and device images of
-// different row layouts; grows as needed, kept for the process.
-std::mutex g_stage_mu;
-void* g_stage = nullptr;
-size_t g_stage_bytes = 0;
-
-void* staging(size_t bytes) {
-    if (bytes > g_stage_bytes) {
-        if (g_stage) FF_HIP(hipHostFree(g_stage));
-        g_stage = nullptr;
-        g_stage_bytes = 0;
-        FF_HIP(hipHostMalloc(&g_stage, bytes, hipHostMallocDefault));
-        g_stage_bytes = bytes;
-    }
-    return g_stage;
-}
-}  // namespace
-
-// ----------------------------------------------------------------- storage
-
-Storage::~Storage() {
-    if (ptr) (void)hipFree(ptr);
-}
-
-size_t pitch_for(int width, int depth, int itemSize) {
-    const size_t row = (size_t)width * depth * itemSize;
-    return (row + 255) / 256 * 256;
-}
-
-Image::Image(int height, int width, int depth, int itemSize) : h_(height), w_(width), d_(depth), item_(itemSize) {
-    allocate();
-}
-
-void Image::allocate() {
-    if (h_ < 0 || w_ < 0 || d_ < 0 || item_ < 0) throw invalid_argument("GPUImage: negative dimension " + shapeString());
-    pitch_ = pitch_for(w_, d_, item_);
-    st_ = std::make_shared<Storage>();
-    st_->home = current_box();
-    const size_t bytes = pitch_ * (size_t)h_;
-    if (bytes > 0) {
-        FF_HIP(hipMalloc(&st_->ptr, bytes));
-        st_->bytes = bytes;
-    }
-}
-
-const std::shared_ptr<StreamBox>& Image::home() const {
-    return st_ && st_->home ? st_->home : StreamBox::library();
-}
-
-hipStream_t Image::homeStream() const { return home()->get(); }
-
-std::string Image::shapeString() const {
-    std::ostringstream os;
-    os << "[" << h_ << ", " << w_ << ", " << d_ << "][" << item_ << "]";
-    return os.str();
-}
-
-std::string Image::shapeString(const ff_image_t& img) {
-    std::ostringstream os;
-    os << "[" << img.height << ", " << img.width << ", " << img.depth << "][" << img.itemSize << "]";
-    return os.str();
-}
-
-// GPUImage::upload (src/gpu/image.cu:81-121): an unallocated image takes the
-// host image's shape; otherwise shapes must match.
-void Image::upload(const ff_image_t& host) {
-    if (!st_) {
-        h_ = host.height;
-        w_ = host.width;
-        d_ = host.depth;
-        item_ = (int)host.itemSize;
-        allocate();
-    }
-    upload(host, homeStream());
-}
-
-void Image::upload(const ff_image_t& host, hipStream_t s) {
-    if (!st_) {
-        h_ = host.height;
-        w_ = host.width;
-        d_ = host.depth;
-        item_ = (int)host.itemSize;
-        allocate();
-    }
-    if (!matches(host))
-        throw invalid_argument("GPUImage::upload(): shapes do not match. Required: " + shapeString() +
-                               ", passed: " + shapeString(host));
-    if (rowBytes() == 0 || h_ == 0) return;
-    if (!host.data) throw invalid_argument("GPUImage::upload(): null host data");
-    if (host.pitch < rowBytes()) throw invalid_argument("GPUImage::upload(): host pitch smaller than a row");
-    const size_t span = (size_t)(h_ - 1) * pitch_ + rowBytes();
-    if (host.pitch == pitch_) {
-        // same row layout: one linear copy
-        FF_HIP(hipMemcpyAsync(data(), host.data, span, hipMemcpyHostToDevice, s));
-        FF_HIP(hipStreamSynchronize(s));
-        return;
-    }
-    // other row layouts go through pinned memory in the device layout (a
-    // 2-D copy from pageable memory moves row by row: ~10 ms for 1088 rows)
-    std::lock_guard<std::mutex> lk(g_stage_mu);
-    char* st = (char*)staging(span);
-    for (int r = 0; r < h_; ++r)
-        std::memcpy(st + (size_t)r * pitch_, (const char*)host.data + (size_t)r * host.pitch, rowBytes());
-    FF_HIP(hipMemcpyAsync(data(), st, span, hipMemcpyHostToDevice, s));
-    FF_HIP(hipStreamSynchronize(s));
-}
-
-// GPUImage::download (src/gpu/image.cu:123-150). The reference prints and
-// returns on an unallocated image; here it is an error.
-void Image::download(ff_image_t& host) const {
-    if (!st_) throw logic_error("GPUImage::download(): unallocated image");
-    if (!matches(host))
-        throw invalid_argument("GPUImage::download(): shapes do not match. Required: " + shapeString() +
-                               ", passed: " + shapeString(host));
-    if (rowBytes() == 0 || h_ == 0) return;
-    if (!host.data) throw invalid_argument("GPUImage::download(): null host data");
-    if (host.pitch < rowBytes()) throw invalid_argument("GPUImage::download(): host pitch smaller than a row");
-    hipStream_t s = homeStream();
-    const size_t span = (size_t)(h_ - 1) * pitch_ + rowBytes();
-    if (host.pitch == pitch_) {
-        FF_HIP(hipMemcpyAsync(host.data, data(), span, hipMemcpyDeviceToHost, s));
-        FF_HIP(hipStreamSynchronize(s));
-        return;
-    }
-    std::lock_guard<std::mutex> lk(g_stage_mu);
-    char* st = (char*)staging(span);
-    FF_HIP(hipMemcpyAsync(st, data(), span, hipMemcpyDeviceToHost, s));
-    FF_HIP(hipStreamSynchronize(s));
-    for (int r = 0; r < h_; ++r)
-        std::memcpy((char*)host.data + (size_t)r * host.pitch, st + (size_t)r * pitch_, rowBytes());
-}
-
-void Image::upload_device(const void* dev, size_t pitch) {
-    if (!st_) throw logic_error("GPUImage::upload_device(): unallocated image");
-    if (rowBytes() == 0 || h_ == 0) return;
-    if (!dev) throw invalid_argument("GPUImage::upload_device(): null device pointer");
-    if (pitch < rowBytes()) throw invalid_argument("GPUImage::upload_device(): pitch smaller than a row");
-    const CopyRegion r{dev, data(), pitch, pitch_, rowBytes(), h_};
-    launch_copy_regions(&r, 1, homeStream());
-    check_launch("GPUImage::upload_device");
-}
-
-void Image::download_device(void* dev, size_t pitch) const {
-    if (!st_) throw logic_error("GPUImage::download_device(): unallocated image");
-    if (rowBytes() == 0 || h_ == 0) return;
-    if (!dev) throw invalid_argument("GPUImage::download_device(): null device pointer");
-    if (pitch < rowBytes()) throw invalid_argument("GPUImage::download_device(): pitch smaller than a row");
-    FF_HIP(hipMemcpy2DAsync(dev, pitch, data(), pitch_, rowBytes(), h_, hipMemcpyDeviceToDevice, homeStream()));
-}
-
-// GPUImage::copyFrom (src/gpu/image.cu:152-170): same shape and pitch
-// required (compareShapeGPU, :207-214). Enqueued as a copy kernel so that it
-// orders with the filter kernels on the library stream.
-void Image::copyFrom(const Image& src) {
-    if (!sameShape(src) || pitch_ != src.pitch_)
-        throw invalid_argument("GPUImage::copyFrom(): shapes do not match. Required: " + shapeString() +
-                               ", passed: " + src.shapeString());
-    if (!st_ || !src.st_) throw logic_error("GPUImage::copyFrom(): unallocated image");
-    if (rowBytes() == 0 || h_ == 0) return;
-    StreamScope sc(home());
-    CrossStream x{&src};
-    launch_copy2d(src.view(), view(), rowBytes(), stream());
-    check_launch("copyFrom");
-}
-
-// GPUImage::clear (src/gpu/image.cu:172-176).
-void Image::clear() {
-    if (!st_ || !st_->ptr) return;
-    FF_HIP(hipMemset2DAsync(data(), pitch_, 0, rowBytes(), h_, homeStream()));
-}
-
-void Image::swapBuffers(Image& o) {
-    if (!sameShape(o) || pitch_ != o.pitch_) throw logic_error("swapBuffers: shapes differ");
-    if (!st_ || !o.st_) throw logic_error("swapBuffers: unallocated image");
-    std::swap(st_->ptr, o.st_->ptr);
-    std::swap(st_->bytes, o.st_->bytes);
-}
-
-// ------------------------------------------------------------- InputFrames
-
-namespace {
 // Copies host images into pinned memory on a few threads. One thread reads
 // pageable memory at ~20 GB/s (2048x1088 u8 frame: 0.1 ms), which is more
 // than a frame's compute; up to eight split it (parts of at least 256 KiB).
@@ -397,7 +216,254 @@ private:
     int pending_ = 0;
     unsigned long long gen_ = 0;
 };
+
+// Host <-> device copies of a whole image through pinned memory, in chunks
+// of rows: the DMA of one chunk runs while the host threads copy the next
+// (or previous) one between the pinned buffer and the host image, in either
+// row layout. Two chunk buffers of about kChunk bytes and their events,
+// made on first use and kept for the process; g_stage_mu serialises their
+// users.
+constexpr size_t kChunk = 4 << 20;
+std::mutex g_stage_mu;
+struct ChunkStage {
+    char* buf[2] = {nullptr, nullptr};
+    size_t bytes = 0;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+};
+ChunkStage g_chunks;
+
+// Rows per chunk for a device pitch, and the buffers sized for them (call
+// with g_stage_mu held).
+int chunk_rows(size_t pitch, size_t row_bytes) {
+    const int rows = (int)std::max<size_t>(1, kChunk / pitch);
+    const size_t need = (size_t)(rows - 1) * pitch + row_bytes;
+    ChunkStage& c = g_chunks;
+    if (c.bytes < need) {
+        for (int k = 0; k < 2; ++k) {
+            if (c.buf[k]) FF_HIP(hipHostFree(c.buf[k]));
+            c.buf[k] = nullptr;
+        }
+        c.bytes = 0;
+        for (int k = 0; k < 2; ++k) FF_HIP(hipHostMalloc((void**)&c.buf[k], need, hipHostMallocDefault));
+        c.bytes = need;
+    }
+    for (int k = 0; k < 2; ++k)
+        if (!c.ev[k]) FF_HIP(hipEventCreateWithFlags(&c.ev[k], hipEventDisableTiming));
+    return rows;
+}
 }  // namespace
+
+// ----------------------------------------------------------------- storage
+
+Storage::~Storage() {
+    if (ptr) (void)hipFree(ptr);
+}
+
+size_t pitch_for(int width, int depth, int itemSize) {
+    const size_t row = (size_t)width * depth * itemSize;
+    return (row + 255) / 256 * 256;
+}
+
+Image::Image(int height, int width, int depth, int itemSize) : h_(height), w_(width), d_(depth), item_(itemSize) {
+    allocate();
+}
+
+void Image::allocate() {
+    if (h_ < 0 || w_ < 0 || d_ < 0 || item_ < 0) throw invalid_argument("GPUImage: negative dimension " + shapeString());
+    pitch_ = pitch_for(w_, d_, item_);
+    st_ = std::make_shared<Storage>();
+    st_->home = current_box();
+    const size_t bytes = pitch_ * (size_t)h_;
+    if (bytes > 0) {
+        FF_HIP(hipMalloc(&st_->ptr, bytes));
+        st_->bytes = bytes;
+    }
+}
+
+const std::shared_ptr<StreamBox>& Image::home() const {
+    return st_ && st_->home ? st_->home : StreamBox::library();
+}
+
+hipStream_t Image::homeStream() const { return home()->get(); }
+
+std::string Image::shapeString() const {
+    std::ostringstream os;
+    os << "[" << h_ << ", " << w_ << ", " << d_ << "][" << item_ << "]";
+    return os.str();
+}
+
+std::string Image::shapeString(const ff_image_t& img) {
+    std::ostringstream os;
+    os << "[" << img.height << ", " << img.width << ", " << img.depth << "][" << img.itemSize << "]";
+    return os.str();
+}
+
+// GPUImage::upload (src/gpu/image.cu:81-121): an unallocated image takes the
+// host image's shape; otherwise shapes must match.
+void Image::upload(const ff_image_t& host) {
+    if (!st_) {
+        h_ = host.height;
+        w_ = host.width;
+        d_ = host.depth;
+        item_ = (int)host.itemSize;
+        allocate();
+    }
+    upload(host, homeStream());
+}
+
+void Image::upload(const ff_image_t& host, hipStream_t s) {
+    if (!st_) {
+        h_ = host.height;
+        w_ = host.width;
+        d_ = host.depth;
+        item_ = (int)host.itemSize;
+        allocate();
+    }
+    if (!matches(host))
+        throw invalid_argument("GPUImage::upload(): shapes do not match. Required: " + shapeString() +
+                               ", passed: " + shapeString(host));
+    if (rowBytes() == 0 || h_ == 0) return;
+    if (!host.data) throw invalid_argument("GPUImage::upload(): null host data");
+    if (host.pitch < rowBytes()) throw invalid_argument("GPUImage::upload(): host pitch smaller than a row");
+    // host rows -> pinned chunk (host threads) -> device (DMA on s), the
+    // copy of chunk c+1 overlapping the DMA of chunk c
+    std::lock_guard<std::mutex> lk(g_stage_mu);
+    const int rows = chunk_rows(pitch_, rowBytes());
+    ChunkStage& cs = g_chunks;
+    for (int r0 = 0, c = 0; r0 < h_; r0 += rows, ++c) {
+        const int k = c & 1, n = std::min(rows, h_ - r0);
+        if (c >= 2) FF_HIP(hipEventSynchronize(cs.ev[k]));
+        HostCopier::get().copy(cs.buf[k], pitch_, (const char*)host.data + (size_t)r0 * host.pitch, host.pitch,
+                               rowBytes(), n);
+        FF_HIP(hipMemcpyAsync((char*)data() + (size_t)r0 * pitch_, cs.buf[k], (size_t)(n - 1) * pitch_ + rowBytes(),
+                              hipMemcpyHostToDevice, s));
+        FF_HIP(hipEventRecord(cs.ev[k], s));
+    }
+    FF_HIP(hipStreamSynchronize(s));
+}
+
+// GPUImage::download (src/gpu/image.cu:123-150). The reference prints and
+// returns on an unallocated image; here it is an error.
+void Image::download(ff_image_t& host) const {
+    if (!st_) throw logic_error("GPUImage::download(): unallocated image");
+    if (!matches(host))
+        throw invalid_argument("GPUImage::download(): shapes do not match. Required: " + shapeString() +
+                               ", passed: " + shapeString(host));
+    if (rowBytes() == 0 || h_ == 0) return;
+    if (!host.data) throw invalid_argument("GPUImage::download(): null host data");
+    if (host.pitch < rowBytes()) throw invalid_argument("GPUImage::download(): host pitch smaller than a row");
+    // device -> pinned chunk (DMA on the home stream) -> host rows (host
+    // threads), the DMA of chunk c+2 issued as soon as chunk c is copied out
+    hipStream_t s = homeStream();
+    std::lock_guard<std::mutex> lk(g_stage_mu);
+    const int rows = chunk_rows(pitch_, rowBytes());
+    ChunkStage& cs = g_chunks;
+    const int n_chunks = (h_ + rows - 1) / rows;
+    auto issue = [&](int c) {
+        const int r0 = c * rows, n = std::min(rows, h_ - r0);
+        FF_HIP(hipMemcpyAsync(cs.buf[c & 1], (const char*)data() + (size_t)r0 * pitch_,
+                              (size_t)(n - 1) * pitch_ + rowBytes(), hipMemcpyDeviceToHost, s));
+        FF_HIP(hipEventRecord(cs.ev[c & 1], s));
+    };
+    for (int c = 0; c < std::min(2, n_chunks); ++c) issue(c);
+    for (int c = 0; c < n_chunks; ++c) {
+        const int r0 = c * rows, n = std::min(rows, h_ - r0);
+        FF_HIP(hipEventSynchronize(cs.ev[c & 1]));
+        HostCopier::get().copy((char*)host.data + (size_t)r0 * host.pitch, host.pitch, cs.buf[c & 1], pitch_,
+                               rowBytes(), n);
+        if (c + 2 < n_chunks) issue(c + 2);
+    }
+}
+
+void Image::upload_device(const void* dev, size_t pitch) {
+    if (!st_) throw logic_error("GPUImage::upload_device(): unallocated image");
+    if (rowBytes() == 0 || h_ == 0) return;
+    if (!dev) throw invalid_argument("GPUImage::upload_device(): null device pointer");
+    if (pitch < rowBytes()) throw invalid_argument("GPUImage::upload_device(): pitch smaller than a row");
+    const CopyRegion r{dev, data(), pitch, pitch_, rowBytes(), h_};
+    launch_copy_regions(&r, 1, homeStream());
+    check_launch("GPUImage::upload_device");
+}
+
+void Image::download_device(void* dev, size_t pitch) const {
+    if (!st_) throw logic_error("GPUImage::download_device(): unallocated image");
+    if (rowBytes() == 0 || h_ == 0) return;
+    if (!dev) throw invalid_argument("GPUImage::download_device(): null device pointer");
+    if (pitch < rowBytes()) throw invalid_argument("GPUImage::download_device(): pitch smaller than a row");
+    FF_HIP(hipMemcpy2DAsync(dev, pitch, data(), pitch_, rowBytes(), h_, hipMemcpyDeviceToDevice, homeStream()));
+}
+
+// GPUImage::copyFrom (src/gpu/image.cu:152-170): same shape and pitch
+// required (compareShapeGPU, :207-214). Enqueued as a copy kernel so that it
+// orders with the filter kernels on the library stream.
+void Image::copyFrom(const Image& src) {
+    if (!sameShape(src) || pitch_ != src.pitch_)
+        throw invalid_argument("GPUImage::copyFrom(): shapes do not match. Required: " + shapeString() +
+                               ", passed: " + src.shapeString());
+    if (!st_ || !src.st_) throw logic_error("GPUImage::copyFrom(): unallocated image");
+    if (rowBytes() == 0 || h_ == 0) return;
+    StreamScope sc(home());
+    CrossStream x{&src};
+    launch_copy2d(src.view(), view(), rowBytes(), stream());
+    check_launch("copyFrom");
+}
+
+// GPUImage::clear (src/gpu/image.cu:172-176).
+void Image::clear() {
+    if (!st_ || !st_->ptr) return;
+    FF_HIP(hipMemset2DAsync(data(), pitch_, 0, rowBytes(), h_, homeStream()));
+}
+
+void Image::swapBuffers(Image& o) {
+    if (!sameShape(o) || pitch_ != o.pitch_) throw logic_error("swapBuffers: shapes differ");
+    if (!st_ || !o.st_) throw logic_error("swapBuffers: unallocated image");
+    std::swap(st_->ptr, o.st_->ptr);
+    std::swap(st_->bytes, o.st_->bytes);
+}
+
+// ------------------------------------------------------------ WaitingLaunch
+
+namespace {
+std::mutex g_waiting_mu;
+std::vector<hipEvent_t> g_waiting_done;  // per device: end of the last waiting launch
+thread_local int t_waiting_depth = 0;
+}  // namespace
+
+WaitingLaunch::WaitingLaunch(hipStream_t s, bool waits) : s_(s) {
+    if (!waits) return;
+    counted_ = true;
+    if (t_waiting_depth++ > 0) return;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    FF_HIP(hipStreamIsCapturing(s, &cs));
+    if (cs != hipStreamCaptureStatusNone) return;
+    int dev = 0;
+    FF_HIP(hipGetDevice(&dev));
+    g_waiting_mu.lock();
+    locked_ = true;
+    try {
+        if ((int)g_waiting_done.size() <= dev) g_waiting_done.resize(dev + 1, nullptr);
+        if (!g_waiting_done[dev]) FF_HIP(hipEventCreateWithFlags(&g_waiting_done[dev], hipEventDisableTiming));
+        e_ = g_waiting_done[dev];
+        FF_HIP(hipStreamWaitEvent(s, e_, 0));
+    } catch (...) {
+        locked_ = false;
+        g_waiting_mu.unlock();
+        --t_waiting_depth;
+        counted_ = false;
+        throw;
+    }
+}
+
+WaitingLaunch::~WaitingLaunch() {
+    if (!counted_) return;
+    --t_waiting_depth;
+    if (!locked_) return;
+    (void)hipEventRecord(e_, s_);
+    g_waiting_mu.unlock();
+}
+
+// ------------------------------------------------------------- InputFrames
+
 
 struct InputFrames::State {
     Image spare;

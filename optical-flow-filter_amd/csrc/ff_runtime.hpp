@@ -203,6 +203,31 @@ private:
     std::vector<std::shared_ptr<StreamBox>> others_;
 };
 
+// Orders fused frame launches in which some tiles wait for others of the
+// same launch (a pyramid's delta level spins until its top level's tiles
+// have signalled) one after another on the device, whatever stream they
+// are on. Inside one launch the producers' blocks come first, and each XCD
+// dispatches its blocks in order, so a launch alone always drains; but two
+// such launches side by side can each fill an XCD with waiting tiles while
+// the other's producers are queued behind them there, and both waits then
+// give up at their spin limit. Construct one around every such launch (a
+// graph launch included): it makes the stream wait for the previous one and
+// records its own end, under a process-wide lock held until destruction.
+// Nested guards on one thread, and guards while the stream is capturing,
+// do nothing. Launches without waits are not ordered.
+class WaitingLaunch {
+public:
+    WaitingLaunch(hipStream_t s, bool waits);
+    ~WaitingLaunch();
+    WaitingLaunch(const WaitingLaunch&) = delete;
+    WaitingLaunch& operator=(const WaitingLaunch&) = delete;
+
+private:
+    hipStream_t s_ = nullptr;
+    hipEvent_t e_ = nullptr;
+    bool counted_ = false, locked_ = false;
+};
+
 // Host frames for a filter's own input image, double buffered: load()
 // copies a host frame into a pinned host buffer (on a few threads), then
 // enqueues its transfer into the spare device buffer on a copy stream of
