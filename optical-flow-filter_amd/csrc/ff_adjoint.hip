@@ -764,18 +764,20 @@ __global__ void __launch_bounds__(64 * kPairBwdRows)
 //   first X pass: as the second, written for lanes 3..60 (columns
 //     x0 .. x0+57, rows y0 .. y0+R-1).
 constexpr int kQuadBwdCols = 58;
-// Workgroup shape of the launches: 8 waves of 4 rows, 32 staged rows for 24
-// output rows, two 40 KB sets of LDS planes (two workgroups per CU). With
-// 16 waves of one row (16 staged rows for 8 output rows, the gradients
-// staged in LDS too, each input recomputing the terms of the three outputs
-// that read it) the 3840x2160 launch took 249 us.
-constexpr int kQuadBwdWaves = 8, kQuadBwdRowsPerWave = 4;
+// Workgroup shape of the launches: 16 waves of 2 rows, 32 staged rows for
+// 24 output rows, two 40 KB sets of LDS planes; two workgroups per CU, so
+// the kernel is held to 64 VGPRs (8 waves per SIMD). With 16 waves of one
+// row (16 staged rows for 8 output rows, the gradients staged in LDS too,
+// each input recomputing the terms of the three outputs that read it) the
+// 3840x2160 launch took 249 us; with 8 waves of 4 rows (99 VGPRs, 4 waves
+// per SIMD) 253 us.
+constexpr int kQuadBwdWaves = 16, kQuadBwdRowsPerWave = 2;
 template <int NW, int RPW>
 struct QuadBwdShape {
     static constexpr int stage = NW * RPW, rows = stage - 8;
 };
 template <bool PAY, int NW, int RPW>
-__global__ void __launch_bounds__(64 * NW)
+__global__ void __launch_bounds__(64 * NW, NW / 2)
     k_ad_prop_quad_bwd(const float2* fin, const float* sin, const float2* vin, const float2* gfo, const float* gso,
                        const float2* gvo, float2* gfi, float* gsi, float2* gvi, Dims d, float dt, int border,
                        int acc) {

@@ -180,3 +180,40 @@ def test_dlpack_export_of_flow(ffo):
     # a capsule nobody consumes is released by its destructor
     cap = GPUImage((8, 8), 4).__dlpack__()
     del cap
+
+
+def test_pyramids_on_many_streams_interleaved():
+    """Sixteen 2- and 3-level pyramids of 1024x544 on sixteen streams, their
+    frames read in place from HBM and launched interleaved for 40 frames:
+    fused launches whose tiles wait on the same launch's top level are
+    ordered one after another on the device (two side by side can each hold
+    an XCD with waiting tiles while the other's producers queue behind them
+    there), so no wait gives up, and every flow is bitwise the flow of the
+    same filter run alone on one stream."""
+    from flowfilter import _native as N
+    from flowfilter.gpu.image import GPUImage
+    h, w, n, F = 544, 1024, 16, 4
+    frames = np.stack(_blobs(h, w, F, seed=11))
+    dev = GPUImage((F * h, w), 1)
+    dev.upload(frames.reshape(F * h, w))
+    ptr = lambda t: dev.data + (t % F) * h * dev.pitch
+    say("16 pyramids on 16 streams, 40 interleaved frames")
+    fs = [_pyr(h, w, 2 + k % 2) for k in range(n)]
+    assert len({f.stream for f in fs}) == n
+    for t in range(40):
+        for f in fs:
+            f.setImageDevice(ptr(t), dev.pitch)
+            f.compute()
+    got = [f.getFlow() for f in fs]
+    assert all(f.waitTimeouts() == 0 for f in fs)
+    say("the same filters alone")
+    for levels in (2, 3):
+        ref = _pyr(h, w, levels)
+        for t in range(40):
+            ref.setImageDevice(ptr(t), dev.pitch)
+            ref.compute()
+        want = ref.getFlow()
+        for k, f in enumerate(fs):
+            if 2 + k % 2 == levels:
+                assert np.array_equal(got[k], want), (k, levels)
+    N.synchronize()
