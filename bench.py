@@ -201,7 +201,7 @@ def batched_bench(args, N, gf, GPUImage):
         res[mode] = px * args.batched_steps / (ms / 1e3) / 1e6
         if batch is not None:
             res["blocks"] = batch.lastBlocks()
-            res["tile"] = {"waves": batch.tileWaves, "top_rows": batch.tileTopRows}
+            res["tile"] = {"waves": batch.tileWaves, "top_rows": batch.tileTopRows, "rows": batch.tileRows}
             res["timeouts"] = batch.waitTimeouts()
         print("%s: %.3f ms/step, %.1f Mpix/s" % (mode, ms / args.batched_steps, res[mode]), file=sys.stderr,
               flush=True)
@@ -523,7 +523,8 @@ def main():
                                "smooth [2,2], u8 frames, step = compute() of a frame read in place from HBM "
                                "(setImageDevice)",
                    "path": "staged" if args.staged else "fused",
-                   "tile": None if args.staged else {"waves": f.tileWaves, "top_rows": f.tileTopRows},
+                   "tile": None if args.staged else {"waves": f.tileWaves, "top_rows": f.tileTopRows,
+                                                     "rows": f.tileRows},
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,

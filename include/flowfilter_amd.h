@@ -422,6 +422,15 @@ FF_API ff_status ff_filter_get_tile_waves(ff_stage* s, int* out);
  * value (0 before the first fused frame). Same filters as above. */
 FF_API ff_status ff_filter_set_tile_top_rows(ff_stage* s, int rows);
 FF_API ff_status ff_filter_get_tile_top_rows(ff_stage* s, int* out);
+/* Rows per thread of the fused kernel's tiles on the other levels (no
+ * reference counterpart): 0 (the default) tunes it with the tile height, 8
+ * or 4 (16- or 15-wave workgroups only: the same tiles in half the
+ * registers per thread, twice as many waves per CU) fixes it; with the tile
+ * height and the top rows both fixed and this 0, 8. Results do not depend
+ * on it. get returns the fixed or tuned value (0 before the first fused
+ * frame). Same filters as above. */
+FF_API ff_status ff_filter_set_tile_rows(ff_stage* s, int rows);
+FF_API ff_status ff_filter_get_tile_rows(ff_stage* s, int* out);
 
 /* Batched filters (no reference counterpart; replaces a caller's loop of
  * PyramidalFlowFilter::compute over camera streams, e.g. the per-camera

@@ -708,6 +708,15 @@ ff_status ff_filter_get_tile_top_rows(ff_stage* s, int* out) {
         on_fused_filter(s, [&](auto& f) { *out = f.tileTopRows(); });
     });
 }
+ff_status ff_filter_set_tile_rows(ff_stage* s, int rows) {
+    return guard_on(s, [&] { on_fused_filter(s, [&](auto& f) { f.setTileRows(rows); }); });
+}
+ff_status ff_filter_get_tile_rows(ff_stage* s, int* out) {
+    return guard_on(s, [&] {
+        out_ptr(out, "rows");
+        on_fused_filter(s, [&](auto& f) { *out = f.tileRows(); });
+    });
+}
 
 // ------------------------------------------------------------------ batches
 ff_status ff_batch_create(ff_stage** out) {

@@ -52,6 +52,8 @@ public:
     int tileWaves() const { return frame_.waves(); }
     void setTileTopRows(int rows) { frame_.setTopRows(rows); }
     int tileTopRows() const { return frame_.topRows(); }
+    int tileRows() const { return frame_.rows(); }
+    void setTileRows(int rows) { frame_.setRows(rows); }
 
     // Used by PyramidalFlowFilter: the same work without the timer.
     void runImageModel();
@@ -112,6 +114,8 @@ public:
     int tileWaves() const { return frame_.waves(); }
     void setTileTopRows(int rows) { frame_.setTopRows(rows); }
     int tileTopRows() const { return frame_.topRows(); }
+    int tileRows() const { return frame_.rows(); }
+    void setTileRows(int rows) { frame_.setRows(rows); }
 
     void runImageModel();
     void runPropagation();
@@ -217,6 +221,8 @@ public:
     int tileWaves() const { return frame_.waves(); }
     void setTileTopRows(int rows) { frame_.setTopRows(rows); }
     int tileTopRows() const { return frame_.topRows(); }
+    int tileRows() const { return frame_.rows(); }
+    void setTileRows(int rows) { frame_.setRows(rows); }
 
 private:
     friend class BatchedFlowFilter;
@@ -287,6 +293,8 @@ public:
     int tileWaves() const { return frame_.waves(); }
     void setTileTopRows(int rows) { frame_.setTopRows(rows); }
     int tileTopRows() const { return frame_.topRows(); }
+    int tileRows() const { return frame_.rows(); }
+    void setTileRows(int rows) { frame_.setRows(rows); }
     // Workgroups of the last fused launch (0 before the first).
     int lastBlocks() const { return last_blocks_; }
 

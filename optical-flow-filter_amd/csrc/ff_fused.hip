@@ -866,17 +866,26 @@ struct GroupEnds {
     int e[6];
 };
 
-// Rows per thread of a tile body: 8, except for top levels reduced on load
-// (the top level of a pyramid, which the level below waits for), which use
-// RTOP, a parameter of the kernel chosen per launch by the tuner among
-// kTopRowChoices: with 4 the tiles are more and shorter and finish sooner
-// (a pyramid launched alone is faster), with 8 they carry less halo (a
-// 64-stream batch is 1.4 % faster).
+// Rows per thread of a tile body: RLOW, except for top levels reduced on
+// load (the top level of a pyramid, which the level below waits for), which
+// use RTOP; both are parameters of the kernel, chosen per launch by the
+// tuner among kShapes. RTOP 4: the top level's tiles are more and shorter
+// and finish sooner (a pyramid launched alone is faster); 8: they carry less
+// halo (a 64-stream batch is 1.4 % faster). RLOW 4 with 16 waves covers
+// the same 64-row tile as RLOW 8 with 8 waves, in half the registers per
+// thread: that kernel is held to 64 VGPRs, so two 16-wave workgroups (32
+// waves) fit a CU instead of two 8-wave ones, for about 8 % more VALU
+// instructions per pixel (each wave exchanges its edge rows for half as
+// many rows).
 constexpr int kRows = 8;
 constexpr int kTopRowChoices[] = {4, 8};
+struct TileShape {
+    int rtop, rlow;
+};
+constexpr TileShape kShapes[] = {{4, 8}, {8, 8}, {4, 4}};
 
-template <int RTOP>
-__global__ void __launch_bounds__(1024)
+template <int RTOP, int RLOW = kRows>
+__global__ void __launch_bounds__(1024, RLOW == 4 ? 8 : 1)
     k_fused_frame(const LevelJob* __restrict__ jobs, const int* __restrict__ block_job, unsigned* counters,
                   unsigned epoch, int timeout_idx, GroupEnds ends, int nw, WaitArgs wa, FrameArgs frames) {
     // row exchange between the waves of the workgroup (2 * nw * 10 * LANES floats)
@@ -888,13 +897,13 @@ __global__ void __launch_bounds__(1024)
     // uniform, so is the selection)
     ImgSrc fr{nullptr, 0};
     if (p.frame_slot >= 0) fr = ImgSrc{frames.ptr[p.frame_slot], frames.pitch[p.frame_slot]};
-    if (b < (unsigned)ends.e[0]) level_tile<false, IMG_F32, kRows>(p, tile, xch, nw, counters, epoch, timeout_idx, wa, fr);
-    else if (b < (unsigned)ends.e[1]) level_tile<false, IMG_U8, kRows>(p, tile, xch, nw, counters, epoch, timeout_idx, wa, fr);
+    if (b < (unsigned)ends.e[0]) level_tile<false, IMG_F32, RLOW>(p, tile, xch, nw, counters, epoch, timeout_idx, wa, fr);
+    else if (b < (unsigned)ends.e[1]) level_tile<false, IMG_U8, RLOW>(p, tile, xch, nw, counters, epoch, timeout_idx, wa, fr);
     else if (b < (unsigned)ends.e[2]) level_tile<false, IMG_U8_DOWN, RTOP>(p, tile, xch, nw, counters, epoch, timeout_idx, wa, fr);
     else if (b < (unsigned)ends.e[3]) level_tile<false, IMG_U8_DOWN2, RTOP>(p, tile, xch, nw, counters, epoch, timeout_idx, wa, fr);
-    else if (b < (unsigned)ends.e[4]) level_tile<true, IMG_F32, kRows>(p, tile, xch, nw, counters, epoch, timeout_idx, wa, fr);
-    else if (b < (unsigned)ends.e[5]) level_tile<true, IMG_U8, kRows>(p, tile, xch, nw, counters, epoch, timeout_idx, wa, fr);
-    else level_tile<true, IMG_U8_DOWN, kRows>(p, tile, xch, nw, counters, epoch, timeout_idx, wa, fr);
+    else if (b < (unsigned)ends.e[4]) level_tile<true, IMG_F32, RLOW>(p, tile, xch, nw, counters, epoch, timeout_idx, wa, fr);
+    else if (b < (unsigned)ends.e[5]) level_tile<true, IMG_U8, RLOW>(p, tile, xch, nw, counters, epoch, timeout_idx, wa, fr);
+    else level_tile<true, IMG_U8_DOWN, RLOW>(p, tile, xch, nw, counters, epoch, timeout_idx, wa, fr);
 }
 
 __global__ void k_clear_counters(unsigned* c, int n) {
@@ -913,7 +922,7 @@ constexpr int kWaveChoices[] = {16, 15, 8, 5, 4};
 bool top_rows_body(const LevelArgs& a) { return !a.delta && (a.img_mode == IMG_U8_DOWN || a.img_mode == IMG_U8_DOWN2); }
 
 // Rows per thread of a job's tile body (see k_fused_frame).
-int rows_per_thread(const LevelArgs& a, int rtop) { return top_rows_body(a) ? rtop : kRows; }
+int rows_per_thread(const LevelArgs& a, int rtop, int rlow = kRows) { return top_rows_body(a) ? rtop : rlow; }
 
 size_t lds_bytes(int nw) { return (size_t)2 * nw * 10 * LANES * sizeof(float); }
 
@@ -1009,7 +1018,7 @@ struct FusedFrame::Impl {
         DeviceArray dev;
         int njobs = 0, blocks = 0, nw = 0;
         GroupEnds ends{};
-        int rtop = 0;
+        int rtop = 0, rlow = kRows;
         size_t map_offset = 0;
         unsigned long long token = 0;
         bool waits = false;  // some job waits for another
@@ -1026,13 +1035,15 @@ struct FusedFrame::Impl {
     // table they count for (0: clear before the next launch)
     unsigned epoch[2] = {0, 0};
     unsigned long long counted[2] = {0, 0};
-    int fixed = 0, fixed_rtop = 0;  // waves per workgroup, top-level rows per thread; 0: autotune
+    // waves per workgroup, top-level rows per thread, other rows per thread;
+    // 0: autotune
+    int fixed = 0, fixed_rtop = 0, fixed_rlow = 0;
     std::vector<int> tuned_for;
-    int tuned = 0, tuned_rtop = 0;
+    int tuned = 0, tuned_rtop = 0, tuned_rlow = kRows;
     std::shared_ptr<WaitWatch> watch = std::make_shared<WaitWatch>();
     int test_spins = 0;  // > 0: the waits give up after this many polls (test hook)
 
-    int table(const std::vector<LevelArgs>& jobs, int nw, int rtop, hipStream_t s);
+    int table(const std::vector<LevelArgs>& jobs, int nw, int rtop, int rlow, hipStream_t s);
     unsigned* region(int si) const { return (unsigned*)counters.data() + (size_t)si * 2 * npairs; }
     int timeout_word() const { return 4 * npairs; }
     void clear_counters(int si, hipStream_t s) const;
@@ -1052,7 +1063,7 @@ FusedFrame& FusedFrame::operator=(FusedFrame&& o) noexcept {
 // thread on reduced top levels: found among the two resident tables or
 // built and uploaded (after waiting for stream s, the stream of every launch
 // of this FusedFrame, since the slot it replaces may be in use).
-int FusedFrame::Impl::table(const std::vector<LevelArgs>& jobs, int nw, int rtop, hipStream_t s) {
+int FusedFrame::Impl::table(const std::vector<LevelArgs>& jobs, int nw, int rtop, int rlow, hipStream_t s) {
     // launch order: grouped by tile body, stable within a group; every top
     // level precedes every delta level, so producers precede consumers
     std::vector<int> order(jobs.size()), pos(jobs.size());
@@ -1065,7 +1076,7 @@ int FusedFrame::Impl::table(const std::vector<LevelArgs>& jobs, int nw, int rtop
     int ends[kGroups - 1] = {0, 0, 0, 0, 0, 0};
     for (size_t i = 0; i < jobs.size(); ++i) {
         const LevelArgs& a = jobs[order[i]];
-        const int rows = nw * rows_per_thread(a, rtop);
+        const int rows = nw * rows_per_thread(a, rtop, rlow);
         if (!fused_level_supported(a) || !tiles_with(a, rows))
             throw logic_error("fused frame: unsupported level parameters");
         LevelJob& p = rec[i];
@@ -1141,6 +1152,7 @@ int FusedFrame::Impl::table(const std::vector<LevelArgs>& jobs, int nw, int rtop
     }
     key.insert(key.end(), reinterpret_cast<const char*>(&nw), reinterpret_cast<const char*>(&nw) + sizeof(int));
     key.insert(key.end(), reinterpret_cast<const char*>(&rtop), reinterpret_cast<const char*>(&rtop) + sizeof(int));
+    key.insert(key.end(), reinterpret_cast<const char*>(&rlow), reinterpret_cast<const char*>(&rlow) + sizeof(int));
 
     for (int i = 0; i < 2; ++i)
         if (slot[i].key == key) return i;
@@ -1169,6 +1181,7 @@ int FusedFrame::Impl::table(const std::vector<LevelArgs>& jobs, int nw, int rtop
     t.blocks = blocks;
     t.nw = nw;
     t.rtop = rtop;
+    t.rlow = rlow;
     t.map_offset = map_offset;
     t.waits = false;
     for (const LevelJob& r : rec) t.waits = t.waits || r.wait_idx >= 0;
@@ -1191,8 +1204,11 @@ void FusedFrame::Impl::run(int ti, hipStream_t s, const FrameRef* frames, int nf
         (const void*)k_fused_frame<4>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes(kMaxWaves));
     static const hipError_t lds_ok8 = hipFuncSetAttribute(
         (const void*)k_fused_frame<8>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes(kMaxWaves));
+    static const hipError_t lds_ok44 = hipFuncSetAttribute(
+        (const void*)k_fused_frame<4, 4>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes(kMaxWaves));
     if (lds_ok4 != hipSuccess) throw_hip(lds_ok4, "hipFuncSetAttribute", __FILE__, __LINE__);
     if (lds_ok8 != hipSuccess) throw_hip(lds_ok8, "hipFuncSetAttribute", __FILE__, __LINE__);
+    if (lds_ok44 != hipSuccess) throw_hip(lds_ok44, "hipFuncSetAttribute", __FILE__, __LINE__);
     // Sequence number of this launch for its slot's completion counters. A
     // launch recorded into a graph clears them itself and counts as the
     // first, so every replay is self-contained; direct launches count up
@@ -1227,7 +1243,11 @@ void FusedFrame::Impl::run(int ti, hipStream_t s, const FrameRef* frames, int nf
     const char* base = (const char*)t.dev.data();
     // the timeout word, relative to this slot's region
     const int timeout_idx = timeout_word() - si * 2 * npairs;
-    if (t.rtop == 8)
+    if (t.rlow == 4)
+        hipLaunchKernelGGL((k_fused_frame<4, 4>), dim3(t.blocks), dim3(t.nw * LANES), lds_bytes(t.nw), s,
+                           (const LevelJob*)base, (const int*)(base + t.map_offset), region(si), e, timeout_idx, t.ends,
+                           t.nw, wa, fa);
+    else if (t.rtop == 8)
         hipLaunchKernelGGL(k_fused_frame<8>, dim3(t.blocks), dim3(t.nw * LANES), lds_bytes(t.nw), s,
                            (const LevelJob*)base, (const int*)(base + t.map_offset), region(si), e, timeout_idx, t.ends,
                            t.nw, wa, fa);
@@ -1246,20 +1266,25 @@ void FusedFrame::Impl::run(int ti, hipStream_t s, const FrameRef* frames, int nf
 void FusedFrame::Impl::tune(const std::vector<LevelArgs>& jobs, hipStream_t s) {
     bool reduced_top = false;
     for (const LevelArgs& a : jobs) reduced_top = reduced_top || top_rows_body(a);
-    int best_nw = 0, best_rtop = 0;
+    int best_nw = 0, best_rtop = 0, best_rlow = kRows;
     float best_ms = 0.0f;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     FF_HIP(hipEventCreate(&e0));
     FF_HIP(hipEventCreate(&e1));
     try {
-        for (int rtop : kTopRowChoices) {
+        for (const TileShape& sh : kShapes) {
+            const int rtop = sh.rtop, rlow = sh.rlow;
             if (fixed_rtop ? rtop != fixed_rtop : (!reduced_top && rtop != kTopRowChoices[0])) continue;
+            if (fixed_rlow && rlow != fixed_rlow) continue;
             for (int nw : kWaveChoices) {
                 if (fixed && nw != fixed) continue;
+                // four rows per thread only in the tallest workgroups (the
+                // 64-row tiles of eight waves of eight rows)
+                if (rlow == 4 && nw < 15) continue;
                 bool ok = true;
-                for (const LevelArgs& a : jobs) ok = ok && tiles_with(a, nw * rows_per_thread(a, rtop));
+                for (const LevelArgs& a : jobs) ok = ok && tiles_with(a, nw * rows_per_thread(a, rtop, rlow));
                 if (!ok) continue;
-                const int t = table(jobs, nw, rtop, s);
+                const int t = table(jobs, nw, rtop, rlow, s);
                 run(t, s);  // warm-up
                 constexpr int kReps = 3;
                 FF_HIP(hipEventRecord(e0, s));
@@ -1271,6 +1296,7 @@ void FusedFrame::Impl::tune(const std::vector<LevelArgs>& jobs, hipStream_t s) {
                 if (!best_nw || ms < best_ms) {
                     best_nw = nw;
                     best_rtop = rtop;
+                    best_rlow = rlow;
                     best_ms = ms;
                 }
             }
@@ -1285,24 +1311,32 @@ void FusedFrame::Impl::tune(const std::vector<LevelArgs>& jobs, hipStream_t s) {
     if (!best_nw) throw logic_error("fused frame: no tiling fits the levels");
     tuned = best_nw;
     tuned_rtop = best_rtop;
+    tuned_rlow = best_rlow;
 }
 
 int FusedFrame::prepare(const std::vector<LevelArgs>& jobs, hipStream_t s) {
     if (jobs.empty()) throw invalid_argument("fused frame: no levels");
-    int nw = impl_->fixed, rtop = impl_->fixed_rtop;
+    // the rows below the top level default to eight when the tile height
+    // and the top rows are both fixed; otherwise the tuner picks them too
+    int nw = impl_->fixed, rtop = impl_->fixed_rtop, rlow = impl_->fixed_rlow ? impl_->fixed_rlow : kRows;
+    if (rlow == 4 && nw && nw < 15)
+        throw invalid_argument("fused frame: four rows per thread need 16 or 15 waves per workgroup, got " +
+                               std::to_string(nw));
     if (!nw || !rtop) {
         // tuned per set of level shapes and per fixed part of the choice
         std::vector<int> k = tune_key(jobs);
         k.push_back(impl_->fixed);
         k.push_back(impl_->fixed_rtop);
+        k.push_back(impl_->fixed_rlow);
         if (k != impl_->tuned_for || !impl_->tuned) {
             impl_->tune(jobs, s);
             impl_->tuned_for = std::move(k);
         }
         nw = impl_->tuned;
         rtop = impl_->tuned_rtop;
+        rlow = impl_->tuned_rlow;
     }
-    return impl_->table(jobs, nw, rtop, s);
+    return impl_->table(jobs, nw, rtop, rlow, s);
 }
 
 void FusedFrame::launch(int table, hipStream_t s, const FrameRef* frames, int nframes) {
@@ -1341,6 +1375,17 @@ void FusedFrame::setTopRows(int rows) {
 }
 
 int FusedFrame::topRows() const { return impl_->fixed_rtop ? impl_->fixed_rtop : impl_->tuned_rtop; }
+int FusedFrame::rows() const {
+    if (impl_->fixed_rlow) return impl_->fixed_rlow;
+    return impl_->fixed && impl_->fixed_rtop ? kRows : impl_->tuned_rlow;
+}
+
+void FusedFrame::setRows(int rows) {
+    if (rows != 0 && rows != 4 && rows != 8)
+        throw invalid_argument("fused frame: rows per thread must be 0 (autotune), 4 or 8, got " +
+                               std::to_string(rows));
+    impl_->fixed_rlow = rows;
+}
 
 unsigned long long FusedFrame::timeouts() const { return impl_->watch->read(); }
 

@@ -219,6 +219,12 @@ public:
     // reports the fixed or tuned value (0 before tuning).
     void setTopRows(int rows);
     int topRows() const;
+    // Rows per thread of the other levels' tiles: 0 (autotune with the
+    // tile height, the default; 8 when both choices above are fixed), 4
+    // (16 or 15 waves per workgroup only) or 8. rows() reads the fixed or
+    // tuned value.
+    void setRows(int rows);
+    int rows() const;
     // Waits that gave up after the spin limit (a scheduling fault; results
     // of such a frame are not valid). Waits for the stream of the launches.
     unsigned long long timeouts() const;

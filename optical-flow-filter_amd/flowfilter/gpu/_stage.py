@@ -87,6 +87,17 @@ class Stage(object):
     def tileTopRows(self, v):
         N.call("ff_filter_set_tile_top_rows", self.handle, int(v))
 
+    @property
+    def tileRows(self):
+        """Filters only: rows per thread of the fused kernel's tiles below
+        the top level (0 = tuned with the tile height, the default; or 4
+        with 16 or 15 waves, 8). Reads the fixed or tuned value."""
+        return N.get_int("ff_filter_get_tile_rows", self.handle)
+
+    @tileRows.setter
+    def tileRows(self, v):
+        N.call("ff_filter_set_tile_rows", self.handle, int(v))
+
 
 def _hnd(img):
     return img.handle if img is not None else None

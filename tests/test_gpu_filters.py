@@ -449,19 +449,25 @@ def test_halo_beyond_a_tile_runs_stage_kernels(ffo, levels):
         check_frame(t, g.getFlow(), o.downloadFlow(), "maxflow 40")
 
 
+_TILES = [(waves, top_rows, 8) for waves in (16, 15, 8, 5, 4) for top_rows in (4, 8)] + \
+    [(waves, 4, 4) for waves in (16, 15)]
+
+
 @pytest.mark.parametrize("levels", [2, 3])
-@pytest.mark.parametrize("top_rows", [4, 8])
-@pytest.mark.parametrize("waves", [16, 15, 8, 5, 4])
-def test_fused_tile_heights_match_oracle(ffo, waves, top_rows, levels):
+@pytest.mark.parametrize("waves,top_rows,rows", _TILES)
+def test_fused_tile_heights_match_oracle(ffo, waves, top_rows, rows, levels):
     """Every tile height of the fused frame kernel, with either row count on
-    the reduced top level, gives the oracle's flow."""
+    the reduced top level and, in 16- and 15-wave workgroups, four rows per
+    thread on every level (the kernel held to 64 VGPRs), gives the oracle's
+    flow."""
     from flowfilter import _native as N
     h, w = 300, 410
-    say("tile waves", waves, "top rows", top_rows, "levels", levels)
+    say("tile waves", waves, "top rows", top_rows, "rows", rows, "levels", levels)
     g, o = _pyr_pair(ffo, h, w, levels, [10.0, 30.0, 60.0][:levels], 4.0, [2, 2, 1][:levels], fused=True)
     g.tileWaves = waves
     g.tileTopRows = top_rows
-    assert (g.tileWaves, g.tileTopRows) == (waves, top_rows)
+    g.tileRows = rows
+    assert (g.tileWaves, g.tileTopRows, g.tileRows) == (waves, top_rows, rows)
     for t, img in enumerate(seq_blobs(h, w, 5)):
         g.loadImage(img)
         N.profile_begin("fused_frame")
@@ -469,7 +475,7 @@ def test_fused_tile_heights_match_oracle(ffo, waves, top_rows, levels):
         _, n = N.profile_end()
         assert n == (0 if t == 0 else 1)
         o.compute(img)
-        check_frame(t, g.getFlow(), o.downloadFlow(), "waves %d top rows %d" % (waves, top_rows))
+        check_frame(t, g.getFlow(), o.downloadFlow(), "waves %d top rows %d rows %d" % (waves, top_rows, rows))
 
 
 def test_fused_autotune_picks_a_tiling(ffo):
@@ -482,8 +488,11 @@ def test_fused_autotune_picks_a_tiling(ffo):
         check_frame(t, g.getFlow(), o.downloadFlow(), "autotuned")
     assert g.tileWaves in (16, 15, 8, 5, 4)
     assert g.tileTopRows in (4, 8)
+    assert g.tileRows == 8 or (g.tileRows == 4 and g.tileWaves in (16, 15))
     from flowfilter import errors
     with pytest.raises(errors.InvalidArgument):
         g.tileWaves = 7
     with pytest.raises(errors.InvalidArgument):
         g.tileTopRows = 6
+    with pytest.raises(errors.InvalidArgument):
+        g.tileRows = 2
