@@ -362,6 +362,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=30)
     ap.add_argument("--frames", type=int, default=24, help="distinct synthetic frames kept in HBM")
     ap.add_argument("--staged", action="store_true", help="one launch per stage kernel instead of fused")
+    ap.add_argument("--tile", default="", help="fix the fused kernel's tile shape: waves,top_rows,rows "
+                                               "(e.g. 16,4,4); default: tuned")
     ap.add_argument("--no-graphs", action="store_true", help="launch the fused kernels one by one")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
@@ -416,6 +418,9 @@ def main():
     f.smoothIterations = SMOOTH
     f.fused = not args.staged
     f.graphs = not args.no_graphs
+    if args.tile:
+        # a fixed tile shape (profiling runs); by default the tuner picks it
+        f.tileWaves, f.tileTopRows, f.tileRows = (int(v) for v in args.tile.split(","))
 
     def step(i):
         f.setImageDevice(base + (i % F) * H * pitch, pitch)
