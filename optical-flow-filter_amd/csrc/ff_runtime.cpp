@@ -33,7 +33,12 @@ void check_launch(const char* what) {
 
 StreamBox::~StreamBox() {
     if (ev_) (void)hipEventDestroy(ev_);
-    if (own_) (void)hipStreamDestroy(own_);
+    if (own_) {
+        // finish first: a stream's handle identifies it to WaitingLaunch,
+        // and a new stream may be given the same handle
+        (void)hipStreamSynchronize(own_);
+        (void)hipStreamDestroy(own_);
+    }
 }
 
 std::shared_ptr<StreamBox> StreamBox::create() {
@@ -424,27 +429,49 @@ void Image::swapBuffers(Image& o) {
 // ------------------------------------------------------------ WaitingLaunch
 
 namespace {
+// Per device: the stream of the last waiting launch, and, once waiting
+// launches have come from more than one stream, an event recorded after
+// each (a record costs about 5 us between two launches of one filter, so a
+// process that launches from one stream records nothing).
+struct WaitingState {
+    hipStream_t last = nullptr;  // the null stream is a stream too
+    hipEvent_t done = nullptr;
+    bool seen = false, multi = false;
+};
 std::mutex g_waiting_mu;
-std::vector<hipEvent_t> g_waiting_done;  // per device: end of the last waiting launch
+std::vector<WaitingState> g_waiting;
 thread_local int t_waiting_depth = 0;
+
+bool capturing(hipStream_t s) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    FF_HIP(hipStreamIsCapturing(s, &cs));
+    return cs != hipStreamCaptureStatusNone;
+}
 }  // namespace
 
 WaitingLaunch::WaitingLaunch(hipStream_t s, bool waits) : s_(s) {
     if (!waits) return;
     counted_ = true;
     if (t_waiting_depth++ > 0) return;
-    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-    FF_HIP(hipStreamIsCapturing(s, &cs));
-    if (cs != hipStreamCaptureStatusNone) return;
-    int dev = 0;
-    FF_HIP(hipGetDevice(&dev));
+    if (capturing(s)) return;
+    FF_HIP(hipGetDevice(&dev_));
     g_waiting_mu.lock();
     locked_ = true;
     try {
-        if ((int)g_waiting_done.size() <= dev) g_waiting_done.resize(dev + 1, nullptr);
-        if (!g_waiting_done[dev]) FF_HIP(hipEventCreateWithFlags(&g_waiting_done[dev], hipEventDisableTiming));
-        e_ = g_waiting_done[dev];
-        FF_HIP(hipStreamWaitEvent(s, e_, 0));
+        if ((int)g_waiting.size() <= dev_) g_waiting.resize(dev_ + 1);
+        WaitingState& w = g_waiting[dev_];
+        if (w.seen && w.last != s) {
+            if (!w.done) FF_HIP(hipEventCreateWithFlags(&w.done, hipEventDisableTiming));
+            if (!w.multi) {
+                // the first launch from a second stream: nothing marks the
+                // end of the launches on the first one, and that stream may
+                // since have been destroyed, so wait for the device once
+                FF_HIP(hipDeviceSynchronize());
+                w.multi = true;
+            } else {
+                FF_HIP(hipStreamWaitEvent(s, w.done, 0));
+            }
+        }
     } catch (...) {
         locked_ = false;
         g_waiting_mu.unlock();
@@ -458,7 +485,10 @@ WaitingLaunch::~WaitingLaunch() {
     if (!counted_) return;
     --t_waiting_depth;
     if (!locked_) return;
-    (void)hipEventRecord(e_, s_);
+    WaitingState& w = g_waiting[dev_];
+    if (w.multi) (void)hipEventRecord(w.done, s_);
+    w.last = s_;
+    w.seen = true;
     g_waiting_mu.unlock();
 }
 

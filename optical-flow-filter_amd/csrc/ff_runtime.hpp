@@ -211,10 +211,17 @@ private:
 // such launches side by side can each fill an XCD with waiting tiles while
 // the other's producers are queued behind them there, and both waits then
 // give up at their spin limit. Construct one around every such launch (a
-// graph launch included): it makes the stream wait for the previous one and
-// records its own end, under a process-wide lock held until destruction.
-// Nested guards on one thread, and guards while the stream is capturing,
-// do nothing. Launches without waits are not ordered.
+// graph launch included): under a process-wide lock held until
+// destruction, it makes the stream wait for the previous such launch when
+// that was on another stream. Launches from one stream are ordered by the
+// stream itself, so ends are recorded (an event per launch, about 5 us
+// between two frames) only after a second stream has launched; that first
+// switch waits on the host for the device, once per device. A stream handed
+// to the library is identified by its handle: it must not be destroyed
+// while a waiting launch on it is pending (the library synchronizes its own
+// streams before destroying them). Nested guards on one thread, and guards
+// while the stream is capturing, do nothing. Launches without waits are not
+// ordered.
 class WaitingLaunch {
 public:
     WaitingLaunch(hipStream_t s, bool waits);
@@ -224,7 +231,7 @@ public:
 
 private:
     hipStream_t s_ = nullptr;
-    hipEvent_t e_ = nullptr;
+    int dev_ = 0;
     bool counted_ = false, locked_ = false;
 };
 
