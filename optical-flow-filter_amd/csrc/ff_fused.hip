@@ -123,25 +123,27 @@ __device__ __forceinline__ float2 ld2(Rsrc r, int off) {
 }
 __device__ __forceinline__ unsigned ldb(Rsrc r, int off) { return __builtin_amdgcn_raw_buffer_load_b8(r, off, 0, 0); }
 __device__ __forceinline__ unsigned ldh(Rsrc r, int off) { return __builtin_amdgcn_raw_buffer_load_b16(r, off, 0, 0); }
+// Cache policy sc1: stores write through this XCD's L2 and such loads do
+// not hit a stale line in its caches (device-coherent, as for agent-scope
+// atomics).
+constexpr int kCoherent = 16;
+// Every store writes through. A frame's state is written once and read by
+// the next launch only, so dirty lines kept in L2 save nothing, and they
+// cost time: 55.8 us per 2048x1088 two-level frame with write-back stores,
+// 52.8 us with write-through ones (streaming stores and loads, policy nt,
+// were slower than either). The write-through also publishes the top
+// level's unsmoothed flow to the tiles of the level below, which read it
+// within the same launch with ld2_coherent.
 __device__ __forceinline__ void st1(Rsrc r, int off, float v) {
-    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, off, 0, 0);
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, off, 0, kCoherent);
 }
 __device__ __forceinline__ void st2(Rsrc r, int off, float a, float b) {
     const u32x2 v = {__float_as_uint(a), __float_as_uint(b)};
-    __builtin_amdgcn_raw_buffer_store_b64(v, r, off, 0, 0);
+    __builtin_amdgcn_raw_buffer_store_b64(v, r, off, 0, kCoherent);
 }
-// Device-coherent forms (cache policy sc1, as for agent-scope atomics): the
-// store is written through to memory and the load does not hit a stale line
-// in this XCD's caches. Used for the one buffer that one level writes and
-// another reads within the same launch (the top level's unsmoothed flow).
-constexpr int kCoherent = 16;
 __device__ __forceinline__ float2 ld2_coherent(Rsrc r, int off) {
     const u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, kCoherent);
     return make_float2(__uint_as_float(v.x), __uint_as_float(v.y));
-}
-__device__ __forceinline__ void st2_coherent(Rsrc r, int off, float a, float b) {
-    const u32x2 v = {__float_as_uint(a), __float_as_uint(b)};
-    __builtin_amdgcn_raw_buffer_store_b64(v, r, off, 0, kCoherent);
 }
 
 // ---------------------------------------------------------- lane shifts
@@ -423,7 +425,7 @@ __device__ __forceinline__ void model_update(State<DELTA, R>& s, const float (&i
             ox = not_finite(ox) ? 0.0f : ox;
             oy = not_finite(oy) ? 0.0f : oy;
             st1(ra0, offa[r], a0);
-            if (p.flow_upd) st2_coherent(rfu, offf[r], ox, oy);
+            if (p.flow_upd) st2(rfu, offf[r], ox, oy);
         }
         s.fu[r] = ox;
         s.fv[r] = oy;
