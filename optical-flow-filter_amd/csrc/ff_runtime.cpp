@@ -10,6 +10,7 @@
 #include <condition_variable>
 #include <cstring>
 #include <mutex>
+#include <set>
 #include <sstream>
 #include <thread>
 #include <vector>
@@ -31,12 +32,20 @@ void check_launch(const char* what) {
 
 // ------------------------------------------------------------------ stream
 
+namespace {
+// The streams of live StreamBoxes, which WaitingLaunch may mark after the
+// fact (defined with WaitingLaunch below).
+void owned_insert(hipStream_t s);
+void owned_erase(hipStream_t s);
+}  // namespace
+
 StreamBox::~StreamBox() {
     if (ev_) (void)hipEventDestroy(ev_);
     if (own_) {
-        // finish first: a stream's handle identifies it to WaitingLaunch,
-        // and a new stream may be given the same handle
+        // finish first: then no launch on it is left to order, and a new
+        // stream may be given the same handle
         (void)hipStreamSynchronize(own_);
+        owned_erase(own_);
         (void)hipStreamDestroy(own_);
     }
 }
@@ -44,6 +53,7 @@ StreamBox::~StreamBox() {
 std::shared_ptr<StreamBox> StreamBox::create() {
     auto b = std::make_shared<StreamBox>();
     FF_HIP(hipStreamCreateWithFlags(&b->own_, hipStreamNonBlocking));
+    owned_insert(b->own_);
     return b;
 }
 
@@ -429,18 +439,46 @@ void Image::swapBuffers(Image& o) {
 // ------------------------------------------------------------ WaitingLaunch
 
 namespace {
-// Per device: the stream of the last waiting launch, and, once waiting
-// launches have come from more than one stream, an event recorded after
-// each (a record costs about 5 us between two launches of one filter, so a
-// process that launches from one stream records nothing).
+// What marks the end of a device's last waiting launch:
+//  NONE:     nothing needs marking (no such launch, or its stream finished);
+//  PENDING:  it ran on a live library stream and is not marked yet: an event
+//            recorded on that stream when another stream launches marks it
+//            (with whatever was queued there since, which only orders more);
+//  RECORDED: `done` was recorded right after it;
+//  UNMARKED: it ran on a caller's stream, which may since have been
+//            destroyed, and was not marked.
+enum class LastMark { NONE, PENDING, RECORDED, UNMARKED };
 struct WaitingState {
     hipStream_t last = nullptr;  // the null stream is a stream too
     hipEvent_t done = nullptr;
-    bool seen = false, multi = false;
+    LastMark mark = LastMark::NONE;
+    bool eager = false;  // record after every launch on a caller's stream
 };
-std::mutex g_waiting_mu;
-std::vector<WaitingState> g_waiting;
+// Kept for the process (never destroyed), like the library stream: boxes
+// may be destroyed during the process's exit.
+struct Waiting {
+    std::mutex mu;
+    std::vector<WaitingState> dev;
+    std::set<hipStream_t> owned;
+};
+Waiting& waiting() {
+    static Waiting* w = new Waiting;
+    return *w;
+}
 thread_local int t_waiting_depth = 0;
+
+void owned_insert(hipStream_t s) {
+    Waiting& g = waiting();
+    std::lock_guard<std::mutex> lk(g.mu);
+    g.owned.insert(s);
+}
+void owned_erase(hipStream_t s) {
+    Waiting& g = waiting();
+    std::lock_guard<std::mutex> lk(g.mu);
+    g.owned.erase(s);
+    for (WaitingState& w : g.dev)
+        if (w.last == s && w.mark != LastMark::RECORDED) w.mark = LastMark::NONE;
+}
 
 bool capturing(hipStream_t s) {
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
@@ -455,26 +493,30 @@ WaitingLaunch::WaitingLaunch(hipStream_t s, bool waits) : s_(s) {
     if (t_waiting_depth++ > 0) return;
     if (capturing(s)) return;
     FF_HIP(hipGetDevice(&dev_));
-    g_waiting_mu.lock();
+    Waiting& g = waiting();
+    g.mu.lock();
     locked_ = true;
     try {
-        if ((int)g_waiting.size() <= dev_) g_waiting.resize(dev_ + 1);
-        WaitingState& w = g_waiting[dev_];
-        if (w.seen && w.last != s) {
+        if ((int)g.dev.size() <= dev_) g.dev.resize(dev_ + 1);
+        WaitingState& w = g.dev[dev_];
+        if (w.last != s && w.mark != LastMark::NONE) {
             if (!w.done) FF_HIP(hipEventCreateWithFlags(&w.done, hipEventDisableTiming));
-            if (!w.multi) {
-                // the first launch from a second stream: nothing marks the
-                // end of the launches on the first one, and that stream may
-                // since have been destroyed, so wait for the device once
-                FF_HIP(hipDeviceSynchronize());
-                w.multi = true;
-            } else {
+            if (w.mark == LastMark::PENDING && !capturing(w.last)) {
+                FF_HIP(hipEventRecord(w.done, w.last));
                 FF_HIP(hipStreamWaitEvent(s, w.done, 0));
+            } else if (w.mark == LastMark::RECORDED) {
+                FF_HIP(hipStreamWaitEvent(s, w.done, 0));
+            } else {
+                // unmarked (or its stream is being captured): wait for the
+                // device, and from now on mark launches on callers' streams
+                // as they end
+                FF_HIP(hipDeviceSynchronize());
+                w.eager = true;
             }
         }
     } catch (...) {
         locked_ = false;
-        g_waiting_mu.unlock();
+        g.mu.unlock();
         --t_waiting_depth;
         counted_ = false;
         throw;
@@ -485,11 +527,20 @@ WaitingLaunch::~WaitingLaunch() {
     if (!counted_) return;
     --t_waiting_depth;
     if (!locked_) return;
-    WaitingState& w = g_waiting[dev_];
-    if (w.multi) (void)hipEventRecord(w.done, s_);
+    Waiting& g = waiting();
+    WaitingState& w = g.dev[dev_];
     w.last = s_;
-    w.seen = true;
-    g_waiting_mu.unlock();
+    if (g.owned.count(s_)) {
+        w.mark = LastMark::PENDING;
+    } else if (w.eager && w.done && hipEventRecord(w.done, s_) == hipSuccess) {
+        w.mark = LastMark::RECORDED;
+    } else if (w.eager && !w.done && hipEventCreateWithFlags(&w.done, hipEventDisableTiming) == hipSuccess &&
+               hipEventRecord(w.done, s_) == hipSuccess) {
+        w.mark = LastMark::RECORDED;
+    } else {
+        w.mark = LastMark::UNMARKED;
+    }
+    g.mu.unlock();
 }
 
 // ------------------------------------------------------------- InputFrames

@@ -212,16 +212,17 @@ private:
 // the other's producers are queued behind them there, and both waits then
 // give up at their spin limit. Construct one around every such launch (a
 // graph launch included): under a process-wide lock held until
-// destruction, it makes the stream wait for the previous such launch when
-// that was on another stream. Launches from one stream are ordered by the
-// stream itself, so ends are recorded (an event per launch, about 5 us
-// between two frames) only after a second stream has launched; that first
-// switch waits on the host for the device, once per device. A stream handed
-// to the library is identified by its handle: it must not be destroyed
-// while a waiting launch on it is pending (the library synchronizes its own
-// streams before destroying them). Nested guards on one thread, and guards
-// while the stream is capturing, do nothing. Launches without waits are not
-// ordered.
+// destruction, it makes the stream wait for the end of the previous such
+// launch when that was on another stream. Launches from one stream are
+// ordered by the stream itself, so nothing is marked while they stay on
+// one: the end of a launch on a library stream (StreamBox) is marked by an
+// event recorded there only when another stream launches (a record between
+// two frames of one filter costs about 5 us). On a caller's stream, which
+// may be destroyed at any time, the first switch away waits on the host for
+// the device, and from then on every launch there records its end. Nested
+// guards on one thread, and guards while the stream is capturing, do
+// nothing; if the previous launch's stream is being captured, the switch
+// waits for the device. Launches without waits are not ordered.
 class WaitingLaunch {
 public:
     WaitingLaunch(hipStream_t s, bool waits);
