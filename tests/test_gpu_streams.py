@@ -217,3 +217,61 @@ def test_pyramids_on_many_streams_interleaved():
             if 2 + k % 2 == levels:
                 assert np.array_equal(got[k], want), (k, levels)
     N.synchronize()
+
+
+def _hip_runtime():
+    """The HIP runtime this process has loaded (torch's copy, which the
+    library binds to as well)."""
+    import ctypes
+    path = next(line.split()[-1] for line in open("/proc/self/maps") if "libamdhip64" in line)
+    hip = ctypes.CDLL(path)
+    hip.hipStreamCreate.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
+    hip.hipStreamDestroy.argtypes = [ctypes.c_void_p]
+    return hip
+
+
+def test_pyramids_on_a_destroyed_caller_stream_and_a_library_stream():
+    """A 2-level pyramid on a caller's stream (made and destroyed with the
+    HIP runtime) interleaved with a 3-level one on its own stream, then the
+    caller's stream destroyed right after the first pyramid's last launch
+    and the second pyramid run on: launches stay ordered across the streams
+    without the destroyed handle being used again, no wait gives up, and
+    both flows are bitwise those of the same filters run alone."""
+    import ctypes
+    from flowfilter.gpu.image import GPUImage
+    hip = _hip_runtime()
+    h, w, F = 544, 1024, 4
+    frames = np.stack(_blobs(h, w, F, seed=13))
+    dev = GPUImage((F * h, w), 1)
+    dev.upload(frames.reshape(F * h, w))
+    ptr = lambda t: dev.data + (t % F) * h * dev.pitch
+
+    def step(f, t):
+        f.setImageDevice(ptr(t), dev.pitch)
+        f.compute()
+
+    raw = ctypes.c_void_p()
+    assert hip.hipStreamCreate(ctypes.byref(raw)) == 0
+    say("pyramid on a caller stream, pyramid on its own stream")
+    a, b = _pyr(h, w, 2), _pyr(h, w, 3)
+    a.stream = raw.value
+    for t in range(12):
+        step(a, t)
+        step(b, t)
+    step(a, 12)
+    fa = a.getFlow()
+    a.stream = None
+    assert hip.hipStreamDestroy(raw) == 0
+    say("caller stream destroyed; the other pyramid goes on")
+    for t in range(12, 20):
+        step(b, t)
+    fb = b.getFlow()
+    assert a.waitTimeouts() == 0 and b.waitTimeouts() == 0
+    say("the same filters alone")
+    ra, rb = _pyr(h, w, 2), _pyr(h, w, 3)
+    for t in range(13):
+        step(ra, t)
+    for t in range(20):
+        step(rb, t)
+    assert np.array_equal(fa, ra.getFlow())
+    assert np.array_equal(fb, rb.getFlow())
