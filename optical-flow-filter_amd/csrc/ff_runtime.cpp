@@ -489,38 +489,36 @@ bool capturing(hipStream_t s) {
 
 WaitingLaunch::WaitingLaunch(hipStream_t s, bool waits) : s_(s) {
     if (!waits) return;
-    counted_ = true;
-    if (t_waiting_depth++ > 0) return;
-    if (capturing(s)) return;
+    // counted (and so undone by the destructor) only once nothing can throw
+    if (t_waiting_depth > 0 || capturing(s)) {
+        ++t_waiting_depth;
+        counted_ = true;
+        return;
+    }
     FF_HIP(hipGetDevice(&dev_));
     Waiting& g = waiting();
-    g.mu.lock();
-    locked_ = true;
-    try {
-        if ((int)g.dev.size() <= dev_) g.dev.resize(dev_ + 1);
-        WaitingState& w = g.dev[dev_];
-        if (w.last != s && w.mark != LastMark::NONE) {
-            if (!w.done) FF_HIP(hipEventCreateWithFlags(&w.done, hipEventDisableTiming));
-            if (w.mark == LastMark::PENDING && !capturing(w.last)) {
-                FF_HIP(hipEventRecord(w.done, w.last));
-                FF_HIP(hipStreamWaitEvent(s, w.done, 0));
-            } else if (w.mark == LastMark::RECORDED) {
-                FF_HIP(hipStreamWaitEvent(s, w.done, 0));
-            } else {
-                // unmarked (or its stream is being captured): wait for the
-                // device, and from now on mark launches on callers' streams
-                // as they end
-                FF_HIP(hipDeviceSynchronize());
-                w.eager = true;
-            }
+    std::unique_lock<std::mutex> lk(g.mu);
+    if ((int)g.dev.size() <= dev_) g.dev.resize(dev_ + 1);
+    WaitingState& w = g.dev[dev_];
+    if (w.last != s && w.mark != LastMark::NONE) {
+        if (!w.done) FF_HIP(hipEventCreateWithFlags(&w.done, hipEventDisableTiming));
+        if (w.mark == LastMark::PENDING && !capturing(w.last)) {
+            FF_HIP(hipEventRecord(w.done, w.last));
+            FF_HIP(hipStreamWaitEvent(s, w.done, 0));
+        } else if (w.mark == LastMark::RECORDED) {
+            FF_HIP(hipStreamWaitEvent(s, w.done, 0));
+        } else {
+            // unmarked (or its stream is being captured): wait for the
+            // device, and from now on mark launches on callers' streams as
+            // they end
+            FF_HIP(hipDeviceSynchronize());
+            w.eager = true;
         }
-    } catch (...) {
-        locked_ = false;
-        g.mu.unlock();
-        --t_waiting_depth;
-        counted_ = false;
-        throw;
     }
+    lk.release();  // held until the destructor
+    ++t_waiting_depth;
+    counted_ = true;
+    locked_ = true;
 }
 
 WaitingLaunch::~WaitingLaunch() {
