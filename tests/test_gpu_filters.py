@@ -496,3 +496,40 @@ def test_fused_autotune_picks_a_tiling(ffo):
         g.tileTopRows = 6
     with pytest.raises(errors.InvalidArgument):
         g.tileRows = 2
+
+
+def _random_config(k):
+    """Seeded configuration k of the sweep below: level count, odd and
+    even sizes (at least 8 rows and columns on the top level), per-level
+    gamma and smoothing, maxflow (hence 1-9 propagation iterations and the
+    halo) and border."""
+    rng = np.random.default_rng(7000 + k)
+    levels = int(rng.integers(1, 5))
+    lo = 8 << (levels - 1)
+    h = int(rng.integers(lo, max(lo + 1, 560)))
+    w = int(rng.integers(lo, max(lo + 1, 720)))
+    gamma = [float(np.round(rng.uniform(1.0, 100.0), 3)) for _ in range(levels)]
+    maxflow = float(np.round(rng.uniform(0.3, 9.0), 2))
+    smooth = [int(rng.integers(1, 4)) for _ in range(levels)]
+    border = int(rng.integers(1, 5))
+    return (h, w), levels, gamma, maxflow, smooth, border
+
+
+@pytest.mark.parametrize("k", range(48))
+def test_fused_random_configurations_match_oracle(ffo, k):
+    """Forty-eight seeded configurations of the pyramidal filter (1-4 levels,
+    odd and even sizes, per-level gamma and smoothing, maxflow 0.3-9,
+    border 1-4), six frames each: every flow and brightness bitwise the
+    oracle's, and no in-launch wait gives up."""
+    (h, w), levels, gamma, maxflow, smooth, border = _random_config(k)
+    say("random configuration", k, (h, w), levels, gamma, maxflow, smooth, border)
+    g, o = _pyr_pair(ffo, h, w, levels, gamma, maxflow, smooth, fused=True)
+    g.propagationBorder = border
+    o.setPropagationBorder(border)
+    for t, img in enumerate(seq_blobs(h, w, 6)):
+        g.loadImage(img)
+        g.compute()
+        o.compute(img)
+        check_frame(t, g.getFlow(), o.downloadFlow(), "random %d" % k)
+        check_frame(t, g.getImage(), o.downloadImage(), "random %d image" % k)
+    assert g.waitTimeouts() == 0
