@@ -26,11 +26,13 @@ passes, calibrated on a copy kernel, read from
 profiles/fused_frame_2048x1088_2lvl_traffic.json; `valu_issue` is the
 kernel's VALU-issue floor, its SQ_INSTS_VALU count from
 profiles/fused_frame_2048x1088_2lvl_valu.json over the CUs' issue rate,
-against the measured launch time: the bound that holds), the 64-stream batched
-measurement, the differentiable-window measurements (forward and forward +
-backward of a 3-level 3840x2160 32-frame window and of 16 sequences of
-512x512 x 8 frames, with peak HBM), and a CPU baseline (the FP32 CPU oracle
-on the host cores, bounded sample).
+against the measured launch time: how busy the VALU is), the 64-stream
+batched measurement, FlowToColor and the rotational predictor, the reference
+notebook's own runtime measurement (640x480, 2 levels, mean elapsedTime()
+beside the numbers the notebook published), the differentiable-window
+measurements (forward and forward + backward of a 3-level 3840x2160
+32-frame window and of 16 sequences of 512x512 x 8 frames, with peak HBM),
+and a CPU baseline (the FP32 CPU oracle on the host cores, bounded sample).
 """
 import argparse
 import json
@@ -222,6 +224,53 @@ def batched_bench(args, N, gf, GPUImage):
                         "(in place, no copy) + advance 64 filters; plain = the same loop of filters on one stream"}
 
 
+def reference_notebook_bench(args, gf):
+    """The runtime measurement of the reference's own notebook
+    (notebooks/GPU_example.ipynb, cells 7 and 10): a 640x480 two-level
+    PyramidalFlowFilter, gamma [10, 50], maxflow 4, smooth [2, 4]; every
+    frame loadImage, compute() and elapsedTime() (the GPU time of compute()),
+    averaged. Cell 10 feeds 1000 all-zero frames (published 0.995867583513
+    ms); cell 7 averages 20 frames of a rendered sequence (published
+    1.17239360809 ms), here 20 frames of the synthetic blob sequence after
+    100 warm-up frames. The reference's hardware is not stated."""
+    import numpy as np
+    from flowfilter import synthetic
+
+    def make():
+        f = gf.PyramidalFlowFilter(480, 640, 2)
+        f.gamma = [10.0, 50.0]
+        f.maxflow = 4.0
+        f.smoothIterations = [2, 4]
+        return f
+
+    f = make()
+    zero = np.zeros((480, 640), np.uint8)
+    et = []
+    for k in range(1000):
+        f.loadImage(zero)
+        f.compute()
+        et.append(f.elapsedTime())
+    zero_ms = float(np.mean(et))
+    f = make()
+    frames = list(synthetic.blob_affine(480, 640, 120, seed=5))
+    et = []
+    for k, img in enumerate(frames):
+        f.loadImage(img)
+        f.compute()
+        if k >= 100:
+            et.append(f.elapsedTime())
+    tex_ms = float(np.mean(et))
+    ref_zero, ref_tex = 0.995867583513, 1.17239360809
+    return {"workload": "640x480 PyramidalFlowFilter, 2 levels, gamma [10,50], maxflow 4, smooth [2,4]; "
+                        "per frame loadImage + compute() + elapsedTime(), mean of elapsedTime()",
+            "zero_frames": {"frames": 1000, "ms_per_compute": round(zero_ms, 5),
+                            "reference_published_ms": ref_zero, "speedup": round(ref_zero / zero_ms, 1)},
+            "textured_frames": {"frames": 20, "ms_per_compute": round(tex_ms, 5),
+                                "reference_published_ms": ref_tex, "speedup": round(ref_tex / tex_ms, 1),
+                                "note": "synthetic blob sequence; the reference's rendered frames are not available"},
+            "reference_hardware": "not stated"}
+
+
 def display_rotation_bench(args, N, GPUImage):
     """The display and image-prediction stages at 2048x1088 (library stream):
     FlowToColor of a flow field (12 B/px: flow 8 in, RGBA 4 out; the wheel
@@ -373,7 +422,7 @@ def main():
                     help="skip the pass that loads each frame with a copy (value_with_copy)")
     ap.add_argument("--batched-steps", type=int, default=60)
     ap.add_argument("--no-display", action="store_true",
-                    help="skip the FlowToColor / rotational predictor measurements")
+                    help="skip the FlowToColor / rotational predictor and the reference notebook measurements")
     ap.add_argument("--no-differentiable", action="store_true",
                     help="skip the differentiable-window measurements (and the torch import they need)")
     ap.add_argument("--ad-reps", type=int, default=2)
@@ -548,6 +597,7 @@ def main():
         out["batched"] = batched_bench(args, N, gf, GPUImage)
     if not args.no_display:
         out["display_rotation"] = display_rotation_bench(args, N, GPUImage)
+        out["reference_notebook"] = reference_notebook_bench(args, gf)
     if not args.no_differentiable:
         out["differentiable"] = differentiable_bench(args)
     if not args.no_cpu:
