@@ -238,3 +238,56 @@ def test_batch_sixty_four_streams_hundred_twenty_steps(in_place):
                 assert np.array_equal(a, b), "stream %d step %d: max |diff| %g" % (i, t, np.abs(a - b).max())
                 assert np.array_equal(batched[i].getImage(), alone[i].getImage()), (i, t)
     assert batch.waitTimeouts() == 0
+
+
+def _random_mix(k):
+    """Seeded batch k of the sweep below: 3-8 streams, each of random size
+    (at least 8 rows and columns on its top level), 1-4 levels, per-level
+    gamma and smoothing, maxflow and border."""
+    rng = np.random.default_rng(9100 + k)
+    streams = []
+    for _ in range(int(rng.integers(3, 9))):
+        levels = int(rng.integers(1, 5))
+        lo = 8 << (levels - 1)
+        h = int(rng.integers(lo, max(lo + 1, 500)))
+        w = int(rng.integers(lo, max(lo + 1, 640)))
+        gamma = [float(np.round(rng.uniform(2.0, 80.0), 3)) for _ in range(levels)]
+        maxflow = float(np.round(rng.uniform(0.5, 6.0), 2))
+        smooth = [int(rng.integers(1, 4)) for _ in range(levels)]
+        border = int(rng.integers(1, 4))
+        streams.append((h, w, levels, gamma, maxflow, smooth, border))
+    return streams
+
+
+@pytest.mark.parametrize("k", range(12))
+def test_batch_random_mixes_match_oracle(ffo, k):
+    """Twelve seeded batches of 3-8 streams of random size and depth
+    (1-4 levels, per-level gamma and smoothing, maxflow 0.5-6, border 1-3),
+    replayed as graphs or launched directly, five steps each: every stream's
+    flow and brightness bitwise the oracle's after every step, and no
+    in-launch wait gives up."""
+    import flowfilter.gpu.flowfilters as gf
+    mix = _random_mix(k)
+    say("random batch", k, [(h, w, levels) for h, w, levels, *_ in mix])
+    batch = gf.BatchedFlowFilter()
+    batch.graphs = k % 2 == 0
+    oracles, seqs = [], []
+    for i, (h, w, levels, gamma, maxflow, smooth, border) in enumerate(mix):
+        f = make(h, w, levels, gamma, maxflow, smooth)
+        f.propagationBorder = border
+        batch.add(f)
+        o = ffo.PyramidalFlowFilter(h, w, levels)
+        o.configure_like(gamma, maxflow, smooth)
+        o.setPropagationBorder(border)
+        oracles.append(o)
+        seqs.append(frames_for(300 + 10 * k + i, h, w, 5))
+    for t in range(5):
+        for i in range(len(mix)):
+            batch[i].loadImage(seqs[i][t])
+        batch.compute()
+        for i, o in enumerate(oracles):
+            o.compute(seqs[i][t])
+            a, b = batch[i].getFlow(), o.downloadFlow()
+            assert np.array_equal(a, b), "batch %d stream %d step %d: max |diff| %g" % (k, i, t, np.abs(a - b).max())
+            assert np.array_equal(batch[i].getImage(), o.downloadImage()), (k, i, t)
+    assert batch.waitTimeouts() == 0

@@ -515,15 +515,17 @@ def _random_config(k):
     return (h, w), levels, gamma, maxflow, smooth, border
 
 
+@pytest.mark.parametrize("fused", [True, False])
 @pytest.mark.parametrize("k", range(48))
-def test_fused_random_configurations_match_oracle(ffo, k):
+def test_fused_random_configurations_match_oracle(ffo, k, fused):
     """Forty-eight seeded configurations of the pyramidal filter (1-4 levels,
     odd and even sizes, per-level gamma and smoothing, maxflow 0.3-9,
-    border 1-4), six frames each: every flow and brightness bitwise the
-    oracle's, and no in-launch wait gives up."""
+    border 1-4), six frames each, on the fused frame kernel and on the
+    stage kernels: every flow and brightness bitwise the oracle's, and no
+    in-launch wait gives up."""
     (h, w), levels, gamma, maxflow, smooth, border = _random_config(k)
-    say("random configuration", k, (h, w), levels, gamma, maxflow, smooth, border)
-    g, o = _pyr_pair(ffo, h, w, levels, gamma, maxflow, smooth, fused=True)
+    say("random configuration", k, (h, w), levels, gamma, maxflow, smooth, border, "fused" if fused else "staged")
+    g, o = _pyr_pair(ffo, h, w, levels, gamma, maxflow, smooth, fused=fused)
     g.propagationBorder = border
     o.setPropagationBorder(border)
     for t, img in enumerate(seq_blobs(h, w, 6)):
