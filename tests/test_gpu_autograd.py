@@ -476,6 +476,52 @@ def test_window_gradients_match_composition(ad, levels):
     assert rel(gg, g64.grad) <= 1e-4
 
 
+def _random_window(k):
+    """Seeded window k of the sweep below: batch, frames, size (at least 8
+    rows and columns on the top level), levels, maxflow, per-level
+    smoothing, border, gains and checkpoint interval. maxflow stays well
+    above the frames' motion (at most 1.5 px per frame along each axis):
+    where an update lands within float32 rounding of the ±maxflow clamp,
+    the clamp's derivative (0 or 1) is decided by rounding, and the kernels
+    and the composition, which round in different orders, can each take
+    either side (with motion close to maxflow such pixels occur; a
+    1e-4 change of maxflow moves them off the threshold)."""
+    r = np.random.default_rng(8800 + k)
+    levels = int(r.integers(1, 4))
+    lo = 8 << (levels - 1)
+    H = int(r.integers(lo, max(lo + 1, 80)))
+    W = int(r.integers(lo, max(lo + 1, 120)))
+    B, T = int(r.integers(1, 4)), int(r.integers(2, 6))
+    maxflow = float(np.round(r.uniform(4.5, 6.0), 2))
+    smooth = [int(r.integers(1, 3)) for _ in range(levels)]
+    border = int(r.integers(1, 4))
+    gamma = [float(np.round(r.uniform(5.0, 60.0), 2)) for _ in range(levels)]
+    checkpoint = int(r.integers(1, T + 1))
+    return B, T, H, W, levels, maxflow, smooth, border, gamma, checkpoint
+
+
+@pytest.mark.parametrize("k", range(16))
+def test_window_gradients_random_configurations(ad, k):
+    """Sixteen seeded windows (batch 1-3, 2-5 frames, odd and even sizes,
+    1-3 levels, maxflow 4.5-6, per-level smoothing, border 1-3, any
+    checkpoint interval): frame and gain gradients of a random weighting of
+    the flows against float64 autograd of the composition, normwise within
+    1e-4."""
+    B, T, H, W, levels, maxflow, smooth, border, gamma, checkpoint = _random_window(k)
+    say("random window", k, (B, T, H, W), levels, maxflow, smooth, border, gamma, checkpoint)
+    frames, _ = _frames(B, T, H, W, 40 + k, speed=1.5)
+    spec = ad.FilterSpec(H, W, levels, maxflow=maxflow, smooth=smooth, border=border)
+    gm = torch.tensor(gamma, device="cuda")
+    weights = torch.from_numpy(rng(50 + k).standard_normal((B, T, H, W, 2)).astype(np.float32)).cuda()
+    gx, gg = _gradients(ad, spec, frames, gm, weights, checkpoint=checkpoint)
+    x64 = frames.double().requires_grad_(True)
+    g64 = gm.double().requires_grad_(True)
+    (tc.window(spec, x64, g64) * weights.double()).sum().backward()
+    say("   frames rel", rel(gx, x64.grad), "gamma rel", rel(gg, g64.grad))
+    assert rel(gx, x64.grad) <= 1e-4
+    assert rel(gg, g64.grad) <= 1e-4
+
+
 def test_checkpoint_interval_changes_nothing(ad):
     B, T, H, W = 1, 6, 48, 64
     frames, _ = _frames(B, T, H, W, 9)
