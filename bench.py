@@ -70,16 +70,31 @@ def algorithmic_bytes():
     }
 
 
-def _cpu_model():
-    """The host CPU's model name (lscpu), else /proc/cpuinfo's."""
+_LSCPU_FIELDS = ("Model name", "BIOS Model name", "CPU family", "Model", "Stepping", "Socket(s)",
+                 "Core(s) per socket", "Thread(s) per core", "CPU max MHz", "NUMA node(s)")
+
+
+def _lscpu():
+    """Identifying fields of lscpu's report ({} without lscpu)."""
     import subprocess
     try:
         out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
-        for line in out.splitlines():
-            if line.strip().startswith("Model name:"):
-                return line.split(":", 1)[1].strip()
     except (OSError, subprocess.SubprocessError):
-        pass
+        return {}
+    info = {}
+    for line in out.splitlines():
+        if ":" in line:
+            k, v = line.split(":", 1)
+            if k.strip() in _LSCPU_FIELDS:
+                info[k.strip()] = v.strip()
+    return info
+
+
+def _cpu_model():
+    """The host CPU's model name (lscpu), else /proc/cpuinfo's."""
+    name = _lscpu().get("Model name")
+    if name:
+        return name
     try:
         with open("/proc/cpuinfo") as fh:
             for line in fh:
@@ -122,6 +137,7 @@ def cpu_baseline(frames, budget_s=12.0, threads=None):
     n1, el1 = run(1, budget_s / 2)
     return {"value": round(H * W * n / el / 1e6, 3), "unit": "Mpix/s", "cores": threads, "kind": "port",
             "single_thread_value": round(H * W * n1 / el1 / 1e6, 3), "cpu_model": _cpu_model(),
+            "cpu_lscpu": _lscpu(),
             "sample": "%d frames (%d threads, %.1f s) and %d frames (1 thread, %.1f s) of the same 2048x1088 "
                       "2-level workload, FP32 CPU oracle (oracle/ff_oracle.hpp, OpenMP over rows)"
                       % (n, threads, el, n1, el1)}
