@@ -97,10 +97,9 @@ __global__ void __launch_bounds__(256) k_copy2d_u32(DevImg src, DevImg dst, size
 // Many 2-D copies in one launch. A region is viewed as rows * (row_bytes /
 // unit) accesses of `unit` bytes (16, or 4 / 1 when rows are not aligned
 // for more); block b of a region copies kCopyPerBlock consecutive accesses,
-// each thread kCopyUnroll of them with all loads issued before the stores.
+// each thread two of them with both loads issued before the stores.
 constexpr int kCopyMax = 64;
-constexpr int kCopyUnroll = 2;
-constexpr int kCopyPerBlock = 256 * kCopyUnroll;
+constexpr int kCopyPerBlock = 256 * 2;
 struct CopyJob {
     const char* src;
     char* dst;
@@ -117,24 +116,28 @@ struct CopyBatch {
 };
 
 template <class T>
+__device__ __forceinline__ const T* copy_src(const CopyJob& c, int i) {
+    const int row = i / c.per_row, col = i - row * c.per_row;
+    return reinterpret_cast<const T*>(c.src + row * c.src_pitch) + col;
+}
+template <class T>
+__device__ __forceinline__ T* copy_dst(const CopyJob& c, int i) {
+    const int row = i / c.per_row, col = i - row * c.per_row;
+    return reinterpret_cast<T*>(c.dst + row * c.dst_pitch) + col;
+}
+
+// The two values stay in named registers: held in a two-element array they
+// were placed in LDS by the compiler, with a wait for each load before the
+// next one was issued (17.7 us for a 2048x1088 u8 frame instead of ~3 us).
+template <class T>
 __device__ __forceinline__ void copy_units(const CopyJob& c, int base) {
-    T v[kCopyUnroll];
-#pragma unroll
-    for (int k = 0; k < kCopyUnroll; ++k) {
-        const int i = base + k * 256;
-        if (i < c.total) {
-            const int row = i / c.per_row, col = i - row * c.per_row;
-            v[k] = reinterpret_cast<const T*>(c.src + row * c.src_pitch)[col];
-        }
-    }
-#pragma unroll
-    for (int k = 0; k < kCopyUnroll; ++k) {
-        const int i = base + k * 256;
-        if (i < c.total) {
-            const int row = i / c.per_row, col = i - row * c.per_row;
-            reinterpret_cast<T*>(c.dst + row * c.dst_pitch)[col] = v[k];
-        }
-    }
+    const int i0 = base, i1 = base + 256;
+    const bool ok0 = i0 < c.total, ok1 = i1 < c.total;
+    T v0{}, v1{};
+    if (ok0) v0 = *copy_src<T>(c, i0);
+    if (ok1) v1 = *copy_src<T>(c, i1);
+    if (ok0) *copy_dst<T>(c, i0) = v0;
+    if (ok1) *copy_dst<T>(c, i1) = v1;
 }
 
 template <int M>
