@@ -102,6 +102,39 @@ def test_gpuimage_device_copies():
     assert np.array_equal(g.download(np.uint8), wide.download(np.uint8)[:, 17:87])
 
 
+@pytest.mark.parametrize("shape,dtype,offset", [
+    ((7, 64), np.uint8, 0),        # 64-byte rows: 16-byte accesses, 28 of them (one partial block)
+    ((1, 16), np.uint8, 0),        # a single 16-byte access (the second of the thread's pair unused)
+    ((300, 2048), np.uint8, 0),    # many blocks of 16-byte accesses
+    ((9, 5), np.float32, 0),       # 20-byte rows: 4-byte accesses
+    ((33, 3, 2), np.float32, 0),   # 24-byte rows (float2): 4-byte accesses, odd row count
+    ((11, 37), np.uint8, 0),       # 37-byte rows: byte accesses
+    ((5, 64), np.uint8, 3),        # 16-byte-wide rows read at a 3-byte offset: byte accesses
+])
+def test_device_copies_every_access_width(shape, dtype, offset):
+    """The region copy kernel behind device uploads: rows copied with 16-,
+    4- or 1-byte accesses depending on alignment, a thread's two accesses
+    possibly past the region's end; every byte lands where it belongs."""
+    from flowfilter.gpu.image import GPUImage
+    from flowfilter import _native as N
+    r = rng(11 + shape[0])
+    if np.issubdtype(dtype, np.integer):
+        a = r.integers(0, 255, shape, dtype=dtype)
+    else:
+        a = r.standard_normal(shape).astype(dtype)
+    item = np.dtype(dtype).itemsize
+    # the source: a wider device image whose rows hold `a` at `offset` bytes
+    row = a.shape[1] * (a.shape[2] if a.ndim == 3 else 1) * item
+    wide = np.zeros((a.shape[0], row + offset + 32), np.uint8)
+    wide[:, offset:offset + row] = a.view(np.uint8).reshape(a.shape[0], row)
+    src = gimg(wide)
+    g = GPUImage(a.shape, item)
+    g.uploadDevice(src.data + offset, src.pitch)
+    N.synchronize()
+    got = g.download(dtype)
+    assert got.shape == a.shape and np.array_equal(got.view(np.uint8), a.view(np.uint8))
+
+
 # ---------------------------------------------------------------- ImageModel
 
 @pytest.mark.parametrize("shape", SIZES)
