@@ -383,7 +383,9 @@ def differentiable_bench(args):
         torch.cuda.synchronize()
         torch.cuda.reset_peak_memory_stats()
         times = []
+        g = gg = None
         for _ in range(reps):
+            g = gg = None  # the peak is one call's, not two calls' gradients
             g, gg = once(times)
         assert bool(torch.isfinite(g).all()) and bool(torch.isfinite(gg).all())
         fwd = min(t[0] for t in times)
