@@ -10,6 +10,9 @@
 // file is compiled with -ffp-contract=off).
 #include <hip/hip_runtime.h>
 
+#include <string>
+#include <vector>
+
 #include "ff_device.hpp"
 #include "ff_kernels.hpp"
 #include "ff_runtime.hpp"
@@ -381,7 +384,36 @@ void launch_copy_batch(const CopyRegion* regions, int n, hipStream_t s) {
 }
 }  // namespace
 
+// Largest band of rows one copy job takes, in bytes. A job counts its
+// accesses in 32-bit integers, and a launch of kCopyMax jobs of one-byte
+// accesses (2^17 blocks of 256 threads each) stays below the 2^32
+// work-items of a grid dimension; larger regions are split into bands.
+constexpr size_t kCopyBandBytes = size_t(1) << 26;
+
 void launch_copy_regions(const CopyRegion* regions, int n, hipStream_t s) {
+    std::vector<CopyRegion> bands;
+    bool split = false;
+    for (int i = 0; i < n; ++i)
+        split = split || (regions[i].rows > 0 && (size_t)regions[i].rows * regions[i].row_bytes > kCopyBandBytes);
+    if (split) {
+        for (int i = 0; i < n; ++i) {
+            const CopyRegion& r = regions[i];
+            if (r.rows <= 0 || r.row_bytes == 0) continue;
+            if (r.row_bytes > kCopyBandBytes)
+                throw invalid_argument("image copy: rows of more than 64 MiB are not supported (" +
+                                       std::to_string(r.row_bytes) + " bytes)");
+            const int per = (int)(kCopyBandBytes / r.row_bytes);
+            for (int r0 = 0; r0 < r.rows; r0 += per) {
+                CopyRegion band = r;
+                band.src = (const char*)r.src + (size_t)r0 * r.src_pitch;
+                band.dst = (char*)r.dst + (size_t)r0 * r.dst_pitch;
+                band.rows = per < r.rows - r0 ? per : r.rows - r0;
+                bands.push_back(band);
+            }
+        }
+        regions = bands.data();
+        n = (int)bands.size();
+    }
     for (int first = 0; first < n; first += kCopyMax) {
         const int m = n - first < kCopyMax ? n - first : kCopyMax;
         if (m == 1) launch_copy_batch<1>(regions + first, m, s);

@@ -97,3 +97,32 @@ def test_set_image_device_errors():
         f.setImageDevice(0, 64)
     with pytest.raises(errors.InvalidArgument):
         f.setImageDevice(4096, 32)
+
+
+def test_frame_in_place_beyond_32_bit_offsets_is_copied():
+    """A frame in place whose pitch x height reaches 2^31 bytes lies beyond
+    the fused kernel's 32-bit buffer offsets: compute() copies it into the
+    input image instead (one copy launch per frame), with the flow of the
+    same frames loaded from the host."""
+    import torch
+    from flowfilter import _native as N, synthetic
+    H, W = 272, 480
+    pitch = ((2 ** 31 // H) + 256) // 256 * 256
+    assert pitch * H >= 2 ** 31
+    say("frame in place with pitch", pitch)
+    frames = list(synthetic.translating_noise(H, W, 3, (1.1, -0.7), seed=9))
+    big = torch.zeros((H, pitch), dtype=torch.uint8, device="cuda")
+    a, b = make(H, W, 2), make(H, W, 2)
+    for t in range(3):
+        big[:, :W] = torch.from_numpy(frames[t]).cuda()
+        torch.cuda.synchronize()
+        N.profile_begin("copy")
+        a.setImageDevice(big.data_ptr(), pitch)
+        a.compute()
+        _, copies = N.profile_end()
+        b.loadImage(frames[t])
+        b.compute()
+        if t >= 1:
+            assert copies == 1, (t, copies)
+        assert np.array_equal(a.getFlow(), b.getFlow()), t
+        assert np.array_equal(a.getImage(), b.getImage()), t

@@ -135,6 +135,29 @@ def test_device_copies_every_access_width(shape, dtype, offset):
     assert got.shape == a.shape and np.array_equal(got.view(np.uint8), a.view(np.uint8))
 
 
+def test_device_copy_of_2_gib_with_byte_accesses():
+    """A 16384 x 131072 u8 image (2^31 bytes) copied from an odd device
+    address, so with one-byte accesses: 2^31 accesses, one more than a
+    32-bit count holds. The copy is split into bands of rows; every byte
+    lands where it belongs (compared on the device)."""
+    import torch
+    from flowfilter.gpu.image import GPUImage
+    from flowfilter import _native as N
+    H, W = 16384, 131072
+    pitch = W + 256
+    print("2 GiB device copy with byte accesses", flush=True)
+    gen = torch.Generator(device="cuda").manual_seed(3)
+    buf = torch.randint(0, 256, (H * pitch + 1,), dtype=torch.uint8, device="cuda", generator=gen)
+    torch.cuda.synchronize()
+    g = GPUImage((H, W), 1)
+    assert g.pitch == W
+    g.uploadDevice(buf.data_ptr() + 1, pitch)
+    N.synchronize()
+    got = torch.from_dlpack(g)
+    want = torch.as_strided(buf, (H, W), (pitch, 1), 1)
+    assert got.shape == (H, W) and torch.equal(got, want)
+
+
 # ---------------------------------------------------------------- ImageModel
 
 @pytest.mark.parametrize("shape", SIZES)
