@@ -431,6 +431,17 @@ FF_API ff_status ff_filter_get_tile_top_rows(ff_stage* s, int* out);
  * frame). Same filters as above. */
 FF_API ff_status ff_filter_set_tile_rows(ff_stage* s, int rows);
 FF_API ff_status ff_filter_get_tile_rows(ff_stage* s, int* out);
+/* Interior tiles of the fused kernel in a tile body of their own (no
+ * reference counterpart): tiles inside the propagation border and away from
+ * the image edges run one branch-free body, in a kernel instantiation of
+ * its own, with 4 rows per thread on the levels below the top (fewer VALU
+ * instructions: a 64-stream batch runs about 2 % faster). 0 (the default)
+ * tunes it with the tiling, 1 turns it on (with 4 rows per thread and 16
+ * or 15 waves per workgroup), 2 off. Results do not depend on it. get
+ * returns the fixed or tuned choice, 1 or 2 (0 before the first fused
+ * frame). Same filters as above. */
+FF_API ff_status ff_filter_set_tile_interior(ff_stage* s, int mode);
+FF_API ff_status ff_filter_get_tile_interior(ff_stage* s, int* out);
 
 /* Batched filters (no reference counterpart; replaces a caller's loop of
  * PyramidalFlowFilter::compute over camera streams, e.g. the per-camera

@@ -717,6 +717,15 @@ ff_status ff_filter_get_tile_rows(ff_stage* s, int* out) {
         on_fused_filter(s, [&](auto& f) { *out = f.tileRows(); });
     });
 }
+ff_status ff_filter_set_tile_interior(ff_stage* s, int mode) {
+    return guard_on(s, [&] { on_fused_filter(s, [&](auto& f) { f.setTileInterior(mode); }); });
+}
+ff_status ff_filter_get_tile_interior(ff_stage* s, int* out) {
+    return guard_on(s, [&] {
+        out_ptr(out, "mode");
+        on_fused_filter(s, [&](auto& f) { *out = f.tileInterior(); });
+    });
+}
 
 // ------------------------------------------------------------------ batches
 ff_status ff_batch_create(ff_stage** out) {

@@ -54,6 +54,8 @@ public:
     int tileTopRows() const { return frame_.topRows(); }
     int tileRows() const { return frame_.rows(); }
     void setTileRows(int rows) { frame_.setRows(rows); }
+    int tileInterior() const { return frame_.interior(); }
+    void setTileInterior(int mode) { frame_.setInterior(mode); }
 
     // Used by PyramidalFlowFilter: the same work without the timer.
     void runImageModel();
@@ -116,6 +118,8 @@ public:
     int tileTopRows() const { return frame_.topRows(); }
     int tileRows() const { return frame_.rows(); }
     void setTileRows(int rows) { frame_.setRows(rows); }
+    int tileInterior() const { return frame_.interior(); }
+    void setTileInterior(int mode) { frame_.setInterior(mode); }
 
     void runImageModel();
     void runPropagation();
@@ -223,6 +227,8 @@ public:
     int tileTopRows() const { return frame_.topRows(); }
     int tileRows() const { return frame_.rows(); }
     void setTileRows(int rows) { frame_.setRows(rows); }
+    int tileInterior() const { return frame_.interior(); }
+    void setTileInterior(int mode) { frame_.setInterior(mode); }
 
 private:
     friend class BatchedFlowFilter;
@@ -295,6 +301,8 @@ public:
     int tileTopRows() const { return frame_.topRows(); }
     int tileRows() const { return frame_.rows(); }
     void setTileRows(int rows) { frame_.setRows(rows); }
+    int tileInterior() const { return frame_.interior(); }
+    void setTileInterior(int mode) { frame_.setInterior(mode); }
     // Workgroups of the last fused launch (0 before the first).
     int lastBlocks() const { return last_blocks_; }
 

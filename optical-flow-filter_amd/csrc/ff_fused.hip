@@ -655,7 +655,15 @@ struct FrameArgs {
     int pitch[kMaxLaunchFrames];
 };
 
-template <bool DELTA, int MODE, int R>
+// INNER: a tile whose every pixel lies inside the border, away from the
+// image edges (every wave inside, no edge lanes, no clamped rows, no
+// column or row beyond the image): one straight-line body with no per-wave
+// choice of pass forms, so no register copies where two forms of a pass
+// would join; it computes what the general body computes for such a tile.
+// ALLOW_INNER: interior tiles take that body (a kernel instantiation of
+// its own, since the second body changes the register allocation of the
+// first). Other tiles choose per wave.
+template <bool DELTA, int MODE, int R, bool ALLOW_INNER, bool INNER = false>
 __device__ __forceinline__ void level_tile(const LevelJob& p, int tile, float* xch, int nw, unsigned* counters, unsigned epoch,
                                            int timeout_idx, const WaitArgs& wa, ImgSrc frame) {
     Geo g;
@@ -668,6 +676,15 @@ __device__ __forceinline__ void level_tile(const LevelJob& p, int tile, float* x
     const int oy0 = tyi * p.TH, oy1 = min(oy0 + p.TH, p.H);
     const int x0 = clampi(ox0 - p.h0, 0, p.W - LANES);
     const int y0 = clampi(oy0 - p.h0, 0, p.H - nw * R);
+    if constexpr (ALLOW_INNER && !INNER) {
+        // at least one column and row from each image edge, also when the
+        // propagation border is 0
+        const int bb = max(p.border, 1);
+        if (x0 >= bb && x0 + LANES - 1 < p.W - bb && y0 >= bb && y0 + nw * R - 1 < p.H - bb) {
+            level_tile<DELTA, MODE, R, true, true>(p, tile, xch, nw, counters, epoch, timeout_idx, wa, frame);
+            return;
+        }
+    }
     g.x = x0 + g.lane;
     g.xl = min(g.x, p.W - 1);
     g.yw = y0 + g.wave * R;  // image row of this thread's first row (wave-uniform)
@@ -722,7 +739,7 @@ __device__ __forceinline__ void level_tile(const LevelJob& p, int tile, float* x
     // a delta level part of its flow output is dead
     auto iteration = [&](auto last) {
         constexpr bool LAST = decltype(last)::value;
-        if (g.wave_in) prop_x<DELTA, false, R, LAST>(s, g, p);
+        if (INNER || g.wave_in) prop_x<DELTA, false, R, LAST>(s, g, p);
         else prop_x<DELTA, true, R, LAST>(s, g, p);
         {
             float* my = xrow(xch, nw, slot, wave);
@@ -772,7 +789,7 @@ __device__ __forceinline__ void level_tile(const LevelJob& p, int tile, float* x
             }
         }
         slot ^= 1;
-        if (g.wave_in) prop_y<DELTA, false, R, LAST>(s, g, p, ea, eb);
+        if (INNER || g.wave_in) prop_y<DELTA, false, R, LAST>(s, g, p, ea, eb);
         else prop_y<DELTA, true, R, LAST>(s, g, p, ea, eb);
     };
     for (int it = 0; it + 1 < p.N; ++it) iteration(std::false_type{});
@@ -805,14 +822,14 @@ __device__ __forceinline__ void level_tile(const LevelJob& p, int tile, float* x
     }
     float im[R + 4];
     finish_image<MODE, R>(ir, im);
-    if (xedge) model_update<DELTA, true, R>(s, im, aold, up, p, rao, rdf, rfu, offa, offf);
+    if (!INNER && xedge) model_update<DELTA, true, R>(s, im, aold, up, p, rao, rdf, rfu, offa, offf);
     else model_update<DELTA, false, R>(s, im, aold, up, p, rao, rdf, rfu, offa, offf);
 
     // ------------------------------------------------------------ smoothing
     // flowSmoothX_k / flowSmoothY_k (flowsmoothing_k.cu:21-87), K times.
     for (int it = 0; it < p.K; ++it) {
-        if (xinside) copy_last_column<R>(s.fu, s.fv, g.lane, xlast);
-        if (xedge) smooth_x<true, R>(s.fu, s.fv);
+        if (!INNER && xinside) copy_last_column<R>(s.fu, s.fv, g.lane, xlast);
+        if (!INNER && xedge) smooth_x<true, R>(s.fu, s.fv);
         else smooth_x<false, R>(s.fu, s.fv);
         {
             float* my = xrow(xch, nw, slot, wave);
@@ -844,7 +861,7 @@ __device__ __forceinline__ void level_tile(const LevelJob& p, int tile, float* x
             b2v = o[3 * LANES + lane];
         }
         slot ^= 1;
-        if (yinside) smooth_y<true, R>(s.fu, s.fv, a2u, a2v, a1u, a1v, b1u, b1v, b2u, b2v, rlast);
+        if (!INNER && yinside) smooth_y<true, R>(s.fu, s.fv, a2u, a2v, a1u, a1v, b1u, b1v, b2u, b2v, rlast);
         else smooth_y<false, R>(s.fu, s.fv, a2u, a2v, a1u, a1v, b1u, b1v, b2u, b2v, rlast);
     }
 
@@ -883,10 +900,11 @@ constexpr int kRows = 8;
 constexpr int kTopRowChoices[] = {4, 8};
 struct TileShape {
     int rtop, rlow;
+    bool inner;  // interior tiles in a body of their own (k_fused_frame's INNER)
 };
-constexpr TileShape kShapes[] = {{4, 8}, {8, 8}, {4, 4}};
+constexpr TileShape kShapes[] = {{4, 8, false}, {8, 8, false}, {4, 4, false}, {4, 4, true}};
 
-template <int RTOP, int RLOW = kRows>
+template <int RTOP, int RLOW = kRows, bool INNER = false>
 __global__ void __launch_bounds__(1024, RLOW == 4 ? 8 : 1)
     k_fused_frame(const LevelJob* __restrict__ jobs, const int* __restrict__ block_job, unsigned* counters,
                   unsigned epoch, int timeout_idx, GroupEnds ends, int nw, WaitArgs wa, FrameArgs frames) {
@@ -899,13 +917,13 @@ __global__ void __launch_bounds__(1024, RLOW == 4 ? 8 : 1)
     // uniform, so is the selection)
     ImgSrc fr{nullptr, 0};
     if (p.frame_slot >= 0) fr = ImgSrc{frames.ptr[p.frame_slot], frames.pitch[p.frame_slot]};
-    if (b < (unsigned)ends.e[0]) level_tile<false, IMG_F32, RLOW>(p, tile, xch, nw, counters, epoch, timeout_idx, wa, fr);
-    else if (b < (unsigned)ends.e[1]) level_tile<false, IMG_U8, RLOW>(p, tile, xch, nw, counters, epoch, timeout_idx, wa, fr);
-    else if (b < (unsigned)ends.e[2]) level_tile<false, IMG_U8_DOWN, RTOP>(p, tile, xch, nw, counters, epoch, timeout_idx, wa, fr);
-    else if (b < (unsigned)ends.e[3]) level_tile<false, IMG_U8_DOWN2, RTOP>(p, tile, xch, nw, counters, epoch, timeout_idx, wa, fr);
-    else if (b < (unsigned)ends.e[4]) level_tile<true, IMG_F32, RLOW>(p, tile, xch, nw, counters, epoch, timeout_idx, wa, fr);
-    else if (b < (unsigned)ends.e[5]) level_tile<true, IMG_U8, RLOW>(p, tile, xch, nw, counters, epoch, timeout_idx, wa, fr);
-    else level_tile<true, IMG_U8_DOWN, RLOW>(p, tile, xch, nw, counters, epoch, timeout_idx, wa, fr);
+    if (b < (unsigned)ends.e[0]) level_tile<false, IMG_F32, RLOW, INNER>(p, tile, xch, nw, counters, epoch, timeout_idx, wa, fr);
+    else if (b < (unsigned)ends.e[1]) level_tile<false, IMG_U8, RLOW, INNER>(p, tile, xch, nw, counters, epoch, timeout_idx, wa, fr);
+    else if (b < (unsigned)ends.e[2]) level_tile<false, IMG_U8_DOWN, RTOP, INNER>(p, tile, xch, nw, counters, epoch, timeout_idx, wa, fr);
+    else if (b < (unsigned)ends.e[3]) level_tile<false, IMG_U8_DOWN2, RTOP, INNER>(p, tile, xch, nw, counters, epoch, timeout_idx, wa, fr);
+    else if (b < (unsigned)ends.e[4]) level_tile<true, IMG_F32, RLOW, INNER>(p, tile, xch, nw, counters, epoch, timeout_idx, wa, fr);
+    else if (b < (unsigned)ends.e[5]) level_tile<true, IMG_U8, RLOW, INNER>(p, tile, xch, nw, counters, epoch, timeout_idx, wa, fr);
+    else level_tile<true, IMG_U8_DOWN, RLOW, INNER>(p, tile, xch, nw, counters, epoch, timeout_idx, wa, fr);
 }
 
 __global__ void k_clear_counters(unsigned* c, int n) {
@@ -1021,6 +1039,7 @@ struct FusedFrame::Impl {
         int njobs = 0, blocks = 0, nw = 0;
         GroupEnds ends{};
         int rtop = 0, rlow = kRows;
+        bool inner = false;  // launched as k_fused_frame<4, 4, true>
         size_t map_offset = 0;
         unsigned long long token = 0;
         bool waits = false;  // some job waits for another
@@ -1040,12 +1059,14 @@ struct FusedFrame::Impl {
     // waves per workgroup, top-level rows per thread, other rows per thread;
     // 0: autotune
     int fixed = 0, fixed_rtop = 0, fixed_rlow = 0;
+    int fixed_inner = 0;  // 0: tuned, 1: interior tiles in their own body, 2: not
     std::vector<int> tuned_for;
     int tuned = 0, tuned_rtop = 0, tuned_rlow = kRows;
+    bool tuned_inner = false;
     std::shared_ptr<WaitWatch> watch = std::make_shared<WaitWatch>();
     int test_spins = 0;  // > 0: the waits give up after this many polls (test hook)
 
-    int table(const std::vector<LevelArgs>& jobs, int nw, int rtop, int rlow, hipStream_t s);
+    int table(const std::vector<LevelArgs>& jobs, int nw, int rtop, int rlow, bool inner, hipStream_t s);
     unsigned* region(int si) const { return (unsigned*)counters.data() + (size_t)si * 2 * npairs; }
     int timeout_word() const { return 4 * npairs; }
     void clear_counters(int si, hipStream_t s) const;
@@ -1065,7 +1086,8 @@ FusedFrame& FusedFrame::operator=(FusedFrame&& o) noexcept {
 // thread on reduced top levels: found among the two resident tables or
 // built and uploaded (after waiting for stream s, the stream of every launch
 // of this FusedFrame, since the slot it replaces may be in use).
-int FusedFrame::Impl::table(const std::vector<LevelArgs>& jobs, int nw, int rtop, int rlow, hipStream_t s) {
+int FusedFrame::Impl::table(const std::vector<LevelArgs>& jobs, int nw, int rtop, int rlow, bool inner,
+                            hipStream_t s) {
     // launch order: grouped by tile body, stable within a group; every top
     // level precedes every delta level, so producers precede consumers
     std::vector<int> order(jobs.size()), pos(jobs.size());
@@ -1155,6 +1177,7 @@ int FusedFrame::Impl::table(const std::vector<LevelArgs>& jobs, int nw, int rtop
     key.insert(key.end(), reinterpret_cast<const char*>(&nw), reinterpret_cast<const char*>(&nw) + sizeof(int));
     key.insert(key.end(), reinterpret_cast<const char*>(&rtop), reinterpret_cast<const char*>(&rtop) + sizeof(int));
     key.insert(key.end(), reinterpret_cast<const char*>(&rlow), reinterpret_cast<const char*>(&rlow) + sizeof(int));
+    key.push_back(inner ? 1 : 0);
 
     for (int i = 0; i < 2; ++i)
         if (slot[i].key == key) return i;
@@ -1184,6 +1207,7 @@ int FusedFrame::Impl::table(const std::vector<LevelArgs>& jobs, int nw, int rtop
     t.nw = nw;
     t.rtop = rtop;
     t.rlow = rlow;
+    t.inner = inner;
     t.map_offset = map_offset;
     t.waits = false;
     for (const LevelJob& r : rec) t.waits = t.waits || r.wait_idx >= 0;
@@ -1208,9 +1232,13 @@ void FusedFrame::Impl::run(int ti, hipStream_t s, const FrameRef* frames, int nf
         (const void*)k_fused_frame<8>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes(kMaxWaves));
     static const hipError_t lds_ok44 = hipFuncSetAttribute(
         (const void*)k_fused_frame<4, 4>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes(kMaxWaves));
+    static const hipError_t lds_ok44i = hipFuncSetAttribute((const void*)k_fused_frame<4, 4, true>,
+                                                            hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                            (int)lds_bytes(kMaxWaves));
     if (lds_ok4 != hipSuccess) throw_hip(lds_ok4, "hipFuncSetAttribute", __FILE__, __LINE__);
     if (lds_ok8 != hipSuccess) throw_hip(lds_ok8, "hipFuncSetAttribute", __FILE__, __LINE__);
     if (lds_ok44 != hipSuccess) throw_hip(lds_ok44, "hipFuncSetAttribute", __FILE__, __LINE__);
+    if (lds_ok44i != hipSuccess) throw_hip(lds_ok44i, "hipFuncSetAttribute", __FILE__, __LINE__);
     // Sequence number of this launch for its slot's completion counters. A
     // launch recorded into a graph clears them itself and counts as the
     // first, so every replay is self-contained; direct launches count up
@@ -1245,7 +1273,11 @@ void FusedFrame::Impl::run(int ti, hipStream_t s, const FrameRef* frames, int nf
     const char* base = (const char*)t.dev.data();
     // the timeout word, relative to this slot's region
     const int timeout_idx = timeout_word() - si * 2 * npairs;
-    if (t.rlow == 4)
+    if (t.rlow == 4 && t.inner)
+        hipLaunchKernelGGL((k_fused_frame<4, 4, true>), dim3(t.blocks), dim3(t.nw * LANES), lds_bytes(t.nw), s,
+                           (const LevelJob*)base, (const int*)(base + t.map_offset), region(si), e, timeout_idx, t.ends,
+                           t.nw, wa, fa);
+    else if (t.rlow == 4)
         hipLaunchKernelGGL((k_fused_frame<4, 4>), dim3(t.blocks), dim3(t.nw * LANES), lds_bytes(t.nw), s,
                            (const LevelJob*)base, (const int*)(base + t.map_offset), region(si), e, timeout_idx, t.ends,
                            t.nw, wa, fa);
@@ -1269,6 +1301,7 @@ void FusedFrame::Impl::tune(const std::vector<LevelArgs>& jobs, hipStream_t s) {
     bool reduced_top = false;
     for (const LevelArgs& a : jobs) reduced_top = reduced_top || top_rows_body(a);
     int best_nw = 0, best_rtop = 0, best_rlow = kRows;
+    bool best_inner = false;
     float best_ms = 0.0f;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     FF_HIP(hipEventCreate(&e0));
@@ -1278,6 +1311,7 @@ void FusedFrame::Impl::tune(const std::vector<LevelArgs>& jobs, hipStream_t s) {
             const int rtop = sh.rtop, rlow = sh.rlow;
             if (fixed_rtop ? rtop != fixed_rtop : (!reduced_top && rtop != kTopRowChoices[0])) continue;
             if (fixed_rlow && rlow != fixed_rlow) continue;
+            if (fixed_inner && sh.inner != (fixed_inner == 1)) continue;
             for (int nw : kWaveChoices) {
                 if (fixed && nw != fixed) continue;
                 // four rows per thread only in the tallest workgroups (the
@@ -1286,7 +1320,7 @@ void FusedFrame::Impl::tune(const std::vector<LevelArgs>& jobs, hipStream_t s) {
                 bool ok = true;
                 for (const LevelArgs& a : jobs) ok = ok && tiles_with(a, nw * rows_per_thread(a, rtop, rlow));
                 if (!ok) continue;
-                const int t = table(jobs, nw, rtop, rlow, s);
+                const int t = table(jobs, nw, rtop, rlow, sh.inner, s);
                 run(t, s);  // warm-up
                 constexpr int kReps = 3;
                 FF_HIP(hipEventRecord(e0, s));
@@ -1299,6 +1333,7 @@ void FusedFrame::Impl::tune(const std::vector<LevelArgs>& jobs, hipStream_t s) {
                     best_nw = nw;
                     best_rtop = rtop;
                     best_rlow = rlow;
+                    best_inner = sh.inner;
                     best_ms = ms;
                 }
             }
@@ -1310,10 +1345,14 @@ void FusedFrame::Impl::tune(const std::vector<LevelArgs>& jobs, hipStream_t s) {
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    if (!best_nw && fixed_inner == 1)
+        throw invalid_argument("fused frame: the interior tile body needs four rows per thread and 16 or 15 waves per "
+                               "workgroup, which the fixed part of the tiling excludes");
     if (!best_nw) throw logic_error("fused frame: no tiling fits the levels");
     tuned = best_nw;
     tuned_rtop = best_rtop;
     tuned_rlow = best_rlow;
+    tuned_inner = best_inner;
 }
 
 int FusedFrame::prepare(const std::vector<LevelArgs>& jobs, hipStream_t s) {
@@ -1321,15 +1360,19 @@ int FusedFrame::prepare(const std::vector<LevelArgs>& jobs, hipStream_t s) {
     // the rows below the top level default to eight when the tile height
     // and the top rows are both fixed; otherwise the tuner picks them too
     int nw = impl_->fixed, rtop = impl_->fixed_rtop, rlow = impl_->fixed_rlow ? impl_->fixed_rlow : kRows;
+    bool inner = impl_->fixed_inner == 1;
     if (rlow == 4 && nw && nw < 15)
         throw invalid_argument("fused frame: four rows per thread need 16 or 15 waves per workgroup, got " +
                                std::to_string(nw));
+    if (inner && nw && rtop && rlow != 4)
+        throw invalid_argument("fused frame: the interior tile body needs four rows per thread below the top level");
     if (!nw || !rtop) {
         // tuned per set of level shapes and per fixed part of the choice
         std::vector<int> k = tune_key(jobs);
         k.push_back(impl_->fixed);
         k.push_back(impl_->fixed_rtop);
         k.push_back(impl_->fixed_rlow);
+        k.push_back(impl_->fixed_inner);
         if (k != impl_->tuned_for || !impl_->tuned) {
             impl_->tune(jobs, s);
             impl_->tuned_for = std::move(k);
@@ -1337,8 +1380,9 @@ int FusedFrame::prepare(const std::vector<LevelArgs>& jobs, hipStream_t s) {
         nw = impl_->tuned;
         rtop = impl_->tuned_rtop;
         rlow = impl_->tuned_rlow;
+        inner = impl_->tuned_inner;
     }
-    return impl_->table(jobs, nw, rtop, rlow, s);
+    return impl_->table(jobs, nw, rtop, rlow, inner, s);
 }
 
 void FusedFrame::launch(int table, hipStream_t s, const FrameRef* frames, int nframes) {
@@ -1380,6 +1424,18 @@ int FusedFrame::topRows() const { return impl_->fixed_rtop ? impl_->fixed_rtop :
 int FusedFrame::rows() const {
     if (impl_->fixed_rlow) return impl_->fixed_rlow;
     return impl_->fixed && impl_->fixed_rtop ? kRows : impl_->tuned_rlow;
+}
+
+void FusedFrame::setInterior(int mode) {
+    if (mode < 0 || mode > 2)
+        throw invalid_argument("fused frame: interior tile body must be 0 (autotune), 1 (on) or 2 (off), got " +
+                               std::to_string(mode));
+    impl_->fixed_inner = mode;
+}
+
+int FusedFrame::interior() const {
+    if (impl_->fixed_inner) return impl_->fixed_inner;
+    return impl_->tuned ? (impl_->tuned_inner ? 1 : 2) : 0;
 }
 
 void FusedFrame::setRows(int rows) {

@@ -225,6 +225,13 @@ public:
     // tuned value.
     void setRows(int rows);
     int rows() const;
+    // Interior tiles (inside the propagation border, away from the image
+    // edges) of a 4-row tiling in a tile body of their own: 0 = autotune
+    // with the tiling (default), 1 = on (needs 4 rows per thread on the
+    // levels below the top), 2 = off. interior() reads the fixed or tuned
+    // choice (0 before tuning).
+    void setInterior(int mode);
+    int interior() const;
     // Waits that gave up after the spin limit (a scheduling fault; results
     // of such a frame are not valid). Waits for the stream of the launches.
     unsigned long long timeouts() const;

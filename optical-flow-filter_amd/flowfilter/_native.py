@@ -238,6 +238,8 @@ SIGNATURES = {
     "ff_filter_get_tile_top_rows": [_S, _pi],
     "ff_filter_set_tile_rows": [_S, _i],
     "ff_filter_get_tile_rows": [_S, _pi],
+    "ff_filter_set_tile_interior": [_S, _i],
+    "ff_filter_get_tile_interior": [_S, _pi],
     "ff_batch_create": [_pS],
     "ff_ad_partial_count": [_i, _i, _pi],
     "ff_ad_image_model": [_v, _v, _v, _i, _i, _i, _v],

@@ -98,6 +98,18 @@ class Stage(object):
     def tileRows(self, v):
         N.call("ff_filter_set_tile_rows", self.handle, int(v))
 
+    @property
+    def tileInterior(self):
+        """Filters only: interior tiles of the fused kernel in a tile body of
+        their own (0 = tuned with the tiling, the default; 1 = on, with 4
+        rows per thread and 16 or 15 waves; 2 = off). Reads the fixed or
+        tuned choice."""
+        return N.get_int("ff_filter_get_tile_interior", self.handle)
+
+    @tileInterior.setter
+    def tileInterior(self, v):
+        N.call("ff_filter_set_tile_interior", self.handle, int(v))
+
 
 def _hnd(img):
     return img.handle if img is not None else None

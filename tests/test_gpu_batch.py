@@ -263,7 +263,8 @@ def _random_mix(k):
 def test_batch_random_mixes_match_oracle(ffo, k):
     """Twelve seeded batches of 3-8 streams of random size and depth
     (1-4 levels, per-level gamma and smoothing, maxflow 0.5-6, border 1-3),
-    replayed as graphs or launched directly, five steps each: every stream's
+    replayed as graphs or launched directly, the interior tiles' own body
+    tuned, on or off, five steps each: every stream's
     flow and brightness bitwise the oracle's after every step, and no
     in-launch wait gives up."""
     import flowfilter.gpu.flowfilters as gf
@@ -271,6 +272,7 @@ def test_batch_random_mixes_match_oracle(ffo, k):
     say("random batch", k, [(h, w, levels) for h, w, levels, *_ in mix])
     batch = gf.BatchedFlowFilter()
     batch.graphs = k % 2 == 0
+    batch.tileInterior = (0, 1, 2)[k % 3]  # interior tile body tuned, on, off
     oracles, seqs = [], []
     for i, (h, w, levels, gamma, maxflow, smooth, border) in enumerate(mix):
         f = make(h, w, levels, gamma, maxflow, smooth)

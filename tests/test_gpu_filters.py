@@ -449,25 +449,26 @@ def test_halo_beyond_a_tile_runs_stage_kernels(ffo, levels):
         check_frame(t, g.getFlow(), o.downloadFlow(), "maxflow 40")
 
 
-_TILES = [(waves, top_rows, 8) for waves in (16, 15, 8, 5, 4) for top_rows in (4, 8)] + \
-    [(waves, 4, 4) for waves in (16, 15)]
+_TILES = [(waves, top_rows, 8, 2) for waves in (16, 15, 8, 5, 4) for top_rows in (4, 8)] + \
+    [(waves, 4, 4, interior) for waves in (16, 15) for interior in (2, 1)]
 
 
 @pytest.mark.parametrize("levels", [2, 3])
-@pytest.mark.parametrize("waves,top_rows,rows", _TILES)
-def test_fused_tile_heights_match_oracle(ffo, waves, top_rows, rows, levels):
+@pytest.mark.parametrize("waves,top_rows,rows,interior", _TILES)
+def test_fused_tile_heights_match_oracle(ffo, waves, top_rows, rows, interior, levels):
     """Every tile height of the fused frame kernel, with either row count on
     the reduced top level and, in 16- and 15-wave workgroups, four rows per
-    thread on every level (the kernel held to 64 VGPRs), gives the oracle's
-    flow."""
+    thread on every level (the kernel held to 64 VGPRs), with and without
+    the interior tiles' own body, gives the oracle's flow."""
     from flowfilter import _native as N
     h, w = 300, 410
-    say("tile waves", waves, "top rows", top_rows, "rows", rows, "levels", levels)
+    say("tile waves", waves, "top rows", top_rows, "rows", rows, "interior", interior, "levels", levels)
     g, o = _pyr_pair(ffo, h, w, levels, [10.0, 30.0, 60.0][:levels], 4.0, [2, 2, 1][:levels], fused=True)
     g.tileWaves = waves
     g.tileTopRows = top_rows
     g.tileRows = rows
-    assert (g.tileWaves, g.tileTopRows, g.tileRows) == (waves, top_rows, rows)
+    g.tileInterior = interior
+    assert (g.tileWaves, g.tileTopRows, g.tileRows, g.tileInterior) == (waves, top_rows, rows, interior)
     for t, img in enumerate(seq_blobs(h, w, 5)):
         g.loadImage(img)
         N.profile_begin("fused_frame")
@@ -489,6 +490,7 @@ def test_fused_autotune_picks_a_tiling(ffo):
     assert g.tileWaves in (16, 15, 8, 5, 4)
     assert g.tileTopRows in (4, 8)
     assert g.tileRows == 8 or (g.tileRows == 4 and g.tileWaves in (16, 15))
+    assert g.tileInterior == 2 or (g.tileInterior == 1 and g.tileRows == 4)
     from flowfilter import errors
     with pytest.raises(errors.InvalidArgument):
         g.tileWaves = 7
@@ -496,6 +498,17 @@ def test_fused_autotune_picks_a_tiling(ffo):
         g.tileTopRows = 6
     with pytest.raises(errors.InvalidArgument):
         g.tileRows = 2
+    with pytest.raises(errors.InvalidArgument):
+        g.tileInterior = 3
+    # the interior body with a fixed 8-row tiling: refused at the next frame
+    f, _ = _pyr_pair(ffo, 272, 512, 2, [10.0, 50.0], 4.0, [2, 2], fused=True)
+    f.tileWaves, f.tileTopRows, f.tileRows, f.tileInterior = 8, 4, 8, 1
+    imgs = list(seq_blobs(272, 512, 2))
+    f.loadImage(imgs[0])
+    f.compute()  # the first frame runs the stage kernels
+    f.loadImage(imgs[1])
+    with pytest.raises(errors.InvalidArgument):
+        f.compute()
 
 
 def _random_config(k):
@@ -520,14 +533,16 @@ def _random_config(k):
 def test_fused_random_configurations_match_oracle(ffo, k, fused):
     """Forty-eight seeded configurations of the pyramidal filter (1-4 levels,
     odd and even sizes, per-level gamma and smoothing, maxflow 0.3-9,
-    border 1-4), six frames each, on the fused frame kernel and on the
-    stage kernels: every flow and brightness bitwise the oracle's, and no
-    in-launch wait gives up."""
+    border 1-4), six frames each, on the fused frame kernel (the interior
+    tiles' own body tuned, on or off) and on the stage kernels: every flow
+    and brightness bitwise the oracle's, and no in-launch wait gives up."""
     (h, w), levels, gamma, maxflow, smooth, border = _random_config(k)
     say("random configuration", k, (h, w), levels, gamma, maxflow, smooth, border, "fused" if fused else "staged")
     g, o = _pyr_pair(ffo, h, w, levels, gamma, maxflow, smooth, fused=fused)
     g.propagationBorder = border
     o.setPropagationBorder(border)
+    if fused:  # interior tile body tuned, forced on, forced off
+        g.tileInterior = (0, 1, 2)[k % 3]
     for t, img in enumerate(seq_blobs(h, w, 6)):
         g.loadImage(img)
         g.compute()
