@@ -220,7 +220,8 @@ def batched_bench(args, N, gf, GPUImage):
         res[mode] = px * args.batched_steps / (ms / 1e3) / 1e6
         if batch is not None:
             res["blocks"] = batch.lastBlocks()
-            res["tile"] = {"waves": batch.tileWaves, "top_rows": batch.tileTopRows, "rows": batch.tileRows}
+            res["tile"] = {"waves": batch.tileWaves, "top_rows": batch.tileTopRows, "rows": batch.tileRows,
+                           "interior": batch.tileInterior}
             res["timeouts"] = batch.waitTimeouts()
         print("%s: %.3f ms/step, %.1f Mpix/s" % (mode, ms / args.batched_steps, res[mode]), file=sys.stderr,
               flush=True)
@@ -604,7 +605,7 @@ def main():
                                "(setImageDevice)",
                    "path": "staged" if args.staged else "fused",
                    "tile": None if args.staged else {"waves": f.tileWaves, "top_rows": f.tileTopRows,
-                                                     "rows": f.tileRows},
+                                                     "rows": f.tileRows, "interior": f.tileInterior},
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
